@@ -1,0 +1,174 @@
+/*
+ * chanflow.h -- C ABI of the MI355X channel-flow DNS time step.
+ *
+ * One run = one incompressible channel flow (periodic x and y, walls in
+ * z) advanced by the reference solver's low-storage RK3 step: staggered
+ * 2nd/4th-order finite differences, implicit wall-normal diffusion and an
+ * FFT + banded-solve pressure projection.  All state lives in device memory
+ * of one GPU; host buffers are borrowed only for the duration of a call.
+ *
+ * Axis names follow the reference: x streamwise (periodic, nx cells),
+ * y spanwise (periodic, ny cells), z wall-normal (walls, nz cells).  A
+ * grid quoted "nx x nz x ny" (streamwise, wall-normal, spanwise, as in the
+ * benchmark tables) is nx, ny, nz here in the order x, y, z.
+ *
+ * Host field layout ("io layout") is the reference's restart-file layout,
+ * io.c:211-296: raw native-endian doubles, C-order [nx][ny][nz], interior
+ * cells only (no ghosts); for w the nz values are the faces above each
+ * cell, the last one being the top wall.
+ *
+ * Errors: every entry point returns a cf_status; cf_last_error() gives a
+ * thread-local message.  Nothing aborts (the reference aborts the whole
+ * MPI job in check(), check.h:8-12).
+ *
+ * Threading: calls on distinct run handles are independent; calls on one
+ * handle must be serialised by the caller.
+ */
+#ifndef CHANFLOW_H
+#define CHANFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cf_status {
+    CF_OK = 0,
+    CF_ERR_ARG = 1,          /* invalid argument / configuration */
+    CF_ERR_UNSUPPORTED = 2,  /* valid but not built (e.g. FFT length with prime factor > 5) */
+    CF_ERR_NOMEM = 3,        /* device or host allocation failed */
+    CF_ERR_DEVICE = 4,       /* HIP runtime error or no GPU */
+    CF_ERR_NUMERIC = 5,      /* zero pivot / non-finite value detected */
+    CF_ERR_STATE = 6         /* call not valid in the handle's current state */
+} cf_status;
+
+enum { CF_FP64 = 0, CF_FP32 = 1 };
+enum { CF_STRETCH_COSINE = 0, CF_STRETCH_TANH = 1, CF_STRETCH_UNIFORM = 2, CF_STRETCH_USER = 3 };
+enum { CF_U = 0, CF_V = 1, CF_W = 2, CF_C = 3, CF_P = 4 };
+
+/* Run configuration: the parameters of chanfast2/chanfast4
+ * (chanfast.h:1-21, main.c:45-61) plus the grid stretching that the
+ * reference hard-codes (chanfast.c:184-189) and the precision mode. */
+typedef struct cf_config {
+    int nx, ny, nz;          /* cells along x, y, z */
+    int order;               /* 2 or 4 */
+    int scalar;              /* 1: advance the scalar c (needed if any betg != 0) */
+    int precision;           /* CF_FP64 or CF_FP32 */
+    double lx, ly, lz;       /* box; walls at z = 0 and z = lz */
+    int stretch;             /* CF_STRETCH_* */
+    double stretch_beta;     /* tanh parameter for CF_STRETCH_TANH */
+    const double *dz_user;   /* nz cell heights for CF_STRETCH_USER (copied at create) */
+    double dt, want_cfl;
+    int cflmode;             /* 1: rescale dt every step so that CFL = want_cfl */
+    double nu, prandtl;
+    double dpdx, want_uvolavg;
+    int uvolavgmode;         /* 1: constant mass flux (bulk u = want_uvolavg) */
+    double want_cvolavg;
+    int cvolavgmode;         /* 1: constant bulk scalar */
+    double uframe, vframe;   /* wall velocities are -uframe, -vframe */
+    double betg_x, betg_y, betg_z;   /* Boussinesq buoyancy coefficients */
+    int device;              /* HIP device ordinal */
+} cf_config;
+
+/* Per-step log record, the values the reference appends to cfl.dat,
+ * pstat.dat and cstat.dat (chanfast.c:392-393, :526-529, :613-616). */
+#define CF_LOG_N 13
+/* [0] completed step number, [1] dt, [2] cfl at step start,
+ * [3] mdpdx, [4] nu dU/dz bottom wall, [5] nu dU/dz top wall, [6] bulk U, [7] centreline U,
+ * [8] dcdt, [9] kappa dC/dz bottom, [10] kappa dC/dz top, [11] bulk C, [12] centreline C */
+
+typedef struct cf_run cf_run;
+
+/* ---- host-only helpers (no GPU needed) ---- */
+
+/* Fill cfg with the reference main.c defaults scaled to a plain channel:
+ * order 2, no scalar, FP64, lx = 2 pi, ly = pi, lz = 2, cosine stretching,
+ * dt = 1e-3, nu = 1/2800, constant mass flux 1. */
+void cf_config_default(cf_config *cfg);
+
+/* Wall-normal cell heights (nz values) for cfg's stretching.
+ * Cosine follows chanfast.c:184-189; tanh places the faces at
+ * z_j = lz/2 (1 + tanh(beta (2j/nz - 1)) / tanh(beta)). */
+cf_status cf_grid_dz(const cf_config *cfg, double *dz);
+
+/* Seeded synthetic initial fields in io layout, not yet projected:
+ * u = 1.5 Ub (1 - (2 zeta/lz - 1)^2) + amp n_u, v = amp n_v, w = amp n_w
+ * (top-wall face 0), c = zeta/lz - 0.5 + camp n_c, with zeta the cell-centre
+ * height and n_* uniform in [-1, 1) from a SplitMix64 counter keyed by
+ * (seed, field, ix, iy, iz).  Any pointer may be NULL. */
+cf_status cf_initial_fields(const cf_config *cfg, uint64_t seed, double amp, double camp,
+                            double *u, double *v, double *w, double *c);
+
+/* Raw restart-file IO in the reference byte format (io.c:211-296) and
+ * its ascii logs (io.c:5-35, "%+e"). */
+cf_status cf_io_write(const char *path, const double *buf, size_t n);
+cf_status cf_io_read(const char *path, double *buf, size_t n);
+cf_status cf_io_append_ascii(const char *path, const double *vals, int n);
+
+const char *cf_last_error(void);
+const char *cf_version(void);
+
+/* ---- run API (replaces the chanfast2/chanfast4 driver, chanfast.c:71-994,
+ *      :995-1916, and its MPI/FFTW pencil machinery) ---- */
+
+cf_status cf_run_create(const cf_config *cfg, cf_run **out);
+cf_status cf_run_destroy(cf_run *run);
+
+/* Copy one field in/out in io layout.  CF_P is download-only (the
+ * pressure of the last projection). */
+cf_status cf_run_upload(cf_run *run, int field, const double *host_io);
+cf_status cf_run_download(cf_run *run, int field, double *host_io);
+
+/* Make u, v, w discretely divergence free: divergence -> FFT -> banded
+ * Poisson with the given dt -> inverse FFT -> projection, as cases.c:417-470
+ * does.  The w wall faces are set to 0 first. */
+cf_status cf_run_project(cf_run *run, double dt);
+
+/* Advance nsteps RK3 steps (asynchronous on the run's stream). */
+cf_status cf_run_step(cf_run *run, int nsteps);
+cf_status cf_run_sync(cf_run *run);
+
+/* Per-step log records of all steps done so far (at most max_steps most
+ * recent), CF_LOG_N doubles each. */
+cf_status cf_run_log(cf_run *run, double *out, int max_steps, int *nsteps_out);
+
+/* Max |divergence| over interior cells of the current velocity. */
+cf_status cf_run_max_divergence(cf_run *run, double *out);
+
+/* Current dt and completed step count. */
+cf_status cf_run_info(cf_run *run, double *dt, int *steps_done);
+
+/* Device bytes held by one run of this configuration (admission control). */
+cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes);
+
+/* Average duration (ms) of the named kernel over the last cf_run_step call
+ * when profiling is on (cf_run_set_profiling); for the roofline figures. */
+cf_status cf_run_set_profiling(cf_run *run, int on);
+cf_status cf_run_kernel_time(cf_run *run, const char *kernel, double *avg_ms, int *launches);
+
+/* ---- operator-level entry points (parity tests of single stages) ---- */
+
+/* Layout transpose io [nx][ny][nz] <-> device [nz][ny][nx] through the
+ * device (upload + LDS-tiled transpose + back); bit-exact data movement.
+ * Stands in for the pencil transposes of transpose.c:409-448. */
+cf_status cf_op_transpose_roundtrip(cf_run *run, const double *in_io, double *dev_layout_out,
+                                    double *io_again_out);
+
+/* Forward FFT3D of a real interior field (io layout, nz levels):
+ * r2c along x scaled 1/nx, c2c along y scaled 1/ny, output
+ * [ix][iy][iz][re,im] with ix < nx/2+1 (chanfast.c:21-37 ordering,
+ * fft.c:161-202 scaling).  Backward is the unscaled inverse. */
+cf_status cf_op_fft3d_fwd(cf_run *run, const double *in_io, double *spec_out);
+cf_status cf_op_fft3d_bwd(cf_run *run, const double *spec_in, double *out_io);
+
+/* Banded Poisson of pressure_poisson2/4 (pressure.c:293-444) on a spectral
+ * array [ix][iy][iz][re,im] (interior levels), in place, through the device. */
+cf_status cf_op_poisson(cf_run *run, double *spec_inout, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,159 @@
+/*
+ * cf_dev.h -- device-side types and helpers of the chanflow HIP library
+ * (gfx950 / CDNA4, wave64).
+ *
+ * Device layout of every field is z-slowest: q = (k * ny + j) * nx + i
+ * over interior cells only (i < nx, j < ny, k < nz).  Periodic x and y
+ * neighbours are taken by index wrap inside the kernels, wall ghosts are
+ * formed on the fly from the boundary closures, so no halo copies exist.
+ * Cell fields (u, v, c, p) hold the nz cells; w holds the nz faces above
+ * them, the last being the top wall (the bottom wall face is implicit 0).
+ * The spectral array holds [nz][ny][nx/2+1] complex values.
+ */
+#ifndef CF_DEV_H
+#define CF_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CF_NGT 3   /* offset of level 0 in the z metric tables (3 ghosts below) */
+
+struct Grid {
+    int nx, ny, nz, nkx;
+    int order;
+    double dx, dy;
+};
+
+/* per-substep coefficients (chanfast.c:459-463) */
+struct SubCoef {
+    double nuadt, nubdt, gamdt, zetdt, abdt;
+    double kadt, kbdt;            /* scalar: (nu/Pr) alpha dt, (nu/Pr) beta dt */
+};
+
+/* Thomas factors of one constant-coefficient tridiagonal system:
+ * d_0 = r_0 ibet_0, d_j = (r_j - a_j d_{j-1}) ibet_j, x_j = d_j - gam_{j+1} x_{j+1} */
+struct Thomas {
+    const double *a, *ibet, *gam;
+};
+
+static __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+static __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+static __device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+static __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+/* multiply by -i*sg (sg = +1 forward, -1 inverse) */
+static __device__ __forceinline__ double2 cmul_mi(double2 a, double sg) { return make_double2(sg * a.y, -sg * a.x); }
+
+static __device__ __forceinline__ int wrapm(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+/* ------------------------------------------------------------------ FFT
+ * In-place mixed-radix Stockham transform of `nlines` lines of length n
+ * held in LDS (line l at buf[l * ld], ld >= n), after the radix-r pass formulation
+ * of Stockham autosort (input i + m n/r, output (i - k) r + k + q p with
+ * k = i mod p).  Radices 2, 3, 4, 5.  tw[j] = exp(-2 pi i j / n) from host
+ * libm.  sg = +1 forward (exp(-i ...)), -1 inverse; unnormalised.  Each
+ * stage reads all of a thread's butterfly inputs into registers, barriers,
+ * then writes, so one LDS buffer suffices; MAXE bounds the elements per
+ * thread (nlines * n <= MAXE * blockDim.x). */
+template <int R>
+static __device__ __forceinline__ void dft_small(double2 *a, double sg)
+{
+    if (R == 2) {
+        double2 t = a[1];
+        a[1] = csub(a[0], t);
+        a[0] = cadd(a[0], t);
+    } else if (R == 4) {
+        double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        double2 t2 = cadd(a[1], a[3]), t3 = cmul_mi(csub(a[1], a[3]), sg);
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    } else if (R == 3) {
+        const double C3 = -0.5, S3 = 0.86602540378443864676;       /* cos, sin of 2 pi / 3 */
+        double2 t = cadd(a[1], a[2]);
+        double2 d = cmul_mi(csub(a[1], a[2]), sg * S3);
+        double2 m0 = make_double2(a[0].x + C3 * t.x, a[0].y + C3 * t.y);
+        a[0] = cadd(a[0], t);
+        a[1] = cadd(m0, d);
+        a[2] = csub(m0, d);
+    } else {  /* R == 5 */
+        const double C51 = 0.30901699437494742410, C52 = -0.80901699437494742410;
+        const double S51 = 0.95105651629515357212, S52 = 0.58778525229247312917;
+        double2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+        double2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+        double2 m1 = make_double2(a[0].x + C51 * t1.x + C52 * t2.x, a[0].y + C51 * t1.y + C52 * t2.y);
+        double2 m2 = make_double2(a[0].x + C52 * t1.x + C51 * t2.x, a[0].y + C52 * t1.y + C51 * t2.y);
+        double2 s1 = cmul_mi(make_double2(S51 * d1.x + S52 * d2.x, S51 * d1.y + S52 * d2.y), sg);
+        double2 s2 = cmul_mi(make_double2(S52 * d1.x - S51 * d2.x, S52 * d1.y - S51 * d2.y), sg);
+        a[0] = cadd(a[0], cadd(t1, t2));
+        a[1] = cadd(m1, s1);
+        a[4] = csub(m1, s1);
+        a[2] = cadd(m2, s2);
+        a[3] = csub(m2, s2);
+    }
+}
+
+template <int R, int MAXE>
+static __device__ __forceinline__ void fft_stage(double2 *buf, int n, int ld, int nlines, int p,
+                                                 const double2 *__restrict__ tw, double sg)
+{
+    constexpr int NB = (MAXE + R - 1) / R;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int T = n / R;
+    const int nb = nlines * T;
+    const int tstride = n / (p * R);
+    double2 y[NB][R];
+#pragma unroll
+    for (int it = 0; it < NB; it++) {
+        const int b = tid + it * nt;
+        if (b < nb) {
+            const int line = b / T, i = b - line * T;
+            const int k = i % p;
+            const double2 *src = buf + line * ld;
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                double2 x = src[i + m * T];
+                if (m > 0 && k > 0) {
+                    double2 w = tw[(k * m * tstride) % n];
+                    w.y *= sg;
+                    x = cmul(x, w);
+                }
+                y[it][m] = x;
+            }
+            dft_small<R>(y[it], sg);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NB; it++) {
+        const int b = tid + it * nt;
+        if (b < nb) {
+            const int line = b / T, i = b - line * T;
+            const int k = i % p;
+            double2 *dstl = buf + line * ld + (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) dstl[q * p] = y[it][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int MAXE>
+static __device__ void lds_fft(double2 *buf, int n, int ld, int nlines, const int *rad, int nrad,
+                               const double2 *__restrict__ tw, double sg)
+{
+    int p = 1;
+    for (int st = 0; st < nrad; st++) {
+        const int r = rad[st];
+        if (r == 4) fft_stage<4, MAXE>(buf, n, ld, nlines, p, tw, sg);
+        else if (r == 2) fft_stage<2, MAXE>(buf, n, ld, nlines, p, tw, sg);
+        else if (r == 3) fft_stage<3, MAXE>(buf, n, ld, nlines, p, tw, sg);
+        else fft_stage<5, MAXE>(buf, n, ld, nlines, p, tw, sg);
+        p *= r;
+    }
+}
+
+#endif

@@ -1,0 +1,79 @@
+/*
+ * cf_kernels.h -- kernel parameter blocks and kernel prototypes shared
+ * by cf_kernels.hip (definitions) and cf_run.hip (launches).
+ */
+#ifndef CF_KERNELS_H
+#define CF_KERNELS_H
+
+#include "cf_dev.h"
+
+/* slots of the per-run device scalar block */
+enum {
+    SC_M0 = 0,      /* mass-flux forcing of substeps 0..2 */
+    SC_MC0 = 3,     /* scalar-mean forcing of substeps 0..2 */
+    SC_STEP = 6,    /* completed steps */
+    SC_CFL = 7,     /* cfl at the start of the current step */
+    SC_ZERO = 8,    /* always 0 */
+    SC_N = 16
+};
+
+#define CF_MAX_RAD 16
+
+/* Everything a substep kernel needs that is fixed for the run: passed by
+ * value (a few hundred bytes of kernel arguments). */
+struct RunDev {
+    Grid g;
+    const double *dzt;          /* nz + 6: dzt[k + 3] = dz of level k (mirrored ghosts) */
+    SubCoef sc[3];
+    double nu, prandtl;
+    double ubc, vbc, cbot, ctop;
+    double lift[3][4];          /* boundary lift of u (bot, top) and v (bot, top), per substep */
+    int m_prev_slot[3];         /* scalar slot holding the previous substep's forcing */
+    const double *th;           /* Thomas factors [s][type 0..3][a, ibet, gam][nz] */
+    const double *stat1;        /* [s][kind][vol1, bot1, top1, cl1] */
+    const double *h1;           /* [s][kind][nz + 2] unit-forcing profile, level k at k + 1 */
+    int cmode[2];
+    double ctarget[2], cfixed[2];
+    int logcap;
+    /* FFT plans */
+    int radx[CF_MAX_RAD], nradx, rady[CF_MAX_RAD], nrady;
+    const double2 *twx, *twy;
+    const double *lamx, *lamy;  /* modified wavenumbers 2 - 2 cos(2 pi k / n) */
+    int fx_pairs;               /* complex x lines per block */
+    int fy_cols;                /* kx columns per y-FFT block */
+};
+
+/* z-tables of the Poisson operator for one dt (pressure.c:318-339) */
+struct PoisTab {
+    const double *a, *c, *bz, *ex, *ey, *dvol;
+    double db_neu, db_dir, db_top;
+};
+
+__global__ void k_io_to_dev(const double *io, double *dev, int nx, int ny, int nz);
+__global__ void k_dev_to_io(const double *dev, double *io, int nx, int ny, int nz);
+__global__ void k_rhs2(const RunDev R, const int s,
+                       const double *u, const double *v, const double *w,
+                       const double *gu, const double *gv, const double *gw, const double *scal,
+                       double *fu, double *fv, double *fw, double *ru, double *rv, double *rw);
+__global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw);
+__global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
+__global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
+                                double *scal, double *logbuf, int kind);
+__global__ void k_cfl_part(const RunDev R, const double *u, const double *v, const double *w,
+                           double dt, double *part, int nb);
+__global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
+                             double *scal, double *logbuf);
+__global__ void k_step_end(double *scal);
+__global__ void k_div_fftx(const RunDev R, const double *us, const double *vs, const double *ws,
+                           const double *h1, const double *mptr, double2 *X);
+__global__ void k_fftx_fwd_plain(const RunDev R, const double *a, double2 *X);
+__global__ void k_ffty(const RunDev R, double2 *X, int fwd);
+__global__ void k_poisson2(const RunDev R, const PoisTab T, double2 *Y, double *G);
+__global__ void k_fftx_inv(const RunDev R, const double2 *X, double *p);
+__global__ void k_project2(const RunDev R, const double *us, const double *vs, const double *ws,
+                           const double *h1, const double *mptr, const double *p, double dtp,
+                           double *u, double *v, double *w);
+__global__ void k_maxdiv_part(const RunDev R, const double *u, const double *v, const double *w,
+                              double *part, int nb);
+
+#endif

@@ -1,0 +1,858 @@
+/*
+ * cf_run.hip -- host runtime of the chanflow library: run handles,
+ * device memory, the column-independent coefficient tables, the
+ * hipGraph-captured RK3 step and the C ABI of include/chanflow.h.
+ *
+ * Replaces the chanfast2 driver's setup and time loop (chanfast.c:71-656)
+ * and its MPI/FFTW pencil machinery with one GPU's worth of kernels.
+ */
+#include "chanflow.h"
+#include "cf_internal_host.h"
+#include "cf_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+/* ------------------------------------------------------------ errors */
+
+static thread_local char g_err[512] = "";
+
+extern "C" void cf_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *cf_last_error(void) { return g_err; }
+extern "C" const char *cf_version(void) { return "chanflow-mi355x 0.1 (gfx950, FP64 order 2)"; }
+
+#define HIPCHK(x)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) {                                                             \
+            cf_set_error("%s:%d: %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));   \
+            return CF_ERR_DEVICE;                                                           \
+        }                                                                                   \
+    } while (0)
+
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+/* RK3 tables, rk3.c:3-6 */
+static const double RK_ALP[3] = { 29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0 };
+static const double RK_BET[3] = { 37.0 / 160.0, 5.0 / 24.0, 1.0 / 6.0 };
+static const double RK_GAM[3] = { 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0 };
+static const double RK_ZET[3] = { 0.0, -17.0 / 60.0, -5.0 / 12.0 };
+
+static const int LOGCAP = 4096;
+static const int MAXE_LINE = 2048;      /* elements per FFT block (8 per thread x 256) */
+
+/* ------------------------------------------------------------ run */
+
+struct PoisHost {
+    std::vector<double> a, c, bz, ex, ey, dvol;
+    double db_neu = 0, db_dir = 0, db_top = 0;
+};
+
+struct cf_run {
+    cf_config cfg;
+    Grid g;
+    int nz_tab;
+    std::vector<double> dz;          /* nz + 6, level k at k + 3 */
+    size_t N, P, M;                  /* cells, plane, spectral modes per level */
+    hipStream_t stream = nullptr;
+    /* device buffers */
+    double *u = nullptr, *v = nullptr, *w = nullptr;
+    double *ru = nullptr, *rv = nullptr, *rw = nullptr;
+    double *fa[3] = {nullptr, nullptr, nullptr}, *fb[3] = {nullptr, nullptr, nullptr};
+    double *p = nullptr, *G = nullptr, *stage = nullptr;
+    double2 *X = nullptr;
+    double *tab = nullptr;           /* all constant tables in one allocation */
+    double *scal = nullptr, *logbuf = nullptr, *part = nullptr;
+    size_t tab_n = 0;
+    int nb_plane = 1, nb_red = 1;
+    RunDev rd;
+    PoisTab pt[4];                   /* 3 substeps + ad-hoc dt */
+    double pt_dt = -1.0;
+    size_t off_pois_adhoc = 0;
+    /* graph */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int steps_done = 0;
+    /* profiling */
+    bool profile = false;
+    std::map<std::string, std::pair<double, int>> ktime;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+/* ------------------------------------------------------------ host tables */
+
+static int factor_radix(int n, int *rad)
+{
+    int nr = 0, m = n;
+    while (m % 4 == 0 && m > 4) { rad[nr++] = 4; m /= 4; }
+    if (m == 4) { rad[nr++] = 4; m = 1; }
+    while (m % 2 == 0) { rad[nr++] = 2; m /= 2; }
+    while (m % 3 == 0) { rad[nr++] = 3; m /= 3; }
+    while (m % 5 == 0) { rad[nr++] = 5; m /= 5; }
+    if (m != 1 || nr > CF_MAX_RAD) return -1;
+    return nr;
+}
+
+/* dz at level k (k in [-3, nz + 2]) */
+#define DZ(k) dzl[(k)]
+
+/* Thomas factors of the wall-reduced 1-D Helmholtz system of a
+ * cell-centred field, 2nd order: diffuseu.c:192-210 with the Dirichlet
+ * ghost rows of :617-682 eliminated as ghost_reduce_*2 does. */
+static void cell2_system(const double *dzl, int nz, double nudt, std::vector<double> &a,
+                         std::vector<double> &b, std::vector<double> &c)
+{
+    a.assign(nz, 0.0); b.assign(nz, 0.0); c.assign(nz, 0.0);
+    for (int k = 0; k < nz; k++) {
+        double dzp1h = 0.5 * (DZ(k + 1) + DZ(k));
+        double dzm1h = 0.5 * (DZ(k - 1) + DZ(k));
+        double dvol = DZ(k);
+        a[k] = +dvol * nudt / (dzm1h * DZ(k));
+        c[k] = +dvol * nudt / (dzp1h * DZ(k));
+        b[k] = -dvol * nudt / (dzm1h * DZ(k)) - dvol * nudt / (dzp1h * DZ(k)) + dvol;
+    }
+    double fac = a[0] / 0.5;            /* bottom ghost row a=0, b=0.5, c=0.5 */
+    a[0] -= 0.5 * fac;
+    b[0] -= 0.5 * fac;
+    fac = c[nz - 1] / 0.5;              /* top ghost row a=0.5, b=0.5, c=0 */
+    c[nz - 1] -= 0.5 * fac;
+    b[nz - 1] -= 0.5 * fac;
+}
+
+/* face-centred w, 2nd order: diffusew.c:198-214 (nz-1 rows), ghost rows b=1 */
+static void face2_system(const double *dzl, int nz, double nudt, std::vector<double> &a,
+                         std::vector<double> &b, std::vector<double> &c)
+{
+    int n = nz - 1;
+    a.assign(n, 0.0); b.assign(n, 0.0); c.assign(n, 0.0);
+    for (int k = 0; k < n; k++) {
+        double dzp1 = DZ(k + 1), dzm0 = DZ(k);
+        double dzp1h = 0.5 * (DZ(k + 1) + DZ(k));
+        double dvol = dzp1h;
+        a[k] = +dvol * nudt / (dzm0 * dzp1h);
+        c[k] = +dvol * nudt / (dzp1 * dzp1h);
+        b[k] = -dvol * nudt / (dzm0 * dzp1h) - dvol * nudt / (dzp1 * dzp1h) + dvol;
+    }
+    double fac = a[0] / 1.0;
+    a[0] -= 1.0 * fac;
+    fac = c[n - 1] / 1.0;
+    c[n - 1] -= 1.0 * fac;
+}
+
+/* tridag.c:4-26 factors: ibet_j = 1 / bet_j, gam_j = c_{j-1} / bet_{j-1} */
+static int thomas_factors(const std::vector<double> &a, const std::vector<double> &b,
+                          const std::vector<double> &c, double *A, double *IB, double *GM)
+{
+    int n = (int) b.size();
+    double bet = b[0];
+    if (bet == 0.0) return -1;
+    A[0] = a[0]; IB[0] = 1.0 / bet; GM[0] = 0.0;
+    for (int j = 1; j < n; j++) {
+        double gam = c[j - 1] / bet;
+        bet = b[j] - a[j] * gam;
+        if (bet == 0.0) return -1;
+        A[j] = a[j]; IB[j] = 1.0 / bet; GM[j] = gam;
+    }
+    return 0;
+}
+
+/* literal Thomas solve (division form) for the host-side profiles */
+static void tridag_host(const std::vector<double> &a, const std::vector<double> &b,
+                        const std::vector<double> &c, const std::vector<double> &r, std::vector<double> &x)
+{
+    int n = (int) b.size();
+    std::vector<double> gam(n);
+    x.assign(n, 0.0);
+    double bet = b[0];
+    x[0] = r[0] / bet;
+    for (int j = 1; j < n; j++) {
+        gam[j] = c[j - 1] / bet;
+        bet = b[j] - a[j] * gam[j];
+        x[j] = (r[j] - a[j] * x[j - 1]) / bet;
+    }
+    for (int j = n - 2; j >= 0; j--) x[j] -= gam[j + 1] * x[j + 1];
+}
+
+/* Poisson z-tables for one dt (pressure.c:318-339 and closures :446-531) */
+static void poisson_tables(const double *dzl, int nz, double dx, double dy, double dt, PoisHost &T)
+{
+    T.a.assign(nz, 0); T.c.assign(nz, 0); T.bz.assign(nz, 0);
+    T.ex.assign(nz, 0); T.ey.assign(nz, 0); T.dvol.assign(nz, 0);
+    for (int k = 0; k < nz; k++) {
+        double dzp1h = 0.5 * (DZ(k + 1) + DZ(k));
+        double dzm1h = 0.5 * (DZ(k - 1) + DZ(k));
+        double dvol = dx * dy * DZ(k);
+        T.a[k] = +dvol * dt / (dzm1h * DZ(k));
+        T.c[k] = +dvol * dt / (dzp1h * DZ(k));
+        T.bz[k] = -dvol * dt / (dzm1h * DZ(k)) - dvol * dt / (dzp1h * DZ(k));
+        T.ex[k] = dvol * dt / (dx * dx);
+        T.ey[k] = dvol * dt / (dy * dy);
+        T.dvol[k] = dvol;
+    }
+    {   /* NEUMANP bottom: a=0, b=-1/h, c=+1/h; b[0] -= c_g * (a[0] / b_g) */
+        double h = 0.5 * (DZ(0) + DZ(-1));
+        double bg = -1.0 / h, cg = +1.0 / h;
+        T.db_neu = cg * (T.a[0] / bg);
+    }
+    T.db_dir = 0.5 * (T.a[0] / 0.5);       /* DIRICHP bottom: b = c = 0.5 */
+    {   /* NEUMANP top: a=-1/h, b=+1/h; b[nz-1] -= a_g * (c[nz-1] / b_g) */
+        double h = 0.5 * (DZ(nz - 1) + DZ(nz));
+        double ag = -1.0 / h, bg = +1.0 / h;
+        T.db_top = ag * (T.c[nz - 1] / bg);
+    }
+}
+
+#undef DZ
+
+/* ------------------------------------------------------------ helpers */
+
+static cf_status validate(const cf_config *c)
+{
+    if (!c) { cf_set_error("null config"); return CF_ERR_ARG; }
+    if (c->nx < 2 || c->ny < 2 || c->nz < 2) { cf_set_error("need nx, ny, nz >= 2"); return CF_ERR_ARG; }
+    if (c->order != 2 && c->order != 4) { cf_set_error("order must be 2 or 4"); return CF_ERR_ARG; }
+    if (!(c->lx > 0) || !(c->ly > 0) || !(c->lz > 0) || !(c->dt > 0) || !(c->nu > 0)) {
+        cf_set_error("lx, ly, lz, dt, nu must be positive");
+        return CF_ERR_ARG;
+    }
+    if (c->scalar && !(c->prandtl > 0)) { cf_set_error("scalar needs prandtl > 0"); return CF_ERR_ARG; }
+    if (!c->scalar && (c->betg_x != 0 || c->betg_y != 0 || c->betg_z != 0)) {
+        cf_set_error("buoyancy (betg != 0) needs scalar = 1");
+        return CF_ERR_ARG;
+    }
+    if (c->order != 2) { cf_set_error("order 4 is not built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->scalar) { cf_set_error("the scalar is not built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->precision != CF_FP64) { cf_set_error("only FP64 is built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->cflmode != 0) { cf_set_error("cflmode 1 (adaptive dt) is not built in this library version"); return CF_ERR_UNSUPPORTED; }
+    int rad[CF_MAX_RAD];
+    if (c->nx % 1 || factor_radix(c->nx, rad) < 0 || c->nx > MAXE_LINE) {
+        cf_set_error("nx = %d: FFT lengths must be 2^a 3^b 5^c and <= %d", c->nx, MAXE_LINE);
+        return CF_ERR_UNSUPPORTED;
+    }
+    if (factor_radix(c->ny, rad) < 0 || c->ny > MAXE_LINE) {
+        cf_set_error("ny = %d: FFT lengths must be 2^a 3^b 5^c and <= %d", c->ny, MAXE_LINE);
+        return CF_ERR_UNSUPPORTED;
+    }
+    if (c->nz > 8000) { cf_set_error("nz > 8000"); return CF_ERR_ARG; }
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes)
+{
+    cf_status st = validate(cfg);
+    if (st != CF_OK) return st;
+    size_t N = (size_t) cfg->nx * cfg->ny * cfg->nz;
+    size_t M = (size_t) (cfg->nx / 2 + 1) * cfg->ny * cfg->nz;
+    /* u v w, ru rv rw, fa x3, fb x3, p, stage: 14 N; X: 2 M; G: M */
+    *bytes = sizeof(double) * (14 * N + 3 * M) + ((size_t) 16 << 20);
+    return CF_OK;
+}
+
+static void free_run(cf_run *r)
+{
+    if (!r) return;
+    if (r->gexec) (void) hipGraphExecDestroy(r->gexec);
+    if (r->graph) (void) hipGraphDestroy(r->graph);
+    double *bufs[] = { r->u, r->v, r->w, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2],
+                       r->fb[0], r->fb[1], r->fb[2], r->p, r->G, r->stage, r->tab, r->scal,
+                       r->logbuf, r->part };
+    for (double *b : bufs) if (b) (void) hipFree(b);
+    if (r->X) (void) hipFree(r->X);
+    for (auto &pe : r->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
+    if (r->stream) (void) hipStreamDestroy(r->stream);
+    delete r;
+}
+
+/* ------------------------------------------------------------ create */
+
+extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
+{
+    if (!out) { cf_set_error("null output handle"); return CF_ERR_ARG; }
+    *out = nullptr;
+    cf_status st = validate(cfg);
+    if (st != CF_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+        cf_set_error("no HIP device %d available", cfg->device);
+        return CF_ERR_DEVICE;
+    }
+    HIPCHK(hipSetDevice(cfg->device));
+
+    cf_run *r = new cf_run();
+    r->cfg = *cfg;
+    r->cfg.dz_user = nullptr;
+    const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    r->g.nx = nx; r->g.ny = ny; r->g.nz = nz; r->g.nkx = nx / 2 + 1; r->g.order = cfg->order;
+    r->g.dx = cfg->lx / (double) nx;            /* chanfast.c:148-149 */
+    r->g.dy = cfg->ly / (double) ny;
+    r->N = (size_t) nx * ny * nz;
+    r->P = (size_t) nx * ny;
+    r->M = (size_t) r->g.nkx * ny;
+
+    /* wall-normal metric with mirrored ghosts (chanfast.c:184-190, ghost.c:31-54) */
+    std::vector<double> dzi(nz);
+    st = cf_grid_dz(cfg, dzi.data());
+    if (st != CF_OK) { delete r; return st; }
+    r->dz.assign(nz + 6, 0.0);
+    double *dzl = r->dz.data() + CF_NGT;
+    for (int k = 0; k < nz; k++) dzl[k] = dzi[k];
+    for (int k = 1; k <= 3; k++) { dzl[-k] = dzl[k - 1]; dzl[nz - 1 + k] = dzl[nz - k]; }
+
+    const double dx = r->g.dx, dy = r->g.dy, dt = cfg->dt, nu = cfg->nu;
+
+    /* ---- host tables ---- */
+    const size_t nzt = (size_t) nz;
+    std::vector<double> th(3 * 4 * 3 * nzt, 0.0), stat1(3 * 2 * 4, 0.0), h1(3 * 2 * (nzt + 2), 0.0);
+    PoisHost ph[4];
+    for (int s = 0; s < 3; s++) {
+        SubCoef &C = r->rd.sc[s];
+        C.nuadt = nu * RK_ALP[s] * dt;
+        C.nubdt = nu * RK_BET[s] * dt;
+        C.abdt = (RK_ALP[s] + RK_BET[s]) * dt;
+        C.gamdt = RK_GAM[s] * dt;
+        C.zetdt = RK_ZET[s] * dt;
+        C.kadt = C.nuadt / (cfg->prandtl > 0 ? cfg->prandtl : 1.0);
+        C.kbdt = C.nubdt / (cfg->prandtl > 0 ? cfg->prandtl : 1.0);
+        for (int type = 0; type < 3; type++) {
+            std::vector<double> a, b, c;
+            if (type == 0) cell2_system(dzl, nz, -C.nubdt, a, b, c);
+            else if (type == 1) cell2_system(dzl, nz, -C.kbdt, a, b, c);
+            else face2_system(dzl, nz, -C.nubdt, a, b, c);
+            double *A = &th[((size_t) (s * 4 + type) * 3 + 0) * nzt];
+            double *IB = &th[((size_t) (s * 4 + type) * 3 + 1) * nzt];
+            double *GM = &th[((size_t) (s * 4 + type) * 3 + 2) * nzt];
+            if (thomas_factors(a, b, c, A, IB, GM)) {
+                cf_set_error("zero pivot in the implicit z system (substep %d)", s);
+                delete r;
+                return CF_ERR_NUMERIC;
+            }
+            if (type < 2) {
+                /* unit-forcing profile h1: h = gam dt, solve (chanfast.c:498-500) */
+                std::vector<double> rr(nz), x;
+                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
+                tridag_host(a, b, c, rr, x);
+                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
+                for (int k = 0; k < nz; k++) H[k + 1] = x[k];
+                H[0] = (0.5 * x[0] - 0.0) / -0.5;              /* ghost_fill_bot2 */
+                H[nz + 1] = (0.5 * x[nz - 1] - 0.0) / -0.5;    /* ghost_fill_top2 */
+                double kap = type == 0 ? nu : nu / cfg->prandtl;
+                double vol = 0.0, lz = 0.0;
+                for (int k = 0; k < nz; k++) { vol += H[k + 1] * dzl[k]; lz += dzl[k]; }
+                double *S = &stat1[(size_t) (s * 2 + type) * 4];
+                S[0] = vol / lz;
+                S[1] = kap * (H[1] - H[0]) / (0.5 * (dzl[0] + dzl[-1]));
+                S[2] = kap * (H[nz + 1] - H[nz]) / (0.5 * (dzl[nz] + dzl[nz - 1]));
+                S[3] = nz % 2 == 0 ? 0.5 * (H[nz / 2] + H[nz / 2 + 1]) : H[nz / 2 + 1];
+            }
+        }
+        /* boundary lift: h -= H1D(bc-only field; -nu b dt) (chanfast.c:472-477) */
+        double bcs[2] = { 0.0 - cfg->uframe, 0.0 - cfg->vframe };
+        for (int f = 0; f < 2; f++) {
+            double bc = bcs[f];
+            double g = (0.5 * 0.0 - bc) / -0.5;                 /* apply_bc_u2 on a zero field */
+            double nudt = -C.nubdt;
+            double dzp1h = 0.5 * (dzl[1] + dzl[0]), dzm1h = 0.5 * (dzl[-1] + dzl[0]);
+            double bot_next = nz > 1 ? 0.0 : g;
+            r->rd.lift[s][2 * f] = 0.0 + nudt * (((bot_next - 0.0) / dzp1h - (0.0 - g) / dzm1h) / dzl[0]);
+            dzp1h = 0.5 * (dzl[nz] + dzl[nz - 1]);
+            dzm1h = 0.5 * (dzl[nz - 2] + dzl[nz - 1]);
+            r->rd.lift[s][2 * f + 1] = 0.0 + nudt * (((g - 0.0) / dzp1h - (0.0 - 0.0) / dzm1h) / dzl[nz - 1]);
+        }
+        poisson_tables(dzl, nz, dx, dy, C.abdt, ph[s]);
+    }
+    poisson_tables(dzl, nz, dx, dy, 1.0, ph[3]);
+    r->pt_dt = 1.0;
+    r->rd.m_prev_slot[0] = SC_M0 + 2;
+    r->rd.m_prev_slot[1] = SC_M0 + 0;
+    r->rd.m_prev_slot[2] = SC_M0 + 1;
+
+    /* FFT plans: radices, twiddles (host libm), modified wavenumbers */
+    int nkx = r->g.nkx;
+    r->rd.nradx = factor_radix(nx, r->rd.radx);
+    r->rd.nrady = factor_radix(ny, r->rd.rady);
+    std::vector<double> twx(2 * nx), twy(2 * ny), lamx(nkx), lamy(ny);
+    for (int j = 0; j < nx; j++) {
+        double th_ = 2.0 * M_PI * (double) j / (double) nx;
+        twx[2 * j] = cos(th_); twx[2 * j + 1] = -sin(th_);
+    }
+    for (int j = 0; j < ny; j++) {
+        double th_ = 2.0 * M_PI * (double) j / (double) ny;
+        twy[2 * j] = cos(th_); twy[2 * j + 1] = -sin(th_);
+    }
+    for (int ix = 0; ix < nkx; ix++) lamx[ix] = 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) ix / (double) nx);
+    for (int iy = 0; iy < ny; iy++) lamy[iy] = 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) iy / (double) ny);
+
+    /* pack tables: dz | th | stat1 | h1 | twx | twy | lamx | lamy | pois x4 */
+    std::vector<double> pack;
+    auto put = [&](const double *src, size_t n) { size_t o = pack.size(); pack.insert(pack.end(), src, src + n); return o; };
+    size_t o_dz = put(r->dz.data(), r->dz.size());
+    size_t o_th = put(th.data(), th.size());
+    size_t o_st = put(stat1.data(), stat1.size());
+    size_t o_h1 = put(h1.data(), h1.size());
+    if (pack.size() % 2) pack.push_back(0.0);       /* 16-byte align the complex twiddles */
+    size_t o_twx = put(twx.data(), twx.size());
+    size_t o_twy = put(twy.data(), twy.size());
+    size_t o_lx = put(lamx.data(), lamx.size());
+    size_t o_ly = put(lamy.data(), lamy.size());
+    size_t o_p[4][6];
+    for (int q = 0; q < 4; q++) {
+        o_p[q][0] = put(ph[q].a.data(), nz); o_p[q][1] = put(ph[q].c.data(), nz);
+        o_p[q][2] = put(ph[q].bz.data(), nz); o_p[q][3] = put(ph[q].ex.data(), nz);
+        o_p[q][4] = put(ph[q].ey.data(), nz); o_p[q][5] = put(ph[q].dvol.data(), nz);
+    }
+    r->off_pois_adhoc = o_p[3][0];
+    r->tab_n = pack.size();
+
+    /* ---- device memory ---- */
+    auto dalloc = [&](double **pp, size_t n) -> bool { return hipMalloc((void **) pp, sizeof(double) * (n ? n : 1)) == hipSuccess; };
+    bool ok = true;
+    ok = ok && dalloc(&r->u, r->N) && dalloc(&r->v, r->N) && dalloc(&r->w, r->N);
+    ok = ok && dalloc(&r->ru, r->N) && dalloc(&r->rv, r->N) && dalloc(&r->rw, r->N);
+    for (int f = 0; f < 3; f++) ok = ok && dalloc(&r->fa[f], r->N) && dalloc(&r->fb[f], r->N);
+    ok = ok && dalloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * nz);
+    ok = ok && hipMalloc((void **) &r->X, sizeof(double2) * r->M * nz) == hipSuccess;
+    ok = ok && dalloc(&r->tab, r->tab_n) && dalloc(&r->scal, SC_N) && dalloc(&r->logbuf, (size_t) LOGCAP * CF_LOG_N);
+    r->nb_plane = (int) ((r->P + 2047) / 2048);
+    if (r->nb_plane > 1024) r->nb_plane = 1024;
+    r->nb_red = 1024;
+    size_t npart = (size_t) r->nb_plane * (nz + 2);
+    if (npart < (size_t) r->nb_red) npart = r->nb_red;
+    ok = ok && dalloc(&r->part, npart);
+    if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
+    }
+    if (hipMemcpy(r->tab, pack.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice) != hipSuccess) {
+        free_run(r); cf_set_error("table upload failed"); return CF_ERR_DEVICE;
+    }
+    /* zero state, tendencies and scalars (finite values: g meets zeta_0 = 0) */
+    std::vector<double> zeros(SC_N, 0.0);
+    bool zok = hipMemcpy(r->scal, zeros.data(), sizeof(double) * SC_N, hipMemcpyHostToDevice) == hipSuccess;
+    double *zb[] = { r->u, r->v, r->w, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2], r->fb[0], r->fb[1], r->fb[2], r->p };
+    for (double *b : zb) zok = zok && hipMemset(b, 0, sizeof(double) * r->N) == hipSuccess;
+    zok = zok && hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N) == hipSuccess;
+    if (!zok || hipDeviceSynchronize() != hipSuccess) { free_run(r); cf_set_error("device init failed"); return CF_ERR_DEVICE; }
+
+    /* ---- kernel parameter block ---- */
+    RunDev &R = r->rd;
+    R.g = r->g;
+    R.dzt = r->tab + o_dz;
+    R.nu = nu;
+    R.prandtl = cfg->prandtl;
+    R.ubc = 0.0 - cfg->uframe;                  /* chanfast.c:59-62 */
+    R.vbc = 0.0 - cfg->vframe;
+    R.cbot = -0.5; R.ctop = 0.5;                /* chanfast.c:67-68 */
+    R.th = r->tab + o_th;
+    R.stat1 = r->tab + o_st;
+    R.h1 = r->tab + o_h1;
+    R.cmode[0] = cfg->uvolavgmode; R.ctarget[0] = cfg->want_uvolavg - cfg->uframe; R.cfixed[0] = -cfg->dpdx;
+    R.cmode[1] = cfg->cvolavgmode; R.ctarget[1] = cfg->want_cvolavg; R.cfixed[1] = 0.0;
+    R.logcap = LOGCAP;
+    R.twx = (const double2 *) (r->tab + o_twx);
+    R.twy = (const double2 *) (r->tab + o_twy);
+    R.lamx = r->tab + o_lx;
+    R.lamy = r->tab + o_ly;
+    R.fx_pairs = MAXE_LINE / nx > 0 ? MAXE_LINE / nx : 1;
+    R.fy_cols = MAXE_LINE / ny >= 8 ? 8 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
+    for (int q = 0; q < 4; q++) {
+        PoisTab &T = r->pt[q];
+        T.a = r->tab + o_p[q][0]; T.c = r->tab + o_p[q][1]; T.bz = r->tab + o_p[q][2];
+        T.ex = r->tab + o_p[q][3]; T.ey = r->tab + o_p[q][4]; T.dvol = r->tab + o_p[q][5];
+        T.db_neu = ph[q].db_neu; T.db_dir = ph[q].db_dir; T.db_top = ph[q].db_top;
+    }
+    *out = r;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_destroy(cf_run *r)
+{
+    if (!r) return CF_OK;
+    (void) hipSetDevice(r->cfg.device);
+    if (r->stream) (void) hipStreamSynchronize(r->stream);
+    free_run(r);
+    return CF_OK;
+}
+
+/* ------------------------------------------------------------ launches */
+
+static cf_status ev_begin(cf_run *r, const char *name)
+{
+    if (!r->profile) return CF_OK;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, r->stream));
+    r->pending.push_back({ std::string(name), { a, b } });
+    return CF_OK;
+}
+
+static cf_status ev_end(cf_run *r)
+{
+    if (!r->profile) return CF_OK;
+    HIPCHK(hipEventRecord(r->pending.back().second.second, r->stream));
+    return CF_OK;
+}
+
+#define KLAUNCH(name, kern, grid, block, lds, ...)                                 \
+    do {                                                                            \
+        cf_status s_ = ev_begin(r, name);                                           \
+        if (s_ != CF_OK) return s_;                                                 \
+        hipLaunchKernelGGL(kern, grid, block, lds, r->stream, __VA_ARGS__);         \
+        LAUNCHCHK();                                                                \
+        s_ = ev_end(r);                                                             \
+        if (s_ != CF_OK) return s_;                                                 \
+    } while (0)
+
+static dim3 cell_grid(const cf_run *r) { return dim3((r->g.nx + 63) / 64, (r->g.ny + 3) / 4, r->g.nz); }
+
+static size_t fx_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fx_pairs * (r->g.nx + 1); }
+static size_t fy_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fy_cols * (r->g.ny + 1); }
+
+/* divergence -> FFT -> Poisson -> inverse FFT -> projection into (u, v, w) */
+static cf_status enqueue_projection(cf_run *r, const double *us, const double *vs, const double *ws,
+                                    const double *h1, const double *mptr, const PoisTab &T, double dtp)
+{
+    const RunDev &R = r->rd;
+    long NL = (long) r->g.ny * r->g.nz;
+    long npairs = (NL + 1) / 2;
+    unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
+    KLAUNCH("div_fftx", k_div_fftx, dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, r->X);
+    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
+    KLAUNCH("ffty_fwd", k_ffty, gy, dim3(256), fy_lds(r), R, r->X, 1);
+    KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, T, r->X, r->G);
+    KLAUNCH("ffty_inv", k_ffty, gy, dim3(256), fy_lds(r), R, r->X, 0);
+    KLAUNCH("fftx_inv", k_fftx_inv, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
+    KLAUNCH("project", k_project2, cell_grid(r), dim3(64, 4), 0, R, us, vs, ws, h1, mptr,
+            (const double *) r->p, dtp, r->u, r->v, r->w);
+    return CF_OK;
+}
+
+static cf_status enqueue_step(cf_run *r)
+{
+    const RunDev &R = r->rd;
+    const int nz = r->g.nz;
+    const double dt = r->cfg.dt;
+    KLAUNCH("cfl", k_cfl_part, dim3(r->nb_red), dim3(256), 0, R, (const double *) r->u,
+            (const double *) r->v, (const double *) r->w, dt, r->part, r->nb_red);
+    KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(64), 0, R, (const double *) r->part, r->nb_red, dt,
+            r->scal, r->logbuf);
+    for (int s = 0; s < 3; s++) {
+        double **fnew = (s % 2 == 0) ? r->fa : r->fb;
+        double **gold = (s % 2 == 0) ? r->fb : r->fa;
+        KLAUNCH("rhs", k_rhs2, cell_grid(r), dim3(64, 4), 0, R, s,
+                (const double *) r->u, (const double *) r->v, (const double *) r->w,
+                (const double *) gold[0], (const double *) gold[1], (const double *) gold[2],
+                (const double *) r->scal, fnew[0], fnew[1], fnew[2], r->ru, r->rv, r->rw);
+        KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s,
+                r->ru, r->rv, r->rw);
+        KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
+                (const double *) r->ru, R.ubc, r->part, r->nb_plane);
+        KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
+                (const double *) r->part, r->nb_plane, r->scal, r->logbuf, 0);
+        const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
+        cf_status st = enqueue_projection(r, r->ru, r->rv, r->rw, h1, r->scal + SC_M0 + s, r->pt[s],
+                                          R.sc[s].abdt);
+        if (st != CF_OK) return st;
+    }
+    KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
+    return CF_OK;
+}
+
+static cf_status collect_profile(cf_run *r)
+{
+    if (!r->profile) return CF_OK;
+    HIPCHK(hipStreamSynchronize(r->stream));
+    for (auto &pe : r->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+        auto &acc = r->ktime[pe.first];
+        acc.first += ms;
+        acc.second += 1;
+        (void) hipEventDestroy(pe.second.first);
+        (void) hipEventDestroy(pe.second.second);
+    }
+    r->pending.clear();
+    return CF_OK;
+}
+
+/* ------------------------------------------------------------ C ABI */
+
+#define CHECK_RUN(r)                                                  \
+    do {                                                              \
+        if (!(r)) { cf_set_error("null run handle"); return CF_ERR_ARG; } \
+        HIPCHK(hipSetDevice((r)->cfg.device));                        \
+    } while (0)
+
+extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
+{
+    CHECK_RUN(r);
+    if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
+    double *dst = field == CF_U ? r->u : field == CF_V ? r->v : field == CF_W ? r->w : nullptr;
+    if (!dst) { cf_set_error("cf_run_upload: field %d is not uploadable in this configuration", field); return CF_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(r->stage, host, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
+    dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
+    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, dst,
+                       r->g.nx, r->g.ny, r->g.nz);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return CF_OK;
+}
+
+static cf_status download_dev(cf_run *r, const double *src, double *host)
+{
+    dim3 grid((r->g.nx + 31) / 32, (r->g.nz + 31) / 32, r->g.ny);
+    hipLaunchKernelGGL(k_dev_to_io, grid, dim3(256), 0, r->stream, src, r->stage, r->g.nx, r->g.ny, r->g.nz);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(host, r->stage, sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
+{
+    CHECK_RUN(r);
+    if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
+    const double *src = field == CF_U ? r->u : field == CF_V ? r->v : field == CF_W ? r->w
+                      : field == CF_P ? r->p : nullptr;
+    if (!src) { cf_set_error("cf_run_download: field %d not held in this configuration", field); return CF_ERR_ARG; }
+    return download_dev(r, src, host);
+}
+
+static cf_status set_adhoc_poisson(cf_run *r, double dt)
+{
+    if (r->pt_dt == dt) return CF_OK;
+    PoisHost ph;
+    poisson_tables(r->dz.data() + CF_NGT, r->g.nz, r->g.dx, r->g.dy, dt, ph);
+    int nz = r->g.nz;
+    std::vector<double> pk;
+    for (auto *vv : { &ph.a, &ph.c, &ph.bz, &ph.ex, &ph.ey, &ph.dvol }) pk.insert(pk.end(), vv->begin(), vv->end());
+    HIPCHK(hipMemcpy(r->tab + r->off_pois_adhoc, pk.data(), sizeof(double) * 6 * nz, hipMemcpyHostToDevice));
+    r->pt[3].db_neu = ph.db_neu; r->pt[3].db_dir = ph.db_dir; r->pt[3].db_top = ph.db_top;
+    r->pt_dt = dt;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_project(cf_run *r, double dt)
+{
+    CHECK_RUN(r);
+    if (!(dt > 0)) { cf_set_error("cf_run_project: dt must be positive"); return CF_ERR_ARG; }
+    cf_status st = set_adhoc_poisson(r, dt);
+    if (st != CF_OK) return st;
+    st = enqueue_projection(r, r->u, r->v, r->w, nullptr, nullptr, r->pt[3], dt);
+    if (st != CF_OK) return st;
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return collect_profile(r);
+}
+
+extern "C" cf_status cf_run_step(cf_run *r, int nsteps)
+{
+    CHECK_RUN(r);
+    if (nsteps < 0) { cf_set_error("negative step count"); return CF_ERR_ARG; }
+    if (nsteps == 0) return CF_OK;
+    if (r->profile) {
+        for (int n = 0; n < nsteps; n++) {
+            cf_status st = enqueue_step(r);
+            if (st != CF_OK) return st;
+        }
+        r->steps_done += nsteps;
+        return collect_profile(r);
+    }
+    if (!r->gexec) {
+        HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        cf_status st = enqueue_step(r);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(r->stream, &gr);
+        if (st != CF_OK) { if (gr) (void) hipGraphDestroy(gr); return st; }
+        HIPCHK(e);
+        r->graph = gr;
+        HIPCHK(hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0));
+    }
+    for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(r->gexec, r->stream));
+    r->steps_done += nsteps;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_sync(cf_run *r)
+{
+    CHECK_RUN(r);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_log(cf_run *r, double *out, int max_steps, int *nsteps_out)
+{
+    CHECK_RUN(r);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    int have = r->steps_done < LOGCAP ? r->steps_done : LOGCAP;
+    int n = have < max_steps ? have : max_steps;
+    if (nsteps_out) *nsteps_out = n;
+    if (!out || n <= 0) return CF_OK;
+    std::vector<double> all((size_t) LOGCAP * CF_LOG_N);
+    HIPCHK(hipMemcpy(all.data(), r->logbuf, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; q++) {
+        int step = r->steps_done - n + q;
+        memcpy(out + (size_t) q * CF_LOG_N, all.data() + (size_t) (step % LOGCAP) * CF_LOG_N,
+               sizeof(double) * CF_LOG_N);
+    }
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_max_divergence(cf_run *r, double *out)
+{
+    CHECK_RUN(r);
+    if (!out) { cf_set_error("null output"); return CF_ERR_ARG; }
+    hipLaunchKernelGGL(k_maxdiv_part, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd,
+                       (const double *) r->u, (const double *) r->v, (const double *) r->w, r->part, r->nb_red);
+    LAUNCHCHK();
+    std::vector<double> part(r->nb_red);
+    HIPCHK(hipMemcpyAsync(part.data(), r->part, sizeof(double) * r->nb_red, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    double m = 0.0;
+    for (double x : part) m = fmax(m, x);
+    *out = m;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_info(cf_run *r, double *dt, int *steps_done)
+{
+    if (!r) { cf_set_error("null run handle"); return CF_ERR_ARG; }
+    if (dt) *dt = r->cfg.dt;
+    if (steps_done) *steps_done = r->steps_done;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_set_profiling(cf_run *r, int on)
+{
+    if (!r) { cf_set_error("null run handle"); return CF_ERR_ARG; }
+    r->profile = on != 0;
+    r->ktime.clear();
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_kernel_time(cf_run *r, const char *kernel, double *avg_ms, int *launches)
+{
+    if (!r || !kernel) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    auto it = r->ktime.find(kernel);
+    if (it == r->ktime.end()) { if (avg_ms) *avg_ms = 0; if (launches) *launches = 0; return CF_OK; }
+    if (avg_ms) *avg_ms = it->second.first / (it->second.second ? it->second.second : 1);
+    if (launches) *launches = it->second.second;
+    return CF_OK;
+}
+
+/* ------------------------------------------------------------ operators */
+
+extern "C" cf_status cf_op_transpose_roundtrip(cf_run *r, const double *in_io, double *dev_out, double *io_out)
+{
+    CHECK_RUN(r);
+    if (!in_io) { cf_set_error("null input"); return CF_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
+    dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
+    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->p,
+                       r->g.nx, r->g.ny, r->g.nz);
+    LAUNCHCHK();
+    if (dev_out) HIPCHK(hipMemcpyAsync(dev_out, r->p, sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    if (io_out) return download_dev(r, r->p, io_out);
+    return CF_OK;
+}
+
+/* host [ix][iy][iz][2] <-> device [iz][iy][ix] complex */
+static void spec_host_to_dev(const cf_run *r, const double *h, std::vector<double2> &d)
+{
+    int nkx = r->g.nkx, ny = r->g.ny, nz = r->g.nz;
+    d.resize(r->M * nz);
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int iz = 0; iz < nz; iz++) {
+                size_t o = (((size_t) ix * ny + iy) * nz + iz) * 2;
+                d[((size_t) iz * ny + iy) * nkx + ix] = make_double2(h[o], h[o + 1]);
+            }
+}
+
+static void spec_dev_to_host(const cf_run *r, const std::vector<double2> &d, double *h)
+{
+    int nkx = r->g.nkx, ny = r->g.ny, nz = r->g.nz;
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int iz = 0; iz < nz; iz++) {
+                size_t o = (((size_t) ix * ny + iy) * nz + iz) * 2;
+                double2 val = d[((size_t) iz * ny + iy) * nkx + ix];
+                h[o] = val.x; h[o + 1] = val.y;
+            }
+}
+
+extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spec_out)
+{
+    CHECK_RUN(r);
+    if (!in_io || !spec_out) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
+    dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
+    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->p,
+                       r->g.nx, r->g.ny, r->g.nz);
+    LAUNCHCHK();
+    const RunDev &R = r->rd;
+    long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
+    unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
+    hipLaunchKernelGGL(k_fftx_fwd_plain, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double *) r->p, r->X);
+    LAUNCHCHK();
+    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
+    hipLaunchKernelGGL(k_ffty, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 1);
+    LAUNCHCHK();
+    std::vector<double2> d(r->M * r->g.nz);
+    HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    spec_dev_to_host(r, d, spec_out);
+    return CF_OK;
+}
+
+extern "C" cf_status cf_op_fft3d_bwd(cf_run *r, const double *spec_in, double *out_io)
+{
+    CHECK_RUN(r);
+    if (!spec_in || !out_io) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    std::vector<double2> d;
+    spec_host_to_dev(r, spec_in, d);
+    HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
+    const RunDev &R = r->rd;
+    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
+    hipLaunchKernelGGL(k_ffty, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 0);
+    LAUNCHCHK();
+    long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
+    unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
+    hipLaunchKernelGGL(k_fftx_inv, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double2 *) r->X, r->p);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return download_dev(r, r->p, out_io);
+}
+
+extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
+{
+    CHECK_RUN(r);
+    if (!spec || !(dt > 0)) { cf_set_error("bad argument"); return CF_ERR_ARG; }
+    cf_status st = set_adhoc_poisson(r, dt);
+    if (st != CF_OK) return st;
+    std::vector<double2> d;
+    spec_host_to_dev(r, spec, d);
+    HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
+    hipLaunchKernelGGL(k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+                       r->rd, r->pt[3], r->X, r->G);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    spec_dev_to_host(r, d, spec);
+    return CF_OK;
+}

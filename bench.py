@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: RK3 steps of the channel DNS at 256 x 193 x 256 on one MI355X.
+
+Workload (x, wall-normal, span = 256, 193, 256 cells; nx, nz, ny in the
+solver's axis names): FP64, 2nd order, Lx = 2 pi, Ly(span) = pi, Lz = 2,
+tanh wall stretching beta = 2, nu = 1/6900, dt = 2.5e-4, constant mass
+flux Ub = 1, seeded laminar + 0.1 noise projected to zero divergence.
+
+Prints one JSON line: Mpt.RK3-steps/s (whole job, inputs resident in HBM),
+the dominant kernel's HBM roofline fraction from HIP-event kernel times,
+and the single-thread CPU baseline (the oracle) on a bounded sample.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ubiquitous-spoon_amd"))
+
+import numpy as np  # noqa: E402
+
+import chanflow as cf  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+STEP_BYTES_PER_PT = 720.0      # algorithmic FP64 bytes per point per RK3 step, order 2
+
+
+def alg_bytes(kernel, N, M):
+    """Algorithmic bytes of one launch (each array read or written once)."""
+    table = {
+        "rhs": 96 * N,              # read u v w g_u g_v g_w, write f_u f_v f_w r_u r_v r_w
+        "thomas": 48 * N,           # read r (3 fields), write solution
+        "plane_sum": 8 * N,         # read u*
+        "div_fftx": 24 * N + 16 * M,
+        "ffty_fwd": 32 * M,
+        "ffty_inv": 32 * M,
+        "poisson": 32 * M,
+        "fftx_inv": 16 * M + 8 * N,
+        "project": 56 * N,          # read u* v* w* p, write u v w
+        "cfl": 24 * N,
+    }
+    return table.get(kernel)
+
+
+def workload_config():
+    return cf.Config(nx=256, ny=256, nz=193, order=2, stretch="tanh", stretch_beta=2.0,
+                     lx=2 * np.pi, ly=np.pi, lz=2.0, nu=1.0 / 6900.0, dt=2.5e-4,
+                     want_uvolavg=1.0, uvolavgmode=1)
+
+
+def cpu_baseline(cfg, init, nsteps):
+    """The CPU oracle (single thread) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    orc = ol.OracleRun(ol.oracle_cfg(cfg, cf.grid_dz(cfg)))
+    for n in "uvw":
+        orc.put(n, init[n])
+    t0 = time.perf_counter()
+    orc.steps(nsteps)
+    t1 = time.perf_counter()
+    orc.close()
+    npts = cfg.nx * cfg.ny * cfg.nz
+    return {"value": npts / 1e6 * nsteps / (t1 - t0), "unit": "Mpt*RK3-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{nsteps} RK3 step(s) of the same 256x193x256 FP64 workload on the "
+                      f"single-threaded C oracle ({t1 - t0:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "multi-GPU sharding is a design note, not built", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = workload_config()
+    N = cfg.nx * cfg.ny * cfg.nz
+    M = (cfg.nx // 2 + 1) * cfg.ny * cfg.nz
+    f = cf.initial_fields(cfg, seed=1234, amp=0.1)
+    run = cf.Run(cfg)
+    for n in "uvw":
+        run.upload(n, f[n])
+    run.project(1.0)
+    init = {n: run.download(n) for n in "uvw"}
+    print(f"[bench] setup done, max div {run.max_divergence():.2e}", file=sys.stderr, flush=True)
+
+    run.step(args.warmup)
+    run.sync()
+    t0 = time.perf_counter()
+    run.step(args.steps)
+    run.sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    ms_per_step = 1e3 * secs / args.steps
+    value = N / 1e6 * args.steps / secs
+    print(f"[bench] {args.steps} steps in {secs:.3f} s ({ms_per_step:.3f} ms/step)", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event times on the run's stream, same workload, right after
+    run.set_profiling(True)
+    run.step(args.profile_steps)
+    run.sync()
+    kt = {}
+    for name in ("cfl", "rhs", "thomas", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
+                 "ffty_inv", "fftx_inv", "project"):
+        ms, n = run.kernel_time(name)
+        if n:
+            kt[name] = (ms, n)
+    run.set_profiling(False)
+    per_step = {k: v[0] * v[1] / args.profile_steps for k, v in kt.items()}
+    dom = max((k for k in per_step if alg_bytes(k, N, M)), key=lambda k: per_step[k])
+    dom_ms = kt[dom][0]
+    dom_bytes = alg_bytes(dom, N, M)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    print("[bench] kernel ms/launch: " + ", ".join(f"{k}={v[0]:.4f}" for k, v in kt.items()),
+          file=sys.stderr, flush=True)
+    div = run.max_divergence()
+    run.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(cfg, init, args.cpu_steps)
+
+    out = {
+        "metric": "Mpt*RK3-steps/s (256x193x256 channel, FP64 order 2)",
+        "value": value,
+        "unit": "Mpt*RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded laminar + 0.1 noise, projected)",
+        "config": {"workload": "channel DNS RK3 step, 256x193x256 (x, wall-normal, span), FP64, order 2, "
+                               "tanh beta=2, nu=1/6900, dt=2.5e-4, Ub=1",
+                   "nx": cfg.nx, "ny_span": cfg.ny, "nz_wall": cfg.nz, "order": 2,
+                   "step_hbm_frac": STEP_BYTES_PER_PT * N * args.steps / secs / (HBM_PEAK_GBS * 1e9),
+                   "max_div_end": div},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "kernel_ms_per_step": per_step,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

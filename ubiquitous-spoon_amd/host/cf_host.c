@@ -45,14 +45,16 @@ cf_status cf_grid_dz(const cf_config *cfg, double *dz)
                                - cos(M_PI * (double) (iz + 1) / (double) nz));
         return CF_OK;
     case CF_STRETCH_TANH: {
+        /* faces z_j = lz/2 (1 + tanh(beta x_j) / tanh(beta)), x_j = (2j - nz) / nz:
+         * x is exactly antisymmetric and tanh odd, so dz mirrors exactly */
         double b = cfg->stretch_beta;
         if (!(b > 0.0)) { cf_set_error("cf_grid_dz: tanh stretching needs beta > 0"); return CF_ERR_ARG; }
         double tb = tanh(b);
-        double zlo = 0.5 * lz * (1.0 + tanh(b * (-1.0)) / tb);
+        double tlo = tanh(b * ((double) (0 - nz) / (double) nz));
         for (int iz = 0; iz < nz; iz++) {
-            double zhi = 0.5 * lz * (1.0 + tanh(b * (2.0 * (double) (iz + 1) / (double) nz - 1.0)) / tb);
-            dz[iz] = zhi - zlo;
-            zlo = zhi;
+            double thi = tanh(b * ((double) (2 * (iz + 1) - nz) / (double) nz));
+            dz[iz] = 0.5 * lz * (thi - tlo) / tb;
+            tlo = thi;
         }
         return CF_OK;
     }

@@ -33,6 +33,7 @@ def alg_bytes(kernel, N, M):
     """Algorithmic bytes of one launch (each array read or written once)."""
     table = {
         "rhs": 96 * N,              # read u v w g_u g_v g_w, write f_u f_v f_w r_u r_v r_w
+        "spike": 48 * N,            # read r (3 fields), write solution
         "thomas": 48 * N,           # read r (3 fields), write solution
         "plane_sum": 8 * N,         # read u*
         "div_fftx": 24 * N + 16 * M,
@@ -111,7 +112,7 @@ def main():
     run.step(args.profile_steps)
     run.sync()
     kt = {}
-    for name in ("cfl", "rhs", "thomas", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
+    for name in ("cfl", "rhs", "spike", "thomas", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
                  "ffty_inv", "fftx_inv", "project"):
         ms, n = run.kernel_time(name)
         if n:

@@ -156,4 +156,103 @@ static __device__ void lds_fft(double2 *buf, int n, int ld, int nlines, const in
     }
 }
 
+/* ---- compile-time plans: the same radix order as the host's factor_radix
+ * (4s first, then 2, 3, 5), so butterfly index math folds to constants */
+constexpr int ct_rad_count(int n)
+{
+    int c = 0;
+    while (n % 4 == 0 && n > 4) { n /= 4; c++; }
+    if (n == 4) return c + 1;
+    while (n % 2 == 0) { n /= 2; c++; }
+    while (n % 3 == 0) { n /= 3; c++; }
+    while (n % 5 == 0) { n /= 5; c++; }
+    return c;
+}
+
+constexpr int ct_rad_at(int n, int idx)
+{
+    int c = 0;
+    while (n % 4 == 0 && n > 4) { if (c == idx) return 4; n /= 4; c++; }
+    if (n == 4) return 4;
+    while (n % 2 == 0) { if (c == idx) return 2; n /= 2; c++; }
+    while (n % 3 == 0) { if (c == idx) return 3; n /= 3; c++; }
+    while (n % 5 == 0) { if (c == idx) return 5; n /= 5; c++; }
+    return 1;
+}
+
+template <int N, int R, int P, int MAXE>
+static __device__ __forceinline__ void fft_stage_ct(double2 *buf, int ld, int nlines,
+                                                    const double2 *__restrict__ tw, double sg)
+{
+    constexpr int NB = (MAXE + R - 1) / R;
+    constexpr int T = N / R;
+    constexpr int TS = N / (P * R);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nb = nlines * T;
+    double2 y[NB][R];
+#pragma unroll
+    for (int it = 0; it < NB; it++) {
+        const int b = tid + it * nt;
+        if (b < nb) {
+            const int line = b / T, i = b - line * T;
+            const int k = i % P;
+            const double2 *src = buf + line * ld;
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                double2 x = src[i + m * T];
+                if (m > 0 && P > 1) {
+                    double2 w = tw[k * m * TS];
+                    w.y *= sg;
+                    x = cmul(x, w);
+                }
+                y[it][m] = x;
+            }
+            dft_small<R>(y[it], sg);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NB; it++) {
+        const int b = tid + it * nt;
+        if (b < nb) {
+            const int line = b / T, i = b - line * T;
+            const int k = i % P;
+            double2 *dstl = buf + line * ld + (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) dstl[q * P] = y[it][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int MAXE, int ST = 0, int P = 1>
+static __device__ __forceinline__ void lds_fft_ct(double2 *buf, int ld, int nlines,
+                                                  const double2 *__restrict__ tw, double sg)
+{
+    if constexpr (ST < ct_rad_count(N)) {
+        constexpr int R = ct_rad_at(N, ST);
+        fft_stage_ct<N, R, P, MAXE>(buf, ld, nlines, tw, sg);
+        lds_fft_ct<N, MAXE, ST + 1, P * R>(buf, ld, nlines, tw, sg);
+    }
+}
+
+/* N > 0: compile-time plan; N == 0: runtime plan (any 2^a 3^b 5^c) */
+template <int N, int MAXE>
+static __device__ __forceinline__ void fft_lines(double2 *buf, int n, int ld, int nlines, const int *rad,
+                                                 int nrad, const double2 *__restrict__ tw, double sg)
+{
+    if constexpr (N > 0) lds_fft_ct<N, MAXE>(buf, ld, nlines, tw, sg);
+    else lds_fft<MAXE>(buf, n, ld, nlines, rad, nrad, tw, sg);
+}
+
+/* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+ * L2); renumber so that consecutive logical blocks share an XCD and the
+ * halo rows they exchange stay in one L2. */
+static __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
+{
+    const unsigned per = nblocks / 8u;
+    if (per == 0 || b >= per * 8u) return b;
+    return (b % 8u) * per + b / 8u;
+}
+
 #endif

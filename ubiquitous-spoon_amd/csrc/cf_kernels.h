@@ -18,6 +18,17 @@ enum {
 };
 
 #define CF_MAX_RAD 16
+#define CF_SPIKE_LMAX 32        /* rows per chunk held in registers */
+#define CF_SPIKE_SMAX 8         /* chunks (waves) per workgroup */
+
+/* Partitioned (spike) solve of one constant-coefficient tridiagonal
+ * system split into S chunks of L rows: per-chunk Thomas factors, the
+ * left/right spike vectors and the inverse of the 2S x 2S interface
+ * system, all column-independent (host-built). */
+struct SpikeTab {
+    int n, S, L;
+    const double *a, *ib, *gam, *vl, *wr, *minv;
+};
 
 /* Everything a substep kernel needs that is fixed for the run: passed by
  * value (a few hundred bytes of kernel arguments). */
@@ -30,6 +41,10 @@ struct RunDev {
     double lift[3][4];          /* boundary lift of u (bot, top) and v (bot, top), per substep */
     int m_prev_slot[3];         /* scalar slot holding the previous substep's forcing */
     const double *th;           /* Thomas factors [s][type 0..3][a, ibet, gam][nz] */
+    SpikeTab sp[3][3];          /* [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
+    const double *rdz;          /* 1 / dz, level-indexed like dzt (offset CF_NGT) */
+    const double *rdzh;         /* 1 / (0.5 (dz_k + dz_k+1)), offset CF_NGT */
+    double rdx, rdy, rdxdy;
     const double *stat1;        /* [s][kind][vol1, bot1, top1, cl1] */
     const double *h1;           /* [s][kind][nz + 2] unit-forcing profile, level k at k + 1 */
     int cmode[2];
@@ -56,6 +71,7 @@ __global__ void k_rhs2(const RunDev R, const int s,
                        const double *gu, const double *gv, const double *gw, const double *scal,
                        double *fu, double *fv, double *fw, double *ru, double *rv, double *rw);
 __global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw);
+__global__ void k_spike(const RunDev R, const int s, double *ru, double *rv, double *rw);
 __global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
                                 double *scal, double *logbuf, int kind);
@@ -64,12 +80,16 @@ __global__ void k_cfl_part(const RunDev R, const double *u, const double *v, con
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);
-__global__ void k_div_fftx(const RunDev R, const double *us, const double *vs, const double *ws,
-                           const double *h1, const double *mptr, double2 *X);
-__global__ void k_fftx_fwd_plain(const RunDev R, const double *a, double2 *X);
-__global__ void k_ffty(const RunDev R, double2 *X, int fwd);
+typedef void (*k_div_fftx_t)(const RunDev, const double *, const double *, const double *,
+                             const double *, const double *, double2 *);
+typedef void (*k_fftx_plain_t)(const RunDev, const double *, double2 *);
+typedef void (*k_ffty_t)(const RunDev, double2 *, int);
+typedef void (*k_fftx_inv_t)(const RunDev, const double2 *, double *);
+k_div_fftx_t pick_div_fftx(int nx);
+k_fftx_plain_t pick_fftx_plain(int nx);
+k_ffty_t pick_ffty(int ny);
+k_fftx_inv_t pick_fftx_inv(int nx);
 __global__ void k_poisson2(const RunDev R, const PoisTab T, double2 *Y, double *G);
-__global__ void k_fftx_inv(const RunDev R, const double2 *X, double *p);
 __global__ void k_project2(const RunDev R, const double *us, const double *vs, const double *ws,
                            const double *h1, const double *mptr, const double *p, double dtp,
                            double *u, double *v, double *w);

@@ -98,9 +98,12 @@ __global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
         return (kk < 0 || kk >= nz - 1) ? 0.0 : AT(w, ii, jj, kk);
     };
 
+    const double *rdz = R.rdz + CF_NGT, *rdzh = R.rdzh + CF_NGT;
     const double dzk = dz[k], dzkp = dz[k + 1];
-    const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
-    const double dzm1h = 0.5 * (dz[k - 1] + dz[k]);
+    const double rzk = rdz[k], rzkp = rdz[k + 1];
+    const double rzp = rdzh[k], rzm = rdzh[k - 1];          /* 1/dzp1h, 1/dzm1h */
+    const double rdx = R.rdx, rdy = R.rdy, rdxdy = R.rdxdy;
+    const double rdx2 = rdx * rdx, rdy2 = rdy * rdy;
     const SubCoef C = R.sc[s];
     const double nu = R.nu;
     const size_t q = (size_t) k * P + (size_t) j * nx + i;
@@ -122,10 +125,9 @@ __global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
         double mzm = 0.5 * (dx * dy * wkm + dx * dy * wipkm);
         double adv = -((mxp * 0.5 * (u0 + uip) - mxm * 0.5 * (u0 + uim))
                      + (myp * 0.5 * (u0 + ujp) - mym * 0.5 * (u0 + ujm))
-                     + (mzp * 0.5 * (u0 + ukp) - mzm * 0.5 * (u0 + ukm))) / (dx * dy * dzk);
-        double f = adv + nu * (((uip - u0) / dx - (u0 - uim) / dx) / dx
-                             + ((ujp - u0) / dy - (u0 - ujm) / dy) / dy);
-        double h = u0 + C.nuadt * (((ukp - u0) / dzp1h - (u0 - ukm) / dzm1h) / dzk);
+                     + (mzp * 0.5 * (u0 + ukp) - mzm * 0.5 * (u0 + ukm))) * (rdxdy * rzk);
+        double f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
+        double h = u0 + C.nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
         double g = gu[q] + scal[R.m_prev_slot[s]];
         h += C.gamdt * f + C.zetdt * g;
         if (k == 0) h -= R.lift[s][0];
@@ -135,19 +137,18 @@ __global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
     }
     /* ---- v ---- */
     {
-        const double uimv = U(im, j, k), uimjpv = U(im, jp, k);
+        const double uimjp = U(im, jp, k);
         double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
-        double mxm = 0.5 * (dy * dzk * uimv + dy * dzk * uimjpv);
+        double mxm = 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
         double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
         double mym = 0.5 * (dx * dzk * vjm + dx * dzk * v0);
         double mzp = 0.5 * (dx * dy * w0 + dx * dy * wjp);
         double mzm = 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
         double adv = -((mxp * 0.5 * (v0 + vip) - mxm * 0.5 * (v0 + vim))
                      + (myp * 0.5 * (v0 + vjp) - mym * 0.5 * (v0 + vjm))
-                     + (mzp * 0.5 * (v0 + vkp) - mzm * 0.5 * (v0 + vkm))) / (dx * dy * dzk);
-        double f = adv + nu * (((vip - v0) / dx - (v0 - vim) / dx) / dx
-                             + ((vjp - v0) / dy - (v0 - vjm) / dy) / dy);
-        double h = v0 + C.nuadt * (((vkp - v0) / dzp1h - (v0 - vkm) / dzm1h) / dzk);
+                     + (mzp * 0.5 * (v0 + vkp) - mzm * 0.5 * (v0 + vkm))) * (rdxdy * rzk);
+        double f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
+        double h = v0 + C.nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
         h += C.gamdt * f + C.zetdt * gv[q];
         if (k == 0) h -= R.lift[s][2];
         if (k == nz - 1) h -= R.lift[s][3];
@@ -156,10 +157,11 @@ __global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
     }
     /* ---- w (faces k < nz - 1; the top wall face carries no equation) ---- */
     if (k < nz - 1) {
-        const double ukp1 = U(i, j, k + 1), uimkp = U(im, j, k + 1);
+        const double uimkp = U(im, j, k + 1);
         const double vjmkp = V(i, jm, k + 1);
         const double wim = W(im, j, k), wjm = W(i, jm, k), wkp = W(i, j, k + 1);
-        double mxp = 0.5 * (dy * dzk * u0 + dy * dzkp * ukp1);
+        const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
+        double mxp = 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
         double mxm = 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
         double myp = 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
         double mym = 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
@@ -167,10 +169,9 @@ __global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
         double mzm = 0.5 * (dx * dy * wkm + dx * dy * w0);
         double adv = -((mxp * 0.5 * (w0 + wip) - mxm * 0.5 * (w0 + wim))
                      + (myp * 0.5 * (w0 + wjp) - mym * 0.5 * (w0 + wjm))
-                     + (mzp * 0.5 * (w0 + wkp) - mzm * 0.5 * (w0 + wkm))) / (dx * dy * dzp1h);
-        double f = adv + nu * (((wip - w0) / dx - (w0 - wim) / dx) / dx
-                             + ((wjp - w0) / dy - (w0 - wjm) / dy) / dy);
-        double h = w0 + C.nuadt * (((wkp - w0) / dzkp - (w0 - wkm) / dzk) / dzp1h);
+                     + (mzp * 0.5 * (w0 + wkp) - mzm * 0.5 * (w0 + wkm))) * (rdxdy * rzp);
+        double f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
+        double h = w0 + C.nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
         h += C.gamdt * f + C.zetdt * gw[q];
         fw[q] = f;
         rw[q] = h * dzp1h;
@@ -237,6 +238,87 @@ __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
         xc[(size_t) k * P] = d;
     }
     if (f == 2) xc[(size_t) (nz - 1) * P] = 0.0;   /* ghost_fill_top2, diffusew.c:683-688 */
+}
+
+/* ---------------------------------------------------------------------
+ * The same constant-coefficient tridiagonal solves as k_thomas, as a
+ * partitioned ("spike") solve held in registers: a workgroup owns 64
+ * columns of one field and S waves, wave c owning rows [c L, c L + L).
+ * Each lane solves its chunk with the chunk-local Thomas factors (y),
+ * publishes the chunk's first and last values, and recovers the two
+ * neighbouring interface unknowns from the precomputed inverse of the
+ * 2S x 2S interface system; x = y - E_{c-1} v - F_{c+1} w with the
+ * column-independent spike vectors v, w.  Every value is read once and
+ * written once (Thomas in global memory reads and writes it twice).
+ * Same system as solve_helmholtz1d_u2 / _w2 (diffuseu.c:173-219,
+ * diffusew.c:175-223); differs from sequential Thomas by round-off.
+ * ------------------------------------------------------------------- */
+__global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
+    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw)
+{
+    __shared__ double itf[2 * CF_SPIKE_SMAX][64];
+    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
+    const size_t P = (size_t) nx * ny;
+    const int f = blockIdx.y;
+    if (f > 2) return;
+    const int type = f == 2 ? 2 : 0;
+    const SpikeTab T = R.sp[s][type];
+    const int lane = threadIdx.x & 63;
+    /* wave-uniform chunk index, so the table reads below are scalar loads */
+    const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t col = (size_t) blockIdx.x * 64 + lane;
+    const bool active = col < P && chunk < T.S;
+    double *x = f == 0 ? ru : (f == 1 ? rv : rw);
+    const int lo = chunk * T.L;
+    const int hi = min(lo + T.L, T.n);
+    double *xc = x + col;
+    double y[CF_SPIKE_LMAX];
+#pragma unroll
+    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+        const int j = lo + t;
+        y[t] = (active && j < hi) ? xc[(size_t) j * P] : 0.0;
+    }
+    double d = 0.0;
+#pragma unroll
+    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+        const int j = lo + t;
+        if (j < hi) {
+            d = (y[t] - T.a[j] * d) * T.ib[j];
+            y[t] = d;
+        }
+    }
+    double yl = 0.0;
+#pragma unroll
+    for (int t = CF_SPIKE_LMAX - 1; t >= 0; t--) {
+        const int j = lo + t;
+        if (j < hi) {
+            if (j < hi - 1) d = y[t] - T.gam[j + 1] * d;
+            else { d = y[t]; yl = d; }
+            y[t] = d;
+        }
+    }
+    if (chunk < T.S) {
+        itf[2 * chunk][lane] = y[0];
+        itf[2 * chunk + 1][lane] = yl;
+    }
+    __syncthreads();
+    if (!active) return;
+    const int m = 2 * T.S;
+    double E = 0.0, F = 0.0;
+    if (chunk > 0) {
+        const double *row = T.minv + (size_t) (2 * (chunk - 1) + 1) * m;
+        for (int q = 0; q < m; q++) E += row[q] * itf[q][lane];
+    }
+    if (chunk < T.S - 1) {
+        const double *row = T.minv + (size_t) (2 * (chunk + 1)) * m;
+        for (int q = 0; q < m; q++) F += row[q] * itf[q][lane];
+    }
+#pragma unroll
+    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+        const int j = lo + t;
+        if (j < hi) xc[(size_t) j * P] = y[t] - E * T.vl[j] - F * T.wr[j];
+    }
+    if (f == 2 && chunk == T.S - 1) xc[(size_t) (nz - 1) * P] = 0.0;   /* top wall face */
 }
 
 /* ---------------------------------------------------------------------
@@ -350,19 +432,26 @@ __global__ __launch_bounds__(256) void k_cfl_part(const RunDev R, const double *
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb, double dt,
-                             double *__restrict__ scal, double *__restrict__ logbuf)
+__global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb,
+                                                    double dt, double *__restrict__ scal, double *__restrict__ logbuf)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double red[256];
     double m = 0.0;
-    for (int b = 0; b < nb; b++) m = fmax(m, part[b]);
+    for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[b]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     long idx = (long) scal[SC_STEP] % R.logcap;
     double *lg = logbuf + (size_t) idx * 13;
     for (int t = 0; t < 13; t++) lg[t] = 0.0;
     lg[0] = scal[SC_STEP] + 1.0;
     lg[1] = dt;
-    lg[2] = m;
-    scal[SC_CFL] = m;
+    lg[2] = red[0];
+    scal[SC_CFL] = red[0];
 }
 
 __global__ void k_step_end(double *__restrict__ scal)
@@ -371,25 +460,51 @@ __global__ void k_step_end(double *__restrict__ scal)
 }
 
 /* ---------------------------------------------------------------------
- * divergence2 (pressure.c:40-61) of the mass-flux-corrected velocity
- * (u + h1 mdpdx, chanfast.c:519-522) fused with the forward real-to-
- * complex x FFT scaled 1/nx (fft.c:161-177).  Two real lines (j, k) are
- * packed into one complex FFT and separated afterwards.
+ * FFT passes.  Spectral layout X[nz][nx/2+1][ny] (ky fastest), so y lines
+ * are contiguous.  x passes pack two real lines (j, j+1 at one level)
+ * into one complex transform.  NX / NY > 0 selects a compile-time plan,
+ * 0 the runtime one.
  * ------------------------------------------------------------------- */
+
+/* unpack two packed real-line spectra and store kx-major: for each kx the
+ * block's 2 nl lines are consecutive ky positions (coalesced) */
+static __device__ __forceinline__ void store_half_spectra(const double2 *sb, int ld, int nl, int nx, int nkx,
+                                                          int ny, long NL, long pair0, double sc,
+                                                          double2 *__restrict__ X)
+{
+    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+        const int kx = e / nl, l = e - kx * nl;
+        const long la = 2 * (pair0 + l), lb = la + 1;
+        if (la >= NL) continue;
+        const double2 Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
+        const long k = la / ny, j = la - k * ny;
+        double2 *dst = X + ((size_t) k * nkx + kx) * ny + j;
+        dst[0] = make_double2((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
+        if (lb < NL) {
+            const long kb = lb / ny, jb = lb - kb * ny;
+            X[((size_t) kb * nkx + kx) * ny + jb] = make_double2((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
+        }
+    }
+}
+
+/* divergence2 (pressure.c:40-61) of the mass-flux-corrected velocity
+ * (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x FFT
+ * scaled 1/nx (fft.c:161-177) */
+template <int NX>
 __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *__restrict__ us,
     const double *__restrict__ vs, const double *__restrict__ ws, const double *__restrict__ h1,
     const double *__restrict__ mptr, double2 *__restrict__ X)
 {
     extern __shared__ double2 sb[];
-    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const int nx = NX > 0 ? NX : R.g.nx;
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = nx + 1;
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
     const size_t P = (size_t) nx * ny;
-    const double *dz = R.dzt + CF_NGT;
+    const double *rdz = R.rdz + CF_NGT;
     const double m = mptr ? *mptr : 0.0;
-    const double dx = R.g.dx, dy = R.g.dy;
-    const long pair0 = (long) blockIdx.x * nl;
+    const long pair0 = (long) xcd_block(blockIdx.x, gridDim.x) * nl;
     auto divg = [&](long line, int i) -> double {
         if (line >= NL) return 0.0;
         int k = (int) (line / ny), j = (int) (line - (long) k * ny);
@@ -400,7 +515,7 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *
         double v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
         double w1 = k < nz - 1 ? ws[q + i] : 0.0;
         double w0 = k > 0 ? ws[q - P + i] : 0.0;
-        return (u1 - u0) / dx + (v1 - v0) / dy + (w1 - w0) / dz[k];
+        return (u1 - u0) * R.rdx + (v1 - v0) * R.rdy + (w1 - w0) * rdz[k];
     };
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
@@ -408,25 +523,18 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *
         sb[l * ld + i] = make_double2(divg(2 * pr, i), divg(2 * pr + 1, i));
     }
     __syncthreads();
-    lds_fft<8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
-    const double sc = 0.5 / (double) nx;
-    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
-        int l = e / nkx, kx = e - l * nkx;
-        long la = 2 * (pair0 + l), lb = la + 1;
-        if (la >= NL) continue;
-        double2 Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
-        X[(size_t) la * nkx + kx] = make_double2((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
-        if (lb < NL) X[(size_t) lb * nkx + kx] = make_double2((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
-    }
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
+    store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
 }
 
-/* plain forward r2c x FFT of a real device-layout field (no divergence),
- * scaled 1/nx: the fftx_d2c stage of FFT3D_FWD on its own */
+/* the fftx_d2c stage of FFT3D_FWD on its own (no divergence), scaled 1/nx */
+template <int NX>
 __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const double *__restrict__ a,
                                                         double2 *__restrict__ X)
 {
     extern __shared__ double2 sb[];
-    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const int nx = NX > 0 ? NX : R.g.nx;
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = nx + 1;
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
@@ -439,45 +547,119 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
         sb[l * ld + i] = make_double2(va, vb);
     }
     __syncthreads();
-    lds_fft<8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
-    const double sc = 0.5 / (double) nx;
-    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
-        int l = e / nkx, kx = e - l * nkx;
-        long la = 2 * (pair0 + l), lb = la + 1;
-        if (la >= NL) continue;
-        double2 Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
-        X[(size_t) la * nkx + kx] = make_double2((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
-        if (lb < NL) X[(size_t) lb * nkx + kx] = make_double2((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
-    }
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
+    store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
 }
 
-/* ---------------------------------------------------------------------
- * Complex y FFT of X[nz][ny][nkx] in place, forward scaled 1/ny
- * (fft.c:186-202) or unscaled inverse (fft.c:205-208).  Block = TK
- * consecutive kx columns at one level.
- * ------------------------------------------------------------------- */
+/* complex y FFT in place, forward scaled 1/ny (fft.c:186-202) or unscaled
+ * inverse (fft.c:205-208); a block transforms TK contiguous lines */
+template <int NY>
 __global__ __launch_bounds__(256) void k_ffty(const RunDev R, double2 *__restrict__ X, int fwd)
 {
     extern __shared__ double2 sb[];
-    const int ny = R.g.ny, nkx = R.g.nkx;
+    const int ny = NY > 0 ? NY : R.g.ny;
+    const long nlines = (long) R.g.nz * R.g.nkx;
     const int TK = R.fy_cols, ld = ny + 1;
-    const int k = blockIdx.y, kx0 = blockIdx.x * TK;
-    double2 *base = X + (size_t) k * ny * nkx;
+    const long L0 = (long) blockIdx.x * TK;
+    const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
+    double2 *base = X + (size_t) L0 * ny;
     for (int e = threadIdx.x; e < TK * ny; e += blockDim.x) {
-        int j = e / TK, t = e - j * TK;
-        int kx = kx0 + t;
-        sb[t * ld + j] = kx < nkx ? base[(size_t) j * nkx + kx] : make_double2(0.0, 0.0);
+        int t = e / ny, j = e - t * ny;
+        sb[t * ld + j] = t < nl ? base[e] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft<8>(sb, ny, ld, TK, R.rady, R.nrady, R.twy, fwd ? 1.0 : -1.0);
+    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, R.twy, fwd ? 1.0 : -1.0);
     const double sc = fwd ? 1.0 / (double) ny : 1.0;
-    for (int e = threadIdx.x; e < TK * ny; e += blockDim.x) {
-        int j = e / TK, t = e - j * TK;
-        int kx = kx0 + t;
-        if (kx < nkx) {
-            double2 val = sb[t * ld + j];
-            base[(size_t) j * nkx + kx] = fwd ? cscale(val, sc) : val;
+    for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
+        int t = e / ny, j = e - t * ny;
+        double2 val = sb[t * ld + j];
+        base[e] = fwd ? cscale(val, sc) : val;
+    }
+}
+
+/* inverse complex-to-real x FFT (fft.c:180-183; FFTW c2r semantics: the
+ * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two real
+ * lines per complex transform, writing p */
+template <int NX>
+__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 *__restrict__ X,
+                                                  double *__restrict__ p)
+{
+    extern __shared__ double2 sb[];
+    const int nx = NX > 0 ? NX : R.g.nx;
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const int ld = nx + 1;
+    const int nl = R.fx_pairs;
+    const long NL = (long) ny * nz;
+    const long pair0 = (long) xcd_block(blockIdx.x, gridDim.x) * nl;
+    const int half = nx / 2;
+    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
+        const int kx = e / nl, l = e - kx * nl;
+        const long la = 2 * (pair0 + l), lb = la + 1;
+        double2 A = make_double2(0.0, 0.0), B = make_double2(0.0, 0.0);
+        if (la < NL) {
+            const long k = la / ny, j = la - k * ny;
+            A = X[((size_t) k * nkx + kx) * ny + j];
         }
+        if (lb < NL) {
+            const long k = lb / ny, j = lb - k * ny;
+            B = X[((size_t) k * nkx + kx) * ny + j];
+        }
+        if (kx == 0 || (nx % 2 == 0 && kx == half)) { A.y = 0.0; B.y = 0.0; }
+        sb[l * ld + kx] = make_double2(A.x - B.y, A.y + B.x);
+        if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = make_double2(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, -1.0);
+    for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
+        int l = e / nx, i = e - l * nx;
+        long la = 2 * (pair0 + l), lb = la + 1;
+        double2 z = sb[l * ld + i];
+        if (la < NL) p[(size_t) la * nx + i] = z.x;
+        if (lb < NL) p[(size_t) lb * nx + i] = z.y;
+    }
+}
+
+/* host-side selectors of the compile-time instances */
+#define CF_FFT_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) \
+    X(120) X(128) X(160) X(192) X(240) X(256) X(320) X(384) X(480) X(512)
+
+k_div_fftx_t pick_div_fftx(int n)
+{
+    switch (n) {
+#define X_(N) case N: return k_div_fftx<N>;
+    CF_FFT_SIZES(X_)
+#undef X_
+    default: return k_div_fftx<0>;
+    }
+}
+
+k_fftx_plain_t pick_fftx_plain(int n)
+{
+    switch (n) {
+#define X_(N) case N: return k_fftx_fwd_plain<N>;
+    CF_FFT_SIZES(X_)
+#undef X_
+    default: return k_fftx_fwd_plain<0>;
+    }
+}
+
+k_ffty_t pick_ffty(int n)
+{
+    switch (n) {
+#define X_(N) case N: return k_ffty<N>;
+    CF_FFT_SIZES(X_)
+#undef X_
+    default: return k_ffty<0>;
+    }
+}
+
+k_fftx_inv_t pick_fftx_inv(int n)
+{
+    switch (n) {
+#define X_(N) case N: return k_fftx_inv<N>;
+    CF_FFT_SIZES(X_)
+#undef X_
+    default: return k_fftx_inv<0>;
     }
 }
 
@@ -495,76 +677,55 @@ __global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab 
     const size_t M = (size_t) ny * nkx;
     const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (qm >= M) return;
-    const int kx = (int) (qm % nkx), ky = (int) (qm / nkx);
+    const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
     const double lx = R.lamx[kx], ly = R.lamy[ky];
     const bool zero = kx == 0 && ky == 0;
     double2 *y = Y + qm;
     double *g = G + qm;
-    double b = T.bz[0] - lx * T.ex[0] - ly * T.ey[0];
-    b -= zero ? T.db_dir : T.db_neu;
-    if (nz == 1) b -= T.db_top;
-    double bet = b;
-    double2 r = cscale(y[0], T.dvol[0]);
-    double2 d = make_double2(r.x / bet, r.y / bet);
-    y[0] = d;
-    for (int k = 1; k < nz; k++) {
-        double bk = T.bz[k] - lx * T.ex[k] - ly * T.ey[k];
-        if (k == nz - 1) bk -= T.db_top;
-        double gam = T.c[k - 1] / bet;
-        bet = bk - T.a[k] * gam;
-        g[(size_t) k * M] = gam;
-        double2 rk = cscale(y[(size_t) k * M], T.dvol[k]);
-        d = make_double2((rk.x - T.a[k] * d.x) / bet, (rk.y - T.a[k] * d.y) / bet);
-        y[(size_t) k * M] = d;
-    }
-    for (int k = nz - 2; k >= 0; k--) {
-        double gk = g[(size_t) (k + 1) * M];
-        double2 yk = y[(size_t) k * M];
-        d = make_double2(yk.x - gk * d.x, yk.y - gk * d.y);
-        y[(size_t) k * M] = d;
-    }
-}
-
-/* ---------------------------------------------------------------------
- * Inverse complex-to-real x FFT (fft.c:180-183, FFTW c2r semantics: the
- * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two
- * real lines per complex transform, writing p.
- * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 *__restrict__ X,
-                                                  double *__restrict__ p)
-{
-    extern __shared__ double2 sb[];
-    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
-    const int ld = nx + 1;
-    const int nl = R.fx_pairs;
-    const long NL = (long) ny * nz;
-    const long pair0 = (long) blockIdx.x * nl;
-    const int half = nx / 2;
-    auto spec = [&](long line, int kx) -> double2 {
-        if (line >= NL) return make_double2(0.0, 0.0);
-        const double2 *row = X + (size_t) line * nkx;
-        if (kx <= half) {
-            double2 a = row[kx];
-            if (kx == 0 || (nx % 2 == 0 && kx == half)) a.y = 0.0;
-            return a;
+    constexpr int B = 8;
+    double ibet = 0.0, cprev = 0.0;
+    double2 d = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < nz; k0 += B) {
+        double2 rb[B];
+#pragma unroll
+        for (int t = 0; t < B; t++)
+            if (k0 + t < nz) rb[t] = y[(size_t) (k0 + t) * M];
+#pragma unroll
+        for (int t = 0; t < B; t++) {
+            const int k = k0 + t;
+            if (k < nz) {
+                double b = T.bz[k] - lx * T.ex[k] - ly * T.ey[k];
+                if (k == 0) b -= zero ? T.db_dir : T.db_neu;
+                if (k == nz - 1) b -= T.db_top;
+                double gam = 0.0, bet = b;
+                if (k > 0) {
+                    gam = cprev * ibet;              /* c_{k-1} / bet_{k-1} */
+                    bet = b - T.a[k] * gam;
+                    g[(size_t) k * M] = gam;
+                }
+                ibet = 1.0 / bet;
+                const double dv = T.dvol[k];
+                const double ak = k > 0 ? T.a[k] : 0.0;
+                d = make_double2((rb[t].x * dv - ak * d.x) * ibet, (rb[t].y * dv - ak * d.y) * ibet);
+                y[(size_t) k * M] = d;
+                cprev = T.c[k];
+            }
         }
-        double2 a = row[nx - kx];
-        return make_double2(a.x, -a.y);
-    };
-    for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
-        int l = e / nx, kx = e - l * nx;
-        long la = 2 * (pair0 + l);
-        double2 A = spec(la, kx), B = spec(la + 1, kx);
-        sb[l * ld + kx] = make_double2(A.x - B.y, A.y + B.x);
     }
-    __syncthreads();
-    lds_fft<8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, -1.0);
-    for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
-        int l = e / nx, i = e - l * nx;
-        long la = 2 * (pair0 + l), lb = la + 1;
-        double2 z = sb[l * ld + i];
-        if (la < NL) p[(size_t) la * nx + i] = z.x;
-        if (lb < NL) p[(size_t) lb * nx + i] = z.y;
+    for (int k0 = nz - 2; k0 >= 0; k0 -= B) {
+        double2 yb[B];
+        double gb[B];
+#pragma unroll
+        for (int t = 0; t < B; t++)
+            if (k0 - t >= 0) { yb[t] = y[(size_t) (k0 - t) * M]; gb[t] = g[(size_t) (k0 - t + 1) * M]; }
+#pragma unroll
+        for (int t = 0; t < B; t++) {
+            const int k = k0 - t;
+            if (k >= 0) {
+                d = make_double2(yb[t].x - gb[t] * d.x, yb[t].y - gb[t] * d.y);
+                y[(size_t) k * M] = d;
+            }
+        }
     }
 }
 
@@ -587,15 +748,14 @@ __global__ __launch_bounds__(256) void k_project2(const RunDev R, const double *
     const size_t P = (size_t) nx * ny;
     const int ip = i + 1 < nx ? i + 1 : 0, jp = j + 1 < ny ? j + 1 : 0;
     const size_t q = (size_t) k * P + (size_t) j * nx + i;
-    const double *dz = R.dzt + CF_NGT;
+    const double *rdzh = R.rdzh + CF_NGT;
     const double m = mptr ? *mptr : 0.0;
     const double p0 = p[q];
     double uc = us[q] + (h1 ? h1[k + 1] * m : 0.0);
-    u[q] = uc - (p[(size_t) k * P + (size_t) j * nx + ip] - p0) * dtp / R.g.dx;
-    v[q] = vs[q] - (p[(size_t) k * P + (size_t) jp * nx + i] - p0) * dtp / R.g.dy;
+    u[q] = uc - (p[(size_t) k * P + (size_t) j * nx + ip] - p0) * (dtp * R.rdx);
+    v[q] = vs[q] - (p[(size_t) k * P + (size_t) jp * nx + i] - p0) * (dtp * R.rdy);
     if (k < nz - 1) {
-        double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
-        w[q] = ws[q] - (p[q + P] - p0) * dtp / dzp1h;
+        w[q] = ws[q] - (p[q + P] - p0) * (dtp * rdzh[k]);
     } else {
         w[q] = 0.0;
     }

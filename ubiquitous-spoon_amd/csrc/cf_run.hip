@@ -17,6 +17,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -88,6 +89,12 @@ struct cf_run {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int steps_done = 0;
+    bool use_spike = false;
+    int spike_waves = 1;
+    k_div_fftx_t kdiv = nullptr;
+    k_fftx_plain_t kfxp = nullptr;
+    k_ffty_t kfy = nullptr;
+    k_fftx_inv_t kfxi = nullptr;
     /* profiling */
     bool profile = false;
     std::map<std::string, std::pair<double, int>> ktime;
@@ -168,6 +175,85 @@ static int thomas_factors(const std::vector<double> &a, const std::vector<double
         if (bet == 0.0) return -1;
         A[j] = a[j]; IB[j] = 1.0 / bet; GM[j] = gam;
     }
+    return 0;
+}
+
+/* Partitioned (spike) solve tables for one tridiagonal system (rows
+ * a_j x_{j-1} + b_j x_j + c_j x_{j+1} = r_j, a_0 = c_{n-1} = 0): chunks of
+ * L rows, chunk-local Thomas factors, spikes v = B^-1 a_lo e_lo and
+ * w = B^-1 c_{hi-1} e_{hi-1}, and the inverse of the interface system
+ * z + K z = y_int over z = (F_0, E_0, F_1, E_1, ...), F/E the first/last
+ * unknown of each chunk. */
+struct SpikeHost {
+    int n = 0, S = 0, L = 0;
+    std::vector<double> a, ib, gam, vl, wr, minv;
+};
+
+static int invert_dense(std::vector<double> &M, int m)
+{
+    std::vector<double> A(M), I((size_t) m * m, 0.0);
+    for (int i = 0; i < m; i++) I[(size_t) i * m + i] = 1.0;
+    for (int c = 0; c < m; c++) {
+        int piv = c;
+        for (int r = c + 1; r < m; r++)
+            if (fabs(A[(size_t) r * m + c]) > fabs(A[(size_t) piv * m + c])) piv = r;
+        if (A[(size_t) piv * m + c] == 0.0) return -1;
+        if (piv != c)
+            for (int q = 0; q < m; q++) {
+                std::swap(A[(size_t) c * m + q], A[(size_t) piv * m + q]);
+                std::swap(I[(size_t) c * m + q], I[(size_t) piv * m + q]);
+            }
+        double inv = 1.0 / A[(size_t) c * m + c];
+        for (int q = 0; q < m; q++) { A[(size_t) c * m + q] *= inv; I[(size_t) c * m + q] *= inv; }
+        for (int r = 0; r < m; r++) {
+            if (r == c) continue;
+            double f = A[(size_t) r * m + c];
+            if (f == 0.0) continue;
+            for (int q = 0; q < m; q++) { A[(size_t) r * m + q] -= f * A[(size_t) c * m + q]; I[(size_t) r * m + q] -= f * I[(size_t) c * m + q]; }
+        }
+    }
+    M = I;
+    return 0;
+}
+
+static int spike_build(const std::vector<double> &a, const std::vector<double> &b,
+                       const std::vector<double> &c, SpikeHost &T)
+{
+    const int n = (int) b.size();
+    T.n = n;
+    T.S = (n + CF_SPIKE_LMAX - 1) / CF_SPIKE_LMAX;
+    if (T.S > CF_SPIKE_SMAX) return -1;
+    T.L = (n + T.S - 1) / T.S;
+    T.a = a; T.ib.assign(n, 0.0); T.gam.assign(n, 0.0); T.vl.assign(n, 0.0); T.wr.assign(n, 0.0);
+    const int S = T.S, L = T.L, m = 2 * S;
+    std::vector<double> M((size_t) m * m, 0.0);
+    for (int i = 0; i < m; i++) M[(size_t) i * m + i] = 1.0;
+    for (int s = 0; s < S; s++) {
+        int lo = s * L, hi = std::min(lo + L, n);
+        double bet = b[lo];
+        if (bet == 0.0) return -2;
+        T.ib[lo] = 1.0 / bet;
+        for (int j = lo + 1; j < hi; j++) {
+            T.gam[j] = c[j - 1] / bet;
+            bet = b[j] - a[j] * T.gam[j];
+            if (bet == 0.0) return -2;
+            T.ib[j] = 1.0 / bet;
+        }
+        auto solve = [&](std::vector<double> &rhs) {   /* in place on [lo, hi) */
+            double d = 0.0;
+            for (int j = lo; j < hi; j++) { d = (rhs[j] - (j > lo ? a[j] * d : 0.0)) * T.ib[j]; rhs[j] = d; }
+            for (int j = hi - 2; j >= lo; j--) { d = rhs[j] - T.gam[j + 1] * d; rhs[j] = d; }
+        };
+        std::vector<double> v(n, 0.0), w(n, 0.0);
+        if (s > 0) { v[lo] = a[lo]; solve(v); }
+        if (s < S - 1) { w[hi - 1] = c[hi - 1]; solve(w); }
+        for (int j = lo; j < hi; j++) { T.vl[j] = v[j]; T.wr[j] = w[j]; }
+        int rf = 2 * s, re = 2 * s + 1;
+        if (s > 0) { M[(size_t) rf * m + 2 * (s - 1) + 1] += v[lo]; M[(size_t) re * m + 2 * (s - 1) + 1] += v[hi - 1]; }
+        if (s < S - 1) { M[(size_t) rf * m + 2 * (s + 1)] += w[lo]; M[(size_t) re * m + 2 * (s + 1)] += w[hi - 1]; }
+    }
+    if (invert_dense(M, m)) return -3;
+    T.minv = M;
     return 0;
 }
 
@@ -318,6 +404,7 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     /* ---- host tables ---- */
     const size_t nzt = (size_t) nz;
     std::vector<double> th(3 * 4 * 3 * nzt, 0.0), stat1(3 * 2 * 4, 0.0), h1(3 * 2 * (nzt + 2), 0.0);
+    SpikeHost sph[3][3];
     PoisHost ph[4];
     for (int s = 0; s < 3; s++) {
         SubCoef &C = r->rd.sc[s];
@@ -341,6 +428,7 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
                 delete r;
                 return CF_ERR_NUMERIC;
             }
+            if (spike_build(a, b, c, sph[s][type])) sph[s][type].n = 0;   /* falls back to Thomas */
             if (type < 2) {
                 /* unit-forcing profile h1: h = gam dt, solve (chanfast.c:498-500) */
                 std::vector<double> rr(nz), x;
@@ -416,6 +504,21 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
         o_p[q][4] = put(ph[q].ey.data(), nz); o_p[q][5] = put(ph[q].dvol.data(), nz);
     }
     r->off_pois_adhoc = o_p[3][0];
+    std::vector<double> rdz(nz + 6), rdzh(nz + 6, 0.0);
+    for (int k = -3; k < nz + 3; k++) rdz[k + 3] = 1.0 / dzl[k];
+    for (int k = -3; k < nz + 2; k++) rdzh[k + 3] = 1.0 / (0.5 * (dzl[k] + dzl[k + 1]));
+    size_t o_rdz = put(rdz.data(), rdz.size());
+    size_t o_rdzh = put(rdzh.data(), rdzh.size());
+    size_t o_sp[3][3][6];
+    bool spike_ok = true;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) {
+            SpikeHost &S = sph[s][t];
+            if (S.n == 0) { spike_ok = false; continue; }
+            o_sp[s][t][0] = put(S.a.data(), S.n); o_sp[s][t][1] = put(S.ib.data(), S.n);
+            o_sp[s][t][2] = put(S.gam.data(), S.n); o_sp[s][t][3] = put(S.vl.data(), S.n);
+            o_sp[s][t][4] = put(S.wr.data(), S.n); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
+        }
     r->tab_n = pack.size();
 
     /* ---- device memory ---- */
@@ -458,6 +561,25 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     R.vbc = 0.0 - cfg->vframe;
     R.cbot = -0.5; R.ctop = 0.5;                /* chanfast.c:67-68 */
     R.th = r->tab + o_th;
+    R.rdz = r->tab + o_rdz;
+    R.rdzh = r->tab + o_rdzh;
+    r->kdiv = pick_div_fftx(nx);
+    r->kfxp = pick_fftx_plain(nx);
+    r->kfy = pick_ffty(ny);
+    r->kfxi = pick_fftx_inv(nx);
+    R.rdx = 1.0 / r->g.dx;
+    R.rdy = 1.0 / r->g.dy;
+    R.rdxdy = 1.0 / (r->g.dx * r->g.dy);
+    r->use_spike = spike_ok;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) {
+            SpikeTab &ST = R.sp[s][t];
+            ST.n = sph[s][t].n; ST.S = sph[s][t].S; ST.L = sph[s][t].L;
+            if (!spike_ok) continue;
+            ST.a = r->tab + o_sp[s][t][0]; ST.ib = r->tab + o_sp[s][t][1]; ST.gam = r->tab + o_sp[s][t][2];
+            ST.vl = r->tab + o_sp[s][t][3]; ST.wr = r->tab + o_sp[s][t][4]; ST.minv = r->tab + o_sp[s][t][5];
+            r->spike_waves = std::max(r->spike_waves, ST.S);
+        }
     R.stat1 = r->tab + o_st;
     R.h1 = r->tab + o_h1;
     R.cmode[0] = cfg->uvolavgmode; R.ctarget[0] = cfg->want_uvolavg - cfg->uframe; R.cfixed[0] = -cfg->dpdx;
@@ -522,6 +644,11 @@ static dim3 cell_grid(const cf_run *r) { return dim3((r->g.nx + 63) / 64, (r->g.
 
 static size_t fx_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fx_pairs * (r->g.nx + 1); }
 static size_t fy_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fy_cols * (r->g.ny + 1); }
+static unsigned fy_blocks(const cf_run *r)
+{
+    long lines = (long) r->g.nz * r->g.nkx;
+    return (unsigned) ((lines + r->rd.fy_cols - 1) / r->rd.fy_cols);
+}
 
 /* divergence -> FFT -> Poisson -> inverse FFT -> projection into (u, v, w) */
 static cf_status enqueue_projection(cf_run *r, const double *us, const double *vs, const double *ws,
@@ -531,12 +658,12 @@ static cf_status enqueue_projection(cf_run *r, const double *us, const double *v
     long NL = (long) r->g.ny * r->g.nz;
     long npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    KLAUNCH("div_fftx", k_div_fftx, dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, r->X);
-    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
-    KLAUNCH("ffty_fwd", k_ffty, gy, dim3(256), fy_lds(r), R, r->X, 1);
+    KLAUNCH("div_fftx", r->kdiv, dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, r->X);
+    dim3 gy(fy_blocks(r));
+    KLAUNCH("ffty_fwd", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 1);
     KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, T, r->X, r->G);
-    KLAUNCH("ffty_inv", k_ffty, gy, dim3(256), fy_lds(r), R, r->X, 0);
-    KLAUNCH("fftx_inv", k_fftx_inv, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
+    KLAUNCH("ffty_inv", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 0);
+    KLAUNCH("fftx_inv", r->kfxi, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
     KLAUNCH("project", k_project2, cell_grid(r), dim3(64, 4), 0, R, us, vs, ws, h1, mptr,
             (const double *) r->p, dtp, r->u, r->v, r->w);
     return CF_OK;
@@ -549,7 +676,7 @@ static cf_status enqueue_step(cf_run *r)
     const double dt = r->cfg.dt;
     KLAUNCH("cfl", k_cfl_part, dim3(r->nb_red), dim3(256), 0, R, (const double *) r->u,
             (const double *) r->v, (const double *) r->w, dt, r->part, r->nb_red);
-    KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(64), 0, R, (const double *) r->part, r->nb_red, dt,
+    KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->part, r->nb_red, dt,
             r->scal, r->logbuf);
     for (int s = 0; s < 3; s++) {
         double **fnew = (s % 2 == 0) ? r->fa : r->fb;
@@ -558,8 +685,12 @@ static cf_status enqueue_step(cf_run *r)
                 (const double *) r->u, (const double *) r->v, (const double *) r->w,
                 (const double *) gold[0], (const double *) gold[1], (const double *) gold[2],
                 (const double *) r->scal, fnew[0], fnew[1], fnew[2], r->ru, r->rv, r->rw);
-        KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s,
-                r->ru, r->rv, r->rw);
+        if (r->use_spike)
+            KLAUNCH("spike", k_spike, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
+                    r->ru, r->rv, r->rw);
+        else
+            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s,
+                    r->ru, r->rv, r->rw);
         KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
                 (const double *) r->ru, R.ubc, r->part, r->nb_plane);
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
@@ -771,7 +902,7 @@ extern "C" cf_status cf_op_transpose_roundtrip(cf_run *r, const double *in_io, d
     return CF_OK;
 }
 
-/* host [ix][iy][iz][2] <-> device [iz][iy][ix] complex */
+/* host [ix][iy][iz][2] <-> device [iz][ix][iy] complex */
 static void spec_host_to_dev(const cf_run *r, const double *h, std::vector<double2> &d)
 {
     int nkx = r->g.nkx, ny = r->g.ny, nz = r->g.nz;
@@ -780,7 +911,7 @@ static void spec_host_to_dev(const cf_run *r, const double *h, std::vector<doubl
         for (int iy = 0; iy < ny; iy++)
             for (int iz = 0; iz < nz; iz++) {
                 size_t o = (((size_t) ix * ny + iy) * nz + iz) * 2;
-                d[((size_t) iz * ny + iy) * nkx + ix] = make_double2(h[o], h[o + 1]);
+                d[((size_t) iz * nkx + ix) * ny + iy] = make_double2(h[o], h[o + 1]);
             }
 }
 
@@ -791,7 +922,7 @@ static void spec_dev_to_host(const cf_run *r, const std::vector<double2> &d, dou
         for (int iy = 0; iy < ny; iy++)
             for (int iz = 0; iz < nz; iz++) {
                 size_t o = (((size_t) ix * ny + iy) * nz + iz) * 2;
-                double2 val = d[((size_t) iz * ny + iy) * nkx + ix];
+                double2 val = d[((size_t) iz * nkx + ix) * ny + iy];
                 h[o] = val.x; h[o + 1] = val.y;
             }
 }
@@ -808,10 +939,10 @@ extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spe
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    hipLaunchKernelGGL(k_fftx_fwd_plain, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double *) r->p, r->X);
+    hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double *) r->p, r->X);
     LAUNCHCHK();
-    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
-    hipLaunchKernelGGL(k_ffty, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 1);
+    dim3 gy(fy_blocks(r));
+    hipLaunchKernelGGL(r->kfy, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 1);
     LAUNCHCHK();
     std::vector<double2> d(r->M * r->g.nz);
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
@@ -828,12 +959,12 @@ extern "C" cf_status cf_op_fft3d_bwd(cf_run *r, const double *spec_in, double *o
     spec_host_to_dev(r, spec_in, d);
     HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
     const RunDev &R = r->rd;
-    dim3 gy((r->g.nkx + R.fy_cols - 1) / R.fy_cols, r->g.nz);
-    hipLaunchKernelGGL(k_ffty, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 0);
+    dim3 gy(fy_blocks(r));
+    hipLaunchKernelGGL(r->kfy, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 0);
     LAUNCHCHK();
     long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    hipLaunchKernelGGL(k_fftx_inv, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double2 *) r->X, r->p);
+    hipLaunchKernelGGL(r->kfxi, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double2 *) r->X, r->p);
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(r->stream));
     return download_dev(r, r->p, out_io);

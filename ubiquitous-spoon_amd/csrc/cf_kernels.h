@@ -18,6 +18,12 @@ enum {
 };
 
 #define CF_MAX_RAD 16
+#ifndef CF_RHS_WAVES
+#define CF_RHS_WAVES 3          /* occupancy target (waves per SIMD) of k_rhs2t */
+#endif
+#ifndef CF_RHS_KC
+#define CF_RHS_KC 24             /* target levels per z-chunk of k_rhs2t */
+#endif
 #define CF_SPIKE_LMAX 32        /* rows per chunk held in registers */
 #define CF_SPIKE_SMAX 8         /* chunks (waves) per workgroup */
 
@@ -58,6 +64,14 @@ struct RunDev {
     int fy_cols;                /* kx columns per y-FFT block */
 };
 
+/* Where a substep's projected velocity comes from: the previous
+ * substep's solve output, its pressure, mass-flux profile and forcing
+ * (h1, m may be null: no correction) and its projection dt. */
+struct ProjSrc {
+    const double *us, *vs, *ws, *p, *h1, *m;
+    double dtp;
+};
+
 /* z-tables of the Poisson operator for one dt (pressure.c:318-339) */
 struct PoisTab {
     const double *a, *c, *bz, *ex, *ey, *dvol;
@@ -66,17 +80,15 @@ struct PoisTab {
 
 __global__ void k_io_to_dev(const double *io, double *dev, int nx, int ny, int nz);
 __global__ void k_dev_to_io(const double *dev, double *io, int nx, int ny, int nz);
-__global__ void k_rhs2(const RunDev R, const int s,
-                       const double *u, const double *v, const double *w,
-                       const double *gu, const double *gv, const double *gw, const double *scal,
-                       double *fu, double *fv, double *fw, double *ru, double *rv, double *rw);
+__global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
+                        const double *gu, const double *gv, const double *gw, const double *scal,
+                        double *fu, double *fv, double *fw, double *ru, double *rv, double *rw,
+                        double *cflpart, double cfldt, int kc);
 __global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw);
 __global__ void k_spike(const RunDev R, const int s, double *ru, double *rv, double *rw);
 __global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
                                 double *scal, double *logbuf, int kind);
-__global__ void k_cfl_part(const RunDev R, const double *u, const double *v, const double *w,
-                           double dt, double *part, int nb);
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);

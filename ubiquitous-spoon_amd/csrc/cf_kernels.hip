@@ -49,137 +49,273 @@ __global__ __launch_bounds__(256) void k_dev_to_io(const double *__restrict__ de
 }
 
 /* ---------------------------------------------------------------------
- * Explicit right-hand side and implicit-solve right-hand side of one
- * substep, 2nd order, one thread per cell (i, j, k).  Fuses, per field:
- *   wall ghosts        apply_bc_u2 / apply_bc_w2   (diffuseu.c:37-69, diffusew.c:37-69)
- *   mass fluxes        massflux{x,y,z}2            (advect.c:6-83)
- *   advection          advect{u,v,w}2              (advect.c:286-463)
- *   x-y diffusion      add_laplacexy_operator_*2   (diffuseu.c:286-313, diffusew.c:290-315)
- *   z-Helmholtz        helmholtz1d_operator_*2     (diffuseu.c:107-132, diffusew.c:107-133)
- *   RK3 combination    h += gam dt f + zet dt g, h -= lift (chanfast.c:465-477, :619-631)
- * Writes f (the new explicit tendency) and r = h * dvol, the right-hand
- * side of the tridiagonal solve (diffuseu.c:202, diffusew.c:207).
- * g_eff = g + m_prev folds the previous substep's mass-flux forcing into
- * the zeta term, as `fu += mdpdx` does (chanfast.c:519-522).
+ * Explicit and implicit-solve right-hand sides of one substep, 2nd order,
+ * 2.5-D blocked and with the
+ * previous substep's projection (project2, pressure.c:120-152, and the
+ * mass-flux correction u += h1 mdpdx, chanfast.c:519-522) applied on the
+ * fly, so the projected velocity is never stored.  A workgroup owns a
+ * 64 x 4 column tile and marches levels [k0, k1); a three-level LDS ring
+ * holds the projected u, v, w of the tile plus a one-cell halo (periodic
+ * wrap in x, y; wall ghosts 2 bc - q_1 for u, v and 0 for w).  Level k + 2
+ * is fetched into registers before level k is computed.  Substep 0 also
+ * reduces the CFL number of the projected state (get_cfl, get.c:390-417).
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_rhs2(const RunDev R, const int s,
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+#define RT_X 64
+#define RT_Y 4
+#define RT_HX (RT_X + 2)
+#define RT_HY (RT_Y + 2)
+#define RT_PX (RT_X + 3)
+#define RT_PY (RT_Y + 3)
+#define RT_NA (RT_HX * RT_HY)      /* 396 halo-tile cells */
+#define RT_NP (RT_PX * RT_PY)      /* 469 pressure cells (one more in +x, +y) */
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVES, 8))) void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
     const double *__restrict__ gu, const double *__restrict__ gv, const double *__restrict__ gw,
     const double *__restrict__ scal,
     double *__restrict__ fu, double *__restrict__ fv, double *__restrict__ fw,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw)
+    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
+    double *__restrict__ cflpart, double cfldt, int kc)
 {
+    __shared__ double SU[3][RT_HY][RT_HX], SV[3][RT_HY][RT_HX], SW[3][RT_HY][RT_HX];
+    __shared__ double SP[2][2][RT_PY][RT_PX];     /* [buffer][p(L), p(L+1)] */
+    __shared__ double red[256];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= nx || j >= ny || k >= nz) return;
     const size_t P = (size_t) nx * ny;
-    const int ip = i + 1 < nx ? i + 1 : 0, im = i > 0 ? i - 1 : nx - 1;
-    const int jp = j + 1 < ny ? j + 1 : 0, jm = j > 0 ? j - 1 : ny - 1;
-    const double dx = R.g.dx, dy = R.g.dy;
+    const int ntx = (nx + RT_X - 1) / RT_X, nty = (ny + RT_Y - 1) / RT_Y;
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tx = (int) (lb % (unsigned) ntx);
+    const unsigned rest = lb / (unsigned) ntx;
+    const int ty = (int) (rest % (unsigned) nty);
+    const int chunk = (int) (rest / (unsigned) nty);
+    const int i0 = tx * RT_X, j0 = ty * RT_Y;
+    const int k0 = chunk * kc;
+    if (k0 >= nz) return;                         /* uniform over the block */
+    const int k1 = min(k0 + kc, nz);
+    const int tid = threadIdx.x;
+    const int lx = tid & 63, ly = tid >> 6;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool act = i < nx && j < ny;
+
+    const double m = S.m ? *S.m : 0.0;
+    const double cx = S.dtp * R.rdx, cy = S.dtp * R.rdy;
     const double *dz = R.dzt + CF_NGT;
+    const double *rdz = R.rdz + CF_NGT, *rdzh = R.rdzh + CF_NGT;
     const double ub = R.ubc, vb = R.vbc;
 
-#define AT(a, ii, jj, kk) (a)[(size_t) (kk) * P + (size_t) (jj) * nx + (ii)]
-    /* wall ghosts: 2nd-order Dirichlet closure gives q_g = 2 bc - q_1 */
-    auto U = [&](int ii, int jj, int kk) -> double {
-        if (kk < 0) return 2.0 * ub - AT(u, ii, jj, 0);
-        if (kk >= nz) return 2.0 * ub - AT(u, ii, jj, nz - 1);
-        return AT(u, ii, jj, kk);
+    /* element e of the halo tile / pressure tile -> global column offset */
+    auto colo = [&](int e, int w) -> size_t {
+        const int r = e / w, c = e - r * w;
+        int ii = i0 - 1 + c, jj = j0 - 1 + r;
+        ii %= nx; if (ii < 0) ii += nx;
+        jj %= ny; if (jj < 0) jj += ny;
+        return (size_t) jj * nx + ii;
     };
-    auto V = [&](int ii, int jj, int kk) -> double {
-        if (kk < 0) return 2.0 * vb - AT(v, ii, jj, 0);
-        if (kk >= nz) return 2.0 * vb - AT(v, ii, jj, nz - 1);
-        return AT(v, ii, jj, kk);
+    size_t oa[2], op[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int e = tid + 256 * t;
+        oa[t] = e < RT_NA ? colo(e, RT_HX) : 0;
+        op[t] = e < RT_NP ? colo(e, RT_PX) : 0;
+    }
+    double fu_[2], fv_[2], fw_[2], pa_[2], pb_[2];
+    int last_wok_level = -100;             /* p(L + 1) of the previous fetch is p(L) of this one */
+    auto fetch = [&](int L) {
+        const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
+        const bool wok = L >= 0 && L <= nz - 2;
+        const bool reuse = last_wok_level == L - 1;
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int e = tid + 256 * t;
+            if (e < RT_NA) {
+                fu_[t] = S.us[(size_t) Lc * P + oa[t]];
+                fv_[t] = S.vs[(size_t) Lc * P + oa[t]];
+                fw_[t] = wok ? S.ws[(size_t) L * P + oa[t]] : 0.0;
+            }
+            if (e < RT_NP) {
+                pa_[t] = reuse ? pb_[t] : S.p[(size_t) Lc * P + op[t]];
+                pb_[t] = wok ? S.p[(size_t) (L + 1) * P + op[t]] : 0.0;
+            }
+        }
+        last_wok_level = wok ? L : -100;
     };
-    /* w: both wall faces carry the Dirichlet value 0 */
-    auto W = [&](int ii, int jj, int kk) -> double {
-        return (kk < 0 || kk >= nz - 1) ? 0.0 : AT(w, ii, jj, kk);
+    auto stage_p = [&](int b) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int e = tid + 256 * t;
+            if (e < RT_NP) {
+                const int r = e / RT_PX, c = e - r * RT_PX;
+                SP[b][0][r][c] = pa_[t];
+                SP[b][1][r][c] = pb_[t];
+            }
+        }
+    };
+    auto build = [&](int L, int sl, int b) {
+        const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
+        const bool wok = L >= 0 && L <= nz - 2;
+        const bool ghost = L < 0 || L >= nz;
+        const double hm = S.h1 ? S.h1[Lc + 1] * m : 0.0;
+        const double cz = wok ? S.dtp * rdzh[L] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int e = tid + 256 * t;
+            if (e < RT_NA) {
+                const int r = e / RT_HX, c = e - r * RT_HX;
+                const double p0 = SP[b][0][r][c];
+                const double uc = fu_[t] + hm;
+                double up = uc - (SP[b][0][r][c + 1] - p0) * cx;
+                double vp = fv_[t] - (SP[b][0][r + 1][c] - p0) * cy;
+                if (ghost) { up = 2.0 * ub - up; vp = 2.0 * vb - vp; }
+                SU[sl][r][c] = up;
+                SV[sl][r][c] = vp;
+                SW[sl][r][c] = wok ? fw_[t] - (SP[b][1][r][c] - p0) * cz : 0.0;
+            }
+        }
     };
 
-    const double *rdz = R.rdz + CF_NGT, *rdzh = R.rdzh + CF_NGT;
-    const double dzk = dz[k], dzkp = dz[k + 1];
-    const double rzk = rdz[k], rzkp = rdz[k + 1];
-    const double rzp = rdzh[k], rzm = rdzh[k - 1];          /* 1/dzp1h, 1/dzm1h */
-    const double rdx = R.rdx, rdy = R.rdy, rdxdy = R.rdxdy;
-    const double rdx2 = rdx * rdx, rdy2 = rdy * rdy;
+    /* prologue: levels k0 - 1, k0, k0 + 1 into ring slots 0, 1, 2 */
+    for (int L = k0 - 1; L <= k0 + 1; L++) {
+        const int b = (L - k0 + 1) & 1;
+        fetch(L);
+        stage_p(b);              /* buffer b was last read two builds ago */
+        __syncthreads();
+        build(L, L - k0 + 1, b);
+    }
+    __syncthreads();
+
     const SubCoef C = R.sc[s];
     const double nu = R.nu;
-    const size_t q = (size_t) k * P + (size_t) j * nx + i;
+    const double dx = R.g.dx, dy = R.g.dy;
+    const double rdx = R.rdx, rdy = R.rdy, rdxdy = R.rdxdy;
+    const double rdx2 = rdx * rdx, rdy2 = rdy * rdy;
+    const double gmprev = scal[R.m_prev_slot[s]];
+    const int r0 = ly + 1, c0 = lx + 1;
+    double cflm = 0.0;
+    /* previous tendencies of this point, one level ahead */
+    const size_t qc = (size_t) j * nx + i;
+    double gnu = 0.0, gnv = 0.0, gnw = 0.0;
+    if (act) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
 
-    /* ---- u ---- */
-    const double u0 = U(i, j, k), uip = U(ip, j, k), uim = U(im, j, k);
-    const double ujp = U(i, jp, k), ujm = U(i, jm, k), ukp = U(i, j, k + 1), ukm = U(i, j, k - 1);
-    const double v0 = V(i, j, k), vip = V(ip, j, k), vim = V(im, j, k);
-    const double vjp = V(i, jp, k), vjm = V(i, jm, k), vkp = V(i, j, k + 1), vkm = V(i, j, k - 1);
-    const double vipjm = V(ip, jm, k);
-    const double w0 = W(i, j, k), wip = W(ip, j, k), wkm = W(i, j, k - 1), wipkm = W(ip, j, k - 1);
-    const double wjp = W(i, jp, k), wjpkm = W(i, jp, k - 1);
-    {
-        double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * uip);
-        double mxm = 0.5 * (dy * dzk * uim + dy * dzk * u0);
-        double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vip);
-        double mym = 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
-        double mzp = 0.5 * (dx * dy * w0 + dx * dy * wip);
-        double mzm = 0.5 * (dx * dy * wkm + dx * dy * wipkm);
-        double adv = -((mxp * 0.5 * (u0 + uip) - mxm * 0.5 * (u0 + uim))
-                     + (myp * 0.5 * (u0 + ujp) - mym * 0.5 * (u0 + ujm))
-                     + (mzp * 0.5 * (u0 + ukp) - mzm * 0.5 * (u0 + ukm))) * (rdxdy * rzk);
-        double f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
-        double h = u0 + C.nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
-        double g = gu[q] + scal[R.m_prev_slot[s]];
-        h += C.gamdt * f + C.zetdt * g;
-        if (k == 0) h -= R.lift[s][0];
-        if (k == nz - 1) h -= R.lift[s][1];
-        fu[q] = f;
-        ru[q] = h * dzk;
+    for (int k = k0; k < k1; k++) {
+        const bool more = k + 2 <= k1;
+        const double gcu = gnu, gcv = gnv, gcw = gnw;
+        if (act && k + 1 < k1) {
+            gnu = gu[(size_t) (k + 1) * P + qc];
+            gnv = gv[(size_t) (k + 1) * P + qc];
+            gnw = gw[(size_t) (k + 1) * P + qc];
+        }
+        if (more) fetch(k + 2);
+        const int sm = (k - k0) % 3, s0 = (k - k0 + 1) % 3, sp = (k - k0 + 2) % 3;
+        if (act) {
+            const size_t q = (size_t) k * P + (size_t) j * nx + i;
+            const double dzk = dz[k], dzkp = dz[k + 1];
+            const double rzk = rdz[k], rzkp = rdz[k + 1];
+            const double rzp = rdzh[k], rzm = rdzh[k - 1];
+#define LU(sl, dr, dc) SU[sl][r0 + (dr)][c0 + (dc)]
+#define LV(sl, dr, dc) SV[sl][r0 + (dr)][c0 + (dc)]
+#define LW(sl, dr, dc) SW[sl][r0 + (dr)][c0 + (dc)]
+/* keeps each component's LDS reads next to their use (register pressure) */
+#define SEQ() asm volatile("" ::: "memory")
+            if (cflpart) {
+                double c;
+                if (cflm < (c = fabs(LU(s0, 0, 0)) * cfldt * rdx)) cflm = c;
+                if (cflm < (c = fabs(LV(s0, 0, 0)) * cfldt * rdy)) cflm = c;
+                if (cflm < (c = fabs(LW(s0, 0, 0)) * cfldt * rzp)) cflm = c;
+            }
+            {   /* u */
+                const double u0 = LU(s0, 0, 0), uip = LU(s0, 0, 1), uim = LU(s0, 0, -1);
+                const double ujp = LU(s0, 1, 0), ujm = LU(s0, -1, 0), ukp = LU(sp, 0, 0), ukm = LU(sm, 0, 0);
+                const double v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vjm = LV(s0, -1, 0), vipjm = LV(s0, -1, 1);
+                const double w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wkm = LW(sm, 0, 0), wipkm = LW(sm, 0, 1);
+                double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * uip);
+                double mxm = 0.5 * (dy * dzk * uim + dy * dzk * u0);
+                double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vip);
+                double mym = 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
+                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wip);
+                double mzm = 0.5 * (dx * dy * wkm + dx * dy * wipkm);
+                double adv = -((mxp * 0.5 * (u0 + uip) - mxm * 0.5 * (u0 + uim))
+                             + (myp * 0.5 * (u0 + ujp) - mym * 0.5 * (u0 + ujm))
+                             + (mzp * 0.5 * (u0 + ukp) - mzm * 0.5 * (u0 + ukm))) * (rdxdy * rzk);
+                double f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
+                double h = u0 + C.nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
+                double g = gcu + gmprev;
+                h += C.gamdt * f + C.zetdt * g;
+                if (k == 0) h -= R.lift[s][0];
+                if (k == nz - 1) h -= R.lift[s][1];
+                fu[q] = f;
+                ru[q] = h * dzk;
+            }
+            SEQ();
+            {   /* v */
+                const double u0 = LU(s0, 0, 0), ujp = LU(s0, 1, 0), uim = LU(s0, 0, -1), uimjp = LU(s0, 1, -1);
+                const double v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vim = LV(s0, 0, -1);
+                const double vjp = LV(s0, 1, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vkm = LV(sm, 0, 0);
+                const double w0 = LW(s0, 0, 0), wjp = LW(s0, 1, 0), wkm = LW(sm, 0, 0), wjpkm = LW(sm, 1, 0);
+                double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
+                double mxm = 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
+                double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
+                double mym = 0.5 * (dx * dzk * vjm + dx * dzk * v0);
+                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wjp);
+                double mzm = 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
+                double adv = -((mxp * 0.5 * (v0 + vip) - mxm * 0.5 * (v0 + vim))
+                             + (myp * 0.5 * (v0 + vjp) - mym * 0.5 * (v0 + vjm))
+                             + (mzp * 0.5 * (v0 + vkp) - mzm * 0.5 * (v0 + vkm))) * (rdxdy * rzk);
+                double f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
+                double h = v0 + C.nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
+                h += C.gamdt * f + C.zetdt * gcv;
+                if (k == 0) h -= R.lift[s][2];
+                if (k == nz - 1) h -= R.lift[s][3];
+                fv[q] = f;
+                rv[q] = h * dzk;
+            }
+            SEQ();
+            if (k < nz - 1) {   /* w on the interior faces */
+                const double u0 = LU(s0, 0, 0), uim = LU(s0, 0, -1), ukp = LU(sp, 0, 0), uimkp = LU(sp, 0, -1);
+                const double v0 = LV(s0, 0, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vjmkp = LV(sp, -1, 0);
+                const double w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wim = LW(s0, 0, -1);
+                const double wjp = LW(s0, 1, 0), wjm = LW(s0, -1, 0), wkp = LW(sp, 0, 0), wkm = LW(sm, 0, 0);
+                const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
+                double mxp = 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
+                double mxm = 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
+                double myp = 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
+                double mym = 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
+                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wkp);
+                double mzm = 0.5 * (dx * dy * wkm + dx * dy * w0);
+                double adv = -((mxp * 0.5 * (w0 + wip) - mxm * 0.5 * (w0 + wim))
+                             + (myp * 0.5 * (w0 + wjp) - mym * 0.5 * (w0 + wjm))
+                             + (mzp * 0.5 * (w0 + wkp) - mzm * 0.5 * (w0 + wkm))) * (rdxdy * rzp);
+                double f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
+                double h = w0 + C.nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
+                h += C.gamdt * f + C.zetdt * gcw;
+                fw[q] = f;
+                rw[q] = h * dzp1h;
+            } else {
+                fw[q] = 0.0;
+                rw[q] = 0.0;
+            }
+#undef SEQ
+#undef LU
+#undef LV
+#undef LW
+        }
+        if (more) {
+            /* pressure buffer of level k + 2 alternates with that of k + 1, so
+             * staging it needs no barrier; the ring slot of level k - 1 does */
+            const int b = (k + 2 - k0 + 1) & 1;
+            stage_p(b);
+            __syncthreads();
+            build(k + 2, sm, b);
+            __syncthreads();
+        }
     }
-    /* ---- v ---- */
-    {
-        const double uimjp = U(im, jp, k);
-        double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
-        double mxm = 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
-        double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
-        double mym = 0.5 * (dx * dzk * vjm + dx * dzk * v0);
-        double mzp = 0.5 * (dx * dy * w0 + dx * dy * wjp);
-        double mzm = 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
-        double adv = -((mxp * 0.5 * (v0 + vip) - mxm * 0.5 * (v0 + vim))
-                     + (myp * 0.5 * (v0 + vjp) - mym * 0.5 * (v0 + vjm))
-                     + (mzp * 0.5 * (v0 + vkp) - mzm * 0.5 * (v0 + vkm))) * (rdxdy * rzk);
-        double f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
-        double h = v0 + C.nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
-        h += C.gamdt * f + C.zetdt * gv[q];
-        if (k == 0) h -= R.lift[s][2];
-        if (k == nz - 1) h -= R.lift[s][3];
-        fv[q] = f;
-        rv[q] = h * dzk;
+    if (cflpart) {
+        red[tid] = cflm;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
+            __syncthreads();
+        }
+        if (tid == 0) cflpart[blockIdx.x] = red[0];
     }
-    /* ---- w (faces k < nz - 1; the top wall face carries no equation) ---- */
-    if (k < nz - 1) {
-        const double uimkp = U(im, j, k + 1);
-        const double vjmkp = V(i, jm, k + 1);
-        const double wim = W(im, j, k), wjm = W(i, jm, k), wkp = W(i, j, k + 1);
-        const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
-        double mxp = 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
-        double mxm = 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
-        double myp = 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
-        double mym = 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
-        double mzp = 0.5 * (dx * dy * w0 + dx * dy * wkp);
-        double mzm = 0.5 * (dx * dy * wkm + dx * dy * w0);
-        double adv = -((mxp * 0.5 * (w0 + wip) - mxm * 0.5 * (w0 + wim))
-                     + (myp * 0.5 * (w0 + wjp) - mym * 0.5 * (w0 + wjm))
-                     + (mzp * 0.5 * (w0 + wkp) - mzm * 0.5 * (w0 + wkm))) * (rdxdy * rzp);
-        double f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
-        double h = w0 + C.nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
-        h += C.gamdt * f + C.zetdt * gw[q];
-        fw[q] = f;
-        rw[q] = h * dzp1h;
-    } else {
-        fw[q] = 0.0;
-        rw[q] = 0.0;
-    }
-#undef AT
 }
 
 /* ---------------------------------------------------------------------
@@ -406,32 +542,6 @@ __global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int
  * then a one-block finish that writes the cfl.dat record and bumps the
  * step counter at the end of the step (k_step_end).
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_cfl_part(const RunDev R, const double *__restrict__ u,
-                                                  const double *__restrict__ v, const double *__restrict__ w,
-                                                  double dt, double *__restrict__ part, int nb)
-{
-    __shared__ double red[256];
-    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const size_t N = (size_t) nx * ny * nz, P = (size_t) nx * ny;
-    const double *dz = R.dzt + CF_NGT;
-    double m = 0.0;
-    for (size_t e = (size_t) blockIdx.x * 256 + threadIdx.x; e < N; e += (size_t) nb * 256) {
-        int k = (int) (e / P);
-        double dzph = 0.5 * (dz[k] + dz[k + 1]);
-        double c;
-        if (m < (c = fabs(u[e]) * dt / R.g.dx)) m = c;
-        if (m < (c = fabs(v[e]) * dt / R.g.dy)) m = c;
-        if (m < (c = fabs(w[e]) * dt / dzph)) m = c;
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
 __global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb,
                                                     double dt, double *__restrict__ scal, double *__restrict__ logbuf)
 {

@@ -85,9 +85,13 @@ struct cf_run {
     PoisTab pt[4];                   /* 3 substeps + ad-hoc dt */
     double pt_dt = -1.0;
     size_t off_pois_adhoc = 0;
-    /* graph */
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    /* graphs: one RK3 step starting from state set 0 or 1 */
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int cur = 0;                     /* state set holding the current solve output */
+    bool proj_pending = false;       /* p / forcings not yet folded into the state */
+    double *cflpart = nullptr;
+    int rhs_blocks = 0, rhs_kc = 1;
     int steps_done = 0;
     bool use_spike = false;
     int spike_waves = 1;
@@ -352,11 +356,13 @@ extern "C" cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes)
 static void free_run(cf_run *r)
 {
     if (!r) return;
-    if (r->gexec) (void) hipGraphExecDestroy(r->gexec);
-    if (r->graph) (void) hipGraphDestroy(r->graph);
+    for (int q = 0; q < 2; q++) {
+        if (r->gexec[q]) (void) hipGraphExecDestroy(r->gexec[q]);
+        if (r->graph[q]) (void) hipGraphDestroy(r->graph[q]);
+    }
     double *bufs[] = { r->u, r->v, r->w, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2],
                        r->fb[0], r->fb[1], r->fb[2], r->p, r->G, r->stage, r->tab, r->scal,
-                       r->logbuf, r->part };
+                       r->logbuf, r->part, r->cflpart };
     for (double *b : bufs) if (b) (void) hipFree(b);
     if (r->X) (void) hipFree(r->X);
     for (auto &pe : r->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
@@ -536,6 +542,13 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     size_t npart = (size_t) r->nb_plane * (nz + 2);
     if (npart < (size_t) r->nb_red) npart = r->nb_red;
     ok = ok && dalloc(&r->part, npart);
+    {
+        int nchunk = (nz + CF_RHS_KC - 1) / CF_RHS_KC;
+        r->rhs_kc = (nz + nchunk - 1) / nchunk;
+        nchunk = (nz + r->rhs_kc - 1) / r->rhs_kc;
+        r->rhs_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nchunk;
+    }
+    ok = ok && dalloc(&r->cflpart, (size_t) r->rhs_blocks);
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
         free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
@@ -650,9 +663,16 @@ static unsigned fy_blocks(const cf_run *r)
     return (unsigned) ((lines + r->rd.fy_cols - 1) / r->rd.fy_cols);
 }
 
-/* divergence -> FFT -> Poisson -> inverse FFT -> projection into (u, v, w) */
-static cf_status enqueue_projection(cf_run *r, const double *us, const double *vs, const double *ws,
-                                    const double *h1, const double *mptr, const PoisTab &T, double dtp)
+static void state_set(cf_run *r, int q, double **u, double **v, double **w)
+{
+    if (q == 0) { *u = r->u; *v = r->v; *w = r->w; }
+    else { *u = r->ru; *v = r->rv; *w = r->rw; }
+}
+
+/* divergence -> FFT -> Poisson -> inverse FFT into p (pressure.c:40-61,
+ * chanfast.c:639-651) */
+static cf_status enqueue_pressure(cf_run *r, const double *us, const double *vs, const double *ws,
+                                  const double *h1, const double *mptr, const PoisTab &T)
 {
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz;
@@ -664,40 +684,67 @@ static cf_status enqueue_projection(cf_run *r, const double *us, const double *v
     KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, T, r->X, r->G);
     KLAUNCH("ffty_inv", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 0);
     KLAUNCH("fftx_inv", r->kfxi, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
-    KLAUNCH("project", k_project2, cell_grid(r), dim3(64, 4), 0, R, us, vs, ws, h1, mptr,
-            (const double *) r->p, dtp, r->u, r->v, r->w);
     return CF_OK;
 }
 
-static cf_status enqueue_step(cf_run *r)
+/* the projection parameters in force for the current state: the last
+ * substep's h1, forcing slot and (alpha + beta) dt */
+static ProjSrc current_projection(cf_run *r)
+{
+    ProjSrc S;
+    state_set(r, r->cur, (double **) &S.us, (double **) &S.vs, (double **) &S.ws);
+    S.p = r->p;
+    S.h1 = r->rd.h1 + (size_t) (2 * 2 + 0) * (r->g.nz + 2);
+    S.m = r->scal + SC_M0 + 2;
+    S.dtp = r->rd.sc[2].abdt;
+    return S;
+}
+
+/* materialise the projected velocity of the current state into (u, v, w) */
+static cf_status enqueue_materialise(cf_run *r, double *u, double *v, double *w)
+{
+    ProjSrc S = current_projection(r);
+    KLAUNCH("materialise", k_project2, cell_grid(r), dim3(64, 4), 0, r->rd, S.us, S.vs, S.ws, S.h1, S.m,
+            S.p, S.dtp, u, v, w);
+    return CF_OK;
+}
+
+/* one RK3 step from state set `from` to the other set */
+static cf_status enqueue_step(cf_run *r, int from)
 {
     const RunDev &R = r->rd;
     const int nz = r->g.nz;
     const double dt = r->cfg.dt;
-    KLAUNCH("cfl", k_cfl_part, dim3(r->nb_red), dim3(256), 0, R, (const double *) r->u,
-            (const double *) r->v, (const double *) r->w, dt, r->part, r->nb_red);
-    KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->part, r->nb_red, dt,
-            r->scal, r->logbuf);
     for (int s = 0; s < 3; s++) {
         double **fnew = (s % 2 == 0) ? r->fa : r->fb;
         double **gold = (s % 2 == 0) ? r->fb : r->fa;
-        KLAUNCH("rhs", k_rhs2, cell_grid(r), dim3(64, 4), 0, R, s,
-                (const double *) r->u, (const double *) r->v, (const double *) r->w,
+        double *iu, *iv, *iw, *ou, *ov, *ow;
+        state_set(r, (from + s) % 2, &iu, &iv, &iw);
+        state_set(r, (from + s + 1) % 2, &ou, &ov, &ow);
+        const int ps = (s + 2) % 3;
+        ProjSrc S;
+        S.us = iu; S.vs = iv; S.ws = iw; S.p = r->p;
+        S.h1 = R.h1 + (size_t) (ps * 2 + 0) * (nz + 2);
+        S.m = r->scal + SC_M0 + ps;
+        S.dtp = R.sc[ps].abdt;
+        KLAUNCH("rhs", k_rhs2t, dim3(r->rhs_blocks), dim3(256), 0, R, s, S,
                 (const double *) gold[0], (const double *) gold[1], (const double *) gold[2],
-                (const double *) r->scal, fnew[0], fnew[1], fnew[2], r->ru, r->rv, r->rw);
+                (const double *) r->scal, fnew[0], fnew[1], fnew[2], ou, ov, ow,
+                s == 0 ? r->cflpart : (double *) nullptr, dt, r->rhs_kc);
+        if (s == 0)
+            KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart,
+                    r->rhs_blocks, dt, r->scal, r->logbuf);
         if (r->use_spike)
             KLAUNCH("spike", k_spike, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
-                    r->ru, r->rv, r->rw);
+                    ou, ov, ow);
         else
-            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s,
-                    r->ru, r->rv, r->rw);
+            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow);
         KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
-                (const double *) r->ru, R.ubc, r->part, r->nb_plane);
+                (const double *) ou, R.ubc, r->part, r->nb_plane);
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
                 (const double *) r->part, r->nb_plane, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
-        cf_status st = enqueue_projection(r, r->ru, r->rv, r->rw, h1, r->scal + SC_M0 + s, r->pt[s],
-                                          R.sc[s].abdt);
+        cf_status st = enqueue_pressure(r, ou, ov, ow, h1, r->scal + SC_M0 + s, r->pt[s]);
         if (st != CF_OK) return st;
     }
     KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
@@ -729,12 +776,36 @@ static cf_status collect_profile(cf_run *r)
         HIPCHK(hipSetDevice((r)->cfg.device));                        \
     } while (0)
 
+/* After an upload or an explicit projection the stored solve output is
+ * the velocity itself: zero the pending pressure and forcings.  The other
+ * fields of the current state are materialised first so they keep their
+ * projected values. */
+static cf_status reset_projection(cf_run *r)
+{
+    if (!r->proj_pending) return CF_OK;
+    double *su, *sv, *sw;
+    state_set(r, r->cur, &su, &sv, &sw);
+    cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+    if (st != CF_OK) return st;
+    HIPCHK(hipMemcpyAsync(su, r->fa[0], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(sv, r->fa[1], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(sw, r->fa[2], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemsetAsync(r->p, 0, sizeof(double) * r->N, r->stream));
+    HIPCHK(hipMemsetAsync(r->scal + SC_M0, 0, sizeof(double) * 6, r->stream));
+    r->proj_pending = false;
+    return CF_OK;
+}
+
 extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
 {
     CHECK_RUN(r);
     if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
-    double *dst = field == CF_U ? r->u : field == CF_V ? r->v : field == CF_W ? r->w : nullptr;
+    double *su, *sv, *sw;
+    state_set(r, r->cur, &su, &sv, &sw);
+    double *dst = field == CF_U ? su : field == CF_V ? sv : field == CF_W ? sw : nullptr;
     if (!dst) { cf_set_error("cf_run_upload: field %d is not uploadable in this configuration", field); return CF_ERR_ARG; }
+    cf_status rs = reset_projection(r);
+    if (rs != CF_OK) return rs;
     HIPCHK(hipMemcpyAsync(r->stage, host, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
     hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, dst,
@@ -758,10 +829,14 @@ extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
 {
     CHECK_RUN(r);
     if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
-    const double *src = field == CF_U ? r->u : field == CF_V ? r->v : field == CF_W ? r->w
-                      : field == CF_P ? r->p : nullptr;
-    if (!src) { cf_set_error("cf_run_download: field %d not held in this configuration", field); return CF_ERR_ARG; }
-    return download_dev(r, src, host);
+    if (field == CF_P) return download_dev(r, r->p, host);
+    if (field != CF_U && field != CF_V && field != CF_W) {
+        cf_set_error("cf_run_download: field %d not held in this configuration", field);
+        return CF_ERR_ARG;
+    }
+    cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+    if (st != CF_OK) return st;
+    return download_dev(r, r->fa[field], host);
 }
 
 static cf_status set_adhoc_poisson(cf_run *r, double dt)
@@ -784,8 +859,18 @@ extern "C" cf_status cf_run_project(cf_run *r, double dt)
     if (!(dt > 0)) { cf_set_error("cf_run_project: dt must be positive"); return CF_ERR_ARG; }
     cf_status st = set_adhoc_poisson(r, dt);
     if (st != CF_OK) return st;
-    st = enqueue_projection(r, r->u, r->v, r->w, nullptr, nullptr, r->pt[3], dt);
+    st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
+    st = enqueue_pressure(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, r->pt[3]);
+    if (st != CF_OK) return st;
+    double *su, *sv, *sw;
+    state_set(r, r->cur, &su, &sv, &sw);
+    KLAUNCH("project", k_project2, cell_grid(r), dim3(64, 4), 0, r->rd, (const double *) r->fa[0],
+            (const double *) r->fa[1], (const double *) r->fa[2], (const double *) nullptr,
+            (const double *) nullptr, (const double *) r->p, dt, su, sv, sw);
+    HIPCHK(hipMemsetAsync(r->p, 0, sizeof(double) * r->N, r->stream));
+    HIPCHK(hipMemsetAsync(r->scal + SC_M0, 0, sizeof(double) * 6, r->stream));
+    r->proj_pending = false;
     HIPCHK(hipStreamSynchronize(r->stream));
     return collect_profile(r);
 }
@@ -797,23 +882,30 @@ extern "C" cf_status cf_run_step(cf_run *r, int nsteps)
     if (nsteps == 0) return CF_OK;
     if (r->profile) {
         for (int n = 0; n < nsteps; n++) {
-            cf_status st = enqueue_step(r);
+            cf_status st = enqueue_step(r, r->cur);
             if (st != CF_OK) return st;
+            r->cur ^= 1;
+            r->proj_pending = true;
         }
         r->steps_done += nsteps;
         return collect_profile(r);
     }
-    if (!r->gexec) {
+    for (int q = 0; q < 2; q++) {
+        if (r->gexec[q]) continue;
         HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-        cf_status st = enqueue_step(r);
+        cf_status st = enqueue_step(r, q);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(r->stream, &gr);
         if (st != CF_OK) { if (gr) (void) hipGraphDestroy(gr); return st; }
         HIPCHK(e);
-        r->graph = gr;
-        HIPCHK(hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0));
+        r->graph[q] = gr;
+        HIPCHK(hipGraphInstantiate(&r->gexec[q], r->graph[q], nullptr, nullptr, 0));
     }
-    for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(r->gexec, r->stream));
+    for (int n = 0; n < nsteps; n++) {
+        HIPCHK(hipGraphLaunch(r->gexec[r->cur], r->stream));
+        r->cur ^= 1;
+    }
+    r->proj_pending = true;
     r->steps_done += nsteps;
     return CF_OK;
 }
@@ -847,8 +939,10 @@ extern "C" cf_status cf_run_max_divergence(cf_run *r, double *out)
 {
     CHECK_RUN(r);
     if (!out) { cf_set_error("null output"); return CF_ERR_ARG; }
+    cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+    if (st != CF_OK) return st;
     hipLaunchKernelGGL(k_maxdiv_part, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd,
-                       (const double *) r->u, (const double *) r->v, (const double *) r->w, r->part, r->nb_red);
+                       (const double *) r->fa[0], (const double *) r->fa[1], (const double *) r->fa[2], r->part, r->nb_red);
     LAUNCHCHK();
     std::vector<double> part(r->nb_red);
     HIPCHK(hipMemcpyAsync(part.data(), r->part, sizeof(double) * r->nb_red, hipMemcpyDeviceToHost, r->stream));
@@ -893,12 +987,12 @@ extern "C" cf_status cf_op_transpose_roundtrip(cf_run *r, const double *in_io, d
     if (!in_io) { cf_set_error("null input"); return CF_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->p,
+    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
                        r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
-    if (dev_out) HIPCHK(hipMemcpyAsync(dev_out, r->p, sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
+    if (dev_out) HIPCHK(hipMemcpyAsync(dev_out, r->fb[0], sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
-    if (io_out) return download_dev(r, r->p, io_out);
+    if (io_out) return download_dev(r, r->fb[0], io_out);
     return CF_OK;
 }
 
@@ -933,13 +1027,13 @@ extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spe
     if (!in_io || !spec_out) { cf_set_error("null argument"); return CF_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->p,
+    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
                        r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double *) r->p, r->X);
+    hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double *) r->fb[0], r->X);
     LAUNCHCHK();
     dim3 gy(fy_blocks(r));
     hipLaunchKernelGGL(r->kfy, gy, dim3(256), fy_lds(r), r->stream, R, r->X, 1);
@@ -964,10 +1058,10 @@ extern "C" cf_status cf_op_fft3d_bwd(cf_run *r, const double *spec_in, double *o
     LAUNCHCHK();
     long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    hipLaunchKernelGGL(r->kfxi, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double2 *) r->X, r->p);
+    hipLaunchKernelGGL(r->kfxi, dim3(gx), dim3(256), fx_lds(r), r->stream, R, (const double2 *) r->X, r->fb[0]);
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(r->stream));
-    return download_dev(r, r->p, out_io);
+    return download_dev(r, r->fb[0], out_io);
 }
 
 extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)

@@ -456,3 +456,60 @@ def test_step_restart_bit_identical():
     c.steps(2)
     for n in "uvw":
         assert np.array_equal(a.get(n), c.get(n))
+
+
+def general_run(order, scalar, nx=12, ny=12, nz=17, seed=4321, cvolavgmode=0, betg_z=0.0):
+    dz = tanh_dz(nz)
+    cfg = dict(nx=nx, ny=ny, nz=nz, order=order, scalar=scalar, lx=2 * np.pi, ly=np.pi,
+               dt=5e-4, want_cfl=0.5, cflmode=0, nu=1 / 3300, prandtl=0.71, dpdx=0.0,
+               want_uvolavg=1.0, uvolavgmode=1, want_cvolavg=0.1, cvolavgmode=cvolavgmode,
+               uframe=0.0, vframe=0.0, betg_x=0.0, betg_y=0.0, betg_z=betg_z, dz=dz)
+    rng = np.random.default_rng(seed)
+    zc = np.cumsum(dz) - 0.5 * dz
+    o = ol.OracleRun(cfg)
+    o.put("u", 1.5 * (1 - (zc - 1.0) ** 2)[None, None, :] + 0.1 * rng.uniform(-1, 1, (nx, ny, nz)))
+    o.put("v", 0.1 * rng.uniform(-1, 1, (nx, ny, nz)))
+    wi = 0.1 * rng.uniform(-1, 1, (nx, ny, nz))
+    wi[:, :, -1] = 0.0
+    o.put("w", wi)
+    if scalar:
+        o.put("c", (zc / 2.0 - 0.5)[None, None, :] + 0.05 * rng.uniform(-1, 1, (nx, ny, nz)))
+    o.project_ic(1.0)
+    return o, cfg
+
+
+@pytest.mark.parametrize("order,scalar", [(4, 0), (2, 1), (4, 1)])
+def test_step_invariants_general(order, scalar):
+    o, cfg = general_run(order, scalar)
+    c0 = o.get("c") if scalar else None
+    log = o.steps(4)
+    assert np.all(np.isfinite(log))
+    assert np.allclose(log[:, 6], 1.0, rtol=0, atol=1e-13)      # bulk velocity held at Ub
+    assert o.max_div() <= 1e-12
+    if scalar:
+        c = o.get("c")
+        # QUICK transport + diffusion of a field bounded by the wall values
+        # stays within a few percent of its initial range
+        assert c.min() >= c0.min() - 0.05 and c.max() <= c0.max() + 0.05
+
+
+def test_scalar_mean_constraint_order4():
+    """cvolavgmode = 1 (chanfast.c:572-617): the bulk scalar is held at
+    the target every step."""
+    o, cfg = general_run(4, 1, cvolavgmode=1)
+    log = o.steps(3)
+    assert np.allclose(log[:, 11], 0.1, rtol=0, atol=1e-13)
+
+
+def test_step_restart_bit_identical_order4_scalar():
+    a, cfg = general_run(4, 1)
+    a.steps(4)
+    b, _ = general_run(4, 1)
+    b.steps(2)
+    mid = {n: b.get(n) for n in "uvwc"}
+    c = ol.OracleRun(cfg)
+    for n in "uvwc":
+        c.put(n, mid[n])
+    c.steps(2)
+    for n in "uvwc":
+        assert np.array_equal(a.get(n), c.get(n)), n

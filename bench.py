@@ -26,7 +26,7 @@ import numpy as np  # noqa: E402
 import chanflow as cf  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
-STEP_BYTES_PER_PT = 720.0      # algorithmic FP64 bytes per point per RK3 step, order 2
+STEP_BYTES_PER_PT = 720.0      # 30 arrays x 8 B x 3 substeps: compulsory FP64 traffic per point and RK3 step, order 2
 
 
 def alg_bytes(kernel, N, M):

@@ -6,6 +6,7 @@
 #define CF_KERNELS_H
 
 #include "cf_dev.h"
+#include "cf_tables.h"
 
 /* slots of the per-run device scalar block */
 enum {
@@ -36,6 +37,28 @@ struct SpikeTab {
     const double *a, *ib, *gam, *vl, *wr, *minv;
 };
 
+/* ghost closures of cell (Dirichlet) and face fields at both walls */
+struct Clos {
+    Clos1 cell[2], face[2];     /* [0] bottom, [1] top */
+};
+
+/* factors of one wall-reduced order-4 z system over n unknowns:
+ * t[0..7n) = a | al | be | iden | p | q | r (cf_tables.h, hept_factor) */
+struct HeptTab {
+    int n;
+    const double *t;
+};
+
+/* order-4 Poisson z-tables for one dt: the wall-reduced bands at zero
+ * horizontal wavenumber for the Neumann bottom (band[0]) and the
+ * Dirichlet-mean bottom of the (0,0) mode (band[1]), 7 nz each (column
+ * offsets -3..+3, level fastest), the diagonal shifts per unit modified
+ * wavenumber and the volumes (pressure.c:357-444) */
+struct Pois4Tab {
+    const double *band[2];
+    const double *ex, *ey, *dvol;
+};
+
 /* Everything a substep kernel needs that is fixed for the run: passed by
  * value (a few hundred bytes of kernel arguments). */
 struct RunDev {
@@ -62,6 +85,13 @@ struct RunDev {
     const double *lamx, *lamy;  /* modified wavenumbers 2 - 2 cos(2 pi k / n) */
     int fx_pairs;               /* complex x lines per block */
     int fy_cols;                /* kx columns per y-FFT block */
+    /* general path (order 4 and/or the scalar) */
+    Clos cl;
+    int scalar;
+    double betg[3];
+    double kap;                 /* scalar diffusivity nu / Pr */
+    const double *liftg;        /* [s][field u, v, w, c][nz] boundary-lift profiles */
+    HeptTab hp[3][3];           /* order-4 implicit z factors [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
 };
 
 /* Where a substep's projected velocity comes from: the previous
@@ -84,8 +114,8 @@ __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
                         const double *gu, const double *gv, const double *gw, const double *scal,
                         double *fu, double *fv, double *fw, double *ru, double *rv, double *rw,
                         double *cflpart, double cfldt, int kc);
-__global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw);
-__global__ void k_spike(const RunDev R, const int s, double *ru, double *rv, double *rw);
+__global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
+__global__ void k_spike(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
 __global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
                                 double *scal, double *logbuf, int kind);
@@ -97,7 +127,7 @@ typedef void (*k_div_fftx_t)(const RunDev, const double *, const double *, const
 typedef void (*k_fftx_plain_t)(const RunDev, const double *, double2 *);
 typedef void (*k_ffty_t)(const RunDev, double2 *, int);
 typedef void (*k_fftx_inv_t)(const RunDev, const double2 *, double *);
-k_div_fftx_t pick_div_fftx(int nx);
+k_div_fftx_t pick_div_fftx(int nx, int order);
 k_fftx_plain_t pick_fftx_plain(int nx);
 k_ffty_t pick_ffty(int ny);
 k_fftx_inv_t pick_fftx_inv(int nx);
@@ -107,5 +137,21 @@ __global__ void k_project2(const RunDev R, const double *us, const double *vs, c
                            double *u, double *v, double *w);
 __global__ void k_maxdiv_part(const RunDev R, const double *u, const double *v, const double *w,
                               double *part, int nb);
+
+/* general path (cf_general.hip) */
+typedef void (*k_grhs_t)(const RunDev, const int, const double *, const double *, const double *,
+                         const double *, const double *, const double *, double *, double *, double *, double);
+struct GRhsSet {
+    k_grhs_t u, v, w, c;
+};
+GRhsSet pick_grhs(int order, int scalar);
+__global__ void k_gsolve4(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
+__global__ void k_gplane(const RunDev R, const double *q0, const double *q1, double *part, int nb);
+__global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);
+__global__ void k_poisson4(const RunDev R, const Pois4Tab T, double2 *Y, double *S);
+typedef void (*k_gproject_t)(const RunDev, const double *, const double *, const double *, const double *,
+                             const double *, const double *, const double *, const double *, const double *,
+                             double, double *, double *, double *, double *);
+k_gproject_t pick_gproject(int order);
 
 #endif

@@ -14,6 +14,9 @@
  * (x, z) plane is transposed through a 32x33 LDS tile (bit-exact moves).
  * Replaces the pencil redistributions of transpose.c:409-448 on one GPU.
  * ------------------------------------------------------------------- */
+static constexpr double A1_ = 1.125 / 1.0;      /* fourth.c:3-6 */
+static constexpr double A2_ = -0.125 / 3.0;
+
 __global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io, double *__restrict__ dev,
                                                    int nx, int ny, int nz)
 {
@@ -324,18 +327,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
  * factors come from the wall-reduced system of solve_helmholtz1d_u2 /
  * _w2 (diffuseu.c:173-219, diffusew.c:175-223 with tridag.c:4-26), built
  * on the host once per substep since they do not depend on the column.
- * Fields: 0 u (nz rows), 1 v (nz rows), 2 w (nz-1 rows; top wall := 0).
+ * Fields: 0 u (nz rows), 1 v (nz rows), 2 w (nz-1 rows; top wall := 0),
+ * 3 the scalar c (nz rows, diffusivity nu / Pr).
  * ------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw)
+    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ rc)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const size_t col = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
-    if (col >= P || f > 2) return;
-    double *x = f == 0 ? ru : (f == 1 ? rv : rw);
-    const int type = f == 2 ? 2 : 0;
+    if (col >= P || f > 3) return;
+    double *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
+    const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
     const int n = f == 2 ? nz - 1 : nz;
     const double *a = R.th + ((size_t) (s * 4 + type) * 3 + 0) * nz;
     const double *ib = R.th + ((size_t) (s * 4 + type) * 3 + 1) * nz;
@@ -390,21 +394,21 @@ __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
  * diffusew.c:175-223); differs from sequential Thomas by round-off.
  * ------------------------------------------------------------------- */
 __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw)
+    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ rc)
 {
     __shared__ double itf[2 * CF_SPIKE_SMAX][64];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int f = blockIdx.y;
-    if (f > 2) return;
-    const int type = f == 2 ? 2 : 0;
+    if (f > 3) return;
+    const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
     const SpikeTab T = R.sp[s][type];
     const int lane = threadIdx.x & 63;
     /* wave-uniform chunk index, so the table reads below are scalar loads */
     const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t col = (size_t) blockIdx.x * 64 + lane;
     const bool active = col < P && chunk < T.S;
-    double *x = f == 0 ? ru : (f == 1 ? rv : rw);
+    double *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int lo = chunk * T.L;
     const int hi = min(lo + T.L, T.n);
     double *xc = x + col;
@@ -597,10 +601,11 @@ static __device__ __forceinline__ void store_half_spectra(const double2 *sb, int
     }
 }
 
-/* divergence2 (pressure.c:40-61) of the mass-flux-corrected velocity
- * (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x FFT
- * scaled 1/nx (fft.c:161-177) */
-template <int NX>
+/* divergence2 / divergence4 (pressure.c:40-118) of the mass-flux-corrected
+ * velocity (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x
+ * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
+ * mirrored oddly about the (zero) wall faces, as divergence4 sets them. */
+template <int NX, int ORD>
 __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *__restrict__ us,
     const double *__restrict__ vs, const double *__restrict__ ws, const double *__restrict__ h1,
     const double *__restrict__ mptr, double2 *__restrict__ X)
@@ -625,7 +630,16 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *
         double v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
         double w1 = k < nz - 1 ? ws[q + i] : 0.0;
         double w0 = k > 0 ? ws[q - P + i] : 0.0;
-        return (u1 - u0) * R.rdx + (v1 - v0) * R.rdy + (w1 - w0) * rdz[k];
+        if (ORD == 2) return (u1 - u0) * R.rdx + (v1 - v0) * R.rdy + (w1 - w0) * rdz[k];
+        const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
+        const int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
+        double u2 = us[q + ip] + hm, um = us[q + im2] + hm;
+        double v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
+        /* w at faces k+1 and k-2: walls at -1 and nz-1 are 0, ghosts odd */
+        double w2 = k + 1 <= nz - 2 ? ws[q + P + i] : (k + 1 == nz - 1 ? 0.0 : -ws[q - P + i]);
+        double wm = k - 2 >= 0 ? ws[q - 2 * P + i] : (k - 2 == -1 ? 0.0 : -ws[q + i]);
+        return (A1_ * (u1 - u0) + A2_ * (u2 - um)) * R.rdx + (A1_ * (v1 - v0) + A2_ * (v2 - vm)) * R.rdy
+             + (A1_ * (w1 - w0) + A2_ * (w2 - wm)) * rdz[k];
     };
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
@@ -733,13 +747,21 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 
 #define CF_FFT_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) \
     X(120) X(128) X(160) X(192) X(240) X(256) X(320) X(384) X(480) X(512)
 
-k_div_fftx_t pick_div_fftx(int n)
+k_div_fftx_t pick_div_fftx(int n, int order)
 {
+    if (order == 4) {
+        switch (n) {
+#define X_(N) case N: return k_div_fftx<N, 4>;
+        CF_FFT_SIZES(X_)
+#undef X_
+        default: return k_div_fftx<0, 4>;
+        }
+    }
     switch (n) {
-#define X_(N) case N: return k_div_fftx<N>;
+#define X_(N) case N: return k_div_fftx<N, 2>;
     CF_FFT_SIZES(X_)
 #undef X_
-    default: return k_div_fftx<0>;
+    default: return k_div_fftx<0, 2>;
     }
 }
 
@@ -888,8 +910,20 @@ __global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const doubl
         size_t row = (size_t) k * P + (size_t) j * nx;
         double w1 = k < nz - 1 ? w[e] : 0.0;
         double w0 = k > 0 ? w[e - P] : 0.0;
-        double d = (u[e] - u[row + im]) / R.g.dx + (v[e] - v[(size_t) k * P + (size_t) jm * nx + i]) / R.g.dy
-                 + (w1 - w0) / dz[k];
+        double d;
+        if (R.g.order == 2) {
+            d = (u[e] - u[row + im]) / R.g.dx + (v[e] - v[(size_t) k * P + (size_t) jm * nx + i]) / R.g.dy
+              + (w1 - w0) / dz[k];
+        } else {                                   /* divergence4, pressure.c:63-118 */
+            int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
+            int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
+            double w2 = k + 1 <= nz - 2 ? w[e + P] : (k + 1 == nz - 1 ? 0.0 : -w[e - P]);
+            double wm = k - 2 >= 0 ? w[e - 2 * P] : (k - 2 == -1 ? 0.0 : -w[e]);
+            d = (A1_ * (u[e] - u[row + im]) + A2_ * (u[row + ip] - u[row + im2])) / R.g.dx
+              + (A1_ * (v[e] - v[(size_t) k * P + (size_t) jm * nx + i])
+                 + A2_ * (v[(size_t) k * P + (size_t) jp * nx + i] - v[(size_t) k * P + (size_t) jm2 * nx + i])) / R.g.dy
+              + (A1_ * (w1 - w0) + A2_ * (w2 - wm)) / dz[k];
+        }
         mx = fmax(mx, fabs(d));
     }
     red[threadIdx.x] = mx;

@@ -35,7 +35,7 @@ extern "C" void cf_set_error(const char *fmt, ...)
 }
 
 extern "C" const char *cf_last_error(void) { return g_err; }
-extern "C" const char *cf_version(void) { return "chanflow-mi355x 0.1 (gfx950, FP64 order 2)"; }
+extern "C" const char *cf_version(void) { return "chanflow-mi355x 0.2 (gfx950, FP64, order 2 and 4, passive scalar)"; }
 
 #define HIPCHK(x)                                                                           \
     do {                                                                                    \
@@ -72,9 +72,10 @@ struct cf_run {
     size_t N, P, M;                  /* cells, plane, spectral modes per level */
     hipStream_t stream = nullptr;
     /* device buffers */
-    double *u = nullptr, *v = nullptr, *w = nullptr;
-    double *ru = nullptr, *rv = nullptr, *rw = nullptr;
-    double *fa[3] = {nullptr, nullptr, nullptr}, *fb[3] = {nullptr, nullptr, nullptr};
+    double *u = nullptr, *v = nullptr, *w = nullptr, *c = nullptr;
+    double *ru = nullptr, *rv = nullptr, *rw = nullptr, *rc = nullptr;
+    double *fa[4] = {nullptr, nullptr, nullptr, nullptr}, *fb[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *S4 = nullptr;            /* order-4 Poisson U factors (3 per spectral value) */
     double *p = nullptr, *G = nullptr, *stage = nullptr;
     double2 *X = nullptr;
     double *tab = nullptr;           /* all constant tables in one allocation */
@@ -83,6 +84,14 @@ struct cf_run {
     int nb_plane = 1, nb_red = 1;
     RunDev rd;
     PoisTab pt[4];                   /* 3 substeps + ad-hoc dt */
+    Pois4Tab pt4[4];
+    size_t off_pois4_adhoc = 0;
+    /* general path: order 4 and/or the scalar (cf_general.hip) */
+    bool general = false;
+    int nfield = 3;
+    GRhsSet grhs;
+    k_gproject_t kproj = nullptr;
+    int gblocks = 0;
     double pt_dt = -1.0;
     size_t off_pois_adhoc = 0;
     /* graphs: one RK3 step starting from state set 0 or 1 */
@@ -309,6 +318,33 @@ static void poisson_tables(const double *dzl, int nz, double dx, double dy, doub
 
 #undef DZ
 
+/* order-4 Poisson z-tables for one dt (pressure.c:357-444 with the
+ * closures of :534-726): wall-reduced bands at zero horizontal
+ * wavenumber, Neumann bottom (variant 0) or the Dirichlet-mean bottom of
+ * the (0,0) mode (variant 1), Neumann top */
+static int pois4_tables(const double *dzl, int nz, double dx, double dy, double dt, std::vector<double> &T)
+{
+    Clos1 nb, db, nt;
+    if (cft::closure(cft::CELL, 4, 0, cft::NEUMANN, dzl, nz, nb) || cft::closure(cft::CELL, 4, 0, cft::DIRICHLET, dzl, nz, db)
+        || cft::closure(cft::CELL, 4, 1, cft::NEUMANN, dzl, nz, nt))
+        return -1;
+    T.assign((size_t) 17 * nz, 0.0);
+    for (int v = 0; v < 2; v++) {
+        std::vector<cft::Row7> rows;
+        cft::op_rows(cft::CELL, 4, dzl, nz, dt, dx * dy, false, rows);
+        cft::eliminate(cft::CELL, nz, v ? db : nb, nt, rows);
+        for (int t = 0; t < 7; t++)
+            for (int k = 0; k < nz; k++) T[(size_t) (v * 7 + t) * nz + k] = rows[k].c[t];
+    }
+    for (int k = 0; k < nz; k++) {
+        double dvol = dx * dy * dzl[k];
+        T[(size_t) 14 * nz + k] = dvol * dt / (dx * dx);
+        T[(size_t) 15 * nz + k] = dvol * dt / (dy * dy);
+        T[(size_t) 16 * nz + k] = dvol;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------ helpers */
 
 static cf_status validate(const cf_config *c)
@@ -325,8 +361,11 @@ static cf_status validate(const cf_config *c)
         cf_set_error("buoyancy (betg != 0) needs scalar = 1");
         return CF_ERR_ARG;
     }
-    if (c->order != 2) { cf_set_error("order 4 is not built in this library version"); return CF_ERR_UNSUPPORTED; }
-    if (c->scalar) { cf_set_error("the scalar is not built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->order == 4 && (c->nx < 6 || c->ny < 6 || c->nz < 6)) {
+        cf_set_error("order 4 needs nx, ny, nz >= 6 (three ghost levels, pressure.c:798)");
+        return CF_ERR_ARG;
+    }
+    if (c->scalar && (c->nx < 4 || c->ny < 4)) { cf_set_error("the scalar needs nx, ny >= 4"); return CF_ERR_ARG; }
     if (c->precision != CF_FP64) { cf_set_error("only FP64 is built in this library version"); return CF_ERR_UNSUPPORTED; }
     if (c->cflmode != 0) { cf_set_error("cflmode 1 (adaptive dt) is not built in this library version"); return CF_ERR_UNSUPPORTED; }
     int rad[CF_MAX_RAD];
@@ -348,8 +387,10 @@ extern "C" cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes)
     if (st != CF_OK) return st;
     size_t N = (size_t) cfg->nx * cfg->ny * cfg->nz;
     size_t M = (size_t) (cfg->nx / 2 + 1) * cfg->ny * cfg->nz;
-    /* u v w, ru rv rw, fa x3, fb x3, p, stage: 14 N; X: 2 M; G: M */
-    *bytes = sizeof(double) * (14 * N + 3 * M) + ((size_t) 16 << 20);
+    size_t nf = cfg->scalar ? 4 : 3;
+    /* state, solve output and two forcing slots per field, p, stage: (4 nf + 2) N;
+     * X: 2 M; G: M; order-4 Poisson factors: 3 M */
+    *bytes = sizeof(double) * ((4 * nf + 2) * N + (cfg->order == 4 ? 6 : 3) * M) + ((size_t) 16 << 20);
     return CF_OK;
 }
 
@@ -360,8 +401,8 @@ static void free_run(cf_run *r)
         if (r->gexec[q]) (void) hipGraphExecDestroy(r->gexec[q]);
         if (r->graph[q]) (void) hipGraphDestroy(r->graph[q]);
     }
-    double *bufs[] = { r->u, r->v, r->w, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2],
-                       r->fb[0], r->fb[1], r->fb[2], r->p, r->G, r->stage, r->tab, r->scal,
+    double *bufs[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
+                       r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p, r->G, r->S4, r->stage, r->tab, r->scal,
                        r->logbuf, r->part, r->cflpart };
     for (double *b : bufs) if (b) (void) hipFree(b);
     if (r->X) (void) hipFree(r->X);
@@ -419,8 +460,9 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
         C.abdt = (RK_ALP[s] + RK_BET[s]) * dt;
         C.gamdt = RK_GAM[s] * dt;
         C.zetdt = RK_ZET[s] * dt;
-        C.kadt = C.nuadt / (cfg->prandtl > 0 ? cfg->prandtl : 1.0);
-        C.kbdt = C.nubdt / (cfg->prandtl > 0 ? cfg->prandtl : 1.0);
+        const double kap = cfg->scalar ? nu / cfg->prandtl : nu;   /* chanfast.c:540-547 */
+        C.kadt = kap * RK_ALP[s] * dt;
+        C.kbdt = kap * RK_BET[s] * dt;
         for (int type = 0; type < 3; type++) {
             std::vector<double> a, b, c;
             if (type == 0) cell2_system(dzl, nz, -C.nubdt, a, b, c);
@@ -471,6 +513,87 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     }
     poisson_tables(dzl, nz, dx, dy, 1.0, ph[3]);
     r->pt_dt = 1.0;
+
+    /* ---- general path: closures, boundary lifts, order-4 systems ---- */
+    const int order = cfg->order;
+    r->general = order == 4 || cfg->scalar;
+    r->nfield = cfg->scalar ? 4 : 3;
+    Clos clo;
+    if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
+        || cft::closure(cft::CELL, order, 1, cft::DIRICHLET, dzl, nz, clo.cell[1])
+        || cft::closure(cft::FACE, order, 0, cft::DIRICHLET, dzl, nz, clo.face[0])
+        || cft::closure(cft::FACE, order, 1, cft::DIRICHLET, dzl, nz, clo.face[1])) {
+        cf_set_error("singular wall closure rows");
+        delete r;
+        return CF_ERR_NUMERIC;
+    }
+    std::vector<double> liftg((size_t) 3 * 4 * nz, 0.0);
+    for (int s = 0; s < 3; s++)
+        for (int f = 0; f < 4; f++) {
+            /* h -= H1D(bc-only field; -kappa b dt) (chanfast.c:472-477, :556-563, :627-632) */
+            const int fam = f == 2 ? cft::FACE : cft::CELL;
+            const int n = fam == cft::CELL ? nz : nz - 1;
+            const double bcb = f == 0 ? 0.0 - cfg->uframe : f == 1 ? 0.0 - cfg->vframe : f == 2 ? 0.0 : -0.5;
+            const double bct = f == 3 ? 0.5 : bcb;
+            const double nudt = f == 3 ? -r->rd.sc[s].kbdt : -r->rd.sc[s].nubdt;
+            const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
+            std::vector<double> col((size_t) n + 6, 0.0), outv((size_t) n, 0.0);
+            for (int m = 0; m < 3; m++) {
+                col[(size_t) (-1 - m + 3)] = cl[0].B[m] * bcb;
+                col[(size_t) (n + m + 3)] = cl[1].B[m] * bct;
+            }
+            cft::helm_apply(fam, order, dzl, nz, nudt, col.data(), outv.data());
+            for (int k = 0; k < n; k++) liftg[((size_t) s * 4 + f) * nz + k] = outv[k];
+        }
+    std::vector<double> hept[3][3], p4[4];
+    if (order == 4) {
+        for (int s = 0; s < 3; s++) {
+            const SubCoef &C = r->rd.sc[s];
+            for (int type = 0; type < 3; type++) {
+                const int fam = type == 2 ? cft::FACE : cft::CELL;
+                const double nudt = type == 1 ? -C.kbdt : -C.nubdt;
+                const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
+                std::vector<cft::Row7> rows;
+                cft::op_rows(fam, 4, dzl, nz, nudt, 1.0, true, rows);
+                cft::eliminate(fam, nz, cl[0], cl[1], rows);
+                if (cft::hept_factor(rows, hept[s][type])) {
+                    cf_set_error("zero pivot in the order-4 implicit z system (substep %d)", s);
+                    delete r;
+                    return CF_ERR_NUMERIC;
+                }
+                if (type == 2) continue;
+                /* unit-forcing profile (chanfast.c:1422-1424) and its statistics */
+                std::vector<double> rr(nz), x;
+                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
+                cft::hept_solve(hept[s][type], nz, rr, x);
+                std::vector<double> A((size_t) nz + 6, 0.0);
+                for (int k = 0; k < nz; k++) A[k + 3] = x[k];
+                for (int m = 0; m < 3; m++) {
+                    A[2 - m] = cl[0].G[m][0] * x[0] + cl[0].G[m][1] * x[1] + cl[0].G[m][2] * x[2];
+                    A[nz + 3 + m] = cl[1].G[m][0] * x[nz - 1] + cl[1].G[m][1] * x[nz - 2] + cl[1].G[m][2] * x[nz - 3];
+                }
+                const double *Ap = A.data() + 3;
+                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
+                for (int k = -1; k <= nz; k++) H[k + 1] = Ap[k];
+                const double kap = type == 0 ? nu : nu / cfg->prandtl;
+                double vol = 0.0, lz = 0.0;
+                for (int k = 0; k < nz; k++) { vol += Ap[k] * dzl[k]; lz += dzl[k]; }
+                const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
+                double *S = &stat1[(size_t) (s * 2 + type) * 4];
+                S[0] = vol / lz;
+                S[1] = kap * ((A1h * (Ap[0] - Ap[-1]) + A2h * (Ap[1] - Ap[-2])) / (0.5 * (dzl[0] + dzl[-1])));
+                S[2] = kap * ((A1h * (Ap[nz] - Ap[nz - 1]) + A2h * (Ap[nz + 1] - Ap[nz - 2])) / (0.5 * (dzl[nz] + dzl[nz - 1])));
+                S[3] = nz % 2 == 0 ? 0.5 * (Ap[nz / 2 - 1] + Ap[nz / 2]) : Ap[nz / 2];
+            }
+        }
+        for (int q = 0; q < 4; q++) {
+            if (pois4_tables(dzl, nz, dx, dy, q < 3 ? r->rd.sc[q].abdt : 1.0, p4[q])) {
+                cf_set_error("singular order-4 pressure closure rows");
+                delete r;
+                return CF_ERR_NUMERIC;
+            }
+        }
+    }
     r->rd.m_prev_slot[0] = SC_M0 + 2;
     r->rd.m_prev_slot[1] = SC_M0 + 0;
     r->rd.m_prev_slot[2] = SC_M0 + 1;
@@ -488,8 +611,18 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
         double th_ = 2.0 * M_PI * (double) j / (double) ny;
         twy[2 * j] = cos(th_); twy[2 * j + 1] = -sin(th_);
     }
-    for (int ix = 0; ix < nkx; ix++) lamx[ix] = 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) ix / (double) nx);
-    for (int iy = 0; iy < ny; iy++) lamy[iy] = 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) iy / (double) ny);
+    /* modified wavenumbers: pressure.c:318-320 (order 2), :382-389 (order 4) */
+    auto lam = [&](int i, int n) -> double {
+        if (order == 2) return 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+        const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
+        double c1 = cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+        double c2 = cos(2.0 * M_PI * 2.0 * (double) i / (double) n);
+        double c3 = cos(2.0 * M_PI * 3.0 * (double) i / (double) n);
+        return (2.0 * A2h * A2h + 2.0 * A1h * A1h) + (4.0 * A1h * A2h - 2.0 * A1h * A1h) * c1
+             + (-4.0 * A1h * A2h) * c2 + (-2.0 * A2h * A2h) * c3;
+    };
+    for (int ix = 0; ix < nkx; ix++) lamx[ix] = lam(ix, nx);
+    for (int iy = 0; iy < ny; iy++) lamy[iy] = lam(iy, ny);
 
     /* pack tables: dz | th | stat1 | h1 | twx | twy | lamx | lamy | pois x4 */
     std::vector<double> pack;
@@ -525,6 +658,14 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
             o_sp[s][t][2] = put(S.gam.data(), S.n); o_sp[s][t][3] = put(S.vl.data(), S.n);
             o_sp[s][t][4] = put(S.wr.data(), S.n); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
         }
+    size_t o_lift = put(liftg.data(), liftg.size());
+    size_t o_hp[3][3] = {{0}}, o_p4[4] = {0};
+    if (order == 4) {
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 3; t++) o_hp[s][t] = put(hept[s][t].data(), hept[s][t].size());
+        for (int q = 0; q < 4; q++) o_p4[q] = put(p4[q].data(), p4[q].size());
+    }
+    r->off_pois4_adhoc = o_p4[3];
     r->tab_n = pack.size();
 
     /* ---- device memory ---- */
@@ -533,13 +674,15 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     ok = ok && dalloc(&r->u, r->N) && dalloc(&r->v, r->N) && dalloc(&r->w, r->N);
     ok = ok && dalloc(&r->ru, r->N) && dalloc(&r->rv, r->N) && dalloc(&r->rw, r->N);
     for (int f = 0; f < 3; f++) ok = ok && dalloc(&r->fa[f], r->N) && dalloc(&r->fb[f], r->N);
+    if (cfg->scalar) ok = ok && dalloc(&r->c, r->N) && dalloc(&r->rc, r->N) && dalloc(&r->fa[3], r->N) && dalloc(&r->fb[3], r->N);
+    if (order == 4) ok = ok && dalloc(&r->S4, 3 * r->M * nz);
     ok = ok && dalloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * nz);
     ok = ok && hipMalloc((void **) &r->X, sizeof(double2) * r->M * nz) == hipSuccess;
     ok = ok && dalloc(&r->tab, r->tab_n) && dalloc(&r->scal, SC_N) && dalloc(&r->logbuf, (size_t) LOGCAP * CF_LOG_N);
     r->nb_plane = (int) ((r->P + 2047) / 2048);
     if (r->nb_plane > 1024) r->nb_plane = 1024;
     r->nb_red = 1024;
-    size_t npart = (size_t) r->nb_plane * (nz + 2);
+    size_t npart = (size_t) r->nb_plane * (2 * nz + 2);
     if (npart < (size_t) r->nb_red) npart = r->nb_red;
     ok = ok && dalloc(&r->part, npart);
     {
@@ -548,7 +691,8 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
         nchunk = (nz + r->rhs_kc - 1) / r->rhs_kc;
         r->rhs_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nchunk;
     }
-    ok = ok && dalloc(&r->cflpart, (size_t) r->rhs_blocks);
+    r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+    ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
         free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
@@ -559,8 +703,9 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     /* zero state, tendencies and scalars (finite values: g meets zeta_0 = 0) */
     std::vector<double> zeros(SC_N, 0.0);
     bool zok = hipMemcpy(r->scal, zeros.data(), sizeof(double) * SC_N, hipMemcpyHostToDevice) == hipSuccess;
-    double *zb[] = { r->u, r->v, r->w, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2], r->fb[0], r->fb[1], r->fb[2], r->p };
-    for (double *b : zb) zok = zok && hipMemset(b, 0, sizeof(double) * r->N) == hipSuccess;
+    double *zb[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
+                     r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p };
+    for (double *b : zb) if (b) zok = zok && hipMemset(b, 0, sizeof(double) * r->N) == hipSuccess;
     zok = zok && hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N) == hipSuccess;
     if (!zok || hipDeviceSynchronize() != hipSuccess) { free_run(r); cf_set_error("device init failed"); return CF_ERR_DEVICE; }
 
@@ -576,7 +721,19 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     R.th = r->tab + o_th;
     R.rdz = r->tab + o_rdz;
     R.rdzh = r->tab + o_rdzh;
-    r->kdiv = pick_div_fftx(nx);
+    r->kdiv = pick_div_fftx(nx, order);
+    r->grhs = pick_grhs(order, cfg->scalar);
+    r->kproj = pick_gproject(order);
+    R.cl = clo;
+    R.scalar = cfg->scalar;
+    R.betg[0] = cfg->betg_x; R.betg[1] = cfg->betg_y; R.betg[2] = cfg->betg_z;
+    R.kap = cfg->scalar ? nu / cfg->prandtl : nu;
+    R.liftg = r->tab + o_lift;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) {
+            R.hp[s][t].n = t == 2 ? nz - 1 : nz;
+            R.hp[s][t].t = order == 4 ? r->tab + o_hp[s][t] : nullptr;
+        }
     r->kfxp = pick_fftx_plain(nx);
     r->kfy = pick_ffty(ny);
     r->kfxi = pick_fftx_inv(nx);
@@ -609,6 +766,12 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
         T.a = r->tab + o_p[q][0]; T.c = r->tab + o_p[q][1]; T.bz = r->tab + o_p[q][2];
         T.ex = r->tab + o_p[q][3]; T.ey = r->tab + o_p[q][4]; T.dvol = r->tab + o_p[q][5];
         T.db_neu = ph[q].db_neu; T.db_dir = ph[q].db_dir; T.db_top = ph[q].db_top;
+        Pois4Tab &T4 = r->pt4[q];
+        if (order == 4) {
+            const double *b = r->tab + o_p4[q];
+            T4.band[0] = b; T4.band[1] = b + (size_t) 7 * nz;
+            T4.ex = b + (size_t) 14 * nz; T4.ey = b + (size_t) 15 * nz; T4.dvol = b + (size_t) 16 * nz;
+        }
     }
     *out = r;
     return CF_OK;
@@ -672,7 +835,7 @@ static void state_set(cf_run *r, int q, double **u, double **v, double **w)
 /* divergence -> FFT -> Poisson -> inverse FFT into p (pressure.c:40-61,
  * chanfast.c:639-651) */
 static cf_status enqueue_pressure(cf_run *r, const double *us, const double *vs, const double *ws,
-                                  const double *h1, const double *mptr, const PoisTab &T)
+                                  const double *h1, const double *mptr, int tq)
 {
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz;
@@ -681,7 +844,10 @@ static cf_status enqueue_pressure(cf_run *r, const double *us, const double *vs,
     KLAUNCH("div_fftx", r->kdiv, dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, r->X);
     dim3 gy(fy_blocks(r));
     KLAUNCH("ffty_fwd", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 1);
-    KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, T, r->X, r->G);
+    if (r->g.order == 4)
+        KLAUNCH("poisson", k_poisson4, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], r->X, r->S4);
+    else
+        KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt[tq], r->X, r->G);
     KLAUNCH("ffty_inv", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 0);
     KLAUNCH("fftx_inv", r->kfxi, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
     return CF_OK;
@@ -736,16 +902,69 @@ static cf_status enqueue_step(cf_run *r, int from)
                     r->rhs_blocks, dt, r->scal, r->logbuf);
         if (r->use_spike)
             KLAUNCH("spike", k_spike, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
-                    ou, ov, ow);
+                    ou, ov, ow, (double *) nullptr);
         else
-            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow);
+            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
+                    (double *) nullptr);
         KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
                 (const double *) ou, R.ubc, r->part, r->nb_plane);
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
                 (const double *) r->part, r->nb_plane, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
-        cf_status st = enqueue_pressure(r, ou, ov, ow, h1, r->scal + SC_M0 + s, r->pt[s]);
+        cf_status st = enqueue_pressure(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
+    }
+    KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
+    return CF_OK;
+}
+
+/* one RK3 step of the general path (order 4 and/or the scalar): the state
+ * (u, v, w, c) is materialised after every substep */
+static cf_status enqueue_step_general(cf_run *r)
+{
+    const RunDev &R = r->rd;
+    const int nz = r->g.nz;
+    const double dt = r->cfg.dt;
+    const bool sc = r->cfg.scalar != 0;
+    const dim3 cg = cell_grid(r), cb(64, 4);
+    for (int s = 0; s < 3; s++) {
+        double **fnew = (s % 2 == 0) ? r->fa : r->fb;
+        double **gold = (s % 2 == 0) ? r->fb : r->fa;
+        const double *u = r->u, *v = r->v, *w = r->w, *c = r->c;
+        KLAUNCH("rhs", r->grhs.u, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[0], (const double *) r->scal,
+                fnew[0], r->ru, s == 0 ? r->cflpart : (double *) nullptr, dt);
+        if (s == 0)
+            KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart, r->gblocks, dt,
+                    r->scal, r->logbuf);
+        KLAUNCH("rhs", r->grhs.v, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[1], (const double *) r->scal,
+                fnew[1], r->rv, (double *) nullptr, dt);
+        KLAUNCH("rhs", r->grhs.w, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[2], (const double *) r->scal,
+                fnew[2], r->rw, (double *) nullptr, dt);
+        if (sc)
+            KLAUNCH("rhs", r->grhs.c, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[3], (const double *) r->scal,
+                    fnew[3], r->rc, (double *) nullptr, dt);
+        const unsigned nf = (unsigned) r->nfield;
+        if (r->g.order == 4)
+            KLAUNCH("solve", k_gsolve4, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, r->ru, r->rv,
+                    r->rw, r->rc);
+        else if (r->use_spike)
+            KLAUNCH("solve", k_spike, dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
+                    r->ru, r->rv, r->rw, r->rc);
+        else
+            KLAUNCH("solve", k_thomas, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, r->ru, r->rv,
+                    r->rw, r->rc);
+        KLAUNCH("plane_sum", k_gplane, dim3(r->nb_plane, nz, sc ? 2 : 1), dim3(256), 0, R, (const double *) r->ru,
+                (const double *) r->rc, r->part, r->nb_plane);
+        KLAUNCH("mass", k_gmass, dim3(sc ? 2 : 1), dim3(256), sizeof(double) * (nz + 6), R, s,
+                (const double *) r->part, r->nb_plane, r->scal, r->logbuf);
+        const double *h1u = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
+        const double *h1c = R.h1 + (size_t) (s * 2 + 1) * (nz + 2);
+        cf_status st = enqueue_pressure(r, r->ru, r->rv, r->rw, h1u, r->scal + SC_M0 + s, s);
+        if (st != CF_OK) return st;
+        KLAUNCH("project", r->kproj, cg, cb, 0, R, (const double *) r->ru, (const double *) r->rv,
+                (const double *) r->rw, (const double *) r->rc, h1u, (const double *) (r->scal + SC_M0 + s),
+                sc ? h1c : (const double *) nullptr, (const double *) (r->scal + SC_MC0 + s), (const double *) r->p,
+                R.sc[s].abdt, r->u, r->v, r->w, sc ? r->c : (double *) nullptr);
     }
     KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
     return CF_OK;
@@ -782,7 +1001,7 @@ static cf_status collect_profile(cf_run *r)
  * projected values. */
 static cf_status reset_projection(cf_run *r)
 {
-    if (!r->proj_pending) return CF_OK;
+    if (r->general || !r->proj_pending) return CF_OK;
     double *su, *sv, *sw;
     state_set(r, r->cur, &su, &sv, &sw);
     cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
@@ -802,7 +1021,7 @@ extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
     if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
     double *su, *sv, *sw;
     state_set(r, r->cur, &su, &sv, &sw);
-    double *dst = field == CF_U ? su : field == CF_V ? sv : field == CF_W ? sw : nullptr;
+    double *dst = field == CF_U ? su : field == CF_V ? sv : field == CF_W ? sw : (field == CF_C ? r->c : nullptr);
     if (!dst) { cf_set_error("cf_run_upload: field %d is not uploadable in this configuration", field); return CF_ERR_ARG; }
     cf_status rs = reset_projection(r);
     if (rs != CF_OK) return rs;
@@ -830,10 +1049,12 @@ extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
     CHECK_RUN(r);
     if (!host) { cf_set_error("null host buffer"); return CF_ERR_ARG; }
     if (field == CF_P) return download_dev(r, r->p, host);
+    if (field == CF_C && r->c) return download_dev(r, r->c, host);
     if (field != CF_U && field != CF_V && field != CF_W) {
         cf_set_error("cf_run_download: field %d not held in this configuration", field);
         return CF_ERR_ARG;
     }
+    if (r->general) return download_dev(r, field == CF_U ? r->u : field == CF_V ? r->v : r->w, host);
     cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
     return download_dev(r, r->fa[field], host);
@@ -842,6 +1063,14 @@ extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
 static cf_status set_adhoc_poisson(cf_run *r, double dt)
 {
     if (r->pt_dt == dt) return CF_OK;
+    if (r->g.order == 4) {
+        std::vector<double> T;
+        if (pois4_tables(r->dz.data() + CF_NGT, r->g.nz, r->g.dx, r->g.dy, dt, T)) {
+            cf_set_error("singular order-4 pressure closure rows");
+            return CF_ERR_NUMERIC;
+        }
+        HIPCHK(hipMemcpy(r->tab + r->off_pois4_adhoc, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    }
     PoisHost ph;
     poisson_tables(r->dz.data() + CF_NGT, r->g.nz, r->g.dx, r->g.dy, dt, ph);
     int nz = r->g.nz;
@@ -859,9 +1088,19 @@ extern "C" cf_status cf_run_project(cf_run *r, double dt)
     if (!(dt > 0)) { cf_set_error("cf_run_project: dt must be positive"); return CF_ERR_ARG; }
     cf_status st = set_adhoc_poisson(r, dt);
     if (st != CF_OK) return st;
+    if (r->general) {
+        st = enqueue_pressure(r, r->u, r->v, r->w, nullptr, nullptr, 3);
+        if (st != CF_OK) return st;
+        KLAUNCH("project", r->kproj, cell_grid(r), dim3(64, 4), 0, r->rd, (const double *) r->u, (const double *) r->v,
+                (const double *) r->w, (const double *) nullptr, (const double *) nullptr, (const double *) nullptr,
+                (const double *) nullptr, (const double *) nullptr, (const double *) r->p, dt, r->u, r->v, r->w,
+                (double *) nullptr);
+        HIPCHK(hipStreamSynchronize(r->stream));
+        return collect_profile(r);
+    }
     st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
-    st = enqueue_pressure(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, r->pt[3]);
+    st = enqueue_pressure(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, 3);
     if (st != CF_OK) return st;
     double *su, *sv, *sw;
     state_set(r, r->cur, &su, &sv, &sw);
@@ -882,24 +1121,31 @@ extern "C" cf_status cf_run_step(cf_run *r, int nsteps)
     if (nsteps == 0) return CF_OK;
     if (r->profile) {
         for (int n = 0; n < nsteps; n++) {
-            cf_status st = enqueue_step(r, r->cur);
+            cf_status st = r->general ? enqueue_step_general(r) : enqueue_step(r, r->cur);
             if (st != CF_OK) return st;
-            r->cur ^= 1;
-            r->proj_pending = true;
+            if (!r->general) {
+                r->cur ^= 1;
+                r->proj_pending = true;
+            }
         }
         r->steps_done += nsteps;
         return collect_profile(r);
     }
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < (r->general ? 1 : 2); q++) {
         if (r->gexec[q]) continue;
         HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-        cf_status st = enqueue_step(r, q);
+        cf_status st = r->general ? enqueue_step_general(r) : enqueue_step(r, q);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(r->stream, &gr);
         if (st != CF_OK) { if (gr) (void) hipGraphDestroy(gr); return st; }
         HIPCHK(e);
         r->graph[q] = gr;
         HIPCHK(hipGraphInstantiate(&r->gexec[q], r->graph[q], nullptr, nullptr, 0));
+    }
+    if (r->general) {
+        for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(r->gexec[0], r->stream));
+        r->steps_done += nsteps;
+        return CF_OK;
     }
     for (int n = 0; n < nsteps; n++) {
         HIPCHK(hipGraphLaunch(r->gexec[r->cur], r->stream));
@@ -939,10 +1185,13 @@ extern "C" cf_status cf_run_max_divergence(cf_run *r, double *out)
 {
     CHECK_RUN(r);
     if (!out) { cf_set_error("null output"); return CF_ERR_ARG; }
-    cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
-    if (st != CF_OK) return st;
-    hipLaunchKernelGGL(k_maxdiv_part, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd,
-                       (const double *) r->fa[0], (const double *) r->fa[1], (const double *) r->fa[2], r->part, r->nb_red);
+    const double *mu = r->u, *mv = r->v, *mw = r->w;
+    if (!r->general) {
+        cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+        if (st != CF_OK) return st;
+        mu = r->fa[0]; mv = r->fa[1]; mw = r->fa[2];
+    }
+    hipLaunchKernelGGL(k_maxdiv_part, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd, mu, mv, mw, r->part, r->nb_red);
     LAUNCHCHK();
     std::vector<double> part(r->nb_red);
     HIPCHK(hipMemcpyAsync(part.data(), r->part, sizeof(double) * r->nb_red, hipMemcpyDeviceToHost, r->stream));
@@ -1073,8 +1322,12 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
     std::vector<double2> d;
     spec_host_to_dev(r, spec, d);
     HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
-    hipLaunchKernelGGL(k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
-                       r->rd, r->pt[3], r->X, r->G);
+    if (r->g.order == 4)
+        hipLaunchKernelGGL(k_poisson4, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+                           r->rd, r->pt4[3], r->X, r->S4);
+    else
+        hipLaunchKernelGGL(k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+                           r->rd, r->pt[3], r->X, r->G);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));

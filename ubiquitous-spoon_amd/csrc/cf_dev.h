@@ -36,15 +36,38 @@ struct Thomas {
     const double *a, *ibet, *gam;
 };
 
-static __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-static __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-static __device__ __forceinline__ double2 cmul(double2 a, double2 b)
+/* real <-> complex type pairs of the two storage precisions */
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { typedef double2 type; };
+template <> struct Cplx<float> { typedef float2 type; };
+template <typename C> struct Real;
+template <> struct Real<double2> { typedef double type; };
+template <> struct Real<float2> { typedef float type; };
+
+template <typename C> static __device__ __forceinline__ C mkc(typename Real<C>::type x, typename Real<C>::type y)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    C c;
+    c.x = x;
+    c.y = y;
+    return c;
 }
-static __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+template <typename C> static __device__ __forceinline__ C cadd(C a, C b) { return mkc<C>(a.x + b.x, a.y + b.y); }
+template <typename C> static __device__ __forceinline__ C csub(C a, C b) { return mkc<C>(a.x - b.x, a.y - b.y); }
+template <typename C> static __device__ __forceinline__ C cmul(C a, C b)
+{
+    return mkc<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C, typename S> static __device__ __forceinline__ C cscale(C a, S s)
+{
+    typedef typename Real<C>::type R;
+    return mkc<C>(a.x * (R) s, a.y * (R) s);
+}
 /* multiply by -i*sg (sg = +1 forward, -1 inverse) */
-static __device__ __forceinline__ double2 cmul_mi(double2 a, double sg) { return make_double2(sg * a.y, -sg * a.x); }
+template <typename C, typename S> static __device__ __forceinline__ C cmul_mi(C a, S sg)
+{
+    typedef typename Real<C>::type R;
+    return mkc<C>((R) sg * a.y, -(R) sg * a.x);
+}
 
 static __device__ __forceinline__ int wrapm(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -57,37 +80,39 @@ static __device__ __forceinline__ int wrapm(int i, int n) { return i < 0 ? i + n
  * stage reads all of a thread's butterfly inputs into registers, barriers,
  * then writes, so one LDS buffer suffices; MAXE bounds the elements per
  * thread (nlines * n <= MAXE * blockDim.x). */
-template <int R>
-static __device__ __forceinline__ void dft_small(double2 *a, double sg)
+template <int R, typename C>
+static __device__ __forceinline__ void dft_small(C *a, double sgd)
 {
+    typedef typename Real<C>::type F;
+    const F sg = (F) sgd;
     if (R == 2) {
-        double2 t = a[1];
+        C t = a[1];
         a[1] = csub(a[0], t);
         a[0] = cadd(a[0], t);
     } else if (R == 4) {
-        double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
-        double2 t2 = cadd(a[1], a[3]), t3 = cmul_mi(csub(a[1], a[3]), sg);
+        C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        C t2 = cadd(a[1], a[3]), t3 = cmul_mi(csub(a[1], a[3]), sg);
         a[0] = cadd(t0, t2);
         a[2] = csub(t0, t2);
         a[1] = cadd(t1, t3);
         a[3] = csub(t1, t3);
     } else if (R == 3) {
-        const double C3 = -0.5, S3 = 0.86602540378443864676;       /* cos, sin of 2 pi / 3 */
-        double2 t = cadd(a[1], a[2]);
-        double2 d = cmul_mi(csub(a[1], a[2]), sg * S3);
-        double2 m0 = make_double2(a[0].x + C3 * t.x, a[0].y + C3 * t.y);
+        const F C3 = (F) -0.5, S3 = (F) 0.86602540378443864676;       /* cos, sin of 2 pi / 3 */
+        C t = cadd(a[1], a[2]);
+        C d = cmul_mi(csub(a[1], a[2]), sg * S3);
+        C m0 = mkc<C>(a[0].x + C3 * t.x, a[0].y + C3 * t.y);
         a[0] = cadd(a[0], t);
         a[1] = cadd(m0, d);
         a[2] = csub(m0, d);
     } else {  /* R == 5 */
-        const double C51 = 0.30901699437494742410, C52 = -0.80901699437494742410;
-        const double S51 = 0.95105651629515357212, S52 = 0.58778525229247312917;
-        double2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
-        double2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
-        double2 m1 = make_double2(a[0].x + C51 * t1.x + C52 * t2.x, a[0].y + C51 * t1.y + C52 * t2.y);
-        double2 m2 = make_double2(a[0].x + C52 * t1.x + C51 * t2.x, a[0].y + C52 * t1.y + C51 * t2.y);
-        double2 s1 = cmul_mi(make_double2(S51 * d1.x + S52 * d2.x, S51 * d1.y + S52 * d2.y), sg);
-        double2 s2 = cmul_mi(make_double2(S52 * d1.x - S51 * d2.x, S52 * d1.y - S51 * d2.y), sg);
+        const F C51 = (F) 0.30901699437494742410, C52 = (F) -0.80901699437494742410;
+        const F S51 = (F) 0.95105651629515357212, S52 = (F) 0.58778525229247312917;
+        C t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+        C d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+        C m1 = mkc<C>(a[0].x + C51 * t1.x + C52 * t2.x, a[0].y + C51 * t1.y + C52 * t2.y);
+        C m2 = mkc<C>(a[0].x + C52 * t1.x + C51 * t2.x, a[0].y + C52 * t1.y + C51 * t2.y);
+        C s1 = cmul_mi(mkc<C>(S51 * d1.x + S52 * d2.x, S51 * d1.y + S52 * d2.y), sg);
+        C s2 = cmul_mi(mkc<C>(S52 * d1.x - S51 * d2.x, S52 * d1.y - S51 * d2.y), sg);
         a[0] = cadd(a[0], cadd(t1, t2));
         a[1] = cadd(m1, s1);
         a[4] = csub(m1, s1);
@@ -96,29 +121,30 @@ static __device__ __forceinline__ void dft_small(double2 *a, double sg)
     }
 }
 
-template <int R, int MAXE>
-static __device__ __forceinline__ void fft_stage(double2 *buf, int n, int ld, int nlines, int p,
-                                                 const double2 *__restrict__ tw, double sg)
+template <int R, int MAXE, typename C>
+static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlines, int p,
+                                                 const C *__restrict__ tw, double sg)
 {
+    typedef typename Real<C>::type F;
     constexpr int NB = (MAXE + R - 1) / R;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int T = n / R;
     const int nb = nlines * T;
     const int tstride = n / (p * R);
-    double2 y[NB][R];
+    C y[NB][R];
 #pragma unroll
     for (int it = 0; it < NB; it++) {
         const int b = tid + it * nt;
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % p;
-            const double2 *src = buf + line * ld;
+            const C *src = buf + line * ld;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                double2 x = src[i + m * T];
+                C x = src[i + m * T];
                 if (m > 0 && k > 0) {
-                    double2 w = tw[(k * m * tstride) % n];
-                    w.y *= sg;
+                    C w = tw[(k * m * tstride) % n];
+                    w.y *= (F) sg;
                     x = cmul(x, w);
                 }
                 y[it][m] = x;
@@ -133,7 +159,7 @@ static __device__ __forceinline__ void fft_stage(double2 *buf, int n, int ld, in
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % p;
-            double2 *dstl = buf + line * ld + (i - k) * R + k;
+            C *dstl = buf + line * ld + (i - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; q++) dstl[q * p] = y[it][q];
         }
@@ -141,9 +167,9 @@ static __device__ __forceinline__ void fft_stage(double2 *buf, int n, int ld, in
     __syncthreads();
 }
 
-template <int MAXE>
-static __device__ void lds_fft(double2 *buf, int n, int ld, int nlines, const int *rad, int nrad,
-                               const double2 *__restrict__ tw, double sg)
+template <int MAXE, typename C>
+static __device__ void lds_fft(C *buf, int n, int ld, int nlines, const int *rad, int nrad,
+                               const C *__restrict__ tw, double sg)
 {
     int p = 1;
     for (int st = 0; st < nrad; st++) {
@@ -180,29 +206,30 @@ constexpr int ct_rad_at(int n, int idx)
     return 1;
 }
 
-template <int N, int R, int P, int MAXE>
-static __device__ __forceinline__ void fft_stage_ct(double2 *buf, int ld, int nlines,
-                                                    const double2 *__restrict__ tw, double sg)
+template <int N, int R, int P, int MAXE, typename C>
+static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
+                                                    const C *__restrict__ tw, double sg)
 {
+    typedef typename Real<C>::type F;
     constexpr int NB = (MAXE + R - 1) / R;
     constexpr int T = N / R;
     constexpr int TS = N / (P * R);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nb = nlines * T;
-    double2 y[NB][R];
+    C y[NB][R];
 #pragma unroll
     for (int it = 0; it < NB; it++) {
         const int b = tid + it * nt;
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % P;
-            const double2 *src = buf + line * ld;
+            const C *src = buf + line * ld;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                double2 x = src[i + m * T];
+                C x = src[i + m * T];
                 if (m > 0 && P > 1) {
-                    double2 w = tw[k * m * TS];
-                    w.y *= sg;
+                    C w = tw[k * m * TS];
+                    w.y *= (F) sg;
                     x = cmul(x, w);
                 }
                 y[it][m] = x;
@@ -217,7 +244,7 @@ static __device__ __forceinline__ void fft_stage_ct(double2 *buf, int ld, int nl
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % P;
-            double2 *dstl = buf + line * ld + (i - k) * R + k;
+            C *dstl = buf + line * ld + (i - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; q++) dstl[q * P] = y[it][q];
         }
@@ -225,9 +252,9 @@ static __device__ __forceinline__ void fft_stage_ct(double2 *buf, int ld, int nl
     __syncthreads();
 }
 
-template <int N, int MAXE, int ST = 0, int P = 1>
-static __device__ __forceinline__ void lds_fft_ct(double2 *buf, int ld, int nlines,
-                                                  const double2 *__restrict__ tw, double sg)
+template <int N, int MAXE, int ST = 0, int P = 1, typename C>
+static __device__ __forceinline__ void lds_fft_ct(C *buf, int ld, int nlines,
+                                                  const C *__restrict__ tw, double sg)
 {
     if constexpr (ST < ct_rad_count(N)) {
         constexpr int R = ct_rad_at(N, ST);
@@ -237,9 +264,9 @@ static __device__ __forceinline__ void lds_fft_ct(double2 *buf, int ld, int nlin
 }
 
 /* N > 0: compile-time plan; N == 0: runtime plan (any 2^a 3^b 5^c) */
-template <int N, int MAXE>
-static __device__ __forceinline__ void fft_lines(double2 *buf, int n, int ld, int nlines, const int *rad,
-                                                 int nrad, const double2 *__restrict__ tw, double sg)
+template <int N, int MAXE, typename C>
+static __device__ __forceinline__ void fft_lines(C *buf, int n, int ld, int nlines, const int *rad,
+                                                 int nrad, const C *__restrict__ tw, double sg)
 {
     if constexpr (N > 0) lds_fft_ct<N, MAXE>(buf, ld, nlines, tw, sg);
     else lds_fft<MAXE>(buf, n, ld, nlines, rad, nrad, tw, sg);

@@ -17,43 +17,43 @@
  */
 #include "cf_kernels.h"
 
-static constexpr double A1 = 1.125 / 1.0;      /* fourth.c:3-6 */
-static constexpr double A2 = -0.125 / 3.0;
-static constexpr double B1 = 1.125 / 2.0;
-static constexpr double B2 = -0.125 / 2.0;
+static constexpr double A1d = 1.125 / 1.0;      /* fourth.c:3-6 */
+static constexpr double A2d = -0.125 / 3.0;
+static constexpr double B1d = 1.125 / 2.0;
+static constexpr double B2d = -0.125 / 2.0;
 
 /* ------------------------------------------------------------ columns */
 
 /* a cell field's values at levels k-3..k+3 of column cc, ghosts from the
  * closure (ghost m of a wall = G[m] . (x0, x1, x2) + B[m] bc) */
-template <int ORD>
-static __device__ __forceinline__ void cell_col7(const double *__restrict__ q, size_t P, size_t cc, int k, int nz,
-                                                 const Clos1 *C, double bcb, double bct, double o[7])
+template <int ORD, typename T>
+static __device__ __forceinline__ void cell_col7(const T *__restrict__ q, size_t P, size_t cc, int k, int nz,
+                                                 const Clos1 *C, double bcb, double bct, T o[7])
 {
 #pragma unroll
     for (int t = 0; t < 7; t++) {
         const int kk = k - 3 + t;
-        o[t] = (kk >= 0 && kk < nz) ? q[(size_t) kk * P + cc] : 0.0;
+        o[t] = (kk >= 0 && kk < nz) ? q[(size_t) kk * P + cc] : (T) 0;
     }
     if (k < 3) {
         const Clos1 &B = C[0];
-        const double x0 = q[cc];
-        const double x1 = ORD == 4 ? q[P + cc] : 0.0, x2 = ORD == 4 ? q[2 * P + cc] : 0.0;
+        const T x0 = q[cc];
+        const T x1 = ORD == 4 ? q[P + cc] : (T) 0, x2 = ORD == 4 ? q[2 * P + cc] : (T) 0;
 #pragma unroll
         for (int m = 0; m < (ORD == 4 ? 3 : 1); m++) {
             const int t = -1 - m - k + 3;
-            if (t >= 0) o[t] = B.G[m][0] * x0 + B.G[m][1] * x1 + B.G[m][2] * x2 + B.B[m] * bcb;
+            if (t >= 0) o[t] = (T) B.G[m][0] * x0 + (T) B.G[m][1] * x1 + (T) B.G[m][2] * x2 + (T) (B.B[m] * bcb);
         }
     }
     if (k > nz - 4) {
-        const Clos1 &T = C[1];
-        const double x0 = q[(size_t) (nz - 1) * P + cc];
-        const double x1 = ORD == 4 ? q[(size_t) (nz - 2) * P + cc] : 0.0;
-        const double x2 = ORD == 4 ? q[(size_t) (nz - 3) * P + cc] : 0.0;
+        const Clos1 &Tp = C[1];
+        const T x0 = q[(size_t) (nz - 1) * P + cc];
+        const T x1 = ORD == 4 ? q[(size_t) (nz - 2) * P + cc] : (T) 0;
+        const T x2 = ORD == 4 ? q[(size_t) (nz - 3) * P + cc] : (T) 0;
 #pragma unroll
         for (int m = 0; m < (ORD == 4 ? 3 : 1); m++) {
             const int t = nz + m - k + 3;
-            if (t <= 6) o[t] = T.G[m][0] * x0 + T.G[m][1] * x1 + T.G[m][2] * x2 + T.B[m] * bct;
+            if (t <= 6) o[t] = (T) Tp.G[m][0] * x0 + (T) Tp.G[m][1] * x1 + (T) Tp.G[m][2] * x2 + (T) (Tp.B[m] * bct);
         }
     }
 }
@@ -61,44 +61,45 @@ static __device__ __forceinline__ void cell_col7(const double *__restrict__ q, s
 /* w at face level kk as the advection terms see it: the wall faces -1 and
  * nz-1 are 0 and the ghost faces mirror oddly about them (apply_bc_w and
  * the in-place ghost rewrite of massflux*4, advect.c:100-131) */
-static __device__ __forceinline__ double wadv(const double *__restrict__ w, size_t P, size_t cc, int kk, int nz)
+template <typename T>
+static __device__ __forceinline__ T wadv(const T *__restrict__ w, size_t P, size_t cc, int kk, int nz)
 {
     if (kk >= 0 && kk <= nz - 2) return w[(size_t) kk * P + cc];
-    if (kk == -1 || kk == nz - 1) return 0.0;
+    if (kk == -1 || kk == nz - 1) return (T) 0;
     if (kk < -1) return -w[(size_t) (-2 - kk) * P + cc];
     return -w[(size_t) (2 * (nz - 1) - kk) * P + cc];
 }
 
 /* w at levels k-3..k+3 for the implicit operator: closure ghosts
  * (apply_bc_w2/4 is re-applied before helmholtz1d_operator_w) */
-template <int ORD>
-static __device__ __forceinline__ void face_col7(const double *__restrict__ w, size_t P, size_t cc, int k, int nz,
-                                                 const Clos1 *C, double o[7])
+template <int ORD, typename T>
+static __device__ __forceinline__ void face_col7(const T *__restrict__ w, size_t P, size_t cc, int k, int nz,
+                                                 const Clos1 *C, T o[7])
 {
 #pragma unroll
     for (int t = 0; t < 7; t++) {
         const int kk = k - 3 + t;
-        o[t] = (kk >= 0 && kk <= nz - 2) ? w[(size_t) kk * P + cc] : 0.0;
+        o[t] = (kk >= 0 && kk <= nz - 2) ? w[(size_t) kk * P + cc] : (T) 0;
     }
     if (k < 3) {
         const Clos1 &B = C[0];
-        const double x0 = w[cc];
-        const double x1 = ORD == 4 ? w[P + cc] : 0.0, x2 = ORD == 4 ? w[2 * P + cc] : 0.0;
+        const T x0 = w[cc];
+        const T x1 = ORD == 4 ? w[P + cc] : (T) 0, x2 = ORD == 4 ? w[2 * P + cc] : (T) 0;
 #pragma unroll
         for (int m = 0; m < (ORD == 4 ? 3 : 1); m++) {
             const int t = -1 - m - k + 3;
-            if (t >= 0) o[t] = B.G[m][0] * x0 + B.G[m][1] * x1 + B.G[m][2] * x2;
+            if (t >= 0) o[t] = (T) B.G[m][0] * x0 + (T) B.G[m][1] * x1 + (T) B.G[m][2] * x2;
         }
     }
     if (k > nz - 5) {
-        const Clos1 &T = C[1];
-        const double x0 = w[(size_t) (nz - 2) * P + cc];
-        const double x1 = ORD == 4 ? w[(size_t) (nz - 3) * P + cc] : 0.0;
-        const double x2 = ORD == 4 ? w[(size_t) (nz - 4) * P + cc] : 0.0;
+        const Clos1 &Tp = C[1];
+        const T x0 = w[(size_t) (nz - 2) * P + cc];
+        const T x1 = ORD == 4 ? w[(size_t) (nz - 3) * P + cc] : (T) 0;
+        const T x2 = ORD == 4 ? w[(size_t) (nz - 4) * P + cc] : (T) 0;
 #pragma unroll
         for (int m = 0; m < (ORD == 4 ? 3 : 1); m++) {
             const int t = nz - 1 + m - k + 3;
-            if (t <= 6) o[t] = T.G[m][0] * x0 + T.G[m][1] * x1 + T.G[m][2] * x2;
+            if (t <= 6) o[t] = (T) Tp.G[m][0] * x0 + (T) Tp.G[m][1] * x1 + (T) Tp.G[m][2] * x2;
         }
     }
 }
@@ -106,61 +107,68 @@ static __device__ __forceinline__ void face_col7(const double *__restrict__ w, s
 static __device__ __forceinline__ size_t rowof(int k, int j, int nx, size_t P) { return (size_t) k * P + (size_t) j * nx; }
 
 /* cell/face 1-D Helmholtz operators (diffuseu.c:107-170, diffusew.c:107-172) */
-template <int ORD>
-static __device__ __forceinline__ double helm_cell(const double *Q, const double *dz, int k, double nudt)
+template <int ORD, typename T>
+static __device__ __forceinline__ T helm_cell(const T *Q, const double *dzd, int k, double nudtd)
 {
-    const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
-    const double dzm1h = 0.5 * (dz[k - 1] + dz[k]);
-    if (ORD == 2) return Q[3] + nudt * (((Q[4] - Q[3]) / dzp1h - (Q[3] - Q[2]) / dzm1h) / dz[k]);
-    const double wp3h = A1 * (Q[5] - Q[4]) + A2 * (Q[6] - Q[3]);
-    const double wp1h = A1 * (Q[4] - Q[3]) + A2 * (Q[5] - Q[2]);
-    const double wm1h = A1 * (Q[3] - Q[2]) + A2 * (Q[4] - Q[1]);
-    const double wm3h = A1 * (Q[2] - Q[1]) + A2 * (Q[3] - Q[0]);
-    const double dzp3h = 0.5 * (dz[k + 2] + dz[k + 1]);
-    const double dzm3h = 0.5 * (dz[k - 2] + dz[k - 1]);
-    return Q[3] + nudt * ((A1 * (wp1h / dzp1h - wm1h / dzm1h) + A2 * (wp3h / dzp3h - wm3h / dzm3h)) / dz[k]);
+    const T A1 = (T) A1d, A2 = (T) A2d;
+    const T nudt = (T) nudtd;
+    auto dz = [&](int kk) -> T { return (T) dzd[kk]; };
+    const T dzp1h = (T) 0.5 * (dz(k + 1) + dz(k));
+    const T dzm1h = (T) 0.5 * (dz(k - 1) + dz(k));
+    if (ORD == 2) return Q[3] + nudt * (((Q[4] - Q[3]) / dzp1h - (Q[3] - Q[2]) / dzm1h) / dz(k));
+    const T wp3h = A1 * (Q[5] - Q[4]) + A2 * (Q[6] - Q[3]);
+    const T wp1h = A1 * (Q[4] - Q[3]) + A2 * (Q[5] - Q[2]);
+    const T wm1h = A1 * (Q[3] - Q[2]) + A2 * (Q[4] - Q[1]);
+    const T wm3h = A1 * (Q[2] - Q[1]) + A2 * (Q[3] - Q[0]);
+    const T dzp3h = (T) 0.5 * (dz(k + 2) + dz(k + 1));
+    const T dzm3h = (T) 0.5 * (dz(k - 2) + dz(k - 1));
+    return Q[3] + nudt * ((A1 * (wp1h / dzp1h - wm1h / dzm1h) + A2 * (wp3h / dzp3h - wm3h / dzm3h)) / dz(k));
 }
 
-template <int ORD>
-static __device__ __forceinline__ double helm_face(const double *Q, const double *dz, int k, double nudt)
+template <int ORD, typename T>
+static __device__ __forceinline__ T helm_face(const T *Q, const double *dzd, int k, double nudtd)
 {
-    const double dzp1 = dz[k + 1], dzm0 = dz[k];
-    const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
+    const T A1 = (T) A1d, A2 = (T) A2d;
+    const T nudt = (T) nudtd;
+    auto dz = [&](int kk) -> T { return (T) dzd[kk]; };
+    const T dzp1 = dz(k + 1), dzm0 = dz(k);
+    const T dzp1h = (T) 0.5 * (dz(k + 1) + dz(k));
     if (ORD == 2) return Q[3] + nudt * (((Q[4] - Q[3]) / dzp1 - (Q[3] - Q[2]) / dzm0) / dzp1h);
-    const double wp3h = A1 * (Q[5] - Q[4]) + A2 * (Q[6] - Q[3]);
-    const double wp1h = A1 * (Q[4] - Q[3]) + A2 * (Q[5] - Q[2]);
-    const double wm1h = A1 * (Q[3] - Q[2]) + A2 * (Q[4] - Q[1]);
-    const double wm3h = A1 * (Q[2] - Q[1]) + A2 * (Q[3] - Q[0]);
-    const double dzp2 = dz[k + 2], dzm1 = dz[k - 1];
+    const T wp3h = A1 * (Q[5] - Q[4]) + A2 * (Q[6] - Q[3]);
+    const T wp1h = A1 * (Q[4] - Q[3]) + A2 * (Q[5] - Q[2]);
+    const T wm1h = A1 * (Q[3] - Q[2]) + A2 * (Q[4] - Q[1]);
+    const T wm3h = A1 * (Q[2] - Q[1]) + A2 * (Q[3] - Q[0]);
+    const T dzp2 = dz(k + 2), dzm1 = dz(k - 1);
     return Q[3] + nudt * ((A1 * (wp1h / dzp1 - wm1h / dzm0) + A2 * (wp3h / dzp2 - wm3h / dzm1)) / dzp1h);
 }
 
 /* x-y diffusion nu (dxx + dyy) q of one point from its x and y lines
  * (add_laplacexy_operator_*, diffuseu.c:286-359) */
-template <int ORD>
-static __device__ __forceinline__ double lapxy(const double *X, const double *Y, double dx, double dy, double nu)
+template <int ORD, typename T>
+static __device__ __forceinline__ T lapxy(const T *X, const T *Y, T dx, T dy, T nu)
 {
+    const T A1 = (T) A1d, A2 = (T) A2d;
     if (ORD == 2)
         return nu * (((X[4] - X[3]) / dx - (X[3] - X[2]) / dx) / dx + ((Y[4] - Y[3]) / dy - (Y[3] - Y[2]) / dy) / dy);
-    const double up3h = A1 * (X[5] - X[4]) + A2 * (X[6] - X[3]);
-    const double up1h = A1 * (X[4] - X[3]) + A2 * (X[5] - X[2]);
-    const double um1h = A1 * (X[3] - X[2]) + A2 * (X[4] - X[1]);
-    const double um3h = A1 * (X[2] - X[1]) + A2 * (X[3] - X[0]);
-    const double vp3h = A1 * (Y[5] - Y[4]) + A2 * (Y[6] - Y[3]);
-    const double vp1h = A1 * (Y[4] - Y[3]) + A2 * (Y[5] - Y[2]);
-    const double vm1h = A1 * (Y[3] - Y[2]) + A2 * (Y[4] - Y[1]);
-    const double vm3h = A1 * (Y[2] - Y[1]) + A2 * (Y[3] - Y[0]);
+    const T up3h = A1 * (X[5] - X[4]) + A2 * (X[6] - X[3]);
+    const T up1h = A1 * (X[4] - X[3]) + A2 * (X[5] - X[2]);
+    const T um1h = A1 * (X[3] - X[2]) + A2 * (X[4] - X[1]);
+    const T um3h = A1 * (X[2] - X[1]) + A2 * (X[3] - X[0]);
+    const T vp3h = A1 * (Y[5] - Y[4]) + A2 * (Y[6] - Y[3]);
+    const T vp1h = A1 * (Y[4] - Y[3]) + A2 * (Y[5] - Y[2]);
+    const T vm1h = A1 * (Y[3] - Y[2]) + A2 * (Y[4] - Y[1]);
+    const T vm3h = A1 * (Y[2] - Y[1]) + A2 * (Y[3] - Y[0]);
     return nu * ((A1 * (up1h / dx - um1h / dx) + A2 * (up3h / dx - um3h / dx)) / dx
                + (A1 * (vp1h / dy - vm1h / dy) + A2 * (vp3h / dy - vm3h / dy)) / dy);
 }
 
 /* wall values of u/v the advection of u/v uses on the wall faces
  * (advect.c:336-341, :539-554); Qb = levels -2..1, Qt = nz-2..nz+1 */
-template <int ORD>
-static __device__ __forceinline__ double wall_interp(double gm2, double gm1, double x0, double x1)
+template <int ORD, typename T>
+static __device__ __forceinline__ T wall_interp(T gm2, T gm1, T x0, T x1)
 {
-    if (ORD == 2) return 0.5 * (gm1 + x0);
-    return 0.5 * A2 * (gm2 + x1) + 0.5 * (A1 + 2.0 * A2) * (gm1 + x0);
+    if (ORD == 2) return (T) 0.5 * (gm1 + x0);
+    return (T) (0.5 * A2d) * (gm2 + x1) + (T) (0.5 * (A1d + 2.0 * A2d)) * (gm1 + x0);
 }
 
 /* block max of the CFL partials (all threads of the block call it) */
@@ -181,11 +189,11 @@ static __device__ __forceinline__ void block_max_store(double v, double *__restr
 
 /* advectu2/4 + laplacexy_u + buoyancy, then the u right-hand side of
  * solve_helmholtz1d_u (chanfast.c:465-477); CFL partials at s == 0 */
-template <int ORD, bool SC>
+template <int ORD, bool SC, typename T>
 __global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ c, const double *__restrict__ g, const double *__restrict__ scal,
-    double *__restrict__ fo, double *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
@@ -193,89 +201,93 @@ __global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
     if (i < nx && j < ny) {
         const size_t P = (size_t) nx * ny;
         const double *dz = R.dzt + CF_NGT;
-        const double dx = R.g.dx, dy = R.g.dy;
+    const T A1 = (T) A1d, A2 = (T) A2d, B1 = (T) B1d, B2 = (T) B2d;
+    (void) A1; (void) A2; (void) B1; (void) B2;
+    auto DZ = [&](int kk) -> T { return (T) dz[kk]; };
+        const T dx = (T) R.g.dx, dy = (T) R.g.dy;
         const size_t cc = (size_t) j * nx + i, q = (size_t) k * P + cc;
         const size_t row = rowof(k, j, nx, P);
-        double U[7], UX[7], UY[7];
+        T U[7], UX[7], UY[7];
         cell_col7<ORD>(u, P, cc, k, nz, R.cl.cell, R.ubc, R.ubc, U);
 #pragma unroll
         for (int t = 0; t < 7; t++) {
-            UX[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : u[row + wrapm(i - 3 + t, nx)];
-            UY[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : u[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
+            UX[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : u[row + wrapm(i - 3 + t, nx)];
+            UY[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : u[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
         }
-        const double Q0 = U[3];
-        const double dzk = dz[k];
+        const T Q0 = U[3];
+        const T dzk = DZ(k);
         /* massfluxx (advect.c:6-99): fluxes interpolated to the u-point in x */
-        auto MU = [&](int d) -> double {     /* at x-position i + d */
-            return ORD == 2 ? 0.5 * (dy * dzk * UX[3 + d] + dy * dzk * UX[4 + d])
+        auto MU = [&](int d) -> T {     /* at x-position i + d */
+            return ORD == 2 ? (T) 0.5 * (dy * dzk * UX[3 + d] + dy * dzk * UX[4 + d])
                             : B1 * (dy * dzk * UX[3 + d] + dy * dzk * UX[4 + d])
                             + B2 * (dy * dzk * UX[2 + d] + dy * dzk * UX[5 + d]);
         };
         const int im = wrapm(i - 1, nx), ip = wrapm(i + 1, nx), ip2 = wrapm(i + 2, nx);
-        auto MV = [&](int e) -> double {     /* at y-position j + e */
+        auto MV = [&](int e) -> T {     /* at y-position j + e */
             const size_t r2 = rowof(k, wrapm(j + e, ny), nx, P);
-            if (ORD == 2) return 0.5 * (dx * dzk * v[r2 + i] + dx * dzk * v[r2 + ip]);
+            if (ORD == 2) return (T) 0.5 * (dx * dzk * v[r2 + i] + dx * dzk * v[r2 + ip]);
             return B1 * (dx * dzk * v[r2 + i] + dx * dzk * v[r2 + ip])
                  + B2 * (dx * dzk * v[r2 + im] + dx * dzk * v[r2 + ip2]);
         };
         const size_t cj = (size_t) j * nx;
-        auto MW = [&](int e) -> double {     /* at face level k + e */
+        auto MW = [&](int e) -> T {     /* at face level k + e */
             const int kk = k + e;
-            if (ORD == 2) return 0.5 * (dx * dy * wadv(w, P, cj + i, kk, nz) + dx * dy * wadv(w, P, cj + ip, kk, nz));
+            if (ORD == 2) return (T) 0.5 * (dx * dy * wadv(w, P, cj + i, kk, nz) + dx * dy * wadv(w, P, cj + ip, kk, nz));
             return B1 * (dx * dy * wadv(w, P, cj + i, kk, nz) + dx * dy * wadv(w, P, cj + ip, kk, nz))
                  + B2 * (dx * dy * wadv(w, P, cj + im, kk, nz) + dx * dy * wadv(w, P, cj + ip2, kk, nz));
         };
-        double f;
-        const double den = dx * dy * dzk;
+        T f;
+        const T den = dx * dy * dzk;
         if (ORD == 2) {
-            double xp = MU(0) * 0.5 * (Q0 + UX[4]), xm = MU(-1) * 0.5 * (Q0 + UX[2]);
-            double yp = MV(0) * 0.5 * (Q0 + UY[4]), ym = MV(-1) * 0.5 * (Q0 + UY[2]);
-            const double mw0 = MW(0), mwm = MW(-1);
-            double zp = mw0 * 0.5 * (Q0 + U[4]), zm = mwm * 0.5 * (Q0 + U[2]);
-            if (k == nz - 1) zp = mw0 * wall_interp<2>(0.0, U[4], U[3], 0.0);
-            if (k == 0) zm = mwm * wall_interp<2>(0.0, U[2], U[3], 0.0);
+            T xp = MU(0) * (T) 0.5 * (Q0 + UX[4]), xm = MU(-1) * (T) 0.5 * (Q0 + UX[2]);
+            T yp = MV(0) * (T) 0.5 * (Q0 + UY[4]), ym = MV(-1) * (T) 0.5 * (Q0 + UY[2]);
+            const T mw0 = MW(0), mwm = MW(-1);
+            T zp = mw0 * (T) 0.5 * (Q0 + U[4]), zm = mwm * (T) 0.5 * (Q0 + U[2]);
+            if (k == nz - 1) zp = mw0 * wall_interp<2, T>((T) 0, U[4], U[3], (T) 0);
+            if (k == 0) zm = mwm * wall_interp<2, T>((T) 0, U[2], U[3], (T) 0);
             f = -((xp - xm) + (yp - ym) + (zp - zm)) / den;
         } else {
-            double xp = MU(0) * 0.5 * (Q0 + UX[4]), xm = MU(-1) * 0.5 * (Q0 + UX[2]);
-            double xp3 = MU(1) * 0.5 * (Q0 + UX[6]), xm3 = MU(-2) * 0.5 * (Q0 + UX[0]);
-            double yp = MV(0) * 0.5 * (Q0 + UY[4]), ym = MV(-1) * 0.5 * (Q0 + UY[2]);
-            double yp3 = MV(1) * 0.5 * (Q0 + UY[6]), ym3 = MV(-2) * 0.5 * (Q0 + UY[0]);
-            const double mw0 = MW(0), mwm = MW(-1), mwp = MW(1), mwm2 = MW(-2);
-            double zp = mw0 * 0.5 * (Q0 + U[4]), zm = mwm * 0.5 * (Q0 + U[2]);
-            double zp3 = mwp * 0.5 * (Q0 + U[6]), zm3 = mwm2 * 0.5 * (Q0 + U[0]);
+            T xp = MU(0) * (T) 0.5 * (Q0 + UX[4]), xm = MU(-1) * (T) 0.5 * (Q0 + UX[2]);
+            T xp3 = MU(1) * (T) 0.5 * (Q0 + UX[6]), xm3 = MU(-2) * (T) 0.5 * (Q0 + UX[0]);
+            T yp = MV(0) * (T) 0.5 * (Q0 + UY[4]), ym = MV(-1) * (T) 0.5 * (Q0 + UY[2]);
+            T yp3 = MV(1) * (T) 0.5 * (Q0 + UY[6]), ym3 = MV(-2) * (T) 0.5 * (Q0 + UY[0]);
+            const T mw0 = MW(0), mwm = MW(-1), mwp = MW(1), mwm2 = MW(-2);
+            T zp = mw0 * (T) 0.5 * (Q0 + U[4]), zm = mwm * (T) 0.5 * (Q0 + U[2]);
+            T zp3 = mwp * (T) 0.5 * (Q0 + U[6]), zm3 = mwm2 * (T) 0.5 * (Q0 + U[0]);
             /* U[t] is level k - 3 + t; wall values from levels -2..1 / nz-2..nz+1 */
             if (k == nz - 1) {
-                const double bct = wall_interp<4>(U[5], U[4], U[3], U[2]);
+                const T bct = wall_interp<4>(U[5], U[4], U[3], U[2]);
                 zp = mw0 * bct;
-                zp3 = -mwm * 0.5 * (U[1] + U[4]) + 2.0 * mw0 * bct;
+                zp3 = -mwm * (T) 0.5 * (U[1] + U[4]) + (T) 2.0 * mw0 * bct;
             }
             if (k == nz - 2) zp3 = mwp * wall_interp<4>(U[6], U[5], U[4], U[3]);
             if (k == 0) {
-                const double bcb = wall_interp<4>(U[1], U[2], U[3], U[4]);
+                const T bcb = wall_interp<4>(U[1], U[2], U[3], U[4]);
                 zm = mwm * bcb;
-                zm3 = -mw0 * 0.5 * (U[5] + U[2]) + 2.0 * mwm * bcb;
+                zm3 = -mw0 * (T) 0.5 * (U[5] + U[2]) + (T) 2.0 * mwm * bcb;
             }
             if (k == 1) zm3 = mwm2 * wall_interp<4>(U[0], U[1], U[2], U[3]);
             f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
                   + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
         }
-        f += lapxy<ORD>(UX, UY, dx, dy, R.nu);
+        f += lapxy<ORD>(UX, UY, dx, dy, (T) R.nu);
         if (SC && R.betg[0] != 0.0) {       /* add_buoyancy2/4, scalar.c:6-76 */
             const size_t r0 = row;
-            const double cx = ORD == 2 ? 0.5 * (c[r0 + i] + c[r0 + ip])
+            const T cx = ORD == 2 ? (T) 0.5 * (c[r0 + i] + c[r0 + ip])
                                        : B1 * (c[r0 + i] + c[r0 + ip]) + B2 * (c[r0 + im] + c[r0 + ip2]);
-            f += cx * R.betg[0];
+            f += cx * (T) R.betg[0];
         }
         const SubCoef &C = R.sc[s];
-        double h = helm_cell<ORD>(U, dz, k, C.nuadt);
-        if (s > 0) h += C.gamdt * f + C.zetdt * (g[q] + scal[R.m_prev_slot[s]]);
-        else h += C.gamdt * f;
-        h -= R.liftg[((size_t) s * 4 + 0) * nz + k];
+        T h = helm_cell<ORD>(U, dz, k, C.nuadt);
+        if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[R.m_prev_slot[s]]);
+        else h += (T) C.gamdt * f;
+        h -= (T) R.liftg[((size_t) s * 4 + 0) * nz + k];
         fo[q] = f;
         ro[q] = h * dzk;                      /* the solve's right-hand side is h dvol (diffuseu.c:202) */
         if (cflpart) {                        /* get_cfl, get.c:390-417 */
             const double dzph = 0.5 * (dz[k] + dz[k + 1]);
-            cflv = fmax(fmax(fabs(Q0) * cfldt / dx, fabs(v[q]) * cfldt / dy), fabs(w[q]) * cfldt / dzph);
+            cflv = fmax(fmax(fabs((double) Q0) * cfldt / R.g.dx, fabs((double) v[q]) * cfldt / R.g.dy),
+                        fabs((double) w[q]) * cfldt / dzph);
         }
     }
     if (cflpart) block_max_store(cflv, cflpart, (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
@@ -283,11 +295,11 @@ __global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
 
 /* ------------------------------------------------------------ v */
 
-template <int ORD, bool SC>
+template <int ORD, bool SC, typename T>
 __global__ __launch_bounds__(256) void k_grhs_v(const RunDev R, const int s,
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ c, const double *__restrict__ g, const double *__restrict__ scal,
-    double *__restrict__ fo, double *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
 {
     (void) cflpart; (void) cfldt; (void) scal;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
@@ -295,95 +307,98 @@ __global__ __launch_bounds__(256) void k_grhs_v(const RunDev R, const int s,
     if (i >= nx || j >= ny) return;
     const size_t P = (size_t) nx * ny;
     const double *dz = R.dzt + CF_NGT;
-    const double dx = R.g.dx, dy = R.g.dy;
+    const T A1 = (T) A1d, A2 = (T) A2d, B1 = (T) B1d, B2 = (T) B2d;
+    (void) A1; (void) A2; (void) B1; (void) B2;
+    auto DZ = [&](int kk) -> T { return (T) dz[kk]; };
+    const T dx = (T) R.g.dx, dy = (T) R.g.dy;
     const size_t cc = (size_t) j * nx + i, q = (size_t) k * P + cc;
     const size_t row = rowof(k, j, nx, P);
-    double V[7], VX[7], VY[7];
+    T V[7], VX[7], VY[7];
     cell_col7<ORD>(v, P, cc, k, nz, R.cl.cell, R.vbc, R.vbc, V);
 #pragma unroll
     for (int t = 0; t < 7; t++) {
-        VX[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : v[row + wrapm(i - 3 + t, nx)];
-        VY[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : v[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
+        VX[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : v[row + wrapm(i - 3 + t, nx)];
+        VY[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : v[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
     }
-    const double Q0 = V[3];
-    const double dzk = dz[k];
+    const T Q0 = V[3];
+    const T dzk = DZ(k);
     const int jm = wrapm(j - 1, ny), jp = wrapm(j + 1, ny), jp2 = wrapm(j + 2, ny);
     /* massfluxy (advect.c:132-200): fluxes interpolated to the v-point in y */
-    auto MU = [&](int d) -> double {     /* at x-position i + d */
+    auto MU = [&](int d) -> T {     /* at x-position i + d */
         const int ii = wrapm(i + d, nx);
-        const double a0 = u[rowof(k, j, nx, P) + ii], a1 = u[rowof(k, jp, nx, P) + ii];
-        if (ORD == 2) return 0.5 * (dy * dzk * a0 + dy * dzk * a1);
-        const double am = u[rowof(k, jm, nx, P) + ii], a2 = u[rowof(k, jp2, nx, P) + ii];
+        const T a0 = u[rowof(k, j, nx, P) + ii], a1 = u[rowof(k, jp, nx, P) + ii];
+        if (ORD == 2) return (T) 0.5 * (dy * dzk * a0 + dy * dzk * a1);
+        const T am = u[rowof(k, jm, nx, P) + ii], a2 = u[rowof(k, jp2, nx, P) + ii];
         return B1 * (dy * dzk * a0 + dy * dzk * a1) + B2 * (dy * dzk * am + dy * dzk * a2);
     };
-    auto MV = [&](int e) -> double {
-        return ORD == 2 ? 0.5 * (dx * dzk * VY[3 + e] + dx * dzk * VY[4 + e])
+    auto MV = [&](int e) -> T {
+        return ORD == 2 ? (T) 0.5 * (dx * dzk * VY[3 + e] + dx * dzk * VY[4 + e])
                         : B1 * (dx * dzk * VY[3 + e] + dx * dzk * VY[4 + e])
                         + B2 * (dx * dzk * VY[2 + e] + dx * dzk * VY[5 + e]);
     };
-    auto MW = [&](int e) -> double {
+    auto MW = [&](int e) -> T {
         const int kk = k + e;
-        const double a0 = wadv(w, P, (size_t) j * nx + i, kk, nz), a1 = wadv(w, P, (size_t) jp * nx + i, kk, nz);
-        if (ORD == 2) return 0.5 * (dx * dy * a0 + dx * dy * a1);
-        const double am = wadv(w, P, (size_t) jm * nx + i, kk, nz), a2 = wadv(w, P, (size_t) jp2 * nx + i, kk, nz);
+        const T a0 = wadv(w, P, (size_t) j * nx + i, kk, nz), a1 = wadv(w, P, (size_t) jp * nx + i, kk, nz);
+        if (ORD == 2) return (T) 0.5 * (dx * dy * a0 + dx * dy * a1);
+        const T am = wadv(w, P, (size_t) jm * nx + i, kk, nz), a2 = wadv(w, P, (size_t) jp2 * nx + i, kk, nz);
         return B1 * (dx * dy * a0 + dx * dy * a1) + B2 * (dx * dy * am + dx * dy * a2);
     };
-    double f;
-    const double den = dx * dy * dzk;
+    T f;
+    const T den = dx * dy * dzk;
     if (ORD == 2) {
-        double xp = MU(0) * 0.5 * (Q0 + VX[4]), xm = MU(-1) * 0.5 * (Q0 + VX[2]);
-        double yp = MV(0) * 0.5 * (Q0 + VY[4]), ym = MV(-1) * 0.5 * (Q0 + VY[2]);
-        const double mw0 = MW(0), mwm = MW(-1);
-        double zp = mw0 * 0.5 * (Q0 + V[4]), zm = mwm * 0.5 * (Q0 + V[2]);
-        if (k == nz - 1) zp = mw0 * wall_interp<2>(0.0, V[4], V[3], 0.0);
-        if (k == 0) zm = mwm * wall_interp<2>(0.0, V[2], V[3], 0.0);
+        T xp = MU(0) * (T) 0.5 * (Q0 + VX[4]), xm = MU(-1) * (T) 0.5 * (Q0 + VX[2]);
+        T yp = MV(0) * (T) 0.5 * (Q0 + VY[4]), ym = MV(-1) * (T) 0.5 * (Q0 + VY[2]);
+        const T mw0 = MW(0), mwm = MW(-1);
+        T zp = mw0 * (T) 0.5 * (Q0 + V[4]), zm = mwm * (T) 0.5 * (Q0 + V[2]);
+        if (k == nz - 1) zp = mw0 * wall_interp<2, T>((T) 0, V[4], V[3], (T) 0);
+        if (k == 0) zm = mwm * wall_interp<2, T>((T) 0, V[2], V[3], (T) 0);
         f = -((xp - xm) + (yp - ym) + (zp - zm)) / den;
     } else {
-        double xp = MU(0) * 0.5 * (Q0 + VX[4]), xm = MU(-1) * 0.5 * (Q0 + VX[2]);
-        double xp3 = MU(1) * 0.5 * (Q0 + VX[6]), xm3 = MU(-2) * 0.5 * (Q0 + VX[0]);
-        double yp = MV(0) * 0.5 * (Q0 + VY[4]), ym = MV(-1) * 0.5 * (Q0 + VY[2]);
-        double yp3 = MV(1) * 0.5 * (Q0 + VY[6]), ym3 = MV(-2) * 0.5 * (Q0 + VY[0]);
-        const double mw0 = MW(0), mwm = MW(-1), mwp = MW(1), mwm2 = MW(-2);
-        double zp = mw0 * 0.5 * (Q0 + V[4]), zm = mwm * 0.5 * (Q0 + V[2]);
-        double zp3 = mwp * 0.5 * (Q0 + V[6]), zm3 = mwm2 * 0.5 * (Q0 + V[0]);
+        T xp = MU(0) * (T) 0.5 * (Q0 + VX[4]), xm = MU(-1) * (T) 0.5 * (Q0 + VX[2]);
+        T xp3 = MU(1) * (T) 0.5 * (Q0 + VX[6]), xm3 = MU(-2) * (T) 0.5 * (Q0 + VX[0]);
+        T yp = MV(0) * (T) 0.5 * (Q0 + VY[4]), ym = MV(-1) * (T) 0.5 * (Q0 + VY[2]);
+        T yp3 = MV(1) * (T) 0.5 * (Q0 + VY[6]), ym3 = MV(-2) * (T) 0.5 * (Q0 + VY[0]);
+        const T mw0 = MW(0), mwm = MW(-1), mwp = MW(1), mwm2 = MW(-2);
+        T zp = mw0 * (T) 0.5 * (Q0 + V[4]), zm = mwm * (T) 0.5 * (Q0 + V[2]);
+        T zp3 = mwp * (T) 0.5 * (Q0 + V[6]), zm3 = mwm2 * (T) 0.5 * (Q0 + V[0]);
         if (k == nz - 1) {
-            const double bct = wall_interp<4>(V[5], V[4], V[3], V[2]);
+            const T bct = wall_interp<4>(V[5], V[4], V[3], V[2]);
             zp = mw0 * bct;
-            zp3 = -mwm * 0.5 * (V[1] + V[4]) + 2.0 * mw0 * bct;
+            zp3 = -mwm * (T) 0.5 * (V[1] + V[4]) + (T) 2.0 * mw0 * bct;
         }
         if (k == nz - 2) zp3 = mwp * wall_interp<4>(V[6], V[5], V[4], V[3]);
         if (k == 0) {
-            const double bcb = wall_interp<4>(V[1], V[2], V[3], V[4]);
+            const T bcb = wall_interp<4>(V[1], V[2], V[3], V[4]);
             zm = mwm * bcb;
-            zm3 = -mw0 * 0.5 * (V[5] + V[2]) + 2.0 * mwm * bcb;
+            zm3 = -mw0 * (T) 0.5 * (V[5] + V[2]) + (T) 2.0 * mwm * bcb;
         }
         if (k == 1) zm3 = mwm2 * wall_interp<4>(V[0], V[1], V[2], V[3]);
         f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
               + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
     }
-    f += lapxy<ORD>(VX, VY, dx, dy, R.nu);
+    f += lapxy<ORD>(VX, VY, dx, dy, (T) R.nu);
     if (SC && R.betg[1] != 0.0) {
-        const double c0 = c[q], c1 = c[rowof(k, jp, nx, P) + i];
-        const double cy = ORD == 2 ? 0.5 * (c0 + c1)
+        const T c0 = c[q], c1 = c[rowof(k, jp, nx, P) + i];
+        const T cy = ORD == 2 ? (T) 0.5 * (c0 + c1)
                                    : B1 * (c0 + c1) + B2 * (c[rowof(k, jm, nx, P) + i] + c[rowof(k, jp2, nx, P) + i]);
-        f += cy * R.betg[1];
+        f += cy * (T) R.betg[1];
     }
     const SubCoef &C = R.sc[s];
-    double h = helm_cell<ORD>(V, dz, k, C.nuadt);
-    if (s > 0) h += C.gamdt * f + C.zetdt * g[q];
-    else h += C.gamdt * f;
-    h -= R.liftg[((size_t) s * 4 + 1) * nz + k];
+    T h = helm_cell<ORD>(V, dz, k, C.nuadt);
+    if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
+    else h += (T) C.gamdt * f;
+    h -= (T) R.liftg[((size_t) s * 4 + 1) * nz + k];
     fo[q] = f;
     ro[q] = h * dzk;
 }
 
 /* ------------------------------------------------------------ w */
 
-template <int ORD, bool SC>
+template <int ORD, bool SC, typename T>
 __global__ __launch_bounds__(256) void k_grhs_w(const RunDev R, const int s,
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ c, const double *__restrict__ g, const double *__restrict__ scal,
-    double *__restrict__ fo, double *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
 {
     (void) cflpart; (void) cfldt; (void) scal;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
@@ -397,95 +412,98 @@ __global__ __launch_bounds__(256) void k_grhs_w(const RunDev R, const int s,
         return;
     }
     const double *dz = R.dzt + CF_NGT;
-    const double dx = R.g.dx, dy = R.g.dy;
+    const T A1 = (T) A1d, A2 = (T) A2d, B1 = (T) B1d, B2 = (T) B2d;
+    (void) A1; (void) A2; (void) B1; (void) B2;
+    auto DZ = [&](int kk) -> T { return (T) dz[kk]; };
+    const T dx = (T) R.g.dx, dy = (T) R.g.dy;
     const size_t row = rowof(k, j, nx, P);
-    double WA[7], WH[7], WX[7], WY[7];
+    T WA[7], WH[7], WX[7], WY[7];
 #pragma unroll
-    for (int t = 0; t < 7; t++) WA[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : wadv(w, P, cc, k - 3 + t, nz);
+    for (int t = 0; t < 7; t++) WA[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : wadv(w, P, cc, k - 3 + t, nz);
     face_col7<ORD>(w, P, cc, k, nz, R.cl.face, WH);
 #pragma unroll
     for (int t = 0; t < 7; t++) {
-        WX[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : w[row + wrapm(i - 3 + t, nx)];
-        WY[t] = (ORD == 2 && (t < 2 || t > 4)) ? 0.0 : w[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
+        WX[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : w[row + wrapm(i - 3 + t, nx)];
+        WY[t] = (ORD == 2 && (t < 2 || t > 4)) ? (T) 0 : w[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
     }
-    const double Q0 = WA[3];
+    const T Q0 = WA[3];
     /* u and v at their own levels with closure ghosts, for the z-interpolated fluxes */
-    auto ucell = [&](int ii, int jj, int kk, const double *fld, double bc) -> double {
+    auto ucell = [&](int ii, int jj, int kk, const T *fld, double bcb, double bct) -> T {
         const size_t c2 = (size_t) jj * nx + ii;
         if (kk >= 0 && kk < nz) return fld[(size_t) kk * P + c2];
         const bool lo = kk < 0;
-        const Clos1 &C = R.cl.cell[lo ? 0 : 1];
+        const Clos1 &Cl = R.cl.cell[lo ? 0 : 1];
         const int m = lo ? -1 - kk : kk - nz;
-        const double x0 = fld[(size_t) (lo ? 0 : nz - 1) * P + c2];
-        const double x1 = ORD == 4 ? fld[(size_t) (lo ? 1 : nz - 2) * P + c2] : 0.0;
-        const double x2 = ORD == 4 ? fld[(size_t) (lo ? 2 : nz - 3) * P + c2] : 0.0;
-        return C.G[m][0] * x0 + C.G[m][1] * x1 + C.G[m][2] * x2 + C.B[m] * bc;
+        const T x0 = fld[(size_t) (lo ? 0 : nz - 1) * P + c2];
+        const T x1 = ORD == 4 ? fld[(size_t) (lo ? 1 : nz - 2) * P + c2] : (T) 0;
+        const T x2 = ORD == 4 ? fld[(size_t) (lo ? 2 : nz - 3) * P + c2] : (T) 0;
+        return (T) Cl.G[m][0] * x0 + (T) Cl.G[m][1] * x1 + (T) Cl.G[m][2] * x2 + (T) (Cl.B[m] * (lo ? bcb : bct));
     };
     /* massfluxz (advect.c:201-283): fluxes interpolated to the w-point in z */
-    auto MUZ = [&](int d) -> double {
+    auto MUZ = [&](int d) -> T {
         const int ii = wrapm(i + d, nx);
-        if (ORD == 2) return 0.5 * (dy * dz[k] * ucell(ii, j, k, u, R.ubc) + dy * dz[k + 1] * ucell(ii, j, k + 1, u, R.ubc));
-        return B1 * (dy * dz[k] * ucell(ii, j, k, u, R.ubc) + dy * dz[k + 1] * ucell(ii, j, k + 1, u, R.ubc))
-             + B2 * (dy * dz[k - 1] * ucell(ii, j, k - 1, u, R.ubc) + dy * dz[k + 2] * ucell(ii, j, k + 2, u, R.ubc));
+        if (ORD == 2) return (T) 0.5 * (dy * DZ(k) * ucell(ii, j, k, u, R.ubc, R.ubc) + dy * DZ(k + 1) * ucell(ii, j, k + 1, u, R.ubc, R.ubc));
+        return B1 * (dy * DZ(k) * ucell(ii, j, k, u, R.ubc, R.ubc) + dy * DZ(k + 1) * ucell(ii, j, k + 1, u, R.ubc, R.ubc))
+             + B2 * (dy * DZ(k - 1) * ucell(ii, j, k - 1, u, R.ubc, R.ubc) + dy * DZ(k + 2) * ucell(ii, j, k + 2, u, R.ubc, R.ubc));
     };
-    auto MVZ = [&](int e) -> double {
+    auto MVZ = [&](int e) -> T {
         const int jj = wrapm(j + e, ny);
-        if (ORD == 2) return 0.5 * (dx * dz[k] * ucell(i, jj, k, v, R.vbc) + dx * dz[k + 1] * ucell(i, jj, k + 1, v, R.vbc));
-        return B1 * (dx * dz[k] * ucell(i, jj, k, v, R.vbc) + dx * dz[k + 1] * ucell(i, jj, k + 1, v, R.vbc))
-             + B2 * (dx * dz[k - 1] * ucell(i, jj, k - 1, v, R.vbc) + dx * dz[k + 2] * ucell(i, jj, k + 2, v, R.vbc));
+        if (ORD == 2) return (T) 0.5 * (dx * DZ(k) * ucell(i, jj, k, v, R.vbc, R.vbc) + dx * DZ(k + 1) * ucell(i, jj, k + 1, v, R.vbc, R.vbc));
+        return B1 * (dx * DZ(k) * ucell(i, jj, k, v, R.vbc, R.vbc) + dx * DZ(k + 1) * ucell(i, jj, k + 1, v, R.vbc, R.vbc))
+             + B2 * (dx * DZ(k - 1) * ucell(i, jj, k - 1, v, R.vbc, R.vbc) + dx * DZ(k + 2) * ucell(i, jj, k + 2, v, R.vbc, R.vbc));
     };
-    auto MWZ = [&](int e) -> double {     /* at face k + e from WA[2+e..5+e] */
-        return ORD == 2 ? 0.5 * (dx * dy * WA[3 + e] + dx * dy * WA[4 + e])
+    auto MWZ = [&](int e) -> T {     /* at face k + e from WA[2+e..5+e] */
+        return ORD == 2 ? (T) 0.5 * (dx * dy * WA[3 + e] + dx * dy * WA[4 + e])
                         : B1 * (dx * dy * WA[3 + e] + dx * dy * WA[4 + e])
                         + B2 * (dx * dy * WA[2 + e] + dx * dy * WA[5 + e]);
     };
-    double f;
-    const double den = dx * dy * (0.5 * (dz[k + 1] + dz[k]));
+    T f;
+    const T den = dx * dy * ((T) 0.5 * (DZ(k + 1) + DZ(k)));
     if (ORD == 2) {
-        double xp = MUZ(0) * 0.5 * (Q0 + WX[4]), xm = MUZ(-1) * 0.5 * (Q0 + WX[2]);
-        double yp = MVZ(0) * 0.5 * (Q0 + WY[4]), ym = MVZ(-1) * 0.5 * (Q0 + WY[2]);
-        double zp = MWZ(0) * 0.5 * (Q0 + WA[4]), zm = MWZ(-1) * 0.5 * (Q0 + WA[2]);
+        T xp = MUZ(0) * (T) 0.5 * (Q0 + WX[4]), xm = MUZ(-1) * (T) 0.5 * (Q0 + WX[2]);
+        T yp = MVZ(0) * (T) 0.5 * (Q0 + WY[4]), ym = MVZ(-1) * (T) 0.5 * (Q0 + WY[2]);
+        T zp = MWZ(0) * (T) 0.5 * (Q0 + WA[4]), zm = MWZ(-1) * (T) 0.5 * (Q0 + WA[2]);
         f = -((xp - xm) + (yp - ym) + (zp - zm)) / den;
     } else {
-        double xp = MUZ(0) * 0.5 * (Q0 + WX[4]), xm = MUZ(-1) * 0.5 * (Q0 + WX[2]);
-        double xp3 = MUZ(1) * 0.5 * (Q0 + WX[6]), xm3 = MUZ(-2) * 0.5 * (Q0 + WX[0]);
-        double yp = MVZ(0) * 0.5 * (Q0 + WY[4]), ym = MVZ(-1) * 0.5 * (Q0 + WY[2]);
-        double yp3 = MVZ(1) * 0.5 * (Q0 + WY[6]), ym3 = MVZ(-2) * 0.5 * (Q0 + WY[0]);
-        double zp = MWZ(0) * 0.5 * (Q0 + WA[4]), zm = MWZ(-1) * 0.5 * (Q0 + WA[2]);
-        double zp3 = MWZ(1) * 0.5 * (Q0 + WA[6]), zm3 = MWZ(-2) * 0.5 * (Q0 + WA[0]);
+        T xp = MUZ(0) * (T) 0.5 * (Q0 + WX[4]), xm = MUZ(-1) * (T) 0.5 * (Q0 + WX[2]);
+        T xp3 = MUZ(1) * (T) 0.5 * (Q0 + WX[6]), xm3 = MUZ(-2) * (T) 0.5 * (Q0 + WX[0]);
+        T yp = MVZ(0) * (T) 0.5 * (Q0 + WY[4]), ym = MVZ(-1) * (T) 0.5 * (Q0 + WY[2]);
+        T yp3 = MVZ(1) * (T) 0.5 * (Q0 + WY[6]), ym3 = MVZ(-2) * (T) 0.5 * (Q0 + WY[0]);
+        T zp = MWZ(0) * (T) 0.5 * (Q0 + WA[4]), zm = MWZ(-1) * (T) 0.5 * (Q0 + WA[2]);
+        T zp3 = MWZ(1) * (T) 0.5 * (Q0 + WA[6]), zm3 = MWZ(-2) * (T) 0.5 * (Q0 + WA[0]);
         f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
               + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
     }
-    f += lapxy<ORD>(WX, WY, dx, dy, R.nu);
+    f += lapxy<ORD>(WX, WY, dx, dy, (T) R.nu);
     if (SC && R.betg[2] != 0.0) {
-        const double c0 = ucell(i, j, k, c, R.cbot), c1 = ucell(i, j, k + 1, c, R.cbot);
-        double cz;
-        if (ORD == 2) cz = 0.5 * (c0 + c1);
+        const T c0 = ucell(i, j, k, c, R.cbot, R.ctop), c1 = ucell(i, j, k + 1, c, R.cbot, R.ctop);
+        T cz;
+        if (ORD == 2) cz = (T) 0.5 * (c0 + c1);
         else {
-            const double cm = k - 1 >= 0 ? ucell(i, j, k - 1, c, R.cbot) : ucell(i, j, k - 1, c, R.cbot);
-            const double c2 = k + 2 < nz ? ucell(i, j, k + 2, c, R.ctop) : ucell(i, j, k + 2, c, R.ctop);
+            const T cm = ucell(i, j, k - 1, c, R.cbot, R.ctop);
+            const T c2 = ucell(i, j, k + 2, c, R.cbot, R.ctop);
             cz = B1 * (c0 + c1) + B2 * (cm + c2);
         }
-        f += cz * R.betg[2];
+        f += cz * (T) R.betg[2];
     }
     const SubCoef &C = R.sc[s];
-    double h = helm_face<ORD>(WH, dz, k, C.nuadt);
-    if (s > 0) h += C.gamdt * f + C.zetdt * g[q];
-    else h += C.gamdt * f;
-    h -= R.liftg[((size_t) s * 4 + 2) * nz + k];
+    T h = helm_face<ORD>(WH, dz, k, C.nuadt);
+    if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
+    else h += (T) C.gamdt * f;
+    h -= (T) R.liftg[((size_t) s * 4 + 2) * nz + k];
     fo[q] = f;
-    ro[q] = h * (0.5 * (dz[k + 1] + dz[k]));   /* times the face volume (diffusew.c:205) */
+    ro[q] = h * ((T) 0.5 * (DZ(k + 1) + DZ(k)));   /* times the face volume (diffusew.c:205) */
 }
 
 /* ------------------------------------------------------------ c */
 
 /* flux_quick + advect_scalar (scalar.c:79-162), laplacexy with nu / Pr,
  * then the scalar's implicit right-hand side (chanfast.c:540-571) */
-template <int ORD, bool SC>
+template <int ORD, bool SC, typename T>
 __global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ c, const double *__restrict__ g, const double *__restrict__ scal,
-    double *__restrict__ fo, double *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
 {
     (void) cflpart; (void) cfldt;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
@@ -493,52 +511,42 @@ __global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
     if (i >= nx || j >= ny) return;
     const size_t P = (size_t) nx * ny;
     const double *dz = R.dzt + CF_NGT;
-    const double dx = R.g.dx, dy = R.g.dy;
+    const T A1 = (T) A1d, A2 = (T) A2d, B1 = (T) B1d, B2 = (T) B2d;
+    (void) A1; (void) A2; (void) B1; (void) B2;
+    auto DZ = [&](int kk) -> T { return (T) dz[kk]; };
+    const T dx = (T) R.g.dx, dy = (T) R.g.dy;
     const size_t cc = (size_t) j * nx + i, q = (size_t) k * P + cc;
     const size_t row = rowof(k, j, nx, P);
-    double Cz[7], CX[7], CY[7];
+    T Cz[7], CX[7], CY[7];
     cell_col7<ORD>(c, P, cc, k, nz, R.cl.cell, R.cbot, R.ctop, Cz);
 #pragma unroll
     for (int t = 0; t < 7; t++) {
-        CX[t] = (t == 0 || t == 6) && ORD == 2 ? 0.0 : c[row + wrapm(i - 3 + t, nx)];
-        CY[t] = (t == 0 || t == 6) && ORD == 2 ? 0.0 : c[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
+        CX[t] = (t == 0 || t == 6) && ORD == 2 ? (T) 0 : c[row + wrapm(i - 3 + t, nx)];
+        CY[t] = (t == 0 || t == 6) && ORD == 2 ? (T) 0 : c[rowof(k, wrapm(j - 3 + t, ny), nx, P) + i];
     }
-    const double qk = -0.125;
+    const T qk = (T) -0.125;
     /* QUICK face value upwinded by the sign of the face volume flux */
-    auto quick = [&](double fl, double cm, double c0, double c1, double c2) -> double {
-        double cf = 0.5 * (c0 + c1);
-        cf += fl > 0.0 ? qk * (c1 - 2.0 * c0 + cm) : qk * (c2 - 2.0 * c1 + c0);
+    auto quick = [&](T fl, T cm, T c0, T c1, T c2) -> T {
+        T cf = (T) 0.5 * (c0 + c1);
+        cf += fl > (T) 0 ? qk * (c1 - (T) 2.0 * c0 + cm) : qk * (c2 - (T) 2.0 * c1 + c0);
         return fl * cf;
     };
     const int im = wrapm(i - 1, nx), jm = wrapm(j - 1, ny);
-    const double fu0 = dy * dz[k] * u[row + i], fum = dy * dz[k] * u[row + im];
-    const double fv0 = dx * dz[k] * v[q], fvm = dx * dz[k] * v[rowof(k, jm, nx, P) + i];
-    const double fw0 = dx * dy * wadv(w, P, cc, k, nz), fwm = dx * dy * wadv(w, P, cc, k - 1, nz);
-    const double mu0 = quick(fu0, CX[2], CX[3], CX[4], CX[5]), mum = quick(fum, CX[1], CX[2], CX[3], CX[4]);
-    const double mv0 = quick(fv0, CY[2], CY[3], CY[4], CY[5]), mvm = quick(fvm, CY[1], CY[2], CY[3], CY[4]);
-    const double mw0 = quick(fw0, Cz[2], Cz[3], Cz[4], Cz[5]), mwm = quick(fwm, Cz[1], Cz[2], Cz[3], Cz[4]);
-    double f = -((mu0 - mum) + (mv0 - mvm) + (mw0 - mwm)) / (dx * dy * dz[k]);
-    f += lapxy<ORD>(CX, CY, dx, dy, R.kap);
+    const T fu0 = dy * DZ(k) * u[row + i], fum = dy * DZ(k) * u[row + im];
+    const T fv0 = dx * DZ(k) * v[q], fvm = dx * DZ(k) * v[rowof(k, jm, nx, P) + i];
+    const T fw0 = dx * dy * wadv(w, P, cc, k, nz), fwm = dx * dy * wadv(w, P, cc, k - 1, nz);
+    const T mu0 = quick(fu0, CX[2], CX[3], CX[4], CX[5]), mum = quick(fum, CX[1], CX[2], CX[3], CX[4]);
+    const T mv0 = quick(fv0, CY[2], CY[3], CY[4], CY[5]), mvm = quick(fvm, CY[1], CY[2], CY[3], CY[4]);
+    const T mw0 = quick(fw0, Cz[2], Cz[3], Cz[4], Cz[5]), mwm = quick(fwm, Cz[1], Cz[2], Cz[3], Cz[4]);
+    T f = -((mu0 - mum) + (mv0 - mvm) + (mw0 - mwm)) / (dx * dy * DZ(k));
+    f += lapxy<ORD>(CX, CY, dx, dy, (T) R.kap);
     const SubCoef &C = R.sc[s];
-    double h = helm_cell<ORD>(Cz, dz, k, C.kadt);
-    if (s > 0) h += C.gamdt * f + C.zetdt * (g[q] + scal[SC_MC0 + (s + 2) % 3]);
-    else h += C.gamdt * f;
-    h -= R.liftg[((size_t) s * 4 + 3) * nz + k];
+    T h = helm_cell<ORD>(Cz, dz, k, C.kadt);
+    if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[SC_MC0 + (s + 2) % 3]);
+    else h += (T) C.gamdt * f;
+    h -= (T) R.liftg[((size_t) s * 4 + 3) * nz + k];
     fo[q] = f;
-    ro[q] = h * dz[k];
-}
-
-GRhsSet pick_grhs(int order, int scalar)
-{
-    GRhsSet S;
-    if (order == 4) {
-        if (scalar) { S.u = k_grhs_u<4, true>; S.v = k_grhs_v<4, true>; S.w = k_grhs_w<4, true>; S.c = k_grhs_c<4, true>; }
-        else { S.u = k_grhs_u<4, false>; S.v = k_grhs_v<4, false>; S.w = k_grhs_w<4, false>; S.c = nullptr; }
-    } else {
-        if (scalar) { S.u = k_grhs_u<2, true>; S.v = k_grhs_v<2, true>; S.w = k_grhs_w<2, true>; S.c = k_grhs_c<2, true>; }
-        else { S.u = k_grhs_u<2, false>; S.v = k_grhs_v<2, false>; S.w = k_grhs_w<2, false>; S.c = nullptr; }
-    }
-    return S;
+    ro[q] = h * DZ(k);
 }
 
 /* ------------------------------------------------------------ solve */
@@ -547,72 +555,77 @@ GRhsSet pick_grhs(int order, int scalar)
  * heptdag.c:5-61), constant coefficients from the host, one thread per
  * (column, field), in place.  Fields 0 u, 1 v, 2 w (nz-1 unknowns, top
  * wall := 0), 3 c. */
+template <typename T>
 __global__ __launch_bounds__(256) void k_gsolve4(const RunDev R, const int s,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ rc)
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const size_t col = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     if (col >= P || f > 3) return;
-    double *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
+    T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
-    const HeptTab T = R.hp[s][type];
-    const int n = T.n;
-    const double *A = T.t, *AL = A + n, *BE = AL + n, *ID = BE + n, *Pp = ID + n, *Qq = Pp + n, *Rr = Qq + n;
-    double *xc = x + col;
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const HeptTab HT = R.hp[s][type];
+    const int n = HT.n;
+    const double *A = HT.t, *AL = A + n, *BE = AL + n, *ID = BE + n, *Pp = ID + n, *Qq = Pp + n, *Rr = Qq + n;
+    T *xc = x + col;
+    T s1 = 0, s2 = 0, s3 = 0;
     constexpr int BT = 8;
     for (int i0 = 0; i0 < n; i0 += BT) {
-        double hb[BT];
+        T hb[BT];
 #pragma unroll
-        for (int t = 0; t < BT; t++) hb[t] = i0 + t < n ? xc[(size_t) (i0 + t) * P] : 0.0;
+        for (int t = 0; t < BT; t++) hb[t] = i0 + t < n ? xc[(size_t) (i0 + t) * P] : (T) 0;
 #pragma unroll
         for (int t = 0; t < BT; t++) {
             const int i = i0 + t;
             if (i < n) {
-                const double si = (hb[t] - A[i] * s3 - AL[i] * s2 - BE[i] * s1) * ID[i];
+                const T si = (hb[t] - (T) A[i] * s3 - (T) AL[i] * s2 - (T) BE[i] * s1) * (T) ID[i];
                 xc[(size_t) i * P] = si;
                 s3 = s2; s2 = s1; s1 = si;
             }
         }
     }
-    double x1 = 0.0, x2 = 0.0, x3 = 0.0;
+    T x1 = 0, x2 = 0, x3 = 0;
     for (int i0 = n - 1; i0 >= 0; i0 -= BT) {
-        double sb[BT];
+        T sb[BT];
 #pragma unroll
-        for (int t = 0; t < BT; t++) sb[t] = i0 - t >= 0 ? xc[(size_t) (i0 - t) * P] : 0.0;
+        for (int t = 0; t < BT; t++) sb[t] = i0 - t >= 0 ? xc[(size_t) (i0 - t) * P] : (T) 0;
 #pragma unroll
         for (int t = 0; t < BT; t++) {
             const int i = i0 - t;
             if (i >= 0) {
-                const double xi = sb[t] - Pp[i] * x1 - Qq[i] * x2 - Rr[i] * x3;
+                const T xi = sb[t] - (T) Pp[i] * x1 - (T) Qq[i] * x2 - (T) Rr[i] * x3;
                 xc[(size_t) i * P] = xi;
                 x3 = x2; x2 = x1; x1 = xi;
             }
         }
     }
-    if (f == 2) xc[(size_t) (nz - 1) * P] = 0.0;
+    if (f == 2) xc[(size_t) (nz - 1) * P] = (T) 0;
 }
+
+template __global__ void k_gsolve4<double>(const RunDev, const int, double *, double *, double *, double *);
+template __global__ void k_gsolve4<float>(const RunDev, const int, float *, float *, float *, float *);
 
 /* ------------------------------------------------------------ constraints */
 
 /* interior plane sums of u* (kind 0, blockIdx.z = 0) and c* (kind 1) for
  * get_xy_avg (get.c:281-307); deterministic fixed-order trees.
  * part[(kind * nz + k) * nb + b] */
-__global__ __launch_bounds__(256) void k_gplane(const RunDev R, const double *__restrict__ q0,
-                                                const double *__restrict__ q1, double *__restrict__ part, int nb)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gplane(const RunDev R, const T *__restrict__ q0,
+                                                const T *__restrict__ q1, double *__restrict__ part, int nb)
 {
     __shared__ double red[256];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int lev = blockIdx.y, kind = blockIdx.z, b = blockIdx.x;
-    const double *q = kind == 0 ? q0 : q1;
+    const T *q = kind == 0 ? q0 : q1;
     const size_t chunk = (P + nb - 1) / nb;
     const size_t lo = (size_t) b * chunk, hi = lo + chunk < P ? lo + chunk : P;
-    const double *pl = q + (size_t) lev * P;
-    double acc = 0.0;
-    for (size_t e = lo + threadIdx.x; e < hi; e += 256) acc += pl[e];
+    const T *pl = q + (size_t) lev * P;
+    double acc = 0.0;                             /* FP64 accumulation for either storage */
+    for (size_t e = lo + threadIdx.x; e < hi; e += 256) acc += (double) pl[e];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -621,6 +634,9 @@ __global__ __launch_bounds__(256) void k_gplane(const RunDev R, const double *__
     }
     if (threadIdx.x == 0) part[((size_t) kind * nz + lev) * nb + b] = red[0];
 }
+
+template __global__ void k_gplane<double>(const RunDev, const double *, const double *, double *, int);
+template __global__ void k_gplane<float>(const RunDev, const float *, const float *, double *, int);
 
 /* mass-flux (kind 0, chanfast.c:485-530) and scalar-mean (kind 1,
  * :572-617) constraints: plane averages, their wall ghosts from the
@@ -659,8 +675,8 @@ __global__ __launch_bounds__(256) void k_gmass(const RunDev R, const int s, cons
         nb_ = A[0] - A[-1];
         nt_ = A[nz] - A[nz - 1];
     } else {
-        nb_ = A1 * (A[0] - A[-1]) + A2 * (A[1] - A[-2]);
-        nt_ = A1 * (A[nz] - A[nz - 1]) + A2 * (A[nz + 1] - A[nz - 2]);
+        nb_ = A1d * (A[0] - A[-1]) + A2d * (A[1] - A[-2]);
+        nt_ = A1d * (A[nz] - A[nz - 1]) + A2d * (A[nz + 1] - A[nz - 2]);
     }
     double bot = kap * (nb_ / (0.5 * (dz[0] + dz[-1])));
     double top = kap * (nt_ / (0.5 * (dz[nz] + dz[nz - 1])));
@@ -686,52 +702,73 @@ __global__ __launch_bounds__(256) void k_gmass(const RunDev R, const int s, cons
  * diagonal, factored in the recurrence of heptdag.c:5-61; the U factors
  * go to S (3 per level), the forward sweep in place in Y.  Real and
  * imaginary parts share the factors. */
-__global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab T,
-                                                  double2 *__restrict__ Y, double *__restrict__ S)
+template <typename Ar, typename C>
+static __device__ __forceinline__ void pois4_mode(const Pois4Tab &T, C *__restrict__ y, double *__restrict__ S,
+                                                  size_t M, int nz, double lxd, double lyd, int variant)
+{
+    const Ar lx = (Ar) lxd, ly = (Ar) lyd;
+    const double *bd = T.band[variant];
+    double *Sp = S, *Sq = S + (size_t) nz * M, *Sr = S + (size_t) 2 * nz * M;
+    Ar p1 = 0, p2 = 0, p3 = 0, q1 = 0, q2 = 0, q3 = 0, r1 = 0, r2 = 0, r3 = 0;
+    Ar s1x = 0, s2x = 0, s3x = 0, s1y = 0, s2y = 0, s3y = 0;
+    for (int i = 0; i < nz; i++) {
+        const Ar a = (Ar) bd[i], b = (Ar) bd[nz + i], c = (Ar) bd[2 * nz + i];
+        const Ar d = (Ar) bd[3 * nz + i] - lx * (Ar) T.ex[i] - ly * (Ar) T.ey[i];
+        const Ar e = (Ar) bd[4 * nz + i], f = (Ar) bd[5 * nz + i], g = (Ar) bd[6 * nz + i];
+        const Ar alp = b - a * p3;
+        const Ar bet = c - a * q3 - alp * p2;
+        const Ar den = d - a * r3 - alp * q2 - bet * p1;
+        const Ar iden = (Ar) 1 / den;
+        const Ar pi = (e - alp * r2 - bet * q1) * iden;
+        const Ar qi = (f - bet * r1) * iden;
+        const Ar ri = g * iden;
+        const C rh = y[(size_t) i * M];
+        const Ar dv = (Ar) T.dvol[i];
+        const Ar six = ((Ar) rh.x * dv - a * s3x - alp * s2x - bet * s1x) * iden;
+        const Ar siy = ((Ar) rh.y * dv - a * s3y - alp * s2y - bet * s1y) * iden;
+        C o;
+        o.x = six; o.y = siy;
+        y[(size_t) i * M] = o;
+        Sp[(size_t) i * M] = pi; Sq[(size_t) i * M] = qi; Sr[(size_t) i * M] = ri;
+        p3 = p2; p2 = p1; p1 = pi;
+        q3 = q2; q2 = q1; q1 = qi;
+        r3 = r2; r2 = r1; r1 = ri;
+        s3x = s2x; s2x = s1x; s1x = six;
+        s3y = s2y; s2y = s1y; s1y = siy;
+    }
+    Ar x1x = 0, x2x = 0, x3x = 0, x1y = 0, x2y = 0, x3y = 0;
+    for (int i = nz - 1; i >= 0; i--) {
+        const Ar pi = (Ar) Sp[(size_t) i * M], qi = (Ar) Sq[(size_t) i * M], ri = (Ar) Sr[(size_t) i * M];
+        const C si = y[(size_t) i * M];
+        const Ar xix = (Ar) si.x - pi * x1x - qi * x2x - ri * x3x;
+        const Ar xiy = (Ar) si.y - pi * x1y - qi * x2y - ri * x3y;
+        C o;
+        o.x = xix; o.y = xiy;
+        y[(size_t) i * M] = o;
+        x3x = x2x; x2x = x1x; x1x = xix;
+        x3y = x2y; x2y = x1y; x1y = xiy;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ S)
 {
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const size_t M = (size_t) ny * nkx;
     const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (qm >= M) return;
     const int kx = (int) (qm / ny), ky = (int) (qm % ny);
-    const double lx = R.lamx[kx], ly = R.lamy[ky];
-    const double *bd = T.band[(kx == 0 && ky == 0) ? 1 : 0];
-    double2 *y = Y + qm;
-    double *Sp = S + qm, *Sq = S + (size_t) nz * M + qm, *Sr = S + (size_t) 2 * nz * M + qm;
-    double p1 = 0, p2 = 0, p3 = 0, q1 = 0, q2 = 0, q3 = 0, r1 = 0, r2 = 0, r3 = 0;
-    double2 s1 = make_double2(0, 0), s2 = s1, s3 = s1;
-    for (int i = 0; i < nz; i++) {
-        const double a = bd[i], b = bd[nz + i], c = bd[2 * nz + i];
-        const double d = bd[3 * nz + i] - lx * T.ex[i] - ly * T.ey[i];
-        const double e = bd[4 * nz + i], f = bd[5 * nz + i], g = bd[6 * nz + i];
-        const double alp = b - a * p3;
-        const double bet = c - a * q3 - alp * p2;
-        const double den = d - a * r3 - alp * q2 - bet * p1;
-        const double iden = 1.0 / den;
-        const double pi = (e - alp * r2 - bet * q1) * iden;
-        const double qi = (f - bet * r1) * iden;
-        const double ri = g * iden;
-        const double2 rh = y[(size_t) i * M];
-        const double dv = T.dvol[i];
-        const double2 si = make_double2((rh.x * dv - a * s3.x - alp * s2.x - bet * s1.x) * iden,
-                                        (rh.y * dv - a * s3.y - alp * s2.y - bet * s1.y) * iden);
-        y[(size_t) i * M] = si;
-        Sp[(size_t) i * M] = pi; Sq[(size_t) i * M] = qi; Sr[(size_t) i * M] = ri;
-        p3 = p2; p2 = p1; p1 = pi;
-        q3 = q2; q2 = q1; q1 = qi;
-        r3 = r2; r2 = r1; r1 = ri;
-        s3 = s2; s2 = s1; s1 = si;
-    }
-    double2 x1 = make_double2(0, 0), x2 = x1, x3 = x1;
-    for (int i = nz - 1; i >= 0; i--) {
-        const double pi = Sp[(size_t) i * M], qi = Sq[(size_t) i * M], ri = Sr[(size_t) i * M];
-        const double2 si = y[(size_t) i * M];
-        const double2 xi = make_double2(si.x - pi * x1.x - qi * x2.x - ri * x3.x,
-                                        si.y - pi * x1.y - qi * x2.y - ri * x3.y);
-        y[(size_t) i * M] = xi;
-        x3 = x2; x2 = x1; x1 = xi;
-    }
+    const int variant = (kx == 0 && ky == 0) ? 1 : 0;
+    const bool low = kx <= CF_F64_KLOW && (ky <= CF_F64_KLOW || ny - ky <= CF_F64_KLOW);
+    if (sizeof(T) == sizeof(double) || low)
+        pois4_mode<double>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant);
+    else
+        pois4_mode<T>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant);
 }
+
+template __global__ void k_poisson4<double>(const RunDev, const Pois4Tab, double2 *, double *);
+template __global__ void k_poisson4<float>(const RunDev, const Pois4Tab, float2 *, double *);
 
 /* ------------------------------------------------------------ projection */
 
@@ -742,13 +779,14 @@ __global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab
  * pressure.c:534-632), which w at the first and last interior face uses. */
 /* (us, u), (vs, v), (ws, w) may be the same arrays (in-place projection):
  * each thread reads and writes only its own point of them */
-template <int ORD>
-__global__ __launch_bounds__(256) void k_gproject(const RunDev R, const double *us,
-    const double *vs, const double *ws, const double *cs,
+template <int ORD, typename T>
+__global__ __launch_bounds__(256) void k_gproject(const RunDev R, const T *us,
+    const T *vs, const T *ws, const T *cs,
     const double *__restrict__ h1u, const double *__restrict__ mu, const double *__restrict__ h1c,
-    const double *__restrict__ mc, const double *__restrict__ p, double dtp,
-    double *u, double *v, double *w, double *c)
+    const double *__restrict__ mc, const T *__restrict__ p, double dtp,
+    T *u, T *v, T *w, T *c)
 {
+    const T A1 = (T) A1d, A2 = (T) A2d;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
@@ -757,10 +795,10 @@ __global__ __launch_bounds__(256) void k_gproject(const RunDev R, const double *
     const size_t P = (size_t) nx * ny;
     const size_t row = rowof(k, j, nx, P), q = row + i;
     const double *rdzh = R.rdzh + CF_NGT;
-    const double m = mu ? *mu : 0.0;
-    const double p0 = p[q];
+    const T m = mu ? (T) *mu : (T) 0;
+    const T p0 = p[q];
     const int ip = wrapm(i + 1, nx), jp = wrapm(j + 1, ny);
-    double gx, gy;
+    T gx, gy;
     if (ORD == 2) {
         gx = p[row + ip] - p0;
         gy = p[rowof(k, jp, nx, P) + i] - p0;
@@ -769,25 +807,48 @@ __global__ __launch_bounds__(256) void k_gproject(const RunDev R, const double *
         gx = A1 * (p[row + ip] - p0) + A2 * (p[row + ip2] - p[row + im]);
         gy = A1 * (p[rowof(k, jp, nx, P) + i] - p0) + A2 * (p[rowof(k, jp2, nx, P) + i] - p[rowof(k, jm, nx, P) + i]);
     }
-    u[q] = (us[q] + (h1u ? h1u[k + 1] * m : 0.0)) - gx * (dtp * R.rdx);
-    v[q] = vs[q] - gy * (dtp * R.rdy);
+    u[q] = (us[q] + (h1u ? (T) h1u[k + 1] * m : (T) 0)) - gx * (T) (dtp * R.rdx);
+    v[q] = vs[q] - gy * (T) (dtp * R.rdy);
     if (k < nz - 1) {
-        const double p1 = p[q + P];
-        double gz;
+        const T p1 = p[q + P];
+        T gz;
         if (ORD == 2) gz = p1 - p0;
         else {
-            const double pm = k > 0 ? p[q - P] : p0;
-            const double p2 = k + 2 < nz ? p[q + 2 * P] : p1;
+            const T pm = k > 0 ? p[q - P] : p0;
+            const T p2 = k + 2 < nz ? p[q + 2 * P] : p1;
             gz = A1 * (p1 - p0) + A2 * (p2 - pm);
         }
-        w[q] = ws[q] - gz * (dtp * rdzh[k]);
+        w[q] = ws[q] - gz * (T) (dtp * rdzh[k]);
     } else {
-        w[q] = 0.0;
+        w[q] = (T) 0;
     }
-    if (c) c[q] = cs[q] + (h1c ? h1c[k + 1] * (mc ? *mc : 0.0) : 0.0);
+    if (c) c[q] = cs[q] + (h1c ? (T) h1c[k + 1] * (mc ? (T) *mc : (T) 0) : (T) 0);
 }
 
-k_gproject_t pick_gproject(int order)
+template <typename T> struct GPick {
+    template <int ORD, bool SC> static void fill(GSel<T> &S)
+    {
+        S.u = k_grhs_u<ORD, SC, T>;
+        S.v = k_grhs_v<ORD, SC, T>;
+        S.w = k_grhs_w<ORD, SC, T>;
+        S.c = SC ? k_grhs_c<ORD, true, T> : nullptr;
+        S.proj = k_gproject<ORD, T>;
+    }
+};
+
+template <typename T>
+GSel<T> pick_general(int order, int scalar)
 {
-    return order == 4 ? k_gproject<4> : k_gproject<2>;
+    GSel<T> S;
+    if (order == 4) {
+        if (scalar) GPick<T>::template fill<4, true>(S);
+        else GPick<T>::template fill<4, false>(S);
+    } else {
+        if (scalar) GPick<T>::template fill<2, true>(S);
+        else GPick<T>::template fill<2, false>(S);
+    }
+    return S;
 }
+
+template GSel<double> pick_general<double>(int, int);
+template GSel<float> pick_general<float>(int, int);

@@ -27,6 +27,7 @@ enum {
 #endif
 #define CF_SPIKE_LMAX 32        /* rows per chunk held in registers */
 #define CF_SPIKE_SMAX 8         /* chunks (waves) per workgroup */
+#define CF_F64_KLOW 4           /* FP32 runs solve Poisson modes |kx|, |ky| <= this in FP64 */
 
 /* Partitioned (spike) solve of one constant-coefficient tridiagonal
  * system split into S chunks of L rows: per-chunk Thomas factors, the
@@ -82,6 +83,7 @@ struct RunDev {
     /* FFT plans */
     int radx[CF_MAX_RAD], nradx, rady[CF_MAX_RAD], nrady;
     const double2 *twx, *twy;
+    const float2 *twx32, *twy32;  /* the same twiddles rounded to FP32 */
     const double *lamx, *lamy;  /* modified wavenumbers 2 - 2 cos(2 pi k / n) */
     int fx_pairs;               /* complex x lines per block */
     int fy_cols;                /* kx columns per y-FFT block */
@@ -108,50 +110,53 @@ struct PoisTab {
     double db_neu, db_dir, db_top;
 };
 
-__global__ void k_io_to_dev(const double *io, double *dev, int nx, int ny, int nz);
-__global__ void k_dev_to_io(const double *dev, double *io, int nx, int ny, int nz);
+template <typename T> __global__ void k_io_to_dev(const double *io, T *dev, int nx, int ny, int nz);
+template <typename T> __global__ void k_dev_to_io(const T *dev, double *io, int nx, int ny, int nz);
 __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
                         const double *gu, const double *gv, const double *gw, const double *scal,
                         double *fu, double *fv, double *fw, double *ru, double *rv, double *rw,
                         double *cflpart, double cfldt, int kc);
-__global__ void k_thomas(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
-__global__ void k_spike(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
+template <typename T> __global__ void k_thomas(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
+template <typename T> __global__ void k_spike(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 __global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
                                 double *scal, double *logbuf, int kind);
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);
-typedef void (*k_div_fftx_t)(const RunDev, const double *, const double *, const double *,
-                             const double *, const double *, double2 *);
+/* kernel pointer types of the FFT passes for storage precision T */
+template <typename T> struct KSel {
+    typedef typename Cplx<T>::type C;
+    typedef void (*div_t)(const RunDev, const T *, const T *, const T *, const double *, const double *, C *);
+    typedef void (*ffty_t)(const RunDev, C *, int);
+    typedef void (*fftxi_t)(const RunDev, const C *, T *);
+};
 typedef void (*k_fftx_plain_t)(const RunDev, const double *, double2 *);
-typedef void (*k_ffty_t)(const RunDev, double2 *, int);
-typedef void (*k_fftx_inv_t)(const RunDev, const double2 *, double *);
-k_div_fftx_t pick_div_fftx(int nx, int order);
+template <typename T> typename KSel<T>::div_t pick_div_fftx(int nx, int order);
 k_fftx_plain_t pick_fftx_plain(int nx);
-k_ffty_t pick_ffty(int ny);
-k_fftx_inv_t pick_fftx_inv(int nx);
-__global__ void k_poisson2(const RunDev R, const PoisTab T, double2 *Y, double *G);
+template <typename T> typename KSel<T>::ffty_t pick_ffty(int ny);
+template <typename T> typename KSel<T>::fftxi_t pick_fftx_inv(int nx);
+template <typename T> __global__ void k_poisson2(const RunDev R, const PoisTab T_, typename Cplx<T>::type *Y, double *G);
 __global__ void k_project2(const RunDev R, const double *us, const double *vs, const double *ws,
                            const double *h1, const double *mptr, const double *p, double dtp,
                            double *u, double *v, double *w);
-__global__ void k_maxdiv_part(const RunDev R, const double *u, const double *v, const double *w,
-                              double *part, int nb);
+template <typename T>
+__global__ void k_maxdiv_part(const RunDev R, const T *u, const T *v, const T *w, double *part, int nb);
 
-/* general path (cf_general.hip) */
-typedef void (*k_grhs_t)(const RunDev, const int, const double *, const double *, const double *,
-                         const double *, const double *, const double *, double *, double *, double *, double);
-struct GRhsSet {
-    k_grhs_t u, v, w, c;
+/* general path (cf_general.hip), storage and arithmetic precision T */
+template <typename T> struct GSel {
+    typedef void (*rhs_t)(const RunDev, const int, const T *, const T *, const T *, const T *, const T *,
+                          const double *, T *, T *, double *, double);
+    typedef void (*proj_t)(const RunDev, const T *, const T *, const T *, const T *, const double *,
+                           const double *, const double *, const double *, const T *, double, T *, T *, T *, T *);
+    rhs_t u, v, w, c;
+    proj_t proj;
 };
-GRhsSet pick_grhs(int order, int scalar);
-__global__ void k_gsolve4(const RunDev R, const int s, double *ru, double *rv, double *rw, double *rc);
-__global__ void k_gplane(const RunDev R, const double *q0, const double *q1, double *part, int nb);
+template <typename T> GSel<T> pick_general(int order, int scalar);
+template <typename T> __global__ void k_gsolve4(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
+template <typename T> __global__ void k_gplane(const RunDev R, const T *q0, const T *q1, double *part, int nb);
 __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);
-__global__ void k_poisson4(const RunDev R, const Pois4Tab T, double2 *Y, double *S);
-typedef void (*k_gproject_t)(const RunDev, const double *, const double *, const double *, const double *,
-                             const double *, const double *, const double *, const double *, const double *,
-                             double, double *, double *, double *, double *);
-k_gproject_t pick_gproject(int order);
+template <typename T>
+__global__ void k_poisson4(const RunDev R, const Pois4Tab T_, typename Cplx<T>::type *Y, double *S);
 
 #endif

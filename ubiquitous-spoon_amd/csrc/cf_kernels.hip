@@ -1,6 +1,8 @@
 /*
  * cf_kernels.hip -- HIP kernels of one RK3 substep of the channel DNS on
- * gfx950, FP64.  Reference semantics per kernel are cited beside it; the
+ * gfx950: the FP64 order-2 fast path, and the solves, FFT passes and
+ * transposes the general path shares (templated on the storage type T,
+ * double or float).  Reference semantics per kernel are cited beside it; the
  * device layout is described in cf_dev.h.  Every kernel bounds-checks its
  * indices against the Grid it is given; the host checks launch shapes.
  */
@@ -9,15 +11,17 @@
 
 #include <math.h>
 
-/* ---------------------------------------------------------------------
- * io layout [nx][ny][nz] <-> device layout [nz][ny][nx]: for each y the
- * (x, z) plane is transposed through a 32x33 LDS tile (bit-exact moves).
- * Replaces the pencil redistributions of transpose.c:409-448 on one GPU.
- * ------------------------------------------------------------------- */
 static constexpr double A1_ = 1.125 / 1.0;      /* fourth.c:3-6 */
 static constexpr double A2_ = -0.125 / 3.0;
 
-__global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io, double *__restrict__ dev,
+/* ---------------------------------------------------------------------
+ * io layout [nx][ny][nz] <-> device layout [nz][ny][nx]: for each y the
+ * (x, z) plane is transposed through a 32x33 LDS tile (bit-exact moves;
+ * a float device field is the io double rounded to nearest).
+ * Replaces the pencil redistributions of transpose.c:409-448 on one GPU.
+ * ------------------------------------------------------------------- */
+template <typename T>
+__global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io, T *__restrict__ dev,
                                                    int nx, int ny, int nz)
 {
     __shared__ double tile[32][33];
@@ -30,11 +34,12 @@ __global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, i = i0 + tx;
-        if (i < nx && k < nz) dev[((size_t) k * ny + j) * nx + i] = tile[tx][r];
+        if (i < nx && k < nz) dev[((size_t) k * ny + j) * nx + i] = (T) tile[tx][r];
     }
 }
 
-__global__ __launch_bounds__(256) void k_dev_to_io(const double *__restrict__ dev, double *__restrict__ io,
+template <typename T>
+__global__ __launch_bounds__(256) void k_dev_to_io(const T *__restrict__ dev, double *__restrict__ io,
                                                    int nx, int ny, int nz)
 {
     __shared__ double tile[32][33];
@@ -42,7 +47,7 @@ __global__ __launch_bounds__(256) void k_dev_to_io(const double *__restrict__ de
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, i = i0 + tx;
-        if (i < nx && k < nz) tile[r][tx] = dev[((size_t) k * ny + j) * nx + i];
+        if (i < nx && k < nz) tile[r][tx] = (double) dev[((size_t) k * ny + j) * nx + i];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
@@ -50,6 +55,11 @@ __global__ __launch_bounds__(256) void k_dev_to_io(const double *__restrict__ de
         if (i < nx && k < nz) io[((size_t) i * ny + j) * nz + k] = tile[tx][r];
     }
 }
+
+template __global__ void k_io_to_dev<double>(const double *, double *, int, int, int);
+template __global__ void k_io_to_dev<float>(const double *, float *, int, int, int);
+template __global__ void k_dev_to_io<double>(const double *, double *, int, int, int);
+template __global__ void k_dev_to_io<float>(const float *, double *, int, int, int);
 
 /* ---------------------------------------------------------------------
  * Explicit and implicit-solve right-hand sides of one substep, 2nd order,
@@ -330,55 +340,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
  * Fields: 0 u (nz rows), 1 v (nz rows), 2 w (nz-1 rows; top wall := 0),
  * 3 the scalar c (nz rows, diffusivity nu / Pr).
  * ------------------------------------------------------------------- */
+template <typename T>
 __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ rc)
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const size_t col = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     if (col >= P || f > 3) return;
-    double *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
+    T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
     const int n = f == 2 ? nz - 1 : nz;
     const double *a = R.th + ((size_t) (s * 4 + type) * 3 + 0) * nz;
     const double *ib = R.th + ((size_t) (s * 4 + type) * 3 + 1) * nz;
     const double *gm = R.th + ((size_t) (s * 4 + type) * 3 + 2) * nz;
-    double *xc = x + col;
-    double d = xc[0] * ib[0];
+    T *xc = x + col;
+    T d = xc[0] * (T) ib[0];
     xc[0] = d;
     int k = 1;
     for (; k + 8 <= n; k += 8) {
-        double rr[8];
+        T rr[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) rr[t] = xc[(size_t) (k + t) * P];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            d = (rr[t] - a[k + t] * d) * ib[k + t];
+            d = (rr[t] - (T) a[k + t] * d) * (T) ib[k + t];
             xc[(size_t) (k + t) * P] = d;
         }
     }
     for (; k < n; k++) {
-        d = (xc[(size_t) k * P] - a[k] * d) * ib[k];
+        d = (xc[(size_t) k * P] - (T) a[k] * d) * (T) ib[k];
         xc[(size_t) k * P] = d;
     }
     k = n - 2;
     for (; k - 7 >= 0; k -= 8) {
-        double rr[8];
+        T rr[8];
 #pragma unroll
         for (int t = 0; t < 8; t++) rr[t] = xc[(size_t) (k - t) * P];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            d = rr[t] - gm[k - t + 1] * d;
+            d = rr[t] - (T) gm[k - t + 1] * d;
             xc[(size_t) (k - t) * P] = d;
         }
     }
     for (; k >= 0; k--) {
-        d = xc[(size_t) k * P] - gm[k + 1] * d;
+        d = xc[(size_t) k * P] - (T) gm[k + 1] * d;
         xc[(size_t) k * P] = d;
     }
-    if (f == 2) xc[(size_t) (nz - 1) * P] = 0.0;   /* ghost_fill_top2, diffusew.c:683-688 */
+    if (f == 2) xc[(size_t) (nz - 1) * P] = (T) 0;   /* ghost_fill_top2, diffusew.c:683-688 */
 }
+
+template __global__ void k_thomas<double>(const RunDev, const int, double *, double *, double *, double *);
+template __global__ void k_thomas<float>(const RunDev, const int, float *, float *, float *, float *);
 
 /* ---------------------------------------------------------------------
  * The same constant-coefficient tridiagonal solves as k_thomas, as a
@@ -393,73 +407,77 @@ __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
  * Same system as solve_helmholtz1d_u2 / _w2 (diffuseu.c:173-219,
  * diffusew.c:175-223); differs from sequential Thomas by round-off.
  * ------------------------------------------------------------------- */
+template <typename T>
 __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ rc)
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
 {
-    __shared__ double itf[2 * CF_SPIKE_SMAX][64];
+    __shared__ T itf[2 * CF_SPIKE_SMAX][64];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int f = blockIdx.y;
     if (f > 3) return;
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
-    const SpikeTab T = R.sp[s][type];
+    const SpikeTab ST = R.sp[s][type];
     const int lane = threadIdx.x & 63;
     /* wave-uniform chunk index, so the table reads below are scalar loads */
     const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t col = (size_t) blockIdx.x * 64 + lane;
-    const bool active = col < P && chunk < T.S;
-    double *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
-    const int lo = chunk * T.L;
-    const int hi = min(lo + T.L, T.n);
-    double *xc = x + col;
-    double y[CF_SPIKE_LMAX];
+    const bool active = col < P && chunk < ST.S;
+    T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
+    const int lo = chunk * ST.L;
+    const int hi = min(lo + ST.L, ST.n);
+    T *xc = x + col;
+    T y[CF_SPIKE_LMAX];
 #pragma unroll
     for (int t = 0; t < CF_SPIKE_LMAX; t++) {
         const int j = lo + t;
-        y[t] = (active && j < hi) ? xc[(size_t) j * P] : 0.0;
+        y[t] = (active && j < hi) ? xc[(size_t) j * P] : (T) 0;
     }
-    double d = 0.0;
+    T d = 0;
 #pragma unroll
     for (int t = 0; t < CF_SPIKE_LMAX; t++) {
         const int j = lo + t;
         if (j < hi) {
-            d = (y[t] - T.a[j] * d) * T.ib[j];
+            d = (y[t] - (T) ST.a[j] * d) * (T) ST.ib[j];
             y[t] = d;
         }
     }
-    double yl = 0.0;
+    T yl = 0;
 #pragma unroll
     for (int t = CF_SPIKE_LMAX - 1; t >= 0; t--) {
         const int j = lo + t;
         if (j < hi) {
-            if (j < hi - 1) d = y[t] - T.gam[j + 1] * d;
+            if (j < hi - 1) d = y[t] - (T) ST.gam[j + 1] * d;
             else { d = y[t]; yl = d; }
             y[t] = d;
         }
     }
-    if (chunk < T.S) {
+    if (chunk < ST.S) {
         itf[2 * chunk][lane] = y[0];
         itf[2 * chunk + 1][lane] = yl;
     }
     __syncthreads();
     if (!active) return;
-    const int m = 2 * T.S;
-    double E = 0.0, F = 0.0;
+    const int m = 2 * ST.S;
+    T E = 0, F = 0;
     if (chunk > 0) {
-        const double *row = T.minv + (size_t) (2 * (chunk - 1) + 1) * m;
-        for (int q = 0; q < m; q++) E += row[q] * itf[q][lane];
+        const double *row = ST.minv + (size_t) (2 * (chunk - 1) + 1) * m;
+        for (int q = 0; q < m; q++) E += (T) row[q] * itf[q][lane];
     }
-    if (chunk < T.S - 1) {
-        const double *row = T.minv + (size_t) (2 * (chunk + 1)) * m;
-        for (int q = 0; q < m; q++) F += row[q] * itf[q][lane];
+    if (chunk < ST.S - 1) {
+        const double *row = ST.minv + (size_t) (2 * (chunk + 1)) * m;
+        for (int q = 0; q < m; q++) F += (T) row[q] * itf[q][lane];
     }
 #pragma unroll
     for (int t = 0; t < CF_SPIKE_LMAX; t++) {
         const int j = lo + t;
-        if (j < hi) xc[(size_t) j * P] = y[t] - E * T.vl[j] - F * T.wr[j];
+        if (j < hi) xc[(size_t) j * P] = y[t] - E * (T) ST.vl[j] - F * (T) ST.wr[j];
     }
-    if (f == 2 && chunk == T.S - 1) xc[(size_t) (nz - 1) * P] = 0.0;   /* top wall face */
+    if (f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
 }
+
+template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
+template __global__ void k_spike<float>(const RunDev, const int, float *, float *, float *, float *);
 
 /* ---------------------------------------------------------------------
  * Plane sums for get_xy_avg (get.c:281-307) of the solved u: levels
@@ -508,7 +526,8 @@ __global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int
                                                        int nb, double *__restrict__ scal, double *__restrict__ logbuf,
                                                        int kind)
 {
-    extern __shared__ double avg[];               /* nz + 2 */
+    extern __shared__ __align__(16) unsigned char smem_[];
+    double *avg = (double *) smem_;               /* nz + 2 */
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const double *dz = R.dzt + CF_NGT;
     const double scale = 1.0 / ((double) nx * (double) ny);
@@ -577,26 +596,35 @@ __global__ void k_step_end(double *__restrict__ scal)
  * FFT passes.  Spectral layout X[nz][nx/2+1][ny] (ky fastest), so y lines
  * are contiguous.  x passes pack two real lines (j, j+1 at one level)
  * into one complex transform.  NX / NY > 0 selects a compile-time plan,
- * 0 the runtime one.
+ * 0 the runtime one.  T is the storage and arithmetic precision; C its
+ * complex type; the twiddles come from host libm in FP64 (rounded for T
+ * = float).
  * ------------------------------------------------------------------- */
+
+template <typename C> static __device__ __forceinline__ const C *twsel(const double2 *d, const float2 *f);
+template <> __device__ __forceinline__ const double2 *twsel<double2>(const double2 *d, const float2 *) { return d; }
+template <> __device__ __forceinline__ const float2 *twsel<float2>(const double2 *, const float2 *f) { return f; }
 
 /* unpack two packed real-line spectra and store kx-major: for each kx the
  * block's 2 nl lines are consecutive ky positions (coalesced) */
-static __device__ __forceinline__ void store_half_spectra(const double2 *sb, int ld, int nl, int nx, int nkx,
-                                                          int ny, long NL, long pair0, double sc,
-                                                          double2 *__restrict__ X)
+template <typename C>
+static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, int nl, int nx, int nkx,
+                                                          int ny, long NL, long pair0, double scd,
+                                                          C *__restrict__ X)
 {
+    typedef typename Real<C>::type F;
+    const F sc = (F) scd;
     for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
         const int kx = e / nl, l = e - kx * nl;
         const long la = 2 * (pair0 + l), lb = la + 1;
         if (la >= NL) continue;
-        const double2 Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
+        const C Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
         const long k = la / ny, j = la - k * ny;
-        double2 *dst = X + ((size_t) k * nkx + kx) * ny + j;
-        dst[0] = make_double2((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
+        C *dst = X + ((size_t) k * nkx + kx) * ny + j;
+        dst[0] = mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
         if (lb < NL) {
             const long kb = lb / ny, jb = lb - kb * ny;
-            X[((size_t) kb * nkx + kx) * ny + jb] = make_double2((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
+            X[((size_t) kb * nkx + kx) * ny + jb] = mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
         }
     }
 }
@@ -605,12 +633,14 @@ static __device__ __forceinline__ void store_half_spectra(const double2 *sb, int
  * velocity (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x
  * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
  * mirrored oddly about the (zero) wall faces, as divergence4 sets them. */
-template <int NX, int ORD>
-__global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *__restrict__ us,
-    const double *__restrict__ vs, const double *__restrict__ ws, const double *__restrict__ h1,
-    const double *__restrict__ mptr, double2 *__restrict__ X)
+template <int NX, int ORD, typename T>
+__global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__restrict__ us,
+    const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
+    const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X)
 {
-    extern __shared__ double2 sb[];
+    typedef typename Cplx<T>::type C;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = nx + 1;
@@ -618,36 +648,38 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const double *
     const long NL = (long) ny * nz;
     const size_t P = (size_t) nx * ny;
     const double *rdz = R.rdz + CF_NGT;
-    const double m = mptr ? *mptr : 0.0;
+    const T m = mptr ? (T) *mptr : (T) 0;
+    const T rdx = (T) R.rdx, rdy = (T) R.rdy;
     const long pair0 = (long) xcd_block(blockIdx.x, gridDim.x) * nl;
-    auto divg = [&](long line, int i) -> double {
-        if (line >= NL) return 0.0;
+    auto divg = [&](long line, int i) -> T {
+        if (line >= NL) return (T) 0;
         int k = (int) (line / ny), j = (int) (line - (long) k * ny);
         int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
         size_t q = (size_t) k * P + (size_t) j * nx;
-        double hm = h1 ? h1[k + 1] * m : 0.0;
-        double u1 = us[q + i] + hm, u0 = us[q + im] + hm;
-        double v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
-        double w1 = k < nz - 1 ? ws[q + i] : 0.0;
-        double w0 = k > 0 ? ws[q - P + i] : 0.0;
-        if (ORD == 2) return (u1 - u0) * R.rdx + (v1 - v0) * R.rdy + (w1 - w0) * rdz[k];
+        T hm = h1 ? (T) h1[k + 1] * m : (T) 0;
+        T u1 = us[q + i] + hm, u0 = us[q + im] + hm;
+        T v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
+        T w1 = k < nz - 1 ? ws[q + i] : (T) 0;
+        T w0 = k > 0 ? ws[q - P + i] : (T) 0;
+        if (ORD == 2) return (u1 - u0) * rdx + (v1 - v0) * rdy + (w1 - w0) * (T) rdz[k];
+        const T a1 = (T) A1_, a2 = (T) A2_;
         const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
         const int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
-        double u2 = us[q + ip] + hm, um = us[q + im2] + hm;
-        double v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
+        T u2 = us[q + ip] + hm, um = us[q + im2] + hm;
+        T v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
         /* w at faces k+1 and k-2: walls at -1 and nz-1 are 0, ghosts odd */
-        double w2 = k + 1 <= nz - 2 ? ws[q + P + i] : (k + 1 == nz - 1 ? 0.0 : -ws[q - P + i]);
-        double wm = k - 2 >= 0 ? ws[q - 2 * P + i] : (k - 2 == -1 ? 0.0 : -ws[q + i]);
-        return (A1_ * (u1 - u0) + A2_ * (u2 - um)) * R.rdx + (A1_ * (v1 - v0) + A2_ * (v2 - vm)) * R.rdy
-             + (A1_ * (w1 - w0) + A2_ * (w2 - wm)) * rdz[k];
+        T w2 = k + 1 <= nz - 2 ? ws[q + P + i] : (k + 1 == nz - 1 ? (T) 0 : -ws[q - P + i]);
+        T wm = k - 2 >= 0 ? ws[q - 2 * P + i] : (k - 2 == -1 ? (T) 0 : -ws[q + i]);
+        return (a1 * (u1 - u0) + a2 * (u2 - um)) * rdx + (a1 * (v1 - v0) + a2 * (v2 - vm)) * rdy
+             + (a1 * (w1 - w0) + a2 * (w2 - wm)) * (T) rdz[k];
     };
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long pr = pair0 + l;
-        sb[l * ld + i] = make_double2(divg(2 * pr, i), divg(2 * pr + 1, i));
+        sb[l * ld + i] = mkc<C>(divg(2 * pr, i), divg(2 * pr + 1, i));
     }
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, twsel<C>(R.twx, R.twx32), 1.0);
     store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
 }
 
@@ -656,7 +688,8 @@ template <int NX>
 __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const double *__restrict__ a,
                                                         double2 *__restrict__ X)
 {
-    extern __shared__ double2 sb[];
+    extern __shared__ __align__(16) unsigned char smem_[];
+    double2 *sb = (double2 *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = nx + 1;
@@ -677,26 +710,28 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
 
 /* complex y FFT in place, forward scaled 1/ny (fft.c:186-202) or unscaled
  * inverse (fft.c:205-208); a block transforms TK contiguous lines */
-template <int NY>
-__global__ __launch_bounds__(256) void k_ffty(const RunDev R, double2 *__restrict__ X, int fwd)
+template <int NY, typename T>
+__global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::type *__restrict__ X, int fwd)
 {
-    extern __shared__ double2 sb[];
+    typedef typename Cplx<T>::type C;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
     const int ny = NY > 0 ? NY : R.g.ny;
     const long nlines = (long) R.g.nz * R.g.nkx;
     const int TK = R.fy_cols, ld = ny + 1;
     const long L0 = (long) blockIdx.x * TK;
     const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
-    double2 *base = X + (size_t) L0 * ny;
+    C *base = X + (size_t) L0 * ny;
     for (int e = threadIdx.x; e < TK * ny; e += blockDim.x) {
         int t = e / ny, j = e - t * ny;
-        sb[t * ld + j] = t < nl ? base[e] : make_double2(0.0, 0.0);
+        sb[t * ld + j] = t < nl ? base[e] : mkc<C>((T) 0, (T) 0);
     }
     __syncthreads();
-    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, R.twy, fwd ? 1.0 : -1.0);
-    const double sc = fwd ? 1.0 / (double) ny : 1.0;
+    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, twsel<C>(R.twy, R.twy32), fwd ? 1.0 : -1.0);
+    const T sc = fwd ? (T) (1.0 / (double) ny) : (T) 1;
     for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
         int t = e / ny, j = e - t * ny;
-        double2 val = sb[t * ld + j];
+        C val = sb[t * ld + j];
         base[e] = fwd ? cscale(val, sc) : val;
     }
 }
@@ -704,11 +739,13 @@ __global__ __launch_bounds__(256) void k_ffty(const RunDev R, double2 *__restric
 /* inverse complex-to-real x FFT (fft.c:180-183; FFTW c2r semantics: the
  * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two real
  * lines per complex transform, writing p */
-template <int NX>
-__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 *__restrict__ X,
-                                                  double *__restrict__ p)
+template <int NX, typename T>
+__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const typename Cplx<T>::type *__restrict__ X,
+                                                  T *__restrict__ p)
 {
-    extern __shared__ double2 sb[];
+    typedef typename Cplx<T>::type C;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = nx + 1;
@@ -719,7 +756,7 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 
     for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
         const int kx = e / nl, l = e - kx * nl;
         const long la = 2 * (pair0 + l), lb = la + 1;
-        double2 A = make_double2(0.0, 0.0), B = make_double2(0.0, 0.0);
+        C A = mkc<C>((T) 0, (T) 0), B = A;
         if (la < NL) {
             const long k = la / ny, j = la - k * ny;
             A = X[((size_t) k * nkx + kx) * ny + j];
@@ -728,16 +765,16 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 
             const long k = lb / ny, j = lb - k * ny;
             B = X[((size_t) k * nkx + kx) * ny + j];
         }
-        if (kx == 0 || (nx % 2 == 0 && kx == half)) { A.y = 0.0; B.y = 0.0; }
-        sb[l * ld + kx] = make_double2(A.x - B.y, A.y + B.x);
-        if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = make_double2(A.x + B.y, B.x - A.y);
+        if (kx == 0 || (nx % 2 == 0 && kx == half)) { A.y = (T) 0; B.y = (T) 0; }
+        sb[l * ld + kx] = mkc<C>(A.x - B.y, A.y + B.x);
+        if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = mkc<C>(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, -1.0);
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, twsel<C>(R.twx, R.twx32), -1.0);
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long la = 2 * (pair0 + l), lb = la + 1;
-        double2 z = sb[l * ld + i];
+        C z = sb[l * ld + i];
         if (la < NL) p[(size_t) la * nx + i] = z.x;
         if (lb < NL) p[(size_t) lb * nx + i] = z.y;
     }
@@ -747,22 +784,21 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const double2 
 #define CF_FFT_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) \
     X(120) X(128) X(160) X(192) X(240) X(256) X(320) X(384) X(480) X(512)
 
-k_div_fftx_t pick_div_fftx(int n, int order)
+template <typename T, int ORD>
+static typename KSel<T>::div_t pick_div_ord(int n)
 {
-    if (order == 4) {
-        switch (n) {
-#define X_(N) case N: return k_div_fftx<N, 4>;
-        CF_FFT_SIZES(X_)
-#undef X_
-        default: return k_div_fftx<0, 4>;
-        }
-    }
     switch (n) {
-#define X_(N) case N: return k_div_fftx<N, 2>;
+#define X_(N) case N: return k_div_fftx<N, ORD, T>;
     CF_FFT_SIZES(X_)
 #undef X_
-    default: return k_div_fftx<0, 2>;
+    default: return k_div_fftx<0, ORD, T>;
     }
+}
+
+template <typename T>
+typename KSel<T>::div_t pick_div_fftx(int n, int order)
+{
+    return order == 4 ? pick_div_ord<T, 4>(n) : pick_div_ord<T, 2>(n);
 }
 
 k_fftx_plain_t pick_fftx_plain(int n)
@@ -775,25 +811,34 @@ k_fftx_plain_t pick_fftx_plain(int n)
     }
 }
 
-k_ffty_t pick_ffty(int n)
+template <typename T>
+typename KSel<T>::ffty_t pick_ffty(int n)
 {
     switch (n) {
-#define X_(N) case N: return k_ffty<N>;
+#define X_(N) case N: return k_ffty<N, T>;
     CF_FFT_SIZES(X_)
 #undef X_
-    default: return k_ffty<0>;
+    default: return k_ffty<0, T>;
     }
 }
 
-k_fftx_inv_t pick_fftx_inv(int n)
+template <typename T>
+typename KSel<T>::fftxi_t pick_fftx_inv(int n)
 {
     switch (n) {
-#define X_(N) case N: return k_fftx_inv<N>;
+#define X_(N) case N: return k_fftx_inv<N, T>;
     CF_FFT_SIZES(X_)
 #undef X_
-    default: return k_fftx_inv<0>;
+    default: return k_fftx_inv<0, T>;
     }
 }
+
+template KSel<double>::div_t pick_div_fftx<double>(int, int);
+template KSel<float>::div_t pick_div_fftx<float>(int, int);
+template KSel<double>::ffty_t pick_ffty<double>(int);
+template KSel<float>::ffty_t pick_ffty<float>(int);
+template KSel<double>::fftxi_t pick_fftx_inv<double>(int);
+template KSel<float>::fftxi_t pick_fftx_inv<float>(int);
 
 /* ---------------------------------------------------------------------
  * pressure_poisson2 (pressure.c:293-355): one thread per (kx, ky) mode,
@@ -801,24 +846,21 @@ k_fftx_inv_t pick_fftx_inv(int n)
  * symmetrisation, Neumann walls except the (0,0) mode whose bottom row
  * is the Dirichlet-mean closure; real and imaginary parts share the
  * matrix.  Thomas with the gam factors kept in G for the back sweep.
+ * Arithmetic in A: FP64 always for T = double; for T = float the modes
+ * with |kx|, |ky| <= CF_F64_KLOW are still solved in FP64 (the mean
+ * pressure and the large-scale modes carry the wall-normal pressure
+ * balance), the rest in FP32.
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T,
-                                                  double2 *__restrict__ Y, double *__restrict__ G)
+template <typename A, typename C>
+static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restrict__ y, double *__restrict__ g,
+                                                  size_t M, int nz, double lxd, double lyd, bool zero)
 {
-    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
-    const size_t M = (size_t) ny * nkx;
-    const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (qm >= M) return;
-    const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
-    const double lx = R.lamx[kx], ly = R.lamy[ky];
-    const bool zero = kx == 0 && ky == 0;
-    double2 *y = Y + qm;
-    double *g = G + qm;
     constexpr int B = 8;
-    double ibet = 0.0, cprev = 0.0;
-    double2 d = make_double2(0.0, 0.0);
+    const A lx = (A) lxd, ly = (A) lyd;
+    A ibet = 0, cprev = 0;
+    A dx_ = 0, dy_ = 0;
     for (int k0 = 0; k0 < nz; k0 += B) {
-        double2 rb[B];
+        C rb[B];
 #pragma unroll
         for (int t = 0; t < B; t++)
             if (k0 + t < nz) rb[t] = y[(size_t) (k0 + t) * M];
@@ -826,26 +868,29 @@ __global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab 
         for (int t = 0; t < B; t++) {
             const int k = k0 + t;
             if (k < nz) {
-                double b = T.bz[k] - lx * T.ex[k] - ly * T.ey[k];
-                if (k == 0) b -= zero ? T.db_dir : T.db_neu;
-                if (k == nz - 1) b -= T.db_top;
-                double gam = 0.0, bet = b;
+                A b = (A) T.bz[k] - lx * (A) T.ex[k] - ly * (A) T.ey[k];
+                if (k == 0) b -= (A) (zero ? T.db_dir : T.db_neu);
+                if (k == nz - 1) b -= (A) T.db_top;
+                A gam = 0, bet = b;
                 if (k > 0) {
                     gam = cprev * ibet;              /* c_{k-1} / bet_{k-1} */
-                    bet = b - T.a[k] * gam;
-                    g[(size_t) k * M] = gam;
+                    bet = b - (A) T.a[k] * gam;
+                    g[(size_t) k * M] = (double) gam;
                 }
-                ibet = 1.0 / bet;
-                const double dv = T.dvol[k];
-                const double ak = k > 0 ? T.a[k] : 0.0;
-                d = make_double2((rb[t].x * dv - ak * d.x) * ibet, (rb[t].y * dv - ak * d.y) * ibet);
-                y[(size_t) k * M] = d;
-                cprev = T.c[k];
+                ibet = (A) 1 / bet;
+                const A dv = (A) T.dvol[k];
+                const A ak = k > 0 ? (A) T.a[k] : (A) 0;
+                dx_ = ((A) rb[t].x * dv - ak * dx_) * ibet;
+                dy_ = ((A) rb[t].y * dv - ak * dy_) * ibet;
+                C o;
+                o.x = dx_; o.y = dy_;
+                y[(size_t) k * M] = o;
+                cprev = (A) T.c[k];
             }
         }
     }
     for (int k0 = nz - 2; k0 >= 0; k0 -= B) {
-        double2 yb[B];
+        C yb[B];
         double gb[B];
 #pragma unroll
         for (int t = 0; t < B; t++)
@@ -854,12 +899,35 @@ __global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab 
         for (int t = 0; t < B; t++) {
             const int k = k0 - t;
             if (k >= 0) {
-                d = make_double2(yb[t].x - gb[t] * d.x, yb[t].y - gb[t] * d.y);
-                y[(size_t) k * M] = d;
+                dx_ = (A) yb[t].x - (A) gb[t] * dx_;
+                dy_ = (A) yb[t].y - (A) gb[t] * dy_;
+                C o;
+                o.x = dx_; o.y = dy_;
+                y[(size_t) k * M] = o;
             }
         }
     }
 }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
+{
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const size_t M = (size_t) ny * nkx;
+    const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (qm >= M) return;
+    const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
+    const bool zero = kx == 0 && ky == 0;
+    const bool low = kx <= CF_F64_KLOW && (ky <= CF_F64_KLOW || ny - ky <= CF_F64_KLOW);
+    if (sizeof(T) == sizeof(double) || low)
+        pois2_mode<double>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
+    else
+        pois2_mode<T>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
+}
+
+template __global__ void k_poisson2<double>(const RunDev, const PoisTab, double2 *, double *);
+template __global__ void k_poisson2<float>(const RunDev, const PoisTab, float2 *, double *);
 
 /* ---------------------------------------------------------------------
  * project2 (pressure.c:120-152) fused with the mass-flux correction:
@@ -893,9 +961,12 @@ __global__ __launch_bounds__(256) void k_project2(const RunDev R, const double *
     }
 }
 
-/* max |div| over interior cells (case1 check, cases.c:473-486) */
-__global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const double *__restrict__ u,
-    const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ part, int nb)
+/* max |div| over interior cells (case1 check, cases.c:473-486), the
+ * divergence of the order in force, evaluated in FP64 whatever the
+ * storage precision */
+template <typename T>
+__global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const T *__restrict__ u,
+    const T *__restrict__ v, const T *__restrict__ w, double *__restrict__ part, int nb)
 {
     __shared__ double red[256];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
@@ -908,20 +979,23 @@ __global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const doubl
         int j = rem / nx, i = rem - j * nx;
         int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
         size_t row = (size_t) k * P + (size_t) j * nx;
-        double w1 = k < nz - 1 ? w[e] : 0.0;
-        double w0 = k > 0 ? w[e - P] : 0.0;
+        double w1 = k < nz - 1 ? (double) w[e] : 0.0;
+        double w0 = k > 0 ? (double) w[e - P] : 0.0;
         double d;
         if (R.g.order == 2) {
-            d = (u[e] - u[row + im]) / R.g.dx + (v[e] - v[(size_t) k * P + (size_t) jm * nx + i]) / R.g.dy
+            d = ((double) u[e] - (double) u[row + im]) / R.g.dx
+              + ((double) v[e] - (double) v[(size_t) k * P + (size_t) jm * nx + i]) / R.g.dy
               + (w1 - w0) / dz[k];
         } else {                                   /* divergence4, pressure.c:63-118 */
             int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
             int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
-            double w2 = k + 1 <= nz - 2 ? w[e + P] : (k + 1 == nz - 1 ? 0.0 : -w[e - P]);
-            double wm = k - 2 >= 0 ? w[e - 2 * P] : (k - 2 == -1 ? 0.0 : -w[e]);
-            d = (A1_ * (u[e] - u[row + im]) + A2_ * (u[row + ip] - u[row + im2])) / R.g.dx
-              + (A1_ * (v[e] - v[(size_t) k * P + (size_t) jm * nx + i])
-                 + A2_ * (v[(size_t) k * P + (size_t) jp * nx + i] - v[(size_t) k * P + (size_t) jm2 * nx + i])) / R.g.dy
+            double w2 = k + 1 <= nz - 2 ? (double) w[e + P] : (k + 1 == nz - 1 ? 0.0 : -(double) w[e - P]);
+            double wm = k - 2 >= 0 ? (double) w[e - 2 * P] : (k - 2 == -1 ? 0.0 : -(double) w[e]);
+            double um = u[row + im], u2 = u[row + ip], um2 = u[row + im2];
+            double vm = v[(size_t) k * P + (size_t) jm * nx + i], v2 = v[(size_t) k * P + (size_t) jp * nx + i];
+            double vm2 = v[(size_t) k * P + (size_t) jm2 * nx + i];
+            d = (A1_ * ((double) u[e] - um) + A2_ * (u2 - um2)) / R.g.dx
+              + (A1_ * ((double) v[e] - vm) + A2_ * (v2 - vm2)) / R.g.dy
               + (A1_ * (w1 - w0) + A2_ * (w2 - wm)) / dz[k];
         }
         mx = fmax(mx, fabs(d));
@@ -934,3 +1008,6 @@ __global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const doubl
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
+
+template __global__ void k_maxdiv_part<double>(const RunDev, const double *, const double *, const double *, double *, int);
+template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const float *, const float *, double *, int);

@@ -89,8 +89,10 @@ struct cf_run {
     /* general path: order 4 and/or the scalar (cf_general.hip) */
     bool general = false;
     int nfield = 3;
-    GRhsSet grhs;
-    k_gproject_t kproj = nullptr;
+    int prec = CF_FP64;
+    size_t esz = sizeof(double);     /* bytes per stored value of the state and work fields */
+    GSel<double> g64;
+    GSel<float> g32;
     int gblocks = 0;
     double pt_dt = -1.0;
     size_t off_pois_adhoc = 0;
@@ -104,10 +106,13 @@ struct cf_run {
     int steps_done = 0;
     bool use_spike = false;
     int spike_waves = 1;
-    k_div_fftx_t kdiv = nullptr;
+    KSel<double>::div_t kdiv = nullptr;
     k_fftx_plain_t kfxp = nullptr;
-    k_ffty_t kfy = nullptr;
-    k_fftx_inv_t kfxi = nullptr;
+    KSel<double>::ffty_t kfy = nullptr;
+    KSel<double>::fftxi_t kfxi = nullptr;
+    KSel<float>::div_t kdiv32 = nullptr;
+    KSel<float>::ffty_t kfy32 = nullptr;
+    KSel<float>::fftxi_t kfxi32 = nullptr;
     /* profiling */
     bool profile = false;
     std::map<std::string, std::pair<double, int>> ktime;
@@ -366,7 +371,7 @@ static cf_status validate(const cf_config *c)
         return CF_ERR_ARG;
     }
     if (c->scalar && (c->nx < 4 || c->ny < 4)) { cf_set_error("the scalar needs nx, ny >= 4"); return CF_ERR_ARG; }
-    if (c->precision != CF_FP64) { cf_set_error("only FP64 is built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->precision != CF_FP64 && c->precision != CF_FP32) { cf_set_error("precision must be CF_FP64 or CF_FP32"); return CF_ERR_ARG; }
     if (c->cflmode != 0) { cf_set_error("cflmode 1 (adaptive dt) is not built in this library version"); return CF_ERR_UNSUPPORTED; }
     int rad[CF_MAX_RAD];
     if (c->nx % 1 || factor_radix(c->nx, rad) < 0 || c->nx > MAXE_LINE) {
@@ -388,9 +393,11 @@ extern "C" cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes)
     size_t N = (size_t) cfg->nx * cfg->ny * cfg->nz;
     size_t M = (size_t) (cfg->nx / 2 + 1) * cfg->ny * cfg->nz;
     size_t nf = cfg->scalar ? 4 : 3;
-    /* state, solve output and two forcing slots per field, p, stage: (4 nf + 2) N;
-     * X: 2 M; G: M; order-4 Poisson factors: 3 M */
-    *bytes = sizeof(double) * ((4 * nf + 2) * N + (cfg->order == 4 ? 6 : 3) * M) + ((size_t) 16 << 20);
+    size_t es = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
+    /* state, solve output and two forcing slots per field, p: (4 nf + 1) N values;
+     * X: 2 M values; FP64 stage N, G M, order-4 Poisson factors 3 M */
+    *bytes = es * ((4 * nf + 1) * N + 2 * M) + sizeof(double) * (N + (cfg->order == 4 ? 4 : 1) * M)
+           + ((size_t) 16 << 20);
     return CF_OK;
 }
 
@@ -516,7 +523,9 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
 
     /* ---- general path: closures, boundary lifts, order-4 systems ---- */
     const int order = cfg->order;
-    r->general = order == 4 || cfg->scalar;
+    r->prec = cfg->precision;
+    r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
+    r->general = order == 4 || cfg->scalar || cfg->precision == CF_FP32;
     r->nfield = cfg->scalar ? 4 : 3;
     Clos clo;
     if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
@@ -634,6 +643,16 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     if (pack.size() % 2) pack.push_back(0.0);       /* 16-byte align the complex twiddles */
     size_t o_twx = put(twx.data(), twx.size());
     size_t o_twy = put(twy.data(), twy.size());
+    /* FP32 copies of the twiddles, two floats per double slot */
+    auto put32 = [&](const std::vector<double> &src) {
+        std::vector<float> f(src.begin(), src.end());
+        if (f.size() % 2) f.push_back(0.f);
+        std::vector<double> slots(f.size() / 2);
+        memcpy(slots.data(), f.data(), sizeof(float) * f.size());
+        return put(slots.data(), slots.size());
+    };
+    size_t o_twx32 = put32(twx);
+    size_t o_twy32 = put32(twy);
     size_t o_lx = put(lamx.data(), lamx.size());
     size_t o_ly = put(lamy.data(), lamy.size());
     size_t o_p[4][6];
@@ -670,14 +689,16 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
 
     /* ---- device memory ---- */
     auto dalloc = [&](double **pp, size_t n) -> bool { return hipMalloc((void **) pp, sizeof(double) * (n ? n : 1)) == hipSuccess; };
+    /* state and work fields in the run's storage precision */
+    auto falloc = [&](double **pp, size_t n) -> bool { return hipMalloc((void **) pp, r->esz * (n ? n : 1)) == hipSuccess; };
     bool ok = true;
-    ok = ok && dalloc(&r->u, r->N) && dalloc(&r->v, r->N) && dalloc(&r->w, r->N);
-    ok = ok && dalloc(&r->ru, r->N) && dalloc(&r->rv, r->N) && dalloc(&r->rw, r->N);
-    for (int f = 0; f < 3; f++) ok = ok && dalloc(&r->fa[f], r->N) && dalloc(&r->fb[f], r->N);
-    if (cfg->scalar) ok = ok && dalloc(&r->c, r->N) && dalloc(&r->rc, r->N) && dalloc(&r->fa[3], r->N) && dalloc(&r->fb[3], r->N);
+    ok = ok && falloc(&r->u, r->N) && falloc(&r->v, r->N) && falloc(&r->w, r->N);
+    ok = ok && falloc(&r->ru, r->N) && falloc(&r->rv, r->N) && falloc(&r->rw, r->N);
+    for (int f = 0; f < 3; f++) ok = ok && falloc(&r->fa[f], r->N) && falloc(&r->fb[f], r->N);
+    if (cfg->scalar) ok = ok && falloc(&r->c, r->N) && falloc(&r->rc, r->N) && falloc(&r->fa[3], r->N) && falloc(&r->fb[3], r->N);
     if (order == 4) ok = ok && dalloc(&r->S4, 3 * r->M * nz);
-    ok = ok && dalloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * nz);
-    ok = ok && hipMalloc((void **) &r->X, sizeof(double2) * r->M * nz) == hipSuccess;
+    ok = ok && falloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * nz);
+    ok = ok && hipMalloc((void **) &r->X, 2 * r->esz * r->M * nz) == hipSuccess;
     ok = ok && dalloc(&r->tab, r->tab_n) && dalloc(&r->scal, SC_N) && dalloc(&r->logbuf, (size_t) LOGCAP * CF_LOG_N);
     r->nb_plane = (int) ((r->P + 2047) / 2048);
     if (r->nb_plane > 1024) r->nb_plane = 1024;
@@ -705,7 +726,7 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     bool zok = hipMemcpy(r->scal, zeros.data(), sizeof(double) * SC_N, hipMemcpyHostToDevice) == hipSuccess;
     double *zb[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                      r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p };
-    for (double *b : zb) if (b) zok = zok && hipMemset(b, 0, sizeof(double) * r->N) == hipSuccess;
+    for (double *b : zb) if (b) zok = zok && hipMemset(b, 0, r->esz * r->N) == hipSuccess;
     zok = zok && hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N) == hipSuccess;
     if (!zok || hipDeviceSynchronize() != hipSuccess) { free_run(r); cf_set_error("device init failed"); return CF_ERR_DEVICE; }
 
@@ -721,9 +742,10 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     R.th = r->tab + o_th;
     R.rdz = r->tab + o_rdz;
     R.rdzh = r->tab + o_rdzh;
-    r->kdiv = pick_div_fftx(nx, order);
-    r->grhs = pick_grhs(order, cfg->scalar);
-    r->kproj = pick_gproject(order);
+    r->kdiv = pick_div_fftx<double>(nx, order);
+    r->kdiv32 = pick_div_fftx<float>(nx, order);
+    r->g64 = pick_general<double>(order, cfg->scalar);
+    r->g32 = pick_general<float>(order, cfg->scalar);
     R.cl = clo;
     R.scalar = cfg->scalar;
     R.betg[0] = cfg->betg_x; R.betg[1] = cfg->betg_y; R.betg[2] = cfg->betg_z;
@@ -735,8 +757,10 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
             R.hp[s][t].t = order == 4 ? r->tab + o_hp[s][t] : nullptr;
         }
     r->kfxp = pick_fftx_plain(nx);
-    r->kfy = pick_ffty(ny);
-    r->kfxi = pick_fftx_inv(nx);
+    r->kfy = pick_ffty<double>(ny);
+    r->kfxi = pick_fftx_inv<double>(nx);
+    r->kfy32 = pick_ffty<float>(ny);
+    r->kfxi32 = pick_fftx_inv<float>(nx);
     R.rdx = 1.0 / r->g.dx;
     R.rdy = 1.0 / r->g.dy;
     R.rdxdy = 1.0 / (r->g.dx * r->g.dy);
@@ -757,6 +781,8 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     R.logcap = LOGCAP;
     R.twx = (const double2 *) (r->tab + o_twx);
     R.twy = (const double2 *) (r->tab + o_twy);
+    R.twx32 = (const float2 *) (r->tab + o_twx32);
+    R.twy32 = (const float2 *) (r->tab + o_twy32);
     R.lamx = r->tab + o_lx;
     R.lamy = r->tab + o_ly;
     R.fx_pairs = MAXE_LINE / nx > 0 ? MAXE_LINE / nx : 1;
@@ -818,8 +844,25 @@ static cf_status ev_end(cf_run *r)
 
 static dim3 cell_grid(const cf_run *r) { return dim3((r->g.nx + 63) / 64, (r->g.ny + 3) / 4, r->g.nz); }
 
-static size_t fx_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fx_pairs * (r->g.nx + 1); }
-static size_t fy_lds(const cf_run *r) { return sizeof(double2) * (size_t) r->rd.fy_cols * (r->g.ny + 1); }
+static size_t fx_lds(const cf_run *r) { return 2 * r->esz * (size_t) r->rd.fx_pairs * (r->g.nx + 1); }
+static size_t fy_lds(const cf_run *r) { return 2 * r->esz * (size_t) r->rd.fy_cols * (r->g.ny + 1); }
+
+/* the kernel sets and typed views of the run's buffers for precision T */
+template <typename T> struct Prec;
+template <> struct Prec<double> {
+    static KSel<double>::div_t div(const cf_run *r) { return r->kdiv; }
+    static KSel<double>::ffty_t ffty(const cf_run *r) { return r->kfy; }
+    static KSel<double>::fftxi_t fftxi(const cf_run *r) { return r->kfxi; }
+    static const GSel<double> &gen(const cf_run *r) { return r->g64; }
+};
+template <> struct Prec<float> {
+    static KSel<float>::div_t div(const cf_run *r) { return r->kdiv32; }
+    static KSel<float>::ffty_t ffty(const cf_run *r) { return r->kfy32; }
+    static KSel<float>::fftxi_t fftxi(const cf_run *r) { return r->kfxi32; }
+    static const GSel<float> &gen(const cf_run *r) { return r->g32; }
+};
+template <typename T> static T *tv(double *p) { return reinterpret_cast<T *>(p); }
+template <typename T> static typename Cplx<T>::type *tx(double2 *p) { return reinterpret_cast<typename Cplx<T>::type *>(p); }
 static unsigned fy_blocks(const cf_run *r)
 {
     long lines = (long) r->g.nz * r->g.nkx;
@@ -834,22 +877,25 @@ static void state_set(cf_run *r, int q, double **u, double **v, double **w)
 
 /* divergence -> FFT -> Poisson -> inverse FFT into p (pressure.c:40-61,
  * chanfast.c:639-651) */
-static cf_status enqueue_pressure(cf_run *r, const double *us, const double *vs, const double *ws,
+template <typename T>
+static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *ws,
                                   const double *h1, const double *mptr, int tq)
 {
+    typedef typename Cplx<T>::type C;
+    C *X = tx<T>(r->X);
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz;
     long npairs = (NL + 1) / 2;
     unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    KLAUNCH("div_fftx", r->kdiv, dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, r->X);
+    KLAUNCH("div_fftx", Prec<T>::div(r), dim3(gx), dim3(256), fx_lds(r), R, us, vs, ws, h1, mptr, X);
     dim3 gy(fy_blocks(r));
-    KLAUNCH("ffty_fwd", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 1);
+    KLAUNCH("ffty_fwd", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 1);
     if (r->g.order == 4)
-        KLAUNCH("poisson", k_poisson4, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], r->X, r->S4);
+        KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], X, r->S4);
     else
-        KLAUNCH("poisson", k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt[tq], r->X, r->G);
-    KLAUNCH("ffty_inv", r->kfy, gy, dim3(256), fy_lds(r), R, r->X, 0);
-    KLAUNCH("fftx_inv", r->kfxi, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) r->X, r->p);
+        KLAUNCH("poisson", k_poisson2<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt[tq], X, r->G);
+    KLAUNCH("ffty_inv", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 0);
+    KLAUNCH("fftx_inv", Prec<T>::fftxi(r), dim3(gx), dim3(256), fx_lds(r), R, (const C *) X, tv<T>(r->p));
     return CF_OK;
 }
 
@@ -901,73 +947,79 @@ static cf_status enqueue_step(cf_run *r, int from)
             KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart,
                     r->rhs_blocks, dt, r->scal, r->logbuf);
         if (r->use_spike)
-            KLAUNCH("spike", k_spike, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
+            KLAUNCH("spike", k_spike<double>, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
                     ou, ov, ow, (double *) nullptr);
         else
-            KLAUNCH("thomas", k_thomas, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
+            KLAUNCH("thomas", k_thomas<double>, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
                     (double *) nullptr);
         KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
                 (const double *) ou, R.ubc, r->part, r->nb_plane);
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
                 (const double *) r->part, r->nb_plane, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
-        cf_status st = enqueue_pressure(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
+        cf_status st = enqueue_pressure<double>(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
     }
     KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
     return CF_OK;
 }
 
-/* one RK3 step of the general path (order 4 and/or the scalar): the state
- * (u, v, w, c) is materialised after every substep */
-static cf_status enqueue_step_general(cf_run *r)
+/* one RK3 step of the general path (order 4, the scalar, or FP32 storage):
+ * the state (u, v, w, c) is materialised after every substep */
+template <typename T>
+static cf_status enqueue_step_general_t(cf_run *r)
 {
     const RunDev &R = r->rd;
     const int nz = r->g.nz;
     const double dt = r->cfg.dt;
     const bool sc = r->cfg.scalar != 0;
     const dim3 cg = cell_grid(r), cb(64, 4);
+    const GSel<T> &K = Prec<T>::gen(r);
+    T *u = tv<T>(r->u), *v = tv<T>(r->v), *w = tv<T>(r->w), *c = tv<T>(r->c);
+    T *ru = tv<T>(r->ru), *rv = tv<T>(r->rv), *rw = tv<T>(r->rw), *rc = tv<T>(r->rc);
     for (int s = 0; s < 3; s++) {
         double **fnew = (s % 2 == 0) ? r->fa : r->fb;
         double **gold = (s % 2 == 0) ? r->fb : r->fa;
-        const double *u = r->u, *v = r->v, *w = r->w, *c = r->c;
-        KLAUNCH("rhs", r->grhs.u, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[0], (const double *) r->scal,
-                fnew[0], r->ru, s == 0 ? r->cflpart : (double *) nullptr, dt);
+        KLAUNCH("rhs", K.u, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[0]), (const double *) r->scal,
+                tv<T>(fnew[0]), ru, s == 0 ? r->cflpart : (double *) nullptr, dt);
         if (s == 0)
             KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart, r->gblocks, dt,
                     r->scal, r->logbuf);
-        KLAUNCH("rhs", r->grhs.v, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[1], (const double *) r->scal,
-                fnew[1], r->rv, (double *) nullptr, dt);
-        KLAUNCH("rhs", r->grhs.w, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[2], (const double *) r->scal,
-                fnew[2], r->rw, (double *) nullptr, dt);
+        KLAUNCH("rhs", K.v, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[1]), (const double *) r->scal,
+                tv<T>(fnew[1]), rv, (double *) nullptr, dt);
+        KLAUNCH("rhs", K.w, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[2]), (const double *) r->scal,
+                tv<T>(fnew[2]), rw, (double *) nullptr, dt);
         if (sc)
-            KLAUNCH("rhs", r->grhs.c, cg, cb, 0, R, s, u, v, w, c, (const double *) gold[3], (const double *) r->scal,
-                    fnew[3], r->rc, (double *) nullptr, dt);
+            KLAUNCH("rhs", K.c, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[3]), (const double *) r->scal,
+                    tv<T>(fnew[3]), rc, (double *) nullptr, dt);
         const unsigned nf = (unsigned) r->nfield;
         if (r->g.order == 4)
-            KLAUNCH("solve", k_gsolve4, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, r->ru, r->rv,
-                    r->rw, r->rc);
+            KLAUNCH("solve", k_gsolve4<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
         else if (r->use_spike)
-            KLAUNCH("solve", k_spike, dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
-                    r->ru, r->rv, r->rw, r->rc);
+            KLAUNCH("solve", k_spike<T>, dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
+                    ru, rv, rw, rc);
         else
-            KLAUNCH("solve", k_thomas, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, r->ru, r->rv,
-                    r->rw, r->rc);
-        KLAUNCH("plane_sum", k_gplane, dim3(r->nb_plane, nz, sc ? 2 : 1), dim3(256), 0, R, (const double *) r->ru,
-                (const double *) r->rc, r->part, r->nb_plane);
+            KLAUNCH("solve", k_thomas<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
+        KLAUNCH("plane_sum", k_gplane<T>, dim3(r->nb_plane, nz, sc ? 2 : 1), dim3(256), 0, R, (const T *) ru,
+                (const T *) rc, r->part, r->nb_plane);
         KLAUNCH("mass", k_gmass, dim3(sc ? 2 : 1), dim3(256), sizeof(double) * (nz + 6), R, s,
                 (const double *) r->part, r->nb_plane, r->scal, r->logbuf);
         const double *h1u = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
         const double *h1c = R.h1 + (size_t) (s * 2 + 1) * (nz + 2);
-        cf_status st = enqueue_pressure(r, r->ru, r->rv, r->rw, h1u, r->scal + SC_M0 + s, s);
+        cf_status st = enqueue_pressure<T>(r, ru, rv, rw, h1u, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
-        KLAUNCH("project", r->kproj, cg, cb, 0, R, (const double *) r->ru, (const double *) r->rv,
-                (const double *) r->rw, (const double *) r->rc, h1u, (const double *) (r->scal + SC_M0 + s),
-                sc ? h1c : (const double *) nullptr, (const double *) (r->scal + SC_MC0 + s), (const double *) r->p,
-                R.sc[s].abdt, r->u, r->v, r->w, sc ? r->c : (double *) nullptr);
+        KLAUNCH("project", K.proj, cg, cb, 0, R, (const T *) ru, (const T *) rv, (const T *) rw, (const T *) rc, h1u,
+                (const double *) (r->scal + SC_M0 + s), sc ? h1c : (const double *) nullptr,
+                (const double *) (r->scal + SC_MC0 + s), (const T *) tv<T>(r->p), R.sc[s].abdt, u, v, w,
+                sc ? c : (T *) nullptr);
     }
     KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
     return CF_OK;
+}
+
+static cf_status enqueue_step_general(cf_run *r)
+{
+    return r->prec == CF_FP32 ? enqueue_step_general_t<float>(r) : enqueue_step_general_t<double>(r);
 }
 
 static cf_status collect_profile(cf_run *r)
@@ -1027,8 +1079,12 @@ extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
     if (rs != CF_OK) return rs;
     HIPCHK(hipMemcpyAsync(r->stage, host, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, dst,
-                       r->g.nx, r->g.ny, r->g.nz);
+    if (r->prec == CF_FP32)
+        hipLaunchKernelGGL(k_io_to_dev<float>, grid, dim3(256), 0, r->stream, (const double *) r->stage, tv<float>(dst),
+                           r->g.nx, r->g.ny, r->g.nz);
+    else
+        hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, dst,
+                           r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(r->stream));
     return CF_OK;
@@ -1037,7 +1093,11 @@ extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
 static cf_status download_dev(cf_run *r, const double *src, double *host)
 {
     dim3 grid((r->g.nx + 31) / 32, (r->g.nz + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_dev_to_io, grid, dim3(256), 0, r->stream, src, r->stage, r->g.nx, r->g.ny, r->g.nz);
+    if (r->prec == CF_FP32)
+        hipLaunchKernelGGL(k_dev_to_io<float>, grid, dim3(256), 0, r->stream, (const float *) src, r->stage,
+                           r->g.nx, r->g.ny, r->g.nz);
+    else
+        hipLaunchKernelGGL(k_dev_to_io<double>, grid, dim3(256), 0, r->stream, src, r->stage, r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(host, r->stage, sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
@@ -1058,6 +1118,19 @@ extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
     cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
     return download_dev(r, r->fa[field], host);
+}
+
+/* divergence -> Poisson(dt) -> projection of the general path's state, in place */
+template <typename T>
+static cf_status project_general(cf_run *r, double dt)
+{
+    T *u = tv<T>(r->u), *v = tv<T>(r->v), *w = tv<T>(r->w);
+    cf_status st = enqueue_pressure<T>(r, u, v, w, nullptr, nullptr, 3);
+    if (st != CF_OK) return st;
+    KLAUNCH("project", Prec<T>::gen(r).proj, cell_grid(r), dim3(64, 4), 0, r->rd, (const T *) u, (const T *) v,
+            (const T *) w, (const T *) nullptr, (const double *) nullptr, (const double *) nullptr,
+            (const double *) nullptr, (const double *) nullptr, (const T *) tv<T>(r->p), dt, u, v, w, (T *) nullptr);
+    return CF_OK;
 }
 
 static cf_status set_adhoc_poisson(cf_run *r, double dt)
@@ -1089,18 +1162,14 @@ extern "C" cf_status cf_run_project(cf_run *r, double dt)
     cf_status st = set_adhoc_poisson(r, dt);
     if (st != CF_OK) return st;
     if (r->general) {
-        st = enqueue_pressure(r, r->u, r->v, r->w, nullptr, nullptr, 3);
+        st = r->prec == CF_FP32 ? project_general<float>(r, dt) : project_general<double>(r, dt);
         if (st != CF_OK) return st;
-        KLAUNCH("project", r->kproj, cell_grid(r), dim3(64, 4), 0, r->rd, (const double *) r->u, (const double *) r->v,
-                (const double *) r->w, (const double *) nullptr, (const double *) nullptr, (const double *) nullptr,
-                (const double *) nullptr, (const double *) nullptr, (const double *) r->p, dt, r->u, r->v, r->w,
-                (double *) nullptr);
         HIPCHK(hipStreamSynchronize(r->stream));
         return collect_profile(r);
     }
     st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
-    st = enqueue_pressure(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, 3);
+    st = enqueue_pressure<double>(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, 3);
     if (st != CF_OK) return st;
     double *su, *sv, *sw;
     state_set(r, r->cur, &su, &sv, &sw);
@@ -1191,7 +1260,12 @@ extern "C" cf_status cf_run_max_divergence(cf_run *r, double *out)
         if (st != CF_OK) return st;
         mu = r->fa[0]; mv = r->fa[1]; mw = r->fa[2];
     }
-    hipLaunchKernelGGL(k_maxdiv_part, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd, mu, mv, mw, r->part, r->nb_red);
+    if (r->prec == CF_FP32)
+        hipLaunchKernelGGL(k_maxdiv_part<float>, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd, (const float *) mu,
+                           (const float *) mv, (const float *) mw, r->part, r->nb_red);
+    else
+        hipLaunchKernelGGL(k_maxdiv_part<double>, dim3(r->nb_red), dim3(256), 0, r->stream, r->rd, mu, mv, mw, r->part,
+                           r->nb_red);
     LAUNCHCHK();
     std::vector<double> part(r->nb_red);
     HIPCHK(hipMemcpyAsync(part.data(), r->part, sizeof(double) * r->nb_red, hipMemcpyDeviceToHost, r->stream));
@@ -1233,10 +1307,11 @@ extern "C" cf_status cf_run_kernel_time(cf_run *r, const char *kernel, double *a
 extern "C" cf_status cf_op_transpose_roundtrip(cf_run *r, const double *in_io, double *dev_out, double *io_out)
 {
     CHECK_RUN(r);
+    if (r->prec != CF_FP64) { cf_set_error("operator entry points take FP64 runs"); return CF_ERR_UNSUPPORTED; }
     if (!in_io) { cf_set_error("null input"); return CF_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
+    hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
                        r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
     if (dev_out) HIPCHK(hipMemcpyAsync(dev_out, r->fb[0], sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
@@ -1273,10 +1348,11 @@ static void spec_dev_to_host(const cf_run *r, const std::vector<double2> &d, dou
 extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spec_out)
 {
     CHECK_RUN(r);
+    if (r->prec != CF_FP64) { cf_set_error("operator entry points take FP64 runs"); return CF_ERR_UNSUPPORTED; }
     if (!in_io || !spec_out) { cf_set_error("null argument"); return CF_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
-    hipLaunchKernelGGL(k_io_to_dev, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
+    hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
                        r->g.nx, r->g.ny, r->g.nz);
     LAUNCHCHK();
     const RunDev &R = r->rd;
@@ -1297,6 +1373,7 @@ extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spe
 extern "C" cf_status cf_op_fft3d_bwd(cf_run *r, const double *spec_in, double *out_io)
 {
     CHECK_RUN(r);
+    if (r->prec != CF_FP64) { cf_set_error("operator entry points take FP64 runs"); return CF_ERR_UNSUPPORTED; }
     if (!spec_in || !out_io) { cf_set_error("null argument"); return CF_ERR_ARG; }
     std::vector<double2> d;
     spec_host_to_dev(r, spec_in, d);
@@ -1316,6 +1393,7 @@ extern "C" cf_status cf_op_fft3d_bwd(cf_run *r, const double *spec_in, double *o
 extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
 {
     CHECK_RUN(r);
+    if (r->prec != CF_FP64) { cf_set_error("operator entry points take FP64 runs"); return CF_ERR_UNSUPPORTED; }
     if (!spec || !(dt > 0)) { cf_set_error("bad argument"); return CF_ERR_ARG; }
     cf_status st = set_adhoc_poisson(r, dt);
     if (st != CF_OK) return st;
@@ -1323,10 +1401,10 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
     spec_host_to_dev(r, spec, d);
     HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
     if (r->g.order == 4)
-        hipLaunchKernelGGL(k_poisson4, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+        hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
                            r->rd, r->pt4[3], r->X, r->S4);
     else
-        hipLaunchKernelGGL(k_poisson2, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+        hipLaunchKernelGGL(k_poisson2<double>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
                            r->rd, r->pt[3], r->X, r->G);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));

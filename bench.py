@@ -32,7 +32,7 @@ STEP_BYTES_PER_PT = 720.0      # 30 arrays x 8 B x 3 substeps: compulsory FP64 t
 def alg_bytes(kernel, N, M):
     """Algorithmic bytes of one launch (each array read or written once)."""
     table = {
-        "rhs": 96 * N,              # read u v w g_u g_v g_w, write f_u f_v f_w r_u r_v r_w
+        "rhs": 104 * N,             # read u* v* w* p g_u g_v g_w, write f_u f_v f_w r_u r_v r_w
         "spike": 48 * N,            # read r (3 fields), write solution
         "thomas": 48 * N,           # read r (3 fields), write solution
         "plane_sum": 8 * N,         # read u*
@@ -47,10 +47,71 @@ def alg_bytes(kernel, N, M):
     return table.get(kernel)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_step_fp64_o2_256x193x256.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs of this workload, scaled by the same counters'
+    reading for a plain copy of known size; None if not measured."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def workload_config():
     return cf.Config(nx=256, ny=256, nz=193, order=2, stretch="tanh", stretch_beta=2.0,
                      lx=2 * np.pi, ly=np.pi, lz=2.0, nu=1.0 / 6900.0, dt=2.5e-4,
                      want_uvolavg=1.0, uvolavgmode=1)
+
+
+def time_form(name, cfg, steps, warmup, names="uvw", bytes_per_pt=None):
+    """Throughput of one more form of the step (same metric), from a seeded
+    projected field; per-kernel HIP-event times of a few extra steps."""
+    f = cf.initial_fields(cfg, seed=1234)
+    N = cfg.nx * cfg.ny * cfg.nz
+    with cf.Run(cfg) as run:
+        for n in names:
+            run.upload(n, f[n])
+        run.project(1.0)
+        run.step(warmup)
+        run.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.sync()
+        secs = time.perf_counter() - t0
+        run.set_profiling(True)
+        run.step(2)
+        run.sync()
+        kt = {}
+        for k in ("rhs", "cfl_finish", "solve", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
+                  "ffty_inv", "fftx_inv", "project"):
+            ms, n = run.kernel_time(k)
+            if n:
+                kt[k] = round(ms * n / 2, 4)
+        div = run.max_divergence()
+    out = {"workload": name, "value": N / 1e6 * steps / secs, "unit": "Mpt*RK3-steps/s",
+           "ms_per_step": 1e3 * secs / steps, "steps": steps, "max_div_end": div, "kernel_ms_per_step": kt}
+    if bytes_per_pt:
+        out["step_hbm_frac"] = bytes_per_pt * N * steps / secs / (HBM_PEAK_GBS * 1e9)
+    print(f"[bench] form {name}: {out['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
+def other_forms(steps):
+    forms = {}
+    c3 = cf.Config(nx=256, ny=256, nz=193, order=2, stretch="tanh", stretch_beta=2.0, lx=2 * np.pi, ly=np.pi,
+                   lz=2.0, nu=1.0 / 6900.0, dt=2.5e-4, precision=cf.CF_FP32)
+    forms["fp32_256x193x256"] = time_form("FP32 form (FP64 islands), 256x193x256 order 2", c3, steps, 5,
+                                          bytes_per_pt=360.0)
+    c2 = cf.Config(nx=192, ny=192, nz=129, order=4, scalar=1, prandtl=0.71, stretch="tanh", stretch_beta=2.0,
+                   lx=2 * np.pi, ly=np.pi, lz=2.0, nu=1.0 / 3300.0, dt=5e-4)
+    forms["order4_scalar_192x129x192"] = time_form("FP64 order 4 + scalar (Pr 0.71), 192x129x192", c2, steps, 5,
+                                                   names="uvwc", bytes_per_pt=816.0)
+    return forms
 
 
 def cpu_baseline(cfg, init, nsteps):
@@ -79,7 +140,20 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form-steps", type=int, default=50, help="timed steps of each other form (0: skip)")
+    ap.add_argument("--copy-calibrate", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+        n = args.copy_calibrate // 8
+        a = torch.full((n,), 1.0, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is a design note, not built", file=sys.stderr)
@@ -128,6 +202,7 @@ def main():
     div = run.max_divergence()
     run.close()
 
+    forms = other_forms(args.form_steps) if args.form_steps > 0 else {}
     cpu = None if args.no_cpu else cpu_baseline(cfg, init, args.cpu_steps)
 
     out = {
@@ -149,9 +224,10 @@ def main():
                    "step_hbm_frac": STEP_BYTES_PER_PT * N * args.steps / secs / (HBM_PEAK_GBS * 1e9),
                    "max_div_end": div},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "kernel_ms_per_step": per_step,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

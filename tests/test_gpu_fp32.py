@@ -62,8 +62,8 @@ def test_fp32_against_fp64_oracle(order, scalar, shape, steps):
         assert e <= 5e-5, (n, e)
     for col in (4, 5):
         assert np.allclose(glog[:, col], olog[:, col], rtol=1e-3, atol=1e-8), cf.LOG_COLUMNS[col]
-    for col in (6, 7):
-        assert np.allclose(glog[:, col], olog[:, col], rtol=1e-5, atol=1e-8), cf.LOG_COLUMNS[col]
+    assert np.allclose(glog[:, 6], olog[:, 6], rtol=1e-5, atol=1e-8), "ubulk"
+    assert np.allclose(glog[:, 7], olog[:, 7], rtol=5e-5, atol=1e-8), "ucl"
     umax = np.abs(want["u"]).max()
     assert div <= 1e-5 * umax / (cfg.lx / cfg.nx)
 

@@ -167,6 +167,33 @@ cf_status cf_op_fft3d_bwd(cf_run *run, const double *spec_in, double *out_io);
  * array [ix][iy][iz][re,im] (interior levels), in place, through the device. */
 cf_status cf_op_poisson(cf_run *run, double *spec_inout, double dt);
 
+/* ---- batches: independent runs stepped together ----
+ *
+ * The reference steps one simulation per process (chanfast2/chanfast4,
+ * chanfast.c:71-994, :995-1916); a batch holds runs of one order, scalar
+ * setting and precision whose grids, viscosities and time steps may all
+ * differ, and steps them together: each stage of the substep is one
+ * persistent launch over all runs' work, the step one captured graph.  At
+ * creation an autotuner times the launch shapes on the batch's own grids
+ * (persistent grid per launch; for order 2 the z-solve shape).  Each run
+ * keeps the semantics of cf_run_* on the same configuration. */
+typedef struct cf_batch cf_batch;
+
+cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch **out);
+cf_status cf_batch_destroy(cf_batch *batch);
+cf_status cf_batch_upload(cf_batch *batch, int run, int field, const double *host_io);
+cf_status cf_batch_download(cf_batch *batch, int run, int field, double *host_io);
+/* project every run's (u, v, w) as cf_run_project does */
+cf_status cf_batch_project(cf_batch *batch, double dt);
+cf_status cf_batch_step(cf_batch *batch, int nsteps);
+cf_status cf_batch_sync(cf_batch *batch);
+cf_status cf_batch_log(cf_batch *batch, int run, double *out, int max_steps, int *nsteps_out);
+cf_status cf_batch_max_divergence(cf_batch *batch, int run, double *out);
+cf_status cf_batch_set_profiling(cf_batch *batch, int on);
+cf_status cf_batch_kernel_time(cf_batch *batch, const char *kernel, double *avg_ms, int *launches);
+/* the autotuner's choices, one line per launch, into buf */
+cf_status cf_batch_tuning(cf_batch *batch, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,6 +142,18 @@ def lib():
         "cf_op_fft3d_fwd": (C.c_int, [P, dp, dp]),
         "cf_op_fft3d_bwd": (C.c_int, [P, dp, dp]),
         "cf_op_poisson": (C.c_int, [P, dp, C.c_double]),
+        "cf_batch_create": (C.c_int, [C.POINTER(_CConfig), C.c_int, C.POINTER(P)]),
+        "cf_batch_destroy": (C.c_int, [P]),
+        "cf_batch_upload": (C.c_int, [P, C.c_int, C.c_int, dp]),
+        "cf_batch_download": (C.c_int, [P, C.c_int, C.c_int, dp]),
+        "cf_batch_project": (C.c_int, [P, C.c_double]),
+        "cf_batch_step": (C.c_int, [P, C.c_int]),
+        "cf_batch_sync": (C.c_int, [P]),
+        "cf_batch_log": (C.c_int, [P, C.c_int, dp, C.c_int, C.POINTER(C.c_int)]),
+        "cf_batch_max_divergence": (C.c_int, [P, C.c_int, dp]),
+        "cf_batch_set_profiling": (C.c_int, [P, C.c_int]),
+        "cf_batch_kernel_time": (C.c_int, [P, C.c_char_p, dp, C.POINTER(C.c_int)]),
+        "cf_batch_tuning": (C.c_int, [P, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -295,3 +307,87 @@ class Run:
         s = np.ascontiguousarray(np.stack([spec.real, spec.imag], -1), dtype=np.float64)
         _check(lib().cf_op_poisson(self._h, _dptr(s), dt))
         return s[..., 0] + 1j * s[..., 1]
+
+
+class Batch:
+    """Independent runs of one order / scalar setting / precision stepped
+    together (a cf_batch handle): every substep stage is one persistent
+    launch over all runs; launch shapes are autotuned at creation."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (_CConfig * len(self.cfgs))()
+        self._keep = []
+        for q, cfg in enumerate(self.cfgs):
+            c, keep = cfg.to_c()
+            arr[q] = c
+            self._keep.append(keep)
+        h = C.c_void_p()
+        _check(lib().cf_batch_create(arr, len(self.cfgs), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def upload(self, run, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.cfgs[run].shape:
+            raise ValueError(f"field {name} has shape {a.shape}, expected {self.cfgs[run].shape}")
+        _check(lib().cf_batch_upload(self._h, run, FIELDS[name], _dptr(a)))
+
+    def download(self, run, name):
+        out = np.zeros(self.cfgs[run].shape)
+        _check(lib().cf_batch_download(self._h, run, FIELDS[name], _dptr(out)))
+        return out
+
+    def project(self, dt=1.0):
+        _check(lib().cf_batch_project(self._h, dt))
+
+    def step(self, n=1):
+        _check(lib().cf_batch_step(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().cf_batch_sync(self._h))
+
+    def log(self, run, max_steps=1 << 20):
+        n = C.c_int(0)
+        _check(lib().cf_batch_log(self._h, run, None, max_steps, C.byref(n)))
+        out = np.zeros((max(n.value, 1), LOG_N))
+        _check(lib().cf_batch_log(self._h, run, _dptr(out), n.value, C.byref(n)))
+        return out[: n.value]
+
+    def max_divergence(self, run):
+        x = C.c_double(0)
+        _check(lib().cf_batch_max_divergence(self._h, run, C.byref(x)))
+        return x.value
+
+    def set_profiling(self, on=True):
+        _check(lib().cf_batch_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        _check(lib().cf_batch_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def tuning(self):
+        buf = C.create_string_buffer(16384)
+        _check(lib().cf_batch_tuning(self._h, buf, len(buf)))
+        return buf.value.decode().splitlines()

@@ -190,13 +190,13 @@ static __device__ __forceinline__ void block_max_store(double v, double *__restr
 /* advectu2/4 + laplacexy_u + buoyancy, then the u right-hand side of
  * solve_helmholtz1d_u (chanfast.c:465-477); CFL partials at s == 0 */
 template <int ORD, bool SC, typename T>
-__global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
+static __device__ __forceinline__ void rhsu_body(const RunDev &R, const int s,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
-    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt, const Blk b)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = b.x * 64 + threadIdx.x, j = b.y * 4 + threadIdx.y, k = b.z;
     double cflv = 0.0;
     if (i < nx && j < ny) {
         const size_t P = (size_t) nx * ny;
@@ -290,20 +290,29 @@ __global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
                         fabs((double) w[q]) * cfldt / dzph);
         }
     }
-    if (cflpart) block_max_store(cflv, cflpart, (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
+    if (cflpart) block_max_store(cflv, cflpart, (b.z * b.ny + b.y) * b.nx + b.x);
+}
+
+template <int ORD, bool SC, typename T>
+__global__ __launch_bounds__(256) void k_grhs_u(const RunDev R, const int s,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+{
+    rhsu_body<ORD, SC, T>(R, s, u, v, w, c, g, scal, fo, ro, cflpart, cfldt, blk_native());
 }
 
 /* ------------------------------------------------------------ v */
 
 template <int ORD, bool SC, typename T>
-__global__ __launch_bounds__(256) void k_grhs_v(const RunDev R, const int s,
+static __device__ __forceinline__ void rhsv_body(const RunDev &R, const int s,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
-    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt, const Blk b)
 {
     (void) cflpart; (void) cfldt; (void) scal;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = b.x * 64 + threadIdx.x, j = b.y * 4 + threadIdx.y, k = b.z;
     if (i >= nx || j >= ny) return;
     const size_t P = (size_t) nx * ny;
     const double *dz = R.dzt + CF_NGT;
@@ -392,17 +401,26 @@ __global__ __launch_bounds__(256) void k_grhs_v(const RunDev R, const int s,
     ro[q] = h * dzk;
 }
 
-/* ------------------------------------------------------------ w */
-
 template <int ORD, bool SC, typename T>
-__global__ __launch_bounds__(256) void k_grhs_w(const RunDev R, const int s,
+__global__ __launch_bounds__(256) void k_grhs_v(const RunDev R, const int s,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
     T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
 {
+    rhsv_body<ORD, SC, T>(R, s, u, v, w, c, g, scal, fo, ro, cflpart, cfldt, blk_native());
+}
+
+/* ------------------------------------------------------------ w */
+
+template <int ORD, bool SC, typename T>
+static __device__ __forceinline__ void rhsw_body(const RunDev &R, const int s,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt, const Blk b)
+{
     (void) cflpart; (void) cfldt; (void) scal;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = b.x * 64 + threadIdx.x, j = b.y * 4 + threadIdx.y, k = b.z;
     if (i >= nx || j >= ny) return;
     const size_t P = (size_t) nx * ny;
     const size_t cc = (size_t) j * nx + i, q = (size_t) k * P + cc;
@@ -495,19 +513,28 @@ __global__ __launch_bounds__(256) void k_grhs_w(const RunDev R, const int s,
     ro[q] = h * ((T) 0.5 * (DZ(k + 1) + DZ(k)));   /* times the face volume (diffusew.c:205) */
 }
 
+template <int ORD, bool SC, typename T>
+__global__ __launch_bounds__(256) void k_grhs_w(const RunDev R, const int s,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+{
+    rhsw_body<ORD, SC, T>(R, s, u, v, w, c, g, scal, fo, ro, cflpart, cfldt, blk_native());
+}
+
 /* ------------------------------------------------------------ c */
 
 /* flux_quick + advect_scalar (scalar.c:79-162), laplacexy with nu / Pr,
  * then the scalar's implicit right-hand side (chanfast.c:540-571) */
 template <int ORD, bool SC, typename T>
-__global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
+static __device__ __forceinline__ void rhsc_body(const RunDev &R, const int s,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
-    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt, const Blk b)
 {
     (void) cflpart; (void) cfldt;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int i = b.x * 64 + threadIdx.x, j = b.y * 4 + threadIdx.y, k = b.z;
     if (i >= nx || j >= ny) return;
     const size_t P = (size_t) nx * ny;
     const double *dz = R.dzt + CF_NGT;
@@ -549,6 +576,15 @@ __global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
     ro[q] = h * DZ(k);
 }
 
+template <int ORD, bool SC, typename T>
+__global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ c, const T *__restrict__ g, const double *__restrict__ scal,
+    T *__restrict__ fo, T *__restrict__ ro, double *__restrict__ cflpart, double cfldt)
+{
+    rhsc_body<ORD, SC, T>(R, s, u, v, w, c, g, scal, fo, ro, cflpart, cfldt, blk_native());
+}
+
 /* ------------------------------------------------------------ solve */
 
 /* order-4 wall-reduced z solves (solve_helmholtz1d_u4 / _w4 with
@@ -556,13 +592,13 @@ __global__ __launch_bounds__(256) void k_grhs_c(const RunDev R, const int s,
  * (column, field), in place.  Fields 0 u, 1 v, 2 w (nz-1 unknowns, top
  * wall := 0), 3 c. */
 template <typename T>
-__global__ __launch_bounds__(256) void k_gsolve4(const RunDev R, const int s,
-    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+static __device__ __forceinline__ void solve4_body(const RunDev &R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
-    const size_t col = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
+    const size_t col = (size_t) b.x * blockDim.x + threadIdx.x;
+    const int f = b.y;
     if (col >= P || f > 3) return;
     T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
@@ -604,6 +640,13 @@ __global__ __launch_bounds__(256) void k_gsolve4(const RunDev R, const int s,
     if (f == 2) xc[(size_t) (nz - 1) * P] = (T) 0;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gsolve4(const RunDev R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+{
+    solve4_body<T>(R, s, ru, rv, rw, rc, blk_native());
+}
+
 template __global__ void k_gsolve4<double>(const RunDev, const int, double *, double *, double *, double *);
 template __global__ void k_gsolve4<float>(const RunDev, const int, float *, float *, float *, float *);
 
@@ -613,16 +656,16 @@ template __global__ void k_gsolve4<float>(const RunDev, const int, float *, floa
  * get_xy_avg (get.c:281-307); deterministic fixed-order trees.
  * part[(kind * nz + k) * nb + b] */
 template <typename T>
-__global__ __launch_bounds__(256) void k_gplane(const RunDev R, const T *__restrict__ q0,
-                                                const T *__restrict__ q1, double *__restrict__ part, int nb)
+static __device__ __forceinline__ void plane_body(const RunDev &R, const T *__restrict__ q0,
+                                                const T *__restrict__ q1, double *__restrict__ part, int nb, const Blk b)
 {
     __shared__ double red[256];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
-    const int lev = blockIdx.y, kind = blockIdx.z, b = blockIdx.x;
+    const int lev = b.y, kind = b.z, bb = b.x;
     const T *q = kind == 0 ? q0 : q1;
     const size_t chunk = (P + nb - 1) / nb;
-    const size_t lo = (size_t) b * chunk, hi = lo + chunk < P ? lo + chunk : P;
+    const size_t lo = (size_t) bb * chunk, hi = lo + chunk < P ? lo + chunk : P;
     const T *pl = q + (size_t) lev * P;
     double acc = 0.0;                             /* FP64 accumulation for either storage */
     for (size_t e = lo + threadIdx.x; e < hi; e += 256) acc += (double) pl[e];
@@ -632,7 +675,14 @@ __global__ __launch_bounds__(256) void k_gplane(const RunDev R, const T *__restr
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[((size_t) kind * nz + lev) * nb + b] = red[0];
+    if (threadIdx.x == 0) part[((size_t) kind * nz + lev) * nb + bb] = red[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gplane(const RunDev R, const T *__restrict__ q0,
+                                                const T *__restrict__ q1, double *__restrict__ part, int nb)
+{
+    plane_body<T>(R, q0, q1, part, nb, blk_native());
 }
 
 template __global__ void k_gplane<double>(const RunDev, const double *, const double *, double *, int);
@@ -642,12 +692,12 @@ template __global__ void k_gplane<float>(const RunDev, const float *, const floa
  * :572-617) constraints: plane averages, their wall ghosts from the
  * closures, bulk value, wall gradients of order 2 or 4 (:489-494,
  * :1413-1418), centreline; forcing from the unit-forcing response. */
-__global__ __launch_bounds__(256) void k_gmass(const RunDev R, const int s, const double *__restrict__ part, int nb,
-                                               double *__restrict__ scal, double *__restrict__ logbuf)
+static __device__ __forceinline__ void mass_body(const RunDev &R, const int s, const double *__restrict__ part, int nb,
+                                               double *__restrict__ scal, double *__restrict__ logbuf, const Blk b)
 {
     extern __shared__ double avg[];               /* nz + 6: level k at k + 3 */
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz, ord = R.g.order;
-    const int kind = blockIdx.x;
+    const int kind = b.x;
     const double *dz = R.dzt + CF_NGT;
     const double scale = 1.0 / ((double) nx * (double) ny);
     for (int l = threadIdx.x; l < nz; l += blockDim.x) {
@@ -693,6 +743,12 @@ __global__ __launch_bounds__(256) void k_gmass(const RunDev R, const int s, cons
         double *lg = logbuf + (size_t) idx * 13 + (kind == 0 ? 3 : 8);
         lg[0] = m; lg[1] = bot; lg[2] = top; lg[3] = vol; lg[4] = cl;
     }
+}
+
+__global__ __launch_bounds__(256) void k_gmass(const RunDev R, const int s, const double *__restrict__ part, int nb,
+                                               double *__restrict__ scal, double *__restrict__ logbuf)
+{
+    mass_body(R, s, part, nb, scal, logbuf, blk_native());
 }
 
 /* ------------------------------------------------------------ Poisson, order 4 */
@@ -751,12 +807,12 @@ static __device__ __forceinline__ void pois4_mode(const Pois4Tab &T, C *__restri
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab T_,
-                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ S)
+static __device__ __forceinline__ void pois4_body(const RunDev &R, const Pois4Tab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ S, const Blk b)
 {
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const size_t M = (size_t) ny * nkx;
-    const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t qm = (size_t) b.x * blockDim.x + threadIdx.x;
     if (qm >= M) return;
     const int kx = (int) (qm / ny), ky = (int) (qm % ny);
     const int variant = (kx == 0 && ky == 0) ? 1 : 0;
@@ -765,6 +821,13 @@ __global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab
         pois4_mode<double>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant);
     else
         pois4_mode<T>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ S)
+{
+    pois4_body<T>(R, T_, Y, S, blk_native());
 }
 
 template __global__ void k_poisson4<double>(const RunDev, const Pois4Tab, double2 *, double *);
@@ -780,17 +843,17 @@ template __global__ void k_poisson4<float>(const RunDev, const Pois4Tab, float2 
 /* (us, u), (vs, v), (ws, w) may be the same arrays (in-place projection):
  * each thread reads and writes only its own point of them */
 template <int ORD, typename T>
-__global__ __launch_bounds__(256) void k_gproject(const RunDev R, const T *us,
+static __device__ __forceinline__ void proj_body(const RunDev &R, const T *us,
     const T *vs, const T *ws, const T *cs,
     const double *__restrict__ h1u, const double *__restrict__ mu, const double *__restrict__ h1c,
     const double *__restrict__ mc, const T *__restrict__ p, double dtp,
-    T *u, T *v, T *w, T *c)
+    T *u, T *v, T *w, T *c, const Blk b)
 {
     const T A1 = (T) A1d, A2 = (T) A2d;
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = b.x * 64 + threadIdx.x;
+    const int j = b.y * 4 + threadIdx.y;
+    const int k = b.z;
     if (i >= nx || j >= ny || k >= nz) return;
     const size_t P = (size_t) nx * ny;
     const size_t row = rowof(k, j, nx, P), q = row + i;
@@ -825,6 +888,16 @@ __global__ __launch_bounds__(256) void k_gproject(const RunDev R, const T *us,
     if (c) c[q] = cs[q] + (h1c ? (T) h1c[k + 1] * (mc ? (T) *mc : (T) 0) : (T) 0);
 }
 
+template <int ORD, typename T>
+__global__ __launch_bounds__(256) void k_gproject(const RunDev R, const T *us,
+    const T *vs, const T *ws, const T *cs,
+    const double *__restrict__ h1u, const double *__restrict__ mu, const double *__restrict__ h1c,
+    const double *__restrict__ mc, const T *__restrict__ p, double dtp,
+    T *u, T *v, T *w, T *c)
+{
+    proj_body<ORD, T>(R, us, vs, ws, cs, h1u, mu, h1c, mc, p, dtp, u, v, w, c, blk_native());
+}
+
 template <typename T> struct GPick {
     template <int ORD, bool SC> static void fill(GSel<T> &S)
     {
@@ -852,3 +925,127 @@ GSel<T> pick_general(int order, int scalar)
 
 template GSel<double> pick_general<double>(int, int);
 template GSel<float> pick_general<float>(int, int);
+
+/* ------------------------------------------------------------ batched launches
+ * Runs of one (order, scalar, precision) with any grid sizes and
+ * coefficients go out together: each launch below is persistent over the
+ * virtual blocks of all runs (batch_loop), each run reading its own
+ * parameter block and buffers from the GArgs table.  Substep index s in
+ * 0..2; s == 3 is a stand-alone projection of the state (dt = dt_adhoc). */
+
+template <int ORD, bool SC, typename T, int F>
+__global__ __launch_bounds__(256) void kb_rhs(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
+                                              int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = A[r];
+        T *const *fnew = (s % 2 == 0) ? a.fa : a.fb;
+        T *const *gold = (s % 2 == 0) ? a.fb : a.fa;
+        const Blk b = blk_of(a, ST_CELL, lb);
+        if (F == 0)
+            rhsu_body<ORD, SC, T>(a.R, s, a.u, a.v, a.w, a.c, gold[0], a.scal, fnew[0], a.ru,
+                                  s == 0 ? a.cflpart : (double *) nullptr, a.dt, b);
+        else if (F == 1)
+            rhsv_body<ORD, SC, T>(a.R, s, a.u, a.v, a.w, a.c, gold[1], a.scal, fnew[1], a.rv, nullptr, a.dt, b);
+        else if (F == 2)
+            rhsw_body<ORD, SC, T>(a.R, s, a.u, a.v, a.w, a.c, gold[2], a.scal, fnew[2], a.rw, nullptr, a.dt, b);
+        else
+            rhsc_body<ORD, SC, T>(a.R, s, a.u, a.v, a.w, a.c, gold[3], a.scal, fnew[3], a.rc, nullptr, a.dt, b);
+    });
+}
+
+template <int ORD, typename T>
+__global__ __launch_bounds__(256) void kb_proj(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
+                                               int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = A[r];
+        const Blk b = blk_of(a, ST_CELL, lb);
+        if (s < 3) {
+            const int nz = a.R.g.nz;
+            const double *h1u = a.R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
+            const double *h1c = a.R.h1 + (size_t) (s * 2 + 1) * (nz + 2);
+            const bool sc = a.R.scalar != 0;
+            proj_body<ORD, T>(a.R, a.ru, a.rv, a.rw, a.rc, h1u, a.scal + SC_M0 + s, sc ? h1c : nullptr,
+                              a.scal + SC_MC0 + s, a.p, a.R.sc[s].abdt, a.u, a.v, a.w, sc ? a.c : nullptr, b);
+        } else {
+            proj_body<ORD, T>(a.R, a.u, a.v, a.w, nullptr, nullptr, nullptr, nullptr, nullptr, a.p, a.dt_adhoc,
+                              a.u, a.v, a.w, nullptr, b);
+        }
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_solve4(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        solve4_body<T>(a.R, s, a.ru, a.rv, a.rw, a.rc, blk_of(a, ST_SOLVE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_plane(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    (void) s;
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        plane_body<T>(a.R, a.ru, a.rc, a.part, a.nb_plane, blk_of(a, ST_PLANE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_mass(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        mass_body(a.R, s, a.part, a.nb_plane, a.scal, a.logbuf, blk_of(a, ST_MASS, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_poisson4(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        pois4_body<T>(a.R, a.pt4[s], a.X, a.S4, blk_of(a, ST_MODE, lb));
+    });
+}
+
+#define CF_INST_B(T)                                                                                              \
+    template __global__ void kb_solve4<T>(const GArgs<T> *, const unsigned *, int, int);                          \
+    template __global__ void kb_plane<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template __global__ void kb_mass<T>(const GArgs<T> *, const unsigned *, int, int);                            \
+    template __global__ void kb_poisson4<T>(const GArgs<T> *, const unsigned *, int, int);
+CF_INST_B(double)
+CF_INST_B(float)
+#undef CF_INST_B
+
+template <typename T> struct BPick {
+    template <int ORD, bool SC> static void fill(BSel<T> &S)
+    {
+        S.u = kb_rhs<ORD, SC, T, 0>;
+        S.v = kb_rhs<ORD, SC, T, 1>;
+        S.w = kb_rhs<ORD, SC, T, 2>;
+        S.c = SC ? kb_rhs<ORD, true, T, 3> : nullptr;
+        S.proj = kb_proj<ORD, T>;
+        S.divx = pick_batched_divx<T>(ORD);
+    }
+};
+
+template <typename T>
+BSel<T> pick_batched(int order, int scalar)
+{
+    BSel<T> S;
+    if (order == 4) {
+        if (scalar) BPick<T>::template fill<4, true>(S);
+        else BPick<T>::template fill<4, false>(S);
+    } else {
+        if (scalar) BPick<T>::template fill<2, true>(S);
+        else BPick<T>::template fill<2, false>(S);
+    }
+    return S;
+}
+
+template BSel<double> pick_batched<double>(int, int);
+template BSel<float> pick_batched<float>(int, int);

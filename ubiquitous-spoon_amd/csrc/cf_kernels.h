@@ -96,6 +96,23 @@ struct RunDev {
     HeptTab hp[3][3];           /* order-4 implicit z factors [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
 };
 
+/* A workgroup's coordinates in one run's launch grid: blockIdx / gridDim
+ * of a single-run launch, or decoded from a batched launch's virtual block. */
+struct Blk {
+    unsigned x, y, z, nx, ny, nz;
+};
+static __device__ __forceinline__ Blk blk_native()
+{
+    Blk b;
+    b.x = blockIdx.x; b.y = blockIdx.y; b.z = blockIdx.z;
+    b.nx = gridDim.x; b.ny = gridDim.y; b.nz = gridDim.z;
+    return b;
+}
+
+/* launch stages of a batched step; each run's grid per stage is dims[stage] */
+enum { ST_CELL = 0, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE, ST_FFTX, ST_FFTY, ST_MODE, CF_NST };
+#define CF_BATCH_MAX 256        /* runs per batched launch */
+
 /* Where a substep's projected velocity comes from: the previous
  * substep's solve output, its pressure, mass-flux profile and forcing
  * (h1, m may be null: no correction) and its projection dt. */
@@ -109,6 +126,61 @@ struct PoisTab {
     const double *a, *c, *bz, *ex, *ey, *dvol;
     double db_neu, db_dir, db_top;
 };
+
+/* One run of a batched launch: its parameter block, buffers (storage
+ * precision T), Poisson tables and per-stage grid shapes.  An array of
+ * these lives in device memory; batched kernels read the entry of the run
+ * a virtual block belongs to. */
+template <typename T> struct GArgs {
+    RunDev R;
+    T *u, *v, *w, *c, *ru, *rv, *rw, *rc, *fa[4], *fb[4], *p;
+    typename Cplx<T>::type *X;
+    double *G, *S4, *scal, *logbuf, *part, *cflpart;
+    double dt;                  /* step */
+    double dt_adhoc;            /* dt of a stand-alone projection (substep index 3) */
+    int nb_plane, gblocks;
+    PoisTab pt[4];              /* substeps 0..2, stand-alone projection */
+    Pois4Tab pt4[4];
+    unsigned dims[CF_NST][3];
+};
+
+/* Persistent loop of a batched launch: vstart[0..nrun] is the prefix of the
+ * runs' virtual block counts for this stage; each workgroup walks virtual
+ * blocks vb = blockIdx.x, + gridDim.x, ..., finds the run by binary search
+ * and calls fn(run, local block).  A barrier separates iterations (bodies
+ * reuse their LDS). */
+template <typename F>
+static __device__ __forceinline__ void batch_loop(const unsigned *__restrict__ vstart, int nrun, F &&fn)
+{
+    __shared__ unsigned lvs[CF_BATCH_MAX + 1];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nt = blockDim.x * blockDim.y;
+    for (int t = tid; t <= nrun; t += nt) lvs[t] = vstart[t];
+    __syncthreads();
+    const unsigned total = lvs[nrun];
+    for (unsigned vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        int lo = 0, hi = nrun - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (lvs[mid] <= vb) lo = mid; else hi = mid - 1;
+        }
+        fn(lo, vb - lvs[lo]);
+        __syncthreads();
+    }
+}
+
+/* local block index -> Blk in a run's grid of the given stage */
+template <typename T>
+static __device__ __forceinline__ Blk blk_of(const GArgs<T> &a, int stage, unsigned lb)
+{
+    Blk b;
+    b.nx = a.dims[stage][0]; b.ny = a.dims[stage][1]; b.nz = a.dims[stage][2];
+    b.x = lb % b.nx;
+    const unsigned r = lb / b.nx;
+    b.y = r % b.ny;
+    b.z = r / b.ny;
+    return b;
+}
 
 template <typename T> __global__ void k_io_to_dev(const double *io, T *dev, int nx, int ny, int nz);
 template <typename T> __global__ void k_dev_to_io(const T *dev, double *io, int nx, int ny, int nz);
@@ -153,6 +225,29 @@ template <typename T> struct GSel {
     proj_t proj;
 };
 template <typename T> GSel<T> pick_general(int order, int scalar);
+
+/* batched launches (cf_general.hip, cf_kernels.hip): a = device array of
+ * nrun GArgs, vs = that stage's virtual-block prefix (nrun + 1) */
+template <typename T> struct BSel {
+    typedef void (*divx_t_)(const GArgs<T> *, const unsigned *, int, int);
+    typedef void (*rhs_t)(const GArgs<T> *, const unsigned *, int, int);
+    rhs_t u, v, w, c;
+    rhs_t proj;
+    divx_t_ divx;
+};
+template <typename T> BSel<T> pick_batched(int order, int scalar);
+template <typename T> typename BSel<T>::divx_t_ pick_batched_divx(int order);
+template <typename T> __global__ void kb_poisson4(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_solve4(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_spike(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_thomas(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_plane(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_mass(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_cfl(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_step_end(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
+template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_poisson(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void k_gsolve4(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> __global__ void k_gplane(const RunDev R, const T *q0, const T *q1, double *part, int nb);
 __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);

@@ -341,13 +341,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
  * 3 the scalar c (nz rows, diffusivity nu / Pr).
  * ------------------------------------------------------------------- */
 template <typename T>
-__global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
-    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+static __device__ __forceinline__ void thomas_body(const RunDev &R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
-    const size_t col = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
+    const size_t col = (size_t) b.x * blockDim.x + threadIdx.x;
+    const int f = b.y;
     if (col >= P || f > 3) return;
     T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
@@ -391,6 +391,13 @@ __global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
     if (f == 2) xc[(size_t) (nz - 1) * P] = (T) 0;   /* ghost_fill_top2, diffusew.c:683-688 */
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_thomas(const RunDev R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+{
+    thomas_body<T>(R, s, ru, rv, rw, rc, blk_native());
+}
+
 template __global__ void k_thomas<double>(const RunDev, const int, double *, double *, double *, double *);
 template __global__ void k_thomas<float>(const RunDev, const int, float *, float *, float *, float *);
 
@@ -408,20 +415,20 @@ template __global__ void k_thomas<float>(const RunDev, const int, float *, float
  * diffusew.c:175-223); differs from sequential Thomas by round-off.
  * ------------------------------------------------------------------- */
 template <typename T>
-__global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
-    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b)
 {
     __shared__ T itf[2 * CF_SPIKE_SMAX][64];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
-    const int f = blockIdx.y;
+    const int f = b.y;
     if (f > 3) return;
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
     const SpikeTab ST = R.sp[s][type];
     const int lane = threadIdx.x & 63;
     /* wave-uniform chunk index, so the table reads below are scalar loads */
     const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t col = (size_t) blockIdx.x * 64 + lane;
+    const size_t col = (size_t) b.x * 64 + lane;
     const bool active = col < P && chunk < ST.S;
     T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int lo = chunk * ST.L;
@@ -474,6 +481,13 @@ __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
         if (j < hi) xc[(size_t) j * P] = y[t] - E * (T) ST.vl[j] - F * (T) ST.wr[j];
     }
     if (f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
+{
+    spike_body<T>(R, s, ru, rv, rw, rc, blk_native());
 }
 
 template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
@@ -565,8 +579,8 @@ __global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int
  * then a one-block finish that writes the cfl.dat record and bumps the
  * step counter at the end of the step (k_step_end).
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb,
-                                                    double dt, double *__restrict__ scal, double *__restrict__ logbuf)
+static __device__ __forceinline__ void cfl_body(const RunDev &R, const double *__restrict__ part, int nb,
+                                                    double dt, double *__restrict__ scal, double *__restrict__ logbuf, const Blk b)
 {
     __shared__ double red[256];
     double m = 0.0;
@@ -585,6 +599,12 @@ __global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double
     lg[1] = dt;
     lg[2] = red[0];
     scal[SC_CFL] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb,
+                                                    double dt, double *__restrict__ scal, double *__restrict__ logbuf)
+{
+    cfl_body(R, part, nb, dt, scal, logbuf, blk_native());
 }
 
 __global__ void k_step_end(double *__restrict__ scal)
@@ -634,9 +654,9 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
  * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
  * mirrored oddly about the (zero) wall faces, as divergence4 sets them. */
 template <int NX, int ORD, typename T>
-__global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__restrict__ us,
+static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__restrict__ us,
     const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
-    const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X)
+    const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X, const Blk b)
 {
     typedef typename Cplx<T>::type C;
     extern __shared__ __align__(16) unsigned char smem_[];
@@ -650,7 +670,7 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__res
     const double *rdz = R.rdz + CF_NGT;
     const T m = mptr ? (T) *mptr : (T) 0;
     const T rdx = (T) R.rdx, rdy = (T) R.rdy;
-    const long pair0 = (long) xcd_block(blockIdx.x, gridDim.x) * nl;
+    const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
     auto divg = [&](long line, int i) -> T {
         if (line >= NL) return (T) 0;
         int k = (int) (line / ny), j = (int) (line - (long) k * ny);
@@ -683,6 +703,14 @@ __global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__res
     store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
 }
 
+template <int NX, int ORD, typename T>
+__global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__restrict__ us,
+    const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
+    const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X)
+{
+    divx_body<NX, ORD, T>(R, us, vs, ws, h1, mptr, X, blk_native());
+}
+
 /* the fftx_d2c stage of FFT3D_FWD on its own (no divergence), scaled 1/nx */
 template <int NX>
 __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const double *__restrict__ a,
@@ -711,7 +739,7 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
 /* complex y FFT in place, forward scaled 1/ny (fft.c:186-202) or unscaled
  * inverse (fft.c:205-208); a block transforms TK contiguous lines */
 template <int NY, typename T>
-__global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::type *__restrict__ X, int fwd)
+static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<T>::type *__restrict__ X, int fwd, const Blk b)
 {
     typedef typename Cplx<T>::type C;
     extern __shared__ __align__(16) unsigned char smem_[];
@@ -719,7 +747,7 @@ __global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::
     const int ny = NY > 0 ? NY : R.g.ny;
     const long nlines = (long) R.g.nz * R.g.nkx;
     const int TK = R.fy_cols, ld = ny + 1;
-    const long L0 = (long) blockIdx.x * TK;
+    const long L0 = (long) b.x * TK;
     const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
     C *base = X + (size_t) L0 * ny;
     for (int e = threadIdx.x; e < TK * ny; e += blockDim.x) {
@@ -736,12 +764,18 @@ __global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::
     }
 }
 
+template <int NY, typename T>
+__global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::type *__restrict__ X, int fwd)
+{
+    ffty_body<NY, T>(R, X, fwd, blk_native());
+}
+
 /* inverse complex-to-real x FFT (fft.c:180-183; FFTW c2r semantics: the
  * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two real
  * lines per complex transform, writing p */
 template <int NX, typename T>
-__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const typename Cplx<T>::type *__restrict__ X,
-                                                  T *__restrict__ p)
+static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typename Cplx<T>::type *__restrict__ X,
+                                                  T *__restrict__ p, const Blk b)
 {
     typedef typename Cplx<T>::type C;
     extern __shared__ __align__(16) unsigned char smem_[];
@@ -751,7 +785,7 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const typename
     const int ld = nx + 1;
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
-    const long pair0 = (long) xcd_block(blockIdx.x, gridDim.x) * nl;
+    const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
     const int half = nx / 2;
     for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
         const int kx = e / nl, l = e - kx * nl;
@@ -778,6 +812,13 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const typename
         if (la < NL) p[(size_t) la * nx + i] = z.x;
         if (lb < NL) p[(size_t) lb * nx + i] = z.y;
     }
+}
+
+template <int NX, typename T>
+__global__ __launch_bounds__(256) void k_fftx_inv(const RunDev R, const typename Cplx<T>::type *__restrict__ X,
+                                                  T *__restrict__ p)
+{
+    fftxi_body<NX, T>(R, X, p, blk_native());
 }
 
 /* host-side selectors of the compile-time instances */
@@ -910,12 +951,12 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T_,
-                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
+static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G, const Blk b)
 {
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const size_t M = (size_t) ny * nkx;
-    const size_t qm = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t qm = (size_t) b.x * blockDim.x + threadIdx.x;
     if (qm >= M) return;
     const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
     const bool zero = kx == 0 && ky == 0;
@@ -924,6 +965,13 @@ __global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab 
         pois2_mode<double>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
     else
         pois2_mode<T>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T_,
+                                                  typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
+{
+    pois2_body<T>(R, T_, Y, G, blk_native());
 }
 
 template __global__ void k_poisson2<double>(const RunDev, const PoisTab, double2 *, double *);
@@ -1011,3 +1059,108 @@ __global__ __launch_bounds__(256) void k_maxdiv_part(const RunDev R, const T *__
 
 template __global__ void k_maxdiv_part<double>(const RunDev, const double *, const double *, const double *, double *, int);
 template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const float *, const float *, double *, int);
+
+/* ------------------------------------------------------------ batched launches
+ * (see cf_general.hip): the shared solves, FFT passes, order-2 Poisson and
+ * the per-step bookkeeping over all runs of a batch; FFTs use the runtime
+ * plan since the runs' lengths differ. */
+
+template <int ORD, typename T>
+__global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
+                                               int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = A[r];
+        const Blk b = blk_of(a, ST_FFTX, lb);
+        if (s < 3)
+            divx_body<0, ORD, T>(a.R, a.ru, a.rv, a.rw, a.R.h1 + (size_t) (s * 2) * (a.R.g.nz + 2),
+                                 a.scal + SC_M0 + s, a.X, b);
+        else
+            divx_body<0, ORD, T>(a.R, a.u, a.v, a.w, nullptr, nullptr, a.X, b);
+    });
+}
+
+template <typename T>
+typename BSel<T>::divx_t_ pick_batched_divx(int order)
+{
+    return order == 4 ? kb_divx<4, T> : kb_divx<2, T>;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_ffty(const GArgs<T> *a_, const unsigned *vs, int nrun, int fwd)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        ffty_body<0, T>(a.R, a.X, fwd, blk_of(a, ST_FFTY, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    (void) s;
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        fftxi_body<0, T>(a.R, a.X, a.p, blk_of(a, ST_FFTX, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_poisson(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        pois2_body<T>(a.R, a.pt[s], a.X, a.G, blk_of(a, ST_MODE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void kb_spike(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        spike_body<T>(a.R, s, a.ru, a.rv, a.rw, a.rc, blk_of(a, ST_SOLVE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_thomas(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        thomas_body<T>(a.R, s, a.ru, a.rv, a.rw, a.rc, blk_of(a, ST_SOLVE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_cfl(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    (void) s;
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<T> &a = a_[r];
+        cfl_body(a.R, a.cflpart, a.gblocks, a.dt, a.scal, a.logbuf, blk_of(a, ST_ONE, lb));
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+{
+    (void) s;
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        (void) lb;
+        if (threadIdx.x == 0) a_[r].scal[SC_STEP] += 1.0;
+    });
+}
+
+#define CF_INST_B(T)                                                                                              \
+    template __global__ void kb_ffty<T>(const GArgs<T> *, const unsigned *, int, int);                            \
+    template __global__ void kb_fftxi<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template __global__ void kb_poisson<T>(const GArgs<T> *, const unsigned *, int, int);                         \
+    template __global__ void kb_spike<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template __global__ void kb_thomas<T>(const GArgs<T> *, const unsigned *, int, int);                          \
+    template __global__ void kb_cfl<T>(const GArgs<T> *, const unsigned *, int, int);                             \
+    template __global__ void kb_step_end<T>(const GArgs<T> *, const unsigned *, int, int);                        \
+    template typename BSel<T>::divx_t_ pick_batched_divx<T>(int);
+CF_INST_B(double)
+CF_INST_B(float)
+#undef CF_INST_B

@@ -420,7 +420,14 @@ static void free_run(cf_run *r)
 
 /* ------------------------------------------------------------ create */
 
+static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out);
+
 extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
+{
+    return create_run(cfg, false, out);
+}
+
+static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out)
 {
     if (!out) { cf_set_error("null output handle"); return CF_ERR_ARG; }
     *out = nullptr;
@@ -525,7 +532,7 @@ extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
     const int order = cfg->order;
     r->prec = cfg->precision;
     r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
-    r->general = order == 4 || cfg->scalar || cfg->precision == CF_FP32;
+    r->general = force_general || order == 4 || cfg->scalar || cfg->precision == CF_FP32;
     r->nfield = cfg->scalar ? 4 : 3;
     Clos clo;
     if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
@@ -1410,5 +1417,492 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
     spec_dev_to_host(r, d, spec);
+    return CF_OK;
+}
+
+/* ------------------------------------------------------------ batches
+ *
+ * A batch steps runs of one (order, scalar, precision) with any grid
+ * sizes, viscosities and time steps together: every stage of the substep
+ * is one persistent launch over the virtual blocks of all runs (the
+ * kb_* kernels), the whole RK3 step one captured graph.  The persistent
+ * grid of each launch and, for order 2, the z solve (several wavefronts
+ * per system, or one lane per system) are chosen by timing the
+ * candidates on the batch's own shapes at creation, while every field is
+ * still zero (each stage then maps zeros to zeros, so tuning leaves the
+ * state untouched).
+ */
+
+enum { KB_RHS_U = 0, KB_RHS_V, KB_RHS_W, KB_RHS_C, KB_SOLVE, KB_PLANE, KB_MASS, KB_CFL, KB_DIVX, KB_FFTY,
+       KB_POIS, KB_FFTXI, KB_PROJ, KB_STEPEND, KB_N };
+static const int KB_STAGE[KB_N] = { ST_CELL, ST_CELL, ST_CELL, ST_CELL, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE,
+                                    ST_FFTX, ST_FFTY, ST_MODE, ST_FFTX, ST_CELL, ST_ONE };
+static const char *KB_NAME[KB_N] = { "rhs", "rhs", "rhs", "rhs", "solve", "plane_sum", "mass", "cfl_finish",
+                                     "div_fftx", "ffty", "poisson", "fftx_inv", "project", "step_end" };
+enum { SOLVE_HEPT = 0, SOLVE_SPIKE = 1, SOLVE_THOMAS = 2 };
+
+struct cf_batch {
+    std::vector<cf_run *> runs;
+    int n = 0, prec = CF_FP64, order = 2, scalar = 0, device = 0;
+    hipStream_t stream = nullptr;
+    void *args = nullptr;                 /* device GArgs<T>[n] */
+    unsigned *vstart = nullptr;           /* device [CF_NST][n + 1] */
+    std::vector<unsigned> hv;
+    int solve_kind = SOLVE_HEPT;
+    unsigned kgrid[KB_N] = {0};
+    int spike_waves = 1, ncu = 256;
+    size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int steps_done = 0;
+    double dt_adhoc = -1.0;
+    bool profile = false;
+    std::map<std::string, std::pair<double, int>> ktime;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<std::string> tuning;      /* human-readable autotuner choices */
+};
+
+template <typename T>
+static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
+{
+    A.assign(b->n, GArgs<T>());
+    for (int q = 0; q < b->n; q++) {
+        cf_run *r = b->runs[q];
+        GArgs<T> &a = A[q];
+        a.R = r->rd;
+        a.u = tv<T>(r->u); a.v = tv<T>(r->v); a.w = tv<T>(r->w); a.c = tv<T>(r->c);
+        a.ru = tv<T>(r->ru); a.rv = tv<T>(r->rv); a.rw = tv<T>(r->rw); a.rc = tv<T>(r->rc);
+        for (int f = 0; f < 4; f++) { a.fa[f] = tv<T>(r->fa[f]); a.fb[f] = tv<T>(r->fb[f]); }
+        a.p = tv<T>(r->p);
+        a.X = tx<T>(r->X);
+        a.G = r->G; a.S4 = r->S4; a.scal = r->scal; a.logbuf = r->logbuf; a.part = r->part; a.cflpart = r->cflpart;
+        a.dt = r->cfg.dt;
+        a.dt_adhoc = r->pt_dt;
+        a.nb_plane = r->nb_plane;
+        a.gblocks = r->gblocks;
+        for (int t = 0; t < 4; t++) { a.pt[t] = r->pt[t]; a.pt4[t] = r->pt4[t]; }
+        const unsigned nx = r->g.nx, ny = r->g.ny, nz = r->g.nz;
+        const unsigned P = (unsigned) r->P, nf = (unsigned) r->nfield, kinds = r->cfg.scalar ? 2u : 1u;
+        auto set = [&](int st, unsigned x, unsigned y, unsigned z) { a.dims[st][0] = x; a.dims[st][1] = y; a.dims[st][2] = z; };
+        set(ST_CELL, (nx + 63) / 64, (ny + 3) / 4, nz);
+        if (b->solve_kind == SOLVE_SPIKE) set(ST_SOLVE, (P + 63) / 64, nf, 1);
+        else set(ST_SOLVE, (P + 255) / 256, nf, 1);
+        set(ST_PLANE, (unsigned) r->nb_plane, nz, kinds);
+        set(ST_MASS, kinds, 1, 1);
+        set(ST_ONE, 1, 1, 1);
+        long NL = (long) ny * nz, npairs = (NL + 1) / 2;
+        set(ST_FFTX, (unsigned) ((npairs + r->rd.fx_pairs - 1) / r->rd.fx_pairs), 1, 1);
+        set(ST_FFTY, fy_blocks(r), 1, 1);
+        set(ST_MODE, (unsigned) ((r->M + 255) / 256), 1, 1);
+    }
+}
+
+template <typename T>
+static cf_status batch_upload_tables(cf_batch *b)
+{
+    std::vector<GArgs<T>> A;
+    batch_fill_args<T>(b, A);
+    b->hv.assign((size_t) CF_NST * (b->n + 1), 0u);
+    for (int st = 0; st < CF_NST; st++) {
+        unsigned acc = 0;
+        for (int q = 0; q < b->n; q++) {
+            b->hv[(size_t) st * (b->n + 1) + q] = acc;
+            acc += A[q].dims[st][0] * A[q].dims[st][1] * A[q].dims[st][2];
+        }
+        b->hv[(size_t) st * (b->n + 1) + b->n] = acc;
+    }
+    HIPCHK(hipMemcpy(b->args, A.data(), sizeof(GArgs<T>) * A.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->vstart, b->hv.data(), sizeof(unsigned) * b->hv.size(), hipMemcpyHostToDevice));
+    return CF_OK;
+}
+
+static unsigned batch_total(const cf_batch *b, int st) { return b->hv[(size_t) st * (b->n + 1) + b->n]; }
+
+static cf_status bev_begin(cf_batch *b, const char *name)
+{
+    if (!b->profile) return CF_OK;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    b->pending.push_back({ std::string(name), { e0, e1 } });
+    return CF_OK;
+}
+
+static cf_status bev_end(cf_batch *b)
+{
+    if (!b->profile) return CF_OK;
+    HIPCHK(hipEventRecord(b->pending.back().second.second, b->stream));
+    return CF_OK;
+}
+
+/* launch kernel slot k of the batch for substep s (or FFT direction) */
+template <typename T>
+static cf_status batch_launch(cf_batch *b, int k, int s)
+{
+    typedef void (*kb_t)(const GArgs<T> *, const unsigned *, int, int);
+    const BSel<T> S = pick_batched<T>(b->order, b->scalar);
+    kb_t fn = nullptr;
+    dim3 block(256);
+    size_t lds = 0;
+    switch (k) {
+    case KB_RHS_U: fn = S.u; block = dim3(64, 4); break;
+    case KB_RHS_V: fn = S.v; block = dim3(64, 4); break;
+    case KB_RHS_W: fn = S.w; block = dim3(64, 4); break;
+    case KB_RHS_C: fn = S.c; block = dim3(64, 4); break;
+    case KB_PROJ: fn = S.proj; block = dim3(64, 4); break;
+    case KB_SOLVE:
+        if (b->solve_kind == SOLVE_HEPT) fn = kb_solve4<T>;
+        else if (b->solve_kind == SOLVE_SPIKE) { fn = kb_spike<T>; block = dim3(64 * b->spike_waves); }
+        else fn = kb_thomas<T>;
+        break;
+    case KB_PLANE: fn = kb_plane<T>; break;
+    case KB_MASS: fn = kb_mass<T>; lds = b->lds_mass; break;
+    case KB_CFL: fn = kb_cfl<T>; break;
+    case KB_DIVX: fn = S.divx; lds = b->lds_fx; break;
+    case KB_FFTY: fn = kb_ffty<T>; lds = b->lds_fy; break;
+    case KB_POIS: fn = b->order == 4 ? kb_poisson4<T> : kb_poisson<T>; break;
+    case KB_FFTXI: fn = kb_fftxi<T>; lds = b->lds_fx; break;
+    case KB_STEPEND: fn = kb_step_end<T>; break;
+    default: cf_set_error("bad batch kernel slot %d", k); return CF_ERR_ARG;
+    }
+    if (!fn) return CF_OK;
+    const int st = KB_STAGE[k];
+    const unsigned total = batch_total(b, st);
+    unsigned grid = b->kgrid[k] ? std::min(b->kgrid[k], total) : total;
+    if (grid == 0) return CF_OK;
+    cf_status e = bev_begin(b, KB_NAME[k]);
+    if (e != CF_OK) return e;
+    hipLaunchKernelGGL(fn, dim3(grid), block, lds, b->stream, (const GArgs<T> *) b->args,
+                       (const unsigned *) (b->vstart + (size_t) st * (b->n + 1)), b->n, s);
+    HIPCHK(hipGetLastError());
+    return bev_end(b);
+}
+
+template <typename T>
+static cf_status batch_enqueue_pressure(cf_batch *b, int s)
+{
+    cf_status st;
+    if ((st = batch_launch<T>(b, KB_DIVX, s)) != CF_OK) return st;
+    if ((st = batch_launch<T>(b, KB_FFTY, 1)) != CF_OK) return st;
+    if ((st = batch_launch<T>(b, KB_POIS, s)) != CF_OK) return st;
+    if ((st = batch_launch<T>(b, KB_FFTY, 0)) != CF_OK) return st;
+    return batch_launch<T>(b, KB_FFTXI, s);
+}
+
+template <typename T>
+static cf_status batch_enqueue_step(cf_batch *b)
+{
+    cf_status st;
+    for (int s = 0; s < 3; s++) {
+        if ((st = batch_launch<T>(b, KB_RHS_U, s)) != CF_OK) return st;
+        if (s == 0 && (st = batch_launch<T>(b, KB_CFL, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_RHS_V, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_RHS_W, s)) != CF_OK) return st;
+        if (b->scalar && (st = batch_launch<T>(b, KB_RHS_C, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_SOLVE, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_PLANE, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_MASS, s)) != CF_OK) return st;
+        if ((st = batch_enqueue_pressure<T>(b, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_PROJ, s)) != CF_OK) return st;
+    }
+    return batch_launch<T>(b, KB_STEPEND, 0);
+}
+
+static cf_status batch_collect(cf_batch *b)
+{
+    if (!b->profile) return CF_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto &pe : b->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+        auto &acc = b->ktime[pe.first];
+        acc.first += ms;
+        acc.second += 1;
+        (void) hipEventDestroy(pe.second.first);
+        (void) hipEventDestroy(pe.second.second);
+    }
+    b->pending.clear();
+    return CF_OK;
+}
+
+/* time one launch slot (median of 3) under the current tables and grid */
+template <typename T>
+static cf_status batch_time_slot(cf_batch *b, int k, float *ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float t[3];
+    cf_status st = batch_launch<T>(b, k, 0);              /* warm */
+    for (int i = 0; i < 3 && st == CF_OK; i++) {
+        HIPCHK(hipEventRecord(e0, b->stream));
+        st = batch_launch<T>(b, k, 0);
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&t[i], e0, e1));
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    if (st != CF_OK) return st;
+    std::sort(t, t + 3);
+    *ms = t[1];
+    return CF_OK;
+}
+
+/* the autotuner: for each launch slot, the persistent grid (one
+ * workgroup per virtual block, or a multiple of the CU count) and for
+ * order 2 the z-solve shape, by measured time on the batch's shapes */
+template <typename T>
+static cf_status batch_autotune(cf_batch *b)
+{
+    cf_status st;
+    b->tuning.clear();
+    if (b->order == 2) {
+        float best = 1e30f;
+        int bk = SOLVE_THOMAS;
+        bool spike_ok = true;
+        for (cf_run *r : b->runs) spike_ok = spike_ok && r->use_spike;
+        for (int kind : { SOLVE_SPIKE, SOLVE_THOMAS }) {
+            if (kind == SOLVE_SPIKE && !spike_ok) continue;
+            b->solve_kind = kind;
+            if ((st = batch_upload_tables<T>(b)) != CF_OK) return st;
+            b->kgrid[KB_SOLVE] = 0;
+            float ms;
+            if ((st = batch_time_slot<T>(b, KB_SOLVE, &ms)) != CF_OK) return st;
+            if (ms < best) { best = ms; bk = kind; }
+        }
+        b->solve_kind = bk;
+        b->tuning.push_back(std::string("solve: ") + (bk == SOLVE_SPIKE ? "spike (chunked, several wavefronts per system)"
+                                                                        : "thomas (one lane per system)"));
+    } else {
+        b->solve_kind = SOLVE_HEPT;
+    }
+    if ((st = batch_upload_tables<T>(b)) != CF_OK) return st;
+    /* slots whose repeated launch is harmless on a zero state and that do real work */
+    const int slots[] = { KB_RHS_U, KB_RHS_V, KB_RHS_W, KB_RHS_C, KB_SOLVE, KB_PLANE, KB_DIVX, KB_FFTY, KB_POIS,
+                          KB_FFTXI, KB_PROJ };
+    for (int k : slots) {
+        if (k == KB_RHS_C && !b->scalar) continue;
+        const unsigned total = batch_total(b, KB_STAGE[k]);
+        std::vector<unsigned> cand = { 0 };
+        for (int m : { 1, 2, 4, 8, 16 })
+            if ((unsigned) (b->ncu * m) < total) cand.push_back((unsigned) (b->ncu * m));
+        float best = 1e30f;
+        unsigned bg = 0;
+        for (unsigned g : cand) {
+            b->kgrid[k] = g;
+            float ms;
+            if ((st = batch_time_slot<T>(b, k, &ms)) != CF_OK) return st;
+            if (ms < best * 0.97f) { best = ms; bg = g; }
+        }
+        b->kgrid[k] = bg;
+        char line[160];
+        snprintf(line, sizeof line, "%s[%d]: grid %u of %u virtual blocks (%.1f us)", KB_NAME[k], k,
+                 bg ? bg : total, total, 1e3 * best);
+        b->tuning.push_back(line);
+    }
+    b->kgrid[KB_RHS_C] = b->kgrid[KB_RHS_C] ? b->kgrid[KB_RHS_C] : b->kgrid[KB_RHS_U];
+    /* the tuning launches wrote forcings, log entries and (through the
+     * scalar's boundary lift) state values: restore the zero state */
+    for (cf_run *r : b->runs) {
+        HIPCHK(hipMemset(r->scal, 0, sizeof(double) * SC_N));
+        HIPCHK(hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N));
+        double *zb[] = { r->u, r->v, r->w, r->c, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
+                         r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p };
+        for (double *z : zb)
+            if (z) HIPCHK(hipMemset(z, 0, r->esz * r->N));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    return CF_OK;
+}
+
+static void free_batch(cf_batch *b)
+{
+    if (!b) return;
+    if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void) hipGraphDestroy(b->graph);
+    if (b->args) (void) hipFree(b->args);
+    if (b->vstart) (void) hipFree(b->vstart);
+    for (auto &pe : b->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
+    if (b->stream) (void) hipStreamDestroy(b->stream);
+    for (cf_run *r : b->runs) free_run(r);
+    delete b;
+}
+
+extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch **out)
+{
+    if (!out || !cfgs) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    *out = nullptr;
+    if (nrun < 1 || nrun > CF_BATCH_MAX) { cf_set_error("a batch holds 1..%d runs", CF_BATCH_MAX); return CF_ERR_ARG; }
+    for (int q = 1; q < nrun; q++) {
+        if (cfgs[q].order != cfgs[0].order || cfgs[q].scalar != cfgs[0].scalar
+            || cfgs[q].precision != cfgs[0].precision || cfgs[q].device != cfgs[0].device) {
+            cf_set_error("run %d: runs of a batch share order, scalar, precision and device", q);
+            return CF_ERR_ARG;
+        }
+    }
+    cf_batch *b = new cf_batch();
+    b->n = nrun;
+    b->prec = cfgs[0].precision;
+    b->order = cfgs[0].order;
+    b->scalar = cfgs[0].scalar;
+    b->device = cfgs[0].device;
+    for (int q = 0; q < nrun; q++) {
+        cf_run *r = nullptr;
+        cf_status st = create_run(&cfgs[q], true, &r);
+        if (st != CF_OK) {
+            char msg[512];
+            snprintf(msg, sizeof msg, "run %d: %s", q, cf_last_error());
+            free_batch(b);
+            cf_set_error("%s", msg);
+            return st;
+        }
+        b->runs.push_back(r);
+        b->spike_waves = std::max(b->spike_waves, r->spike_waves);
+        b->lds_fx = std::max(b->lds_fx, fx_lds(r));
+        b->lds_fy = std::max(b->lds_fy, fy_lds(r));
+        b->lds_mass = std::max(b->lds_mass, sizeof(double) * (size_t) (r->g.nz + 6));
+    }
+    HIPCHK(hipSetDevice(b->device));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->ncu = prop.multiProcessorCount;
+    size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
+    if (hipMalloc(&b->args, asz * nrun) != hipSuccess
+        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_NST * (nrun + 1)) != hipSuccess
+        || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_batch(b);
+        cf_set_error("batch allocation failed");
+        return CF_ERR_NOMEM;
+    }
+    cf_status st = b->prec == CF_FP32 ? batch_autotune<float>(b) : batch_autotune<double>(b);
+    if (st != CF_OK) { free_batch(b); return st; }
+    *out = b;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_batch_destroy(cf_batch *b)
+{
+    if (!b) return CF_OK;
+    (void) hipSetDevice(b->device);
+    if (b->stream) (void) hipStreamSynchronize(b->stream);
+    free_batch(b);
+    return CF_OK;
+}
+
+#define CHECK_BATCH(b, run)                                                              \
+    do {                                                                                 \
+        if (!(b)) { cf_set_error("null batch handle"); return CF_ERR_ARG; }              \
+        if ((run) < 0 || (run) >= (b)->n) { cf_set_error("run %d out of range", (int) (run)); return CF_ERR_ARG; } \
+        HIPCHK(hipSetDevice((b)->device));                                               \
+        HIPCHK(hipStreamSynchronize((b)->stream));                                       \
+    } while (0)
+
+extern "C" cf_status cf_batch_upload(cf_batch *b, int run, int field, const double *host_io)
+{
+    CHECK_BATCH(b, run);
+    return cf_run_upload(b->runs[run], field, host_io);
+}
+
+extern "C" cf_status cf_batch_download(cf_batch *b, int run, int field, double *host_io)
+{
+    CHECK_BATCH(b, run);
+    return cf_run_download(b->runs[run], field, host_io);
+}
+
+extern "C" cf_status cf_batch_project(cf_batch *b, double dt)
+{
+    CHECK_BATCH(b, 0);
+    if (!(dt > 0)) { cf_set_error("cf_batch_project: dt must be positive"); return CF_ERR_ARG; }
+    for (cf_run *r : b->runs) {
+        cf_status st = set_adhoc_poisson(r, dt);
+        if (st != CF_OK) return st;
+    }
+    b->dt_adhoc = dt;
+    cf_status st = b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
+    if (st != CF_OK) return st;
+    if (b->prec == CF_FP32) {
+        if ((st = batch_enqueue_pressure<float>(b, 3)) != CF_OK) return st;
+        if ((st = batch_launch<float>(b, KB_PROJ, 3)) != CF_OK) return st;
+    } else {
+        if ((st = batch_enqueue_pressure<double>(b, 3)) != CF_OK) return st;
+        if ((st = batch_launch<double>(b, KB_PROJ, 3)) != CF_OK) return st;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return batch_collect(b);
+}
+
+extern "C" cf_status cf_batch_step(cf_batch *b, int nsteps)
+{
+    CHECK_BATCH(b, 0);
+    if (nsteps < 0) { cf_set_error("negative step count"); return CF_ERR_ARG; }
+    if (nsteps == 0) return CF_OK;
+    cf_status st;
+    if (b->profile) {
+        for (int n = 0; n < nsteps; n++) {
+            st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            if (st != CF_OK) return st;
+        }
+    } else {
+        if (!b->gexec) {
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &gr);
+            if (st != CF_OK) { if (gr) (void) hipGraphDestroy(gr); return st; }
+            HIPCHK(e);
+            b->graph = gr;
+            HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        for (int n = 0; n < nsteps; n++) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    }
+    b->steps_done += nsteps;
+    for (cf_run *r : b->runs) r->steps_done += nsteps;
+    return b->profile ? batch_collect(b) : CF_OK;
+}
+
+extern "C" cf_status cf_batch_sync(cf_batch *b)
+{
+    CHECK_BATCH(b, 0);
+    return CF_OK;
+}
+
+extern "C" cf_status cf_batch_log(cf_batch *b, int run, double *out, int max_steps, int *nsteps_out)
+{
+    CHECK_BATCH(b, run);
+    return cf_run_log(b->runs[run], out, max_steps, nsteps_out);
+}
+
+extern "C" cf_status cf_batch_max_divergence(cf_batch *b, int run, double *out)
+{
+    CHECK_BATCH(b, run);
+    return cf_run_max_divergence(b->runs[run], out);
+}
+
+extern "C" cf_status cf_batch_set_profiling(cf_batch *b, int on)
+{
+    if (!b) { cf_set_error("null batch handle"); return CF_ERR_ARG; }
+    b->profile = on != 0;
+    b->ktime.clear();
+    return CF_OK;
+}
+
+extern "C" cf_status cf_batch_kernel_time(cf_batch *b, const char *kernel, double *avg_ms, int *launches)
+{
+    if (!b || !kernel) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    auto it = b->ktime.find(kernel);
+    if (it == b->ktime.end()) { if (avg_ms) *avg_ms = 0; if (launches) *launches = 0; return CF_OK; }
+    if (avg_ms) *avg_ms = it->second.first / (it->second.second ? it->second.second : 1);
+    if (launches) *launches = it->second.second;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_batch_tuning(cf_batch *b, char *buf, size_t len)
+{
+    if (!b || !buf || len == 0) { cf_set_error("null argument"); return CF_ERR_ARG; }
+    std::string all;
+    for (auto &l : b->tuning) { all += l; all += "\n"; }
+    snprintf(buf, len, "%s", all.c_str());
     return CF_OK;
 }

@@ -114,6 +114,70 @@ def other_forms(steps):
     return forms
 
 
+def batch48(steps):
+    """The uneven batch: 16 runs each of 64x65x64, 96x97x96, 128x129x128
+    (x, wall-normal, span), nu seeded uniform in [1/6000, 1/2000], dt 5e-4,
+    order 2 FP64, stepped together; against the same 48 runs stepped one at
+    a time on the single-run path."""
+    rng = np.random.default_rng(2024)
+    cfgs = [cf.Config(nx=n, ny=n, nz=n + 1, stretch="tanh", nu=float(rng.uniform(1 / 6000, 1 / 2000)), dt=5e-4)
+            for n in (64, 96, 128) for _ in range(16)]
+    cells = sum(c.nx * c.ny * c.nz for c in cfgs)
+    fields = [cf.initial_fields(c, seed=300 + q) for q, c in enumerate(cfgs)]
+    t_create = time.perf_counter()
+    with cf.Batch(cfgs) as b:
+        t_create = time.perf_counter() - t_create
+        for q in range(len(cfgs)):
+            for n in "uvw":
+                b.upload(q, n, fields[q][n])
+        b.project(1.0)
+        b.step(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        secs = time.perf_counter() - t0
+        tuning = b.tuning()
+    t1 = 0.0
+    for q, c in enumerate(cfgs):
+        with cf.Run(c) as r:
+            for n in "uvw":
+                r.upload(n, fields[q][n])
+            r.project(1.0)
+            r.step(2)
+            r.sync()
+            t0 = time.perf_counter()
+            r.step(steps)
+            r.sync()
+            t1 += time.perf_counter() - t0
+    out = {"workload": "48 runs: 16 each of 64x65x64, 96x97x96, 128x129x128, FP64 order 2, one batch",
+           "value": cells / 1e6 * steps / secs, "unit": "Mpt*RK3-steps/s", "steps": steps,
+           "ms_per_step": 1e3 * secs / steps, "create_and_autotune_s": t_create,
+           "one_at_a_time": {"value": cells / 1e6 * steps / t1, "ms_per_step": 1e3 * t1 / steps},
+           "speedup_vs_one_at_a_time": t1 / secs, "autotuner": tuning}
+    print(f"[bench] batch48: {out['ms_per_step']:.3f} ms/step, one at a time {1e3 * t1 / steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    return out
+
+
+def served_stream(nreq):
+    """The served stream: Poisson arrivals at 20 req/s (seeded) of runs with
+    grid uniform over 64x65x64, 128x97x128, 192x129x192, 256x193x256, order
+    2 or 4 and FP32 or FP64 with p = 1/2, 50..500 steps, 200 GB HBM budget;
+    against the plain per-run loop on the same requests."""
+    from chanflow import serve
+    reqs = serve.poisson_stream(nreq, rate=20.0, seed=7)
+    res = serve.serve(reqs, budget=200 << 30, quantum=20)
+    res["server"].close()
+    s = serve.summary(res)
+    base = serve.summary(serve.plain_loop(reqs))
+    out = {"workload": f"{nreq} Poisson arrivals at 20 req/s, mixed grids / order 2|4 / FP32|FP64, 50-500 steps",
+           **s, "plain_per_run_loop": base, "throughput_vs_plain": s["value"] / base["value"]}
+    print(f"[bench] served: p50 {s['p50_s']:.2f} s p99 {s['p99_s']:.2f} s {s['value']:.0f} Mpt*steps/s; "
+          f"plain loop {base['value']:.0f}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(cfg, init, nsteps):
     """The CPU oracle (single thread) on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -141,6 +205,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=50, help="timed steps of each other form (0: skip)")
+    ap.add_argument("--served", type=int, default=24, help="requests of the served stream (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -203,6 +268,10 @@ def main():
     run.close()
 
     forms = other_forms(args.form_steps) if args.form_steps > 0 else {}
+    if args.form_steps > 0:
+        forms["batch48"] = batch48(20)
+    if args.served > 0:
+        forms["served_stream"] = served_stream(args.served)
     cpu = None if args.no_cpu else cpu_baseline(cfg, init, args.cpu_steps)
 
     out = {

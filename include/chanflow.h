@@ -194,6 +194,32 @@ cf_status cf_batch_kernel_time(cf_batch *batch, const char *kernel, double *avg_
 /* the autotuner's choices, one line per launch, into buf */
 cf_status cf_batch_tuning(cf_batch *batch, char *buf, size_t len);
 
+/* ---- serving: continuous batching of independent runs ----
+ *
+ * No counterpart in the reference (one run per process).  Requests are
+ * queued by cf_server_submit; each cf_server_tick admits queued requests in
+ * arrival order while their footprint (cf_run_footprint) fits the HBM
+ * budget, projects them once (dt = 1, as cf_run_project), steps every group
+ * of active runs sharing order / scalar / precision in batched launches
+ * (groups on separate streams), and retires runs that reached their step
+ * count.  Times are seconds since cf_server_create.  A run's result does not
+ * depend on which other runs share its ticks. */
+typedef struct cf_server cf_server;
+
+cf_status cf_server_create(size_t hbm_budget_bytes, int device, int quantum_steps, cf_server **out);
+cf_status cf_server_destroy(cf_server *server);
+/* copies the io-layout initial fields (c may be NULL without the scalar);
+ * retain != 0 keeps the finished run resident for cf_server_result */
+cf_status cf_server_submit(cf_server *server, const cf_config *cfg, const double *u, const double *v,
+                           const double *w, const double *c, int nsteps, int retain, int *request_id);
+cf_status cf_server_tick(cf_server *server, int *n_active, int *n_queued, int *n_done);
+cf_status cf_server_result(cf_server *server, int request_id, int field, double *host_io);
+cf_status cf_server_log(cf_server *server, int request_id, double *out, int max_steps, int *nsteps_out);
+cf_status cf_server_release(cf_server *server, int request_id);
+cf_status cf_server_request_info(cf_server *server, int request_id, double *t_submit, double *t_admit,
+                                 double *t_done, int *steps_done);
+cf_status cf_server_stats(cf_server *server, double *seconds, double *cell_steps, size_t *hbm_in_use);
+
 #ifdef __cplusplus
 }
 #endif

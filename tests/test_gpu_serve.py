@@ -1,0 +1,102 @@
+"""GPU tests of the served stream (cf_server_*): continuous batching of
+independent runs under an HBM budget.
+
+A run's result is bit-identical alone and in a mix that it joins while
+other runs are mid-flight (per-run reductions, fixed solve shape), and
+matches the oracle (FP64, <= 1e-12 relative L2); admission never exceeds
+the budget; a mixed stream of orders and precisions completes with its
+latencies reported.
+"""
+import numpy as np
+import pytest
+
+import chanflow as cf
+from chanflow import serve
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cfg_of(shape, order=2, prec=cf.CF_FP64, **kw):
+    base = dict(stretch="tanh", nu=1.0 / 2800.0, dt=5e-4)
+    base.update(kw)
+    return cf.Config(nx=shape[0], ny=shape[1], nz=shape[2], order=order, precision=prec, **base)
+
+
+def drain(srv):
+    while True:
+        a, q, d = srv.tick()
+        if a == 0 and q == 0:
+            return d
+
+
+def test_same_alone_and_in_the_mix():
+    mine = cfg_of((32, 24, 33))
+    f = cf.initial_fields(mine, seed=5)
+    with serve.Server(64 << 30) as srv:
+        a = srv.submit(mine, f, 30, retain=True)
+        drain(srv)
+        alone = {n: srv.result(a, n) for n in "uvw"}
+        alone_log = srv.log(a)
+    with serve.Server(64 << 30, quantum=7) as srv:
+        b = srv.submit(cfg_of((48, 32, 33), nu=1.0 / 4000.0), cf.initial_fields(cfg_of((48, 32, 33)), seed=6), 60)
+        c = srv.submit(cfg_of((24, 20, 33), order=4, prec=cf.CF_FP32), cf.initial_fields(cfg_of((24, 20, 33)), seed=7), 40)
+        srv.tick()
+        srv.tick()
+        a2 = srv.submit(mine, f, 30, retain=True)        # joins while b and c are mid-flight
+        d = srv.submit(cfg_of((16, 12, 17), nu=1.0 / 2000.0), cf.initial_fields(cfg_of((16, 12, 17)), seed=8), 25)
+        drain(srv)
+        mixed = {n: srv.result(a2, n) for n in "uvw"}
+        mixed_log = srv.log(a2)
+        for rid in (b, c, d):
+            assert srv.info(rid)[3] > 0
+    for n in "uvw":
+        assert np.array_equal(alone[n], mixed[n]), n
+    assert np.array_equal(alone_log, mixed_log)
+    dz = cf.grid_dz(mine)
+    orc = ol.OracleRun(ol.oracle_cfg(mine, dz))
+    for n in "uvw":
+        orc.put(n, f[n])
+    orc.project_ic(1.0)
+    orc.steps(30)
+    for n in "uvw":
+        assert rel(alone[n], orc.get(n)) <= 1e-12, n
+
+
+def test_admission_respects_the_budget():
+    cfgs = [cfg_of((48, 32, 33)) for _ in range(5)]
+    per = cf.footprint(cfgs[0])
+    with serve.Server(2 * per + per // 2) as srv:          # room for two at a time
+        for q, c in enumerate(cfgs):
+            srv.submit(c, cf.initial_fields(c, seed=q), 15 + 5 * q)
+        peak = 0
+        while True:
+            a, qd, d = srv.tick()
+            peak = max(peak, a)
+            _, _, used = srv.stats()
+            assert used <= 2 * per + per // 2
+            if a == 0 and qd == 0:
+                break
+        assert d == 5
+        assert peak == 2
+
+
+def test_mixed_stream_completes():
+    reqs = serve.poisson_stream(12, rate=50.0, seed=3, grids=((32, 24, 33), (48, 32, 33), (64, 48, 65)),
+                                steps=(20, 60))
+    res = serve.serve(reqs, budget=64 << 30, quantum=10, retain=True)
+    srv = res["server"]
+    s = serve.summary(res)
+    print("served stream:", s)
+    assert s["requests"] == 12 and s["p99_s"] >= s["p50_s"] > 0
+    for q, r in enumerate(reqs):
+        lg = srv.log(res["ids"][q])
+        assert lg.shape[0] == r.nsteps
+        assert np.allclose(lg[:, 6], 1.0, rtol=0, atol=1e-5 if r.cfg.precision == cf.CF_FP32 else 1e-12)
+        u = srv.result(res["ids"][q], "u")
+        assert np.isfinite(u).all()
+    srv.close()

@@ -892,58 +892,93 @@ template KSel<float>::fftxi_t pick_fftx_inv<float>(int);
  * pressure and the large-scale modes carry the wall-normal pressure
  * balance), the rest in FP32.
  * ------------------------------------------------------------------- */
+/* One mode's Thomas solve in two passes over Y without storing the
+ * factors: the forward sweep keeps only the recurrence state (d, 1/bet)
+ * entering every chunk of CF_P2_CH levels; the backward pass walks the
+ * chunks from the top, recomputes each chunk's forward sweep from its
+ * checkpoint into registers and back-substitutes.  HBM traffic is two
+ * reads and one write of Y (48 B per complex value) instead of writing
+ * the forward sweep and the factors and reading both back (80 B).  G holds
+ * the checkpoints: level slot 3 (c - 1) + {0, 1, 2} for chunk c >= 1. */
+#define CF_P2_CH 16
 template <typename A, typename C>
 static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restrict__ y, double *__restrict__ g,
                                                   size_t M, int nz, double lxd, double lyd, bool zero)
 {
-    constexpr int B = 8;
     const A lx = (A) lxd, ly = (A) lyd;
-    A ibet = 0, cprev = 0;
-    A dx_ = 0, dy_ = 0;
-    for (int k0 = 0; k0 < nz; k0 += B) {
-        C rb[B];
-#pragma unroll
-        for (int t = 0; t < B; t++)
-            if (k0 + t < nz) rb[t] = y[(size_t) (k0 + t) * M];
-#pragma unroll
-        for (int t = 0; t < B; t++) {
-            const int k = k0 + t;
-            if (k < nz) {
-                A b = (A) T.bz[k] - lx * (A) T.ex[k] - ly * (A) T.ey[k];
-                if (k == 0) b -= (A) (zero ? T.db_dir : T.db_neu);
-                if (k == nz - 1) b -= (A) T.db_top;
-                A gam = 0, bet = b;
-                if (k > 0) {
-                    gam = cprev * ibet;              /* c_{k-1} / bet_{k-1} */
-                    bet = b - (A) T.a[k] * gam;
-                    g[(size_t) k * M] = (double) gam;
-                }
-                ibet = (A) 1 / bet;
-                const A dv = (A) T.dvol[k];
-                const A ak = k > 0 ? (A) T.a[k] : (A) 0;
-                dx_ = ((A) rb[t].x * dv - ak * dx_) * ibet;
-                dy_ = ((A) rb[t].y * dv - ak * dy_) * ibet;
-                C o;
-                o.x = dx_; o.y = dy_;
-                y[(size_t) k * M] = o;
-                cprev = (A) T.c[k];
+    const int nch = (nz + CF_P2_CH - 1) / CF_P2_CH;
+    /* one level of the forward recurrence: (d, ibet) of level k-1 -> level k */
+    auto fwd = [&](int k, C rk, A &dx_, A &dy_, A &ibet, A &gam) {
+        A b = (A) T.bz[k] - lx * (A) T.ex[k] - ly * (A) T.ey[k];
+        if (k == 0) b -= (A) (zero ? T.db_dir : T.db_neu);
+        if (k == nz - 1) b -= (A) T.db_top;
+        A bet = b;
+        gam = 0;
+        if (k > 0) {
+            gam = (A) T.c[k - 1] * ibet;              /* c_{k-1} / bet_{k-1} */
+            bet = b - (A) T.a[k] * gam;
+        }
+        ibet = (A) 1 / bet;
+        const A dv = (A) T.dvol[k];
+        const A ak = k > 0 ? (A) T.a[k] : (A) 0;
+        dx_ = ((A) rk.x * dv - ak * dx_) * ibet;
+        dy_ = ((A) rk.y * dv - ak * dy_) * ibet;
+    };
+    {   /* pass 1: forward sweep, checkpoints only */
+        A dx_ = 0, dy_ = 0, ibet = 0, gam;
+        for (int c = 0; c < nch; c++) {
+            const int k0 = c * CF_P2_CH;
+            if (c > 0) {
+                g[(size_t) (3 * (c - 1) + 0) * M] = (double) dx_;
+                g[(size_t) (3 * (c - 1) + 1) * M] = (double) dy_;
+                g[(size_t) (3 * (c - 1) + 2) * M] = (double) ibet;
             }
+            C rb[CF_P2_CH];
+#pragma unroll
+            for (int t = 0; t < CF_P2_CH; t++)
+                if (k0 + t < nz) rb[t] = y[(size_t) (k0 + t) * M];
+#pragma unroll
+            for (int t = 0; t < CF_P2_CH; t++)
+                if (k0 + t < nz) fwd(k0 + t, rb[t], dx_, dy_, ibet, gam);
         }
     }
-    for (int k0 = nz - 2; k0 >= 0; k0 -= B) {
-        C yb[B];
-        double gb[B];
+    /* pass 2: chunks from the top, recompute then back-substitute */
+    A xnx = 0, xny = 0, gnext = 0;
+    for (int c = nch - 1; c >= 0; c--) {
+        const int k0 = c * CF_P2_CH;
+        A dx_ = 0, dy_ = 0, ibet = 0;
+        if (c > 0) {
+            dx_ = (A) g[(size_t) (3 * (c - 1) + 0) * M];
+            dy_ = (A) g[(size_t) (3 * (c - 1) + 1) * M];
+            ibet = (A) g[(size_t) (3 * (c - 1) + 2) * M];
+        }
+        C rb[CF_P2_CH];
+        A dxs[CF_P2_CH], dys[CF_P2_CH], gs[CF_P2_CH];
 #pragma unroll
-        for (int t = 0; t < B; t++)
-            if (k0 - t >= 0) { yb[t] = y[(size_t) (k0 - t) * M]; gb[t] = g[(size_t) (k0 - t + 1) * M]; }
+        for (int t = 0; t < CF_P2_CH; t++)
+            if (k0 + t < nz) rb[t] = y[(size_t) (k0 + t) * M];
 #pragma unroll
-        for (int t = 0; t < B; t++) {
-            const int k = k0 - t;
-            if (k >= 0) {
-                dx_ = (A) yb[t].x - (A) gb[t] * dx_;
-                dy_ = (A) yb[t].y - (A) gb[t] * dy_;
+        for (int t = 0; t < CF_P2_CH; t++) {
+            if (k0 + t < nz) {
+                A gam;
+                fwd(k0 + t, rb[t], dx_, dy_, ibet, gam);
+                dxs[t] = dx_; dys[t] = dy_; gs[t] = gam;
+            }
+        }
+#pragma unroll
+        for (int t = CF_P2_CH - 1; t >= 0; t--) {
+            const int k = k0 + t;
+            if (k < nz) {
+                if (k < nz - 1) {
+                    xnx = dxs[t] - gnext * xnx;
+                    xny = dys[t] - gnext * xny;
+                } else {
+                    xnx = dxs[t];
+                    xny = dys[t];
+                }
+                gnext = gs[t];
                 C o;
-                o.x = dx_; o.y = dy_;
+                o.x = xnx; o.y = xny;
                 y[(size_t) k * M] = o;
             }
         }

@@ -704,7 +704,9 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     for (int f = 0; f < 3; f++) ok = ok && falloc(&r->fa[f], r->N) && falloc(&r->fb[f], r->N);
     if (cfg->scalar) ok = ok && falloc(&r->c, r->N) && falloc(&r->rc, r->N) && falloc(&r->fa[3], r->N) && falloc(&r->fb[3], r->N);
     if (order == 4) ok = ok && dalloc(&r->S4, 3 * r->M * nz);
-    ok = ok && falloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * nz);
+    /* G: the order-2 Poisson's chunk checkpoints (3 per chunk of 16 levels) per mode */
+    const size_t glev = std::max((size_t) nz, (size_t) 3 * ((nz + 15) / 16));
+    ok = ok && falloc(&r->p, r->N) && dalloc(&r->stage, r->N) && dalloc(&r->G, r->M * glev);
     ok = ok && hipMalloc((void **) &r->X, 2 * r->esz * r->M * nz) == hipSuccess;
     ok = ok && dalloc(&r->tab, r->tab_n) && dalloc(&r->scal, SC_N) && dalloc(&r->logbuf, (size_t) LOGCAP * CF_LOG_N);
     r->nb_plane = (int) ((r->P + 2047) / 2048);
@@ -1904,5 +1906,288 @@ extern "C" cf_status cf_batch_tuning(cf_batch *b, char *buf, size_t len)
     std::string all;
     for (auto &l : b->tuning) { all += l; all += "\n"; }
     snprintf(buf, len, "%s", all.c_str());
+    return CF_OK;
+}
+
+/* ------------------------------------------------------------ serving
+ *
+ * Continuous batching of independent channel runs.  Requests queue on
+ * submit; every tick admits queued requests in arrival order while their
+ * device footprint fits the HBM budget (fields stay resident until the
+ * run retires), then steps each group of active runs that share order,
+ * scalar setting and precision with one batched launch per stage, on the
+ * group's own stream so groups overlap.  A tick advances a group by the
+ * smallest remaining step count of its runs (at most the quantum), so runs
+ * retire exactly at their step count and new arrivals join at the next
+ * tick.  Per-run arithmetic does not depend on the other runs of the
+ * group (per-run reductions, fixed solve shape), so a run's result is the
+ * same alone and in any mix.
+ */
+
+#include <chrono>
+
+struct cf_server {
+    size_t budget = 0, used = 0;
+    int device = 0, quantum = 20;
+    std::chrono::steady_clock::time_point t0;
+    struct Req {
+        cf_config cfg;
+        std::vector<double> init[4];
+        int nsteps = 0, done = 0, state = 0;       /* 0 queued, 1 active, 2 done (retained), 3 released */
+        bool retain = false;
+        size_t bytes = 0;
+        cf_run *run = nullptr;
+        double t_submit = 0, t_admit = -1, t_done = -1;
+    };
+    std::vector<Req> reqs;
+    std::vector<int> queue;
+    std::map<int, cf_batch *> groups;              /* key -> non-owning batch of the group's active runs */
+    double pt_steps = 0.0;                         /* completed cell-steps */
+};
+
+static double srv_now(const cf_server *s)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+static int group_key(const cf_config &c) { return (c.order == 4 ? 4 : 0) + (c.scalar ? 2 : 0) + (c.precision == CF_FP32 ? 1 : 0); }
+
+extern "C" cf_status cf_server_create(size_t hbm_budget_bytes, int device, int quantum_steps, cf_server **out)
+{
+    if (!out || hbm_budget_bytes == 0 || quantum_steps < 1) { cf_set_error("bad server arguments"); return CF_ERR_ARG; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        cf_set_error("no HIP device %d available", device);
+        return CF_ERR_DEVICE;
+    }
+    cf_server *s = new cf_server();
+    s->budget = hbm_budget_bytes;
+    s->device = device;
+    s->quantum = quantum_steps;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return CF_OK;
+}
+
+static void srv_drop_group(cf_batch *b)
+{
+    if (!b) return;
+    b->runs.clear();                               /* runs belong to their requests */
+    free_batch(b);
+}
+
+extern "C" cf_status cf_server_destroy(cf_server *s)
+{
+    if (!s) return CF_OK;
+    (void) hipSetDevice(s->device);
+    (void) hipDeviceSynchronize();
+    for (auto &g : s->groups) srv_drop_group(g.second);
+    for (auto &r : s->reqs)
+        if (r.run) free_run(r.run);
+    delete s;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_server_submit(cf_server *s, const cf_config *cfg, const double *u, const double *v,
+                                      const double *w, const double *c, int nsteps, int retain, int *request_id)
+{
+    if (!s || !cfg || !u || !v || !w || nsteps < 1) { cf_set_error("bad submit arguments"); return CF_ERR_ARG; }
+    if (cfg->scalar && !c) { cf_set_error("a scalar run needs its c field"); return CF_ERR_ARG; }
+    if (cfg->device != s->device) { cf_set_error("request device %d differs from the server's", cfg->device); return CF_ERR_ARG; }
+    size_t bytes = 0;
+    cf_status st = cf_run_footprint(cfg, &bytes);
+    if (st != CF_OK) return st;
+    if (bytes > s->budget) { cf_set_error("request needs %zu bytes, more than the whole budget", bytes); return CF_ERR_NOMEM; }
+    cf_server::Req r;
+    r.cfg = *cfg;
+    r.cfg.dz_user = nullptr;
+    if (cfg->stretch == CF_STRETCH_USER) { cf_set_error("the server takes built-in stretchings only"); return CF_ERR_UNSUPPORTED; }
+    const size_t n = (size_t) cfg->nx * cfg->ny * cfg->nz;
+    const double *src[4] = { u, v, w, cfg->scalar ? c : nullptr };
+    for (int f = 0; f < 4; f++)
+        if (src[f]) r.init[f].assign(src[f], src[f] + n);
+    r.nsteps = nsteps;
+    r.retain = retain != 0;
+    r.bytes = bytes;
+    r.t_submit = srv_now(s);
+    s->reqs.push_back(std::move(r));
+    s->queue.push_back((int) s->reqs.size() - 1);
+    if (request_id) *request_id = (int) s->reqs.size() - 1;
+    return CF_OK;
+}
+
+/* rebuild the group batch of key k from the currently active runs */
+static cf_status srv_regroup(cf_server *s, int k)
+{
+    std::vector<cf_run *> runs;
+    for (auto &r : s->reqs)
+        if (r.state == 1 && group_key(r.cfg) == k) runs.push_back(r.run);
+    auto it = s->groups.find(k);
+    if (runs.empty()) {
+        if (it != s->groups.end()) { srv_drop_group(it->second); s->groups.erase(it); }
+        return CF_OK;
+    }
+    cf_batch *b = it != s->groups.end() ? it->second : nullptr;
+    if (!b) {
+        b = new cf_batch();
+        b->prec = runs[0]->cfg.precision;
+        b->order = runs[0]->cfg.order;
+        b->scalar = runs[0]->cfg.scalar;
+        b->device = s->device;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
+            b->ncu = prop.multiProcessorCount;
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            cf_set_error("stream creation failed");
+            return CF_ERR_DEVICE;
+        }
+        s->groups[k] = b;
+    }
+    if (b->args) { (void) hipFree(b->args); b->args = nullptr; }
+    if (b->vstart) { (void) hipFree(b->vstart); b->vstart = nullptr; }
+    b->runs = runs;
+    b->n = (int) runs.size();
+    if (b->n > CF_BATCH_MAX) { cf_set_error("more than %d active runs in one group", CF_BATCH_MAX); return CF_ERR_UNSUPPORTED; }
+    b->spike_waves = 1;
+    b->lds_fx = b->lds_fy = b->lds_mass = 0;
+    bool spike_ok = true;
+    for (cf_run *r : runs) {
+        b->spike_waves = std::max(b->spike_waves, r->spike_waves);
+        b->lds_fx = std::max(b->lds_fx, fx_lds(r));
+        b->lds_fy = std::max(b->lds_fy, fy_lds(r));
+        b->lds_mass = std::max(b->lds_mass, sizeof(double) * (size_t) (r->g.nz + 6));
+        spike_ok = spike_ok && r->use_spike;
+    }
+    b->solve_kind = b->order == 4 ? SOLVE_HEPT : (spike_ok ? SOLVE_SPIKE : SOLVE_THOMAS);
+    for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
+    size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
+    HIPCHK(hipMalloc(&b->args, asz * b->n));
+    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_NST * (b->n + 1)));
+    return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
+}
+
+extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, int *n_done)
+{
+    if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->device));
+    cf_status st;
+    std::vector<int> changed;
+    /* admit in arrival order while the budget allows (head-of-line: a large
+     * request waits for memory rather than being overtaken indefinitely) */
+    while (!s->queue.empty()) {
+        cf_server::Req &r = s->reqs[s->queue.front()];
+        if (s->used + r.bytes > s->budget) break;
+        cf_run *run = nullptr;
+        if ((st = create_run(&r.cfg, true, &run)) != CF_OK) return st;
+        for (int f = 0; f < 4; f++) {
+            if (r.init[f].empty()) continue;
+            if ((st = cf_run_upload(run, f, r.init[f].data())) != CF_OK) { free_run(run); return st; }
+            std::vector<double>().swap(r.init[f]);
+        }
+        if ((st = cf_run_project(run, 1.0)) != CF_OK) { free_run(run); return st; }
+        r.run = run;
+        r.state = 1;
+        r.t_admit = srv_now(s);
+        s->used += r.bytes;
+        changed.push_back(group_key(r.cfg));
+        s->queue.erase(s->queue.begin());
+    }
+    for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
+    /* one quantum per group, all groups in flight together */
+    std::map<int, int> adv;
+    for (auto &g : s->groups) {
+        cf_batch *b = g.second;
+        int k = s->quantum;
+        for (auto &r : s->reqs)
+            if (r.state == 1 && group_key(r.cfg) == g.first) k = std::min(k, r.nsteps - r.done);
+        for (int n = 0; n < k; n++) {
+            st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            if (st != CF_OK) return st;
+        }
+        adv[g.first] = k;
+    }
+    for (auto &g : s->groups) HIPCHK(hipStreamSynchronize(g.second->stream));
+    std::vector<int> retire;
+    const double now = srv_now(s);
+    for (auto &r : s->reqs) {
+        if (r.state != 1) continue;
+        const int k = adv[group_key(r.cfg)];
+        r.done += k;
+        r.run->steps_done += k;
+        s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
+        if (r.done >= r.nsteps) {
+            r.t_done = now;
+            r.state = r.retain ? 2 : 3;
+            if (!r.retain) {
+                free_run(r.run);
+                r.run = nullptr;
+                s->used -= r.bytes;
+            }
+            retire.push_back(group_key(r.cfg));
+        }
+    }
+    for (int k : retire) if ((st = srv_regroup(s, k)) != CF_OK) return st;
+    int na = 0, nd = 0;
+    for (auto &r : s->reqs) { na += r.state == 1; nd += r.state >= 2; }
+    if (n_active) *n_active = na;
+    if (n_queued) *n_queued = (int) s->queue.size();
+    if (n_done) *n_done = nd;
+    return CF_OK;
+}
+
+static cf_server::Req *srv_req(cf_server *s, int id)
+{
+    if (!s || id < 0 || id >= (int) s->reqs.size()) { cf_set_error("bad request id %d", id); return nullptr; }
+    return &s->reqs[id];
+}
+
+extern "C" cf_status cf_server_result(cf_server *s, int request_id, int field, double *host_io)
+{
+    cf_server::Req *r = srv_req(s, request_id);
+    if (!r) return CF_ERR_ARG;
+    if (r->state != 2 || !r->run) { cf_set_error("request %d holds no retained result", request_id); return CF_ERR_STATE; }
+    return cf_run_download(r->run, field, host_io);
+}
+
+extern "C" cf_status cf_server_log(cf_server *s, int request_id, double *out, int max_steps, int *nsteps_out)
+{
+    cf_server::Req *r = srv_req(s, request_id);
+    if (!r) return CF_ERR_ARG;
+    if (!r->run) { cf_set_error("request %d is not resident", request_id); return CF_ERR_STATE; }
+    return cf_run_log(r->run, out, max_steps, nsteps_out);
+}
+
+extern "C" cf_status cf_server_release(cf_server *s, int request_id)
+{
+    cf_server::Req *r = srv_req(s, request_id);
+    if (!r) return CF_ERR_ARG;
+    if (r->state == 2 && r->run) {
+        free_run(r->run);
+        r->run = nullptr;
+        s->used -= r->bytes;
+        r->state = 3;
+    }
+    return CF_OK;
+}
+
+extern "C" cf_status cf_server_request_info(cf_server *s, int request_id, double *t_submit, double *t_admit,
+                                            double *t_done, int *steps_done)
+{
+    cf_server::Req *r = srv_req(s, request_id);
+    if (!r) return CF_ERR_ARG;
+    if (t_submit) *t_submit = r->t_submit;
+    if (t_admit) *t_admit = r->t_admit;
+    if (t_done) *t_done = r->t_done;
+    if (steps_done) *steps_done = r->done;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_server_stats(cf_server *s, double *seconds, double *cell_steps, size_t *hbm_in_use)
+{
+    if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
+    if (seconds) *seconds = srv_now(s);
+    if (cell_steps) *cell_steps = s->pt_steps;
+    if (hbm_in_use) *hbm_in_use = s->used;
     return CF_OK;
 }

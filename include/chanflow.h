@@ -201,13 +201,19 @@ cf_status cf_batch_tuning(cf_batch *batch, char *buf, size_t len);
  * arrival order while their footprint (cf_run_footprint) fits the HBM
  * budget, projects them once (dt = 1, as cf_run_project), steps every group
  * of active runs sharing order / scalar / precision in batched launches
- * (groups on separate streams), and retires runs that reached their step
- * count.  Times are seconds since cf_server_create.  A run's result does not
- * depend on which other runs share its ticks. */
+ * (groups on separate streams), steps each run of at least the solo size
+ * (cf_server_set_solo_points, default 4 Mi cells) through its own
+ * single-run graph on its own stream, all in flight together, and retires
+ * runs that reached their step count.  Times are seconds since
+ * cf_server_create.  A run's result does not depend on which other runs
+ * share its ticks. */
 typedef struct cf_server cf_server;
 
 cf_status cf_server_create(size_t hbm_budget_bytes, int device, int quantum_steps, cf_server **out);
 cf_status cf_server_destroy(cf_server *server);
+/* runs with nx ny nz >= points step on their own (0: every run; SIZE_MAX:
+ * none); applies to requests admitted afterwards */
+cf_status cf_server_set_solo_points(cf_server *server, size_t points);
 /* copies the io-layout initial fields (c may be NULL without the scalar);
  * retain != 0 keeps the finished run resident for cf_server_result */
 cf_status cf_server_submit(cf_server *server, const cf_config *cfg, const double *u, const double *v,

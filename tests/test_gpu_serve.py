@@ -73,15 +73,24 @@ def test_admission_respects_the_budget():
     with serve.Server(2 * per + per // 2) as srv:          # room for two at a time
         for q, c in enumerate(cfgs):
             srv.submit(c, cf.initial_fields(c, seed=q), 15 + 5 * q)
-        peak = 0
         while True:
             a, qd, d = srv.tick()
-            peak = max(peak, a)
             _, _, used = srv.stats()
             assert used <= 2 * per + per // 2
             if a == 0 and qd == 0:
                 break
         assert d == 5
+        # tick() reports the count after its retirements, so read concurrency
+        # from the residency intervals [t_admit, t_done] instead
+        ev = []
+        for q in range(5):
+            _, ta, td, n = srv.info(q)
+            assert n == 15 + 5 * q
+            ev += [(ta, 1), (td, -1)]
+        peak = cur = 0
+        for _, e in sorted(ev, key=lambda x: (x[0], x[1])):
+            cur += e
+            peak = max(peak, cur)
         assert peak == 2
 
 

@@ -23,6 +23,7 @@ def _bind():
     sig = {
         "cf_server_create": (C.c_int, [C.c_size_t, C.c_int, C.c_int, C.POINTER(P)]),
         "cf_server_destroy": (C.c_int, [P]),
+        "cf_server_set_solo_points": (C.c_int, [P, C.c_size_t]),
         "cf_server_submit": (C.c_int, [P, C.POINTER(_CConfig), dp, dp, dp, dp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "cf_server_tick": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "cf_server_result": (C.c_int, [P, C.c_int, C.c_int, dp]),
@@ -41,14 +42,18 @@ def _bind():
 
 class Server:
     """A cf_server handle: HBM-budgeted admission, resident state,
-    continuous batching per (order, scalar, precision) group."""
+    continuous batching per (order, scalar, precision) group; runs of at
+    least solo_points cells (None: the library default) step through their
+    own graphs beside the batches."""
 
-    def __init__(self, hbm_budget_bytes=200 << 30, device=0, quantum=20):
+    def __init__(self, hbm_budget_bytes=200 << 30, device=0, quantum=20, solo_points=None):
         L = _bind()
         h = C.c_void_p()
         _check(L.cf_server_create(int(hbm_budget_bytes), device, quantum, C.byref(h)))
         self._h = h
         self.cfgs = {}
+        if solo_points is not None:
+            _check(L.cf_server_set_solo_points(h, int(solo_points)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -138,12 +143,12 @@ def poisson_stream(n, rate=20.0, seed=7, grids=GRIDS, steps=(50, 500), p_order4=
     return out
 
 
-def serve(requests, budget=200 << 30, quantum=20, retain=False, time_scale=1.0):
+def serve(requests, budget=200 << 30, quantum=20, retain=False, time_scale=1.0, solo_points=None):
     """Drive a Server in real time: each request is submitted once its
     arrival time (scaled by time_scale) has passed; tick until all are
     done.  Returns per-request latencies (completion - arrival, s), total
     wall time, cell-steps and the server for retained results."""
-    srv = Server(budget, quantum=quantum)
+    srv = Server(budget, quantum=quantum, solo_points=solo_points)
     pending = sorted(range(len(requests)), key=lambda q: requests[q].t_arrive)
     ids = {}
     t0 = time.perf_counter()

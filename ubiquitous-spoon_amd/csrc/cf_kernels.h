@@ -127,6 +127,26 @@ struct PoisTab {
     double db_neu, db_dir, db_top;
 };
 
+/* order-2 Poisson: the partitioned solve (at most 16 z-chunks of L rows
+ * per mode, 8 modes per 256-thread workgroup), L a compiled chunk length
+ * >= nz / 16, covers nz <= 512; beyond that one thread per mode (256
+ * modes per workgroup). */
+#define CF_P2S_LSET(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+                       X(20) X(24) X(28) X(32)
+static inline int pois2_L_of(int need)
+{
+    static const int set[] = { 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 20, 24, 28, 32 };
+    for (int l : set)
+        if (l >= need) return l;
+    return 0;
+}
+/* chunk length for nz (0: one thread per mode) */
+static inline int pois2_L(int nz) { return pois2_L_of((nz + 15) / 16); }
+static inline unsigned pois2_blocks(int L, size_t M)
+{
+    return (unsigned) (L > 0 ? (M + 7) / 8 : (M + 255) / 256);
+}
+
 /* One run of a batched launch: its parameter block, buffers (storage
  * precision T), Poisson tables and per-stage grid shapes.  An array of
  * these lives in device memory; batched kernels read the entry of the run
@@ -190,9 +210,10 @@ __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
                         double *cflpart, double cfldt, int kc);
 template <typename T> __global__ void k_thomas(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> __global__ void k_spike(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
-__global__ void k_plane_sum(const RunDev R, const double *q, double bc, double *part, int nb);
-__global__ void k_mass_finalize(const RunDev R, const int s, const double *part, int nb,
-                                double *scal, double *logbuf, int kind);
+__global__ void k_spike_sums(const RunDev R, const int s, double *ru, double *rv, double *rw, double *plane);
+__global__ void k_level_sum(const double *plane, int nbx, double *lsum);
+__global__ void k_mass_finalize(const RunDev R, const int s, const double *lsum, double *scal, double *logbuf,
+                                int kind);
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);
@@ -202,13 +223,14 @@ template <typename T> struct KSel {
     typedef void (*div_t)(const RunDev, const T *, const T *, const T *, const double *, const double *, C *);
     typedef void (*ffty_t)(const RunDev, C *, int);
     typedef void (*fftxi_t)(const RunDev, const C *, T *);
+    typedef void (*pois2_t)(const RunDev, const PoisTab, C *, double *);
 };
 typedef void (*k_fftx_plain_t)(const RunDev, const double *, double2 *);
 template <typename T> typename KSel<T>::div_t pick_div_fftx(int nx, int order);
 k_fftx_plain_t pick_fftx_plain(int nx);
 template <typename T> typename KSel<T>::ffty_t pick_ffty(int ny);
 template <typename T> typename KSel<T>::fftxi_t pick_fftx_inv(int nx);
-template <typename T> __global__ void k_poisson2(const RunDev R, const PoisTab T_, typename Cplx<T>::type *Y, double *G);
+template <typename T> typename KSel<T>::pois2_t pick_poisson2(int lmax);
 __global__ void k_project2(const RunDev R, const double *us, const double *vs, const double *ws,
                            const double *h1, const double *mptr, const double *p, double dtp,
                            double *u, double *v, double *w);
@@ -234,6 +256,7 @@ template <typename T> struct BSel {
     rhs_t u, v, w, c;
     rhs_t proj;
     divx_t_ divx;
+    typedef void (*kb_t)(const GArgs<T> *, const unsigned *, int, int);
 };
 template <typename T> BSel<T> pick_batched(int order, int scalar);
 template <typename T> typename BSel<T>::divx_t_ pick_batched_divx(int order);
@@ -247,7 +270,7 @@ template <typename T> __global__ void kb_cfl(const GArgs<T> *a, const unsigned *
 template <typename T> __global__ void kb_step_end(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
 template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
-template <typename T> __global__ void kb_poisson(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);
 template <typename T> __global__ void k_gsolve4(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> __global__ void k_gplane(const RunDev R, const T *q0, const T *q1, double *part, int nb);
 __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);

@@ -414,9 +414,22 @@ template __global__ void k_thomas<float>(const RunDev, const int, float *, float
  * Same system as solve_helmholtz1d_u2 / _w2 (diffuseu.c:173-219,
  * diffusew.c:175-223); differs from sequential Thomas by round-off.
  * ------------------------------------------------------------------- */
+static __device__ __forceinline__ double wave_sum(double v)
+{
+    /* fixed butterfly over the 64 lanes: every lane ends with the same,
+     * order-independent-of-scheduling sum */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* plane != null (field 0 only): also write, per level, the sum of the
+ * solved u* over the workgroup's 64 columns to plane[level * b.nx + b.x],
+ * the first stage of get_xy_avg (get.c:281-307) */
 template <typename T>
 static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
-    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b)
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b,
+    double *__restrict__ plane = nullptr)
 {
     __shared__ T itf[2 * CF_SPIKE_SMAX][64];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
@@ -464,23 +477,32 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
         itf[2 * chunk + 1][lane] = yl;
     }
     __syncthreads();
-    if (!active) return;
+    if (chunk >= ST.S) return;                    /* wave-uniform */
+    const bool sums = plane != nullptr && f == 0;
+    if (!active && !sums) return;
     const int m = 2 * ST.S;
     T E = 0, F = 0;
-    if (chunk > 0) {
+    if (active && chunk > 0) {
         const double *row = ST.minv + (size_t) (2 * (chunk - 1) + 1) * m;
         for (int q = 0; q < m; q++) E += (T) row[q] * itf[q][lane];
     }
-    if (chunk < ST.S - 1) {
+    if (active && chunk < ST.S - 1) {
         const double *row = ST.minv + (size_t) (2 * (chunk + 1)) * m;
         for (int q = 0; q < m; q++) F += (T) row[q] * itf[q][lane];
     }
 #pragma unroll
     for (int t = 0; t < CF_SPIKE_LMAX; t++) {
         const int j = lo + t;
-        if (j < hi) xc[(size_t) j * P] = y[t] - E * (T) ST.vl[j] - F * (T) ST.wr[j];
+        if (j < hi) {
+            const T val = y[t] - E * (T) ST.vl[j] - F * (T) ST.wr[j];
+            if (active) xc[(size_t) j * P] = val;
+            if (sums) {
+                const double tot = wave_sum(active ? (double) val : 0.0);
+                if (lane == 0) plane[(size_t) j * b.nx + b.x] = tot;
+            }
+        }
     }
-    if (f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
+    if (active && f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
 }
 
 template <typename T>
@@ -490,69 +512,60 @@ __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
     spike_body<T>(R, s, ru, rv, rw, rc, blk_native());
 }
 
-template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
-template __global__ void k_spike<float>(const RunDev, const int, float *, float *, float *, float *);
+/* the fast path's solve: k_spike with the plane sums of u* fused in */
+__global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s, double *__restrict__ ru,
+    double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ plane)
+{
+    spike_body<double>(R, s, ru, rv, rw, nullptr, blk_native(), plane);
+}
 
-/* ---------------------------------------------------------------------
- * Plane sums for get_xy_avg (get.c:281-307) of the solved u: levels
- * -1 and nz are the wall ghosts 2 bc - u (apply_bc_u2 after the solve,
- * chanfast.c:482-483).  Deterministic: block b sums a fixed chunk of the
- * plane in a fixed tree order; the final sum over blocks is sequential.
- * part[(k + 1) * nb + b].
- * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_plane_sum(const RunDev R, const double *__restrict__ q,
-                                                   double bc, double *__restrict__ part, int nb)
+/* second stage of the plane sums: level blockIdx.x, nbx partials in a
+ * fixed tree */
+__global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ plane, int nbx,
+                                                   double *__restrict__ lsum)
 {
     __shared__ double red[256];
-    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
-    const size_t P = (size_t) nx * ny;
-    const int lev = blockIdx.y - 1;              /* -1 .. nz */
-    const int b = blockIdx.x;
-    if (lev > nz || b >= nb) return;
-    const size_t chunk = (P + nb - 1) / nb;
-    const size_t lo = (size_t) b * chunk, hi = lo + chunk < P ? lo + chunk : P;
-    const int src = lev < 0 ? 0 : (lev >= nz ? nz - 1 : lev);
-    const bool ghost = lev < 0 || lev >= nz;
-    const double *pl = q + (size_t) src * P;
+    const int l = blockIdx.x;
     double acc = 0.0;
-    for (size_t e = lo + threadIdx.x; e < hi; e += 256) {
-        double val = pl[e];
-        acc += ghost ? (2.0 * bc - val) : val;
-    }
+    for (int q = threadIdx.x; q < nbx; q += 256) acc += plane[(size_t) l * nbx + q];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[(size_t) (lev + 1) * nb + b] = red[0];
+    if (threadIdx.x == 0) lsum[l] = red[0];
 }
+
+template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
+template __global__ void k_spike<float>(const RunDev, const int, float *, float *, float *, float *);
+
 
 /* ---------------------------------------------------------------------
  * Mass-flux constraint (chanfast.c:485-530): plane averages -> bulk
  * velocity (get_z_avg, get.c:490-502), wall shear, centreline; forcing
  * mdpdx = (Ub - uframe - U0) / U1 with U1 from the column-independent
- * unit-forcing solution h1 (host-precomputed per substep).  Writes the
- * forcing into scal[slot]; on the last substep writes the pstat record.
- * One block.
+ * unit-forcing solution h1 (host-precomputed per substep).  Input: the
+ * per-level plane sums of the solved field (k_spike_sums, k_level_sum);
+ * the wall-ghost averages are 2 bc - the adjacent level (apply_bc_u2).
+ * Writes the forcing into scal[slot]; on the last substep writes the
+ * pstat record.  One block.
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int s, const double *__restrict__ part,
-                                                       int nb, double *__restrict__ scal, double *__restrict__ logbuf,
-                                                       int kind)
+__global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int s, const double *__restrict__ lsum,
+                                                       double *__restrict__ scal, double *__restrict__ logbuf, int kind)
 {
     extern __shared__ __align__(16) unsigned char smem_[];
     double *avg = (double *) smem_;               /* nz + 2 */
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const double *dz = R.dzt + CF_NGT;
     const double scale = 1.0 / ((double) nx * (double) ny);
-    for (int l = threadIdx.x; l < nz + 2; l += blockDim.x) {
-        double acc = 0.0;
-        for (int b = 0; b < nb; b++) acc += part[(size_t) l * nb + b];
-        avg[l] = acc * scale;
-    }
+    for (int l = threadIdx.x; l < nz; l += blockDim.x) avg[l + 1] = lsum[l] * scale;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const double *A = avg + 1;                    /* A[-1 .. nz] */
+    double *A = avg + 1;                          /* A[-1 .. nz] */
+    const double bc = kind == 0 ? R.ubc : R.cbot;
+    A[-1] = 2.0 * bc - A[0];                      /* apply_bc_u2 ghosts of the solved field */
+    A[nz] = 2.0 * (kind == 0 ? R.ubc : R.ctop) - A[nz - 1];
     double vol = 0.0, lz = 0.0;
     for (int k = 0; k < nz; k++) { vol += A[k] * dz[k]; lz += dz[k]; }
     vol = vol / lz;
@@ -985,12 +998,166 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
     }
 }
 
-template <typename T>
+/* Partitioned form of the same solve, for nz <= 16 * 32.  A workgroup of
+ * 256 threads owns CF_P2S_MW modes; thread t takes part t & 1 (real or
+ * imaginary: both parts share the matrix), mode (t >> 1) & 7 of the group
+ * and z-chunk t >> 4 of L rows (L a compile-time choice >= nz / 16).  Rows
+ * above nz are identity rows (x = 0), so every chunk has L - 1 interior
+ * rows and a separator (its last row).  Each thread holds its chunk in
+ * registers and solves the interior as a Dirichlet problem in the two
+ * neighbouring separators (x = y + P X_below + Q X_above, Thomas in
+ * registers); the separators' Schur complement (tridiagonal, one unknown
+ * per chunk) is solved in LDS by one thread per (mode, part), then every
+ * thread writes its rows.  Every value is read once and written once (32 B
+ * per complex FP64 value), and the 16-fold parallelism over z hides the
+ * latency of the level recurrences, which bounds the one-thread-per-mode
+ * sweep at these mode counts.  Differs from sequential Thomas by
+ * round-off. */
+#define CF_P2S_NC 16
+#define CF_P2S_MW 8
+
+template <int L, typename A, typename T>
+static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Real<typename Cplx<T>::type>::type *__restrict__ yr,
+                                                  size_t M, int nz, int nch, size_t idx, bool act,
+                                                  double lxd, double lyd, bool zero, double (*sh)[CF_P2S_NC][16],
+                                                  double (*xs)[16])
+{
+    typedef typename Real<typename Cplx<T>::type>::type S;
+    const int g = threadIdx.x & 15, chunk = threadIdx.x >> 4;
+    const A lx = (A) lxd, ly = (A) lyd;
+    const bool live = chunk < nch;
+    const int lo = chunk * L;
+    /* row coefficients, identity beyond the top wall */
+    auto cb = [&](int k) {
+        if (k >= nz) return (A) 1;
+        A b = (A) T_.bz[k] - lx * (A) T_.ex[k] - ly * (A) T_.ey[k];
+        if (k == 0) b -= (A) (zero ? T_.db_dir : T_.db_neu);
+        if (k == nz - 1) b -= (A) T_.db_top;
+        return b;
+    };
+    auto ca = [&](int k) { return (k > 0 && k < nz) ? (A) T_.a[k] : (A) 0; };
+    auto cc = [&](int k) { return k < nz - 1 ? (A) T_.c[k] : (A) 0; };
+    A d[L - 1], ph[L - 1], gm[L - 1];
+    A ds = 0;
+    const bool ld = live && act;
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const int k = lo + t;
+        const A r = (ld && k < nz) ? (A) yr[(size_t) k * 2 * M + idx] * (A) T_.dvol[k] : (A) 0;
+        if (t < L - 1) d[t] = r;
+        else ds = r;
+    }
+    /* forward sweep of the interior; the separator below enters through
+     * a_lo, carried as ph */
+    A ibet = 0, dp = 0, pp = 0;
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) {
+        const int k = lo + t;
+        A b = cb(k), gam = 0;
+        if (t > 0) {
+            gam = cc(k - 1) * ibet;
+            b -= ca(k) * gam;
+        }
+        ibet = (A) 1 / b;
+        if (t == 0) { dp = d[t] * ibet; pp = -ca(k) * ibet; }
+        else { dp = (d[t] - ca(k) * dp) * ibet; pp = -ca(k) * pp * ibet; }
+        d[t] = dp; ph[t] = pp; gm[t] = gam;
+    }
+    /* back substitution of the particular (y), below (P) and above (Q)
+     * responses, in place: d <- y, ph <- P, gm <- Q */
+    A yv = d[L - 2], pv = ph[L - 2], qv = -cc(lo + L - 2) * ibet;
+    const A yl = yv, pl = pv, ql = qv;
+    A gnext = gm[L - 2];
+    gm[L - 2] = qv;
+#pragma unroll
+    for (int t = L - 3; t >= 0; t--) {
+        yv = d[t] - gnext * yv;
+        pv = ph[t] - gnext * pv;
+        qv = -gnext * qv;
+        gnext = gm[t];
+        d[t] = yv; ph[t] = pv; gm[t] = qv;
+    }
+    if (live) {
+        sh[0][chunk][g] = (double) yv; sh[1][chunk][g] = (double) pv; sh[2][chunk][g] = (double) qv;
+        sh[3][chunk][g] = (double) yl; sh[4][chunk][g] = (double) pl; sh[5][chunk][g] = (double) ql;
+        sh[6][chunk][g] = (double) ds;
+    }
+    __syncthreads();
+    if (chunk == 0) {
+        /* separator s_j = (j + 1) L - 1 above chunk j:
+         * a x_{s-1} + b x_s + c x_{s+1} = d_s with x_{s-1} chunk j's top
+         * interior row and x_{s+1} chunk j + 1's bottom one */
+        A cp = 0, dq = 0;
+        for (int j = 0; j < nch; j++) {
+            const int s = (j + 1) * L - 1;
+            const bool nxt = j < nch - 1;
+            const A as = ca(s), cs = cc(s);
+            const A sub = as * (A) sh[4][j][g];
+            const A dia = cb(s) + as * (A) sh[5][j][g] + (nxt ? cs * (A) sh[1][j + 1][g] : (A) 0);
+            const A sup = nxt ? cs * (A) sh[2][j + 1][g] : (A) 0;
+            const A rhs = (A) sh[6][j][g] - as * (A) sh[3][j][g] - (nxt ? cs * (A) sh[0][j + 1][g] : (A) 0);
+            const A iden = (A) 1 / (dia - sub * cp);
+            cp = sup * iden;
+            dq = (rhs - sub * dq) * iden;
+            sh[3][j][g] = (double) cp;            /* chunk j's top row is consumed */
+            sh[4][j][g] = (double) dq;
+        }
+        A X = 0;
+        for (int j = nch - 1; j >= 0; j--) {
+            X = (A) sh[4][j][g] - (A) sh[3][j][g] * X;
+            xs[j][g] = (double) X;
+        }
+    }
+    __syncthreads();
+    if (!ld) return;
+    const A XB = chunk > 0 ? (A) xs[chunk - 1][g] : (A) 0;
+    const A XA = (A) xs[chunk][g];
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) {
+        const int k = lo + t;
+        if (k < nz) yr[(size_t) k * 2 * M + idx] = (S) (d[t] + ph[t] * XB + gm[t] * XA);
+    }
+    if (lo + L - 1 < nz) yr[(size_t) (lo + L - 1) * 2 * M + idx] = (S) XA;
+}
+
+/* L = pois2_L(nz) (or larger): the partitioned solve with chunks of L
+ * rows; L = 0: one thread per mode */
+template <typename T, int L>
 static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab T_,
                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G, const Blk b)
 {
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const size_t M = (size_t) ny * nkx;
+    if (L > 0) {
+        __shared__ double sh[7][CF_P2S_NC][16];
+        __shared__ double xs[CF_P2S_NC][16];
+        const int nch = (nz + L - 1) / (L > 0 ? L : 1);
+        const size_t m0 = (size_t) b.x * CF_P2S_MW;
+        const size_t mode = m0 + ((threadIdx.x >> 1) & (CF_P2S_MW - 1));
+        const bool act = mode < M;
+        const size_t mc = act ? mode : M - 1;
+        const int kx = (int) (mc / ny), ky = (int) (mc % ny);
+        const bool zero = kx == 0 && ky == 0;
+        /* FP32 storage: workgroups holding any mode with |kx|, |ky| <=
+         * CF_F64_KLOW solve in FP64 (a workgroup-uniform choice) */
+        bool low = false;
+        if (sizeof(T) != sizeof(double)) {
+            const size_t m1 = min(m0 + CF_P2S_MW, M) - 1;
+            for (size_t q = m0; q <= m1; q++) {
+                const int qx = (int) (q / ny), qy = (int) (q % ny);
+                low = low || (qx <= CF_F64_KLOW && (qy <= CF_F64_KLOW || ny - qy <= CF_F64_KLOW));
+            }
+        }
+        typename Real<typename Cplx<T>::type>::type *yr = (typename Real<typename Cplx<T>::type>::type *) Y;
+        const size_t idx = 2 * mc + (threadIdx.x & 1);
+        const double lx = R.lamx[kx], ly = R.lamy[ky];
+        constexpr int LL = L > 1 ? L : 2;
+        if (sizeof(T) == sizeof(double) || low)
+            pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs);
+        else
+            pois2s_run<LL, T, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs);
+        return;
+    }
     const size_t qm = (size_t) b.x * blockDim.x + threadIdx.x;
     if (qm >= M) return;
     const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
@@ -1002,15 +1169,25 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
         pois2_mode<T>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T_,
                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
 {
-    pois2_body<T>(R, T_, Y, G, blk_native());
+    pois2_body<T, L>(R, T_, Y, G, blk_native());
 }
 
-template __global__ void k_poisson2<double>(const RunDev, const PoisTab, double2 *, double *);
-template __global__ void k_poisson2<float>(const RunDev, const PoisTab, float2 *, double *);
+template <typename T>
+typename KSel<T>::pois2_t pick_poisson2(int L)
+{
+    switch (L) {
+#define X_(N) case N: return k_poisson2<T, N>;
+    CF_P2S_LSET(X_)
+#undef X_
+    default: return k_poisson2<T, 0>;
+    }
+}
+template KSel<double>::pois2_t pick_poisson2<double>(int);
+template KSel<float>::pois2_t pick_poisson2<float>(int);
 
 /* ---------------------------------------------------------------------
  * project2 (pressure.c:120-152) fused with the mass-flux correction:
@@ -1140,13 +1317,24 @@ __global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *a_, const unsign
     });
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(256) void kb_poisson(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
-        pois2_body<T>(a.R, a.pt[s], a.X, a.G, blk_of(a, ST_MODE, lb));
+        pois2_body<T, L>(a.R, a.pt[s], a.X, a.G, blk_of(a, ST_MODE, lb));
     });
+}
+
+template <typename T>
+typename BSel<T>::kb_t pick_batched_poisson(int L)
+{
+    switch (L) {
+#define X_(N) case N: return kb_poisson<T, N>;
+    CF_P2S_LSET(X_)
+#undef X_
+    default: return kb_poisson<T, 0>;
+    }
 }
 
 template <typename T>
@@ -1190,7 +1378,7 @@ __global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *a_, const uns
 #define CF_INST_B(T)                                                                                              \
     template __global__ void kb_ffty<T>(const GArgs<T> *, const unsigned *, int, int);                            \
     template __global__ void kb_fftxi<T>(const GArgs<T> *, const unsigned *, int, int);                           \
-    template __global__ void kb_poisson<T>(const GArgs<T> *, const unsigned *, int, int);                         \
+    template typename BSel<T>::kb_t pick_batched_poisson<T>(int);                                                  \
     template __global__ void kb_spike<T>(const GArgs<T> *, const unsigned *, int, int);                           \
     template __global__ void kb_thomas<T>(const GArgs<T> *, const unsigned *, int, int);                          \
     template __global__ void kb_cfl<T>(const GArgs<T> *, const unsigned *, int, int);                             \

@@ -79,7 +79,7 @@ struct cf_run {
     double *p = nullptr, *G = nullptr, *stage = nullptr;
     double2 *X = nullptr;
     double *tab = nullptr;           /* all constant tables in one allocation */
-    double *scal = nullptr, *logbuf = nullptr, *part = nullptr;
+    double *scal = nullptr, *logbuf = nullptr, *part = nullptr, *lsum = nullptr;
     size_t tab_n = 0;
     int nb_plane = 1, nb_red = 1;
     RunDev rd;
@@ -410,7 +410,7 @@ static void free_run(cf_run *r)
     }
     double *bufs[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                        r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p, r->G, r->S4, r->stage, r->tab, r->scal,
-                       r->logbuf, r->part, r->cflpart };
+                       r->logbuf, r->part, r->lsum, r->cflpart };
     for (double *b : bufs) if (b) (void) hipFree(b);
     if (r->X) (void) hipFree(r->X);
     for (auto &pe : r->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
@@ -712,9 +712,9 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     r->nb_plane = (int) ((r->P + 2047) / 2048);
     if (r->nb_plane > 1024) r->nb_plane = 1024;
     r->nb_red = 1024;
-    size_t npart = (size_t) r->nb_plane * (2 * nz + 2);
+    size_t npart = std::max((size_t) r->nb_plane * (2 * nz + 2), (size_t) ((r->P + 63) / 64) * nz);
     if (npart < (size_t) r->nb_red) npart = r->nb_red;
-    ok = ok && dalloc(&r->part, npart);
+    ok = ok && dalloc(&r->part, npart) && dalloc(&r->lsum, (size_t) nz + 2);
     {
         int nchunk = (nz + CF_RHS_KC - 1) / CF_RHS_KC;
         r->rhs_kc = (nz + nchunk - 1) / nchunk;
@@ -902,7 +902,8 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
     if (r->g.order == 4)
         KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], X, r->S4);
     else
-        KLAUNCH("poisson", k_poisson2<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt[tq], X, r->G);
+        KLAUNCH("poisson", pick_poisson2<T>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
+                dim3(256), 0, R, r->pt[tq], X, r->G);
     KLAUNCH("ffty_inv", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 0);
     KLAUNCH("fftx_inv", Prec<T>::fftxi(r), dim3(gx), dim3(256), fx_lds(r), R, (const C *) X, tv<T>(r->p));
     return CF_OK;
@@ -955,16 +956,23 @@ static cf_status enqueue_step(cf_run *r, int from)
         if (s == 0)
             KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart,
                     r->rhs_blocks, dt, r->scal, r->logbuf);
-        if (r->use_spike)
-            KLAUNCH("spike", k_spike<double>, dim3((unsigned) ((r->P + 63) / 64), 3), dim3(64 * r->spike_waves), 0, R, s,
-                    ou, ov, ow, (double *) nullptr);
-        else
+        const int nbx = (int) ((r->P + 63) / 64);
+        if (r->use_spike) {
+            /* the solve with the first stage of the plane sums of u* fused in */
+            KLAUNCH("spike", k_spike_sums, dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
+                    ou, ov, ow, r->part);
+            KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, nbx,
+                    r->lsum);
+        } else {
             KLAUNCH("thomas", k_thomas<double>, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
                     (double *) nullptr);
-        KLAUNCH("plane_sum", k_plane_sum, dim3(r->nb_plane, nz + 2), dim3(256), 0, R,
-                (const double *) ou, R.ubc, r->part, r->nb_plane);
+            KLAUNCH("plane_sum", k_gplane<double>, dim3(r->nb_plane, nz, 1), dim3(256), 0, R, (const double *) ou,
+                    (const double *) nullptr, r->part, r->nb_plane);
+            KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, r->nb_plane,
+                    r->lsum);
+        }
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
-                (const double *) r->part, r->nb_plane, r->scal, r->logbuf, 0);
+                (const double *) r->lsum, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
         cf_status st = enqueue_pressure<double>(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
@@ -1413,7 +1421,8 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
         hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
                            r->rd, r->pt4[3], r->X, r->S4);
     else
-        hipLaunchKernelGGL(k_poisson2<double>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+        hipLaunchKernelGGL(pick_poisson2<double>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
+                           dim3(256), 0, r->stream,
                            r->rd, r->pt[3], r->X, r->G);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
@@ -1448,9 +1457,10 @@ struct cf_batch {
     int n = 0, prec = CF_FP64, order = 2, scalar = 0, device = 0;
     hipStream_t stream = nullptr;
     void *args = nullptr;                 /* device GArgs<T>[n] */
-    unsigned *vstart = nullptr;           /* device [CF_NST][n + 1] */
+    unsigned *vstart = nullptr;           /* device [CF_NST + CF_P2_NG][n + 1] */
     std::vector<unsigned> hv;
     int solve_kind = SOLVE_HEPT;
+    std::vector<int> p2L;                 /* distinct order-2 Poisson chunk lengths, one launch each */
     unsigned kgrid[KB_N] = {0};
     int spike_waves = 1, ncu = 256;
     size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
@@ -1463,6 +1473,22 @@ struct cf_batch {
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<std::string> tuning;      /* human-readable autotuner choices */
 };
+
+/* The order-2 Poisson of a batch goes out as one persistent launch per
+ * distinct chunk length L (pois2_L of each run's nz), so that every run is
+ * solved with its own partition, bit for bit as when stepped alone; the
+ * prefix table of group g is row CF_NST + g of vstart. */
+#define CF_P2_NG 20
+static std::vector<int> batch_p2L(const std::vector<cf_run *> &runs)
+{
+    std::vector<int> ls;
+    for (const cf_run *r : runs) {
+        const int l = pois2_L(r->g.nz);
+        if (std::find(ls.begin(), ls.end(), l) == ls.end()) ls.push_back(l);
+    }
+    std::sort(ls.begin(), ls.end());
+    return ls;
+}
 
 template <typename T>
 static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
@@ -1495,7 +1521,7 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         long NL = (long) ny * nz, npairs = (NL + 1) / 2;
         set(ST_FFTX, (unsigned) ((npairs + r->rd.fx_pairs - 1) / r->rd.fx_pairs), 1, 1);
         set(ST_FFTY, fy_blocks(r), 1, 1);
-        set(ST_MODE, (unsigned) ((r->M + 255) / 256), 1, 1);
+        set(ST_MODE, b->order == 2 ? pois2_blocks(pois2_L(r->g.nz), r->M) : (unsigned) ((r->M + 255) / 256), 1, 1);
     }
 }
 
@@ -1504,12 +1530,16 @@ static cf_status batch_upload_tables(cf_batch *b)
 {
     std::vector<GArgs<T>> A;
     batch_fill_args<T>(b, A);
-    b->hv.assign((size_t) CF_NST * (b->n + 1), 0u);
-    for (int st = 0; st < CF_NST; st++) {
+    const int ng = (int) b->p2L.size();
+    if (ng > CF_P2_NG) { cf_set_error("too many Poisson chunk lengths in one batch"); return CF_ERR_UNSUPPORTED; }
+    b->hv.assign((size_t) (CF_NST + ng) * (b->n + 1), 0u);
+    for (int st = 0; st < CF_NST + ng; st++) {
         unsigned acc = 0;
         for (int q = 0; q < b->n; q++) {
             b->hv[(size_t) st * (b->n + 1) + q] = acc;
-            acc += A[q].dims[st][0] * A[q].dims[st][1] * A[q].dims[st][2];
+            const int sd = st < CF_NST ? st : ST_MODE;
+            const bool in = st < CF_NST || pois2_L(b->runs[q]->g.nz) == b->p2L[st - CF_NST];
+            if (in) acc += A[q].dims[sd][0] * A[q].dims[sd][1] * A[q].dims[sd][2];
         }
         b->hv[(size_t) st * (b->n + 1) + b->n] = acc;
     }
@@ -1563,7 +1593,24 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     case KB_CFL: fn = kb_cfl<T>; break;
     case KB_DIVX: fn = S.divx; lds = b->lds_fx; break;
     case KB_FFTY: fn = kb_ffty<T>; lds = b->lds_fy; break;
-    case KB_POIS: fn = b->order == 4 ? kb_poisson4<T> : kb_poisson<T>; break;
+    case KB_POIS:
+        if (b->order == 4) { fn = kb_poisson4<T>; break; }
+        {
+            /* one launch per chunk-length group, each over its own prefix table */
+            cf_status e = bev_begin(b, KB_NAME[k]);
+            if (e != CF_OK) return e;
+            for (size_t g = 0; g < b->p2L.size(); g++) {
+                const int row = CF_NST + (int) g;
+                const unsigned tg = batch_total(b, row);
+                const unsigned grid = b->kgrid[k] ? std::min(b->kgrid[k], tg) : tg;
+                if (grid == 0) continue;
+                hipLaunchKernelGGL(pick_batched_poisson<T>(b->p2L[g]), dim3(grid), block, 0, b->stream,
+                                   (const GArgs<T> *) b->args, (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)),
+                                   b->n, s);
+                HIPCHK(hipGetLastError());
+            }
+            return bev_end(b);
+        }
     case KB_FFTXI: fn = kb_fftxi<T>; lds = b->lds_fx; break;
     case KB_STEPEND: fn = kb_step_end<T>; break;
     default: cf_set_error("bad batch kernel slot %d", k); return CF_ERR_ARG;
@@ -1766,13 +1813,14 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
         b->lds_fy = std::max(b->lds_fy, fy_lds(r));
         b->lds_mass = std::max(b->lds_mass, sizeof(double) * (size_t) (r->g.nz + 6));
     }
+    b->p2L = batch_p2L(b->runs);
     HIPCHK(hipSetDevice(b->device));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
         b->ncu = prop.multiProcessorCount;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     if (hipMalloc(&b->args, asz * nrun) != hipSuccess
-        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_NST * (nrun + 1)) != hipSuccess
+        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG) * (nrun + 1)) != hipSuccess
         || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         free_batch(b);
         cf_set_error("batch allocation failed");
@@ -1935,6 +1983,7 @@ struct cf_server {
         std::vector<double> init[4];
         int nsteps = 0, done = 0, state = 0;       /* 0 queued, 1 active, 2 done (retained), 3 released */
         bool retain = false;
+        bool solo = false;                         /* stepped through its own graph, not a group batch */
         size_t bytes = 0;
         cf_run *run = nullptr;
         double t_submit = 0, t_admit = -1, t_done = -1;
@@ -1942,6 +1991,7 @@ struct cf_server {
     std::vector<Req> reqs;
     std::vector<int> queue;
     std::map<int, cf_batch *> groups;              /* key -> non-owning batch of the group's active runs */
+    size_t solo_points = (size_t) 4 << 20;         /* runs of at least this many cells step on their own */
     double pt_steps = 0.0;                         /* completed cell-steps */
 };
 
@@ -1951,6 +2001,13 @@ static double srv_now(const cf_server *s)
 }
 
 static int group_key(const cf_config &c) { return (c.order == 4 ? 4 : 0) + (c.scalar ? 2 : 0) + (c.precision == CF_FP32 ? 1 : 0); }
+
+extern "C" cf_status cf_server_set_solo_points(cf_server *s, size_t points)
+{
+    if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
+    s->solo_points = points;
+    return CF_OK;
+}
 
 extern "C" cf_status cf_server_create(size_t hbm_budget_bytes, int device, int quantum_steps, cf_server **out)
 {
@@ -2021,7 +2078,7 @@ static cf_status srv_regroup(cf_server *s, int k)
 {
     std::vector<cf_run *> runs;
     for (auto &r : s->reqs)
-        if (r.state == 1 && group_key(r.cfg) == k) runs.push_back(r.run);
+        if (r.state == 1 && !r.solo && group_key(r.cfg) == k) runs.push_back(r.run);
     auto it = s->groups.find(k);
     if (runs.empty()) {
         if (it != s->groups.end()) { srv_drop_group(it->second); s->groups.erase(it); }
@@ -2060,10 +2117,11 @@ static cf_status srv_regroup(cf_server *s, int k)
         spike_ok = spike_ok && r->use_spike;
     }
     b->solve_kind = b->order == 4 ? SOLVE_HEPT : (spike_ok ? SOLVE_SPIKE : SOLVE_THOMAS);
+    b->p2L = batch_p2L(runs);
     for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     HIPCHK(hipMalloc(&b->args, asz * b->n));
-    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_NST * (b->n + 1)));
+    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG) * (b->n + 1)));
     return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
 }
 
@@ -2079,7 +2137,12 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         cf_server::Req &r = s->reqs[s->queue.front()];
         if (s->used + r.bytes > s->budget) break;
         cf_run *run = nullptr;
-        if ((st = create_run(&r.cfg, true, &run)) != CF_OK) return st;
+        /* large runs fill the GPU on their own: they keep the single-run
+         * graph (the order-2 FP64 fast path where it applies) on their own
+         * stream; small ones join their group's batched launches.  The
+         * choice depends on the request alone, never on the mix. */
+        r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
+        if ((st = create_run(&r.cfg, !r.solo, &run)) != CF_OK) return st;
         for (int f = 0; f < 4; f++) {
             if (r.init[f].empty()) continue;
             if ((st = cf_run_upload(run, f, r.init[f].data())) != CF_OK) { free_run(run); return st; }
@@ -2090,31 +2153,42 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         r.state = 1;
         r.t_admit = srv_now(s);
         s->used += r.bytes;
-        changed.push_back(group_key(r.cfg));
+        if (!r.solo) changed.push_back(group_key(r.cfg));
         s->queue.erase(s->queue.begin());
     }
     for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
-    /* one quantum per group, all groups in flight together */
+    /* one quantum per group and per solo run, all in flight together on
+     * their own streams */
     std::map<int, int> adv;
     for (auto &g : s->groups) {
         cf_batch *b = g.second;
         int k = s->quantum;
         for (auto &r : s->reqs)
-            if (r.state == 1 && group_key(r.cfg) == g.first) k = std::min(k, r.nsteps - r.done);
+            if (r.state == 1 && !r.solo && group_key(r.cfg) == g.first) k = std::min(k, r.nsteps - r.done);
         for (int n = 0; n < k; n++) {
             st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
             if (st != CF_OK) return st;
         }
         adv[g.first] = k;
     }
+    std::vector<int> solo_adv(s->reqs.size(), 0);
+    for (size_t q = 0; q < s->reqs.size(); q++) {
+        cf_server::Req &r = s->reqs[q];
+        if (r.state != 1 || !r.solo) continue;
+        solo_adv[q] = std::min(s->quantum, r.nsteps - r.done);
+        if ((st = cf_run_step(r.run, solo_adv[q])) != CF_OK) return st;   /* counts its own steps */
+    }
     for (auto &g : s->groups) HIPCHK(hipStreamSynchronize(g.second->stream));
+    for (auto &r : s->reqs)
+        if (r.state == 1 && r.solo) HIPCHK(hipStreamSynchronize(r.run->stream));
     std::vector<int> retire;
     const double now = srv_now(s);
-    for (auto &r : s->reqs) {
+    for (size_t q = 0; q < s->reqs.size(); q++) {
+        cf_server::Req &r = s->reqs[q];
         if (r.state != 1) continue;
-        const int k = adv[group_key(r.cfg)];
+        const int k = r.solo ? solo_adv[q] : adv[group_key(r.cfg)];
         r.done += k;
-        r.run->steps_done += k;
+        if (!r.solo) r.run->steps_done += k;
         s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
         if (r.done >= r.nsteps) {
             r.t_done = now;
@@ -2124,7 +2198,7 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
                 r.run = nullptr;
                 s->used -= r.bytes;
             }
-            retire.push_back(group_key(r.cfg));
+            if (!r.solo) retire.push_back(group_key(r.cfg));
         }
     }
     for (int k : retire) if ((st = srv_regroup(s, k)) != CF_OK) return st;

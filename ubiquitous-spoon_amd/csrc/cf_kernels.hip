@@ -1027,23 +1027,30 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
     const A lx = (A) lxd, ly = (A) lyd;
     const bool live = chunk < nch;
     const int lo = chunk * L;
-    /* row coefficients, identity beyond the top wall */
+    /* row coefficients, identity beyond the top wall; every read is at a
+     * clamped (valid) level and the padding is applied by selects, so the
+     * loads issue without branches */
+    const int ktop = nz - 1;
+    const A dbot = (A) (zero ? T_.db_dir : T_.db_neu), dtop = (A) T_.db_top;
     auto cb = [&](int k) {
-        if (k >= nz) return (A) 1;
-        A b = (A) T_.bz[k] - lx * (A) T_.ex[k] - ly * (A) T_.ey[k];
-        if (k == 0) b -= (A) (zero ? T_.db_dir : T_.db_neu);
-        if (k == nz - 1) b -= (A) T_.db_top;
-        return b;
+        const int kc = min(k, ktop);
+        A b = (A) T_.bz[kc] - lx * (A) T_.ex[kc] - ly * (A) T_.ey[kc];
+        b -= k == 0 ? dbot : (A) 0;
+        b -= k == ktop ? dtop : (A) 0;
+        return k > ktop ? (A) 1 : b;
     };
-    auto ca = [&](int k) { return (k > 0 && k < nz) ? (A) T_.a[k] : (A) 0; };
-    auto cc = [&](int k) { return k < nz - 1 ? (A) T_.c[k] : (A) 0; };
+    auto ca = [&](int k) { const A v = (A) T_.a[min(k, ktop)]; return (k > 0 && k <= ktop) ? v : (A) 0; };
+    auto cc = [&](int k) { const A v = (A) T_.c[min(k, ktop)]; return k < ktop ? v : (A) 0; };
     A d[L - 1], ph[L - 1], gm[L - 1];
     A ds = 0;
     const bool ld = live && act;
+    const size_t stride = 2 * M;
+    S *__restrict__ col = yr + idx;
 #pragma unroll
     for (int t = 0; t < L; t++) {
-        const int k = lo + t;
-        const A r = (ld && k < nz) ? (A) yr[(size_t) k * 2 * M + idx] * (A) T_.dvol[k] : (A) 0;
+        const int k = lo + t, kc = min(k, ktop);
+        const A v = (A) col[(size_t) kc * stride] * (A) T_.dvol[kc];
+        const A r = (ld && k <= ktop) ? v : (A) 0;
         if (t < L - 1) d[t] = r;
         else ds = r;
     }
@@ -1112,12 +1119,16 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
     if (!ld) return;
     const A XB = chunk > 0 ? (A) xs[chunk - 1][g] : (A) 0;
     const A XA = (A) xs[chunk][g];
+    S *__restrict__ out = col + (size_t) lo * stride;
+    if (lo + L - 1 <= ktop) {                     /* a full chunk */
 #pragma unroll
-    for (int t = 0; t < L - 1; t++) {
-        const int k = lo + t;
-        if (k < nz) yr[(size_t) k * 2 * M + idx] = (S) (d[t] + ph[t] * XB + gm[t] * XA);
+        for (int t = 0; t < L - 1; t++) out[(size_t) t * stride] = (S) (d[t] + ph[t] * XB + gm[t] * XA);
+        out[(size_t) (L - 1) * stride] = (S) XA;
+    } else {
+#pragma unroll
+        for (int t = 0; t < L - 1; t++)
+            if (lo + t <= ktop) out[(size_t) t * stride] = (S) (d[t] + ph[t] * XB + gm[t] * XA);
     }
-    if (lo + L - 1 < nz) yr[(size_t) (lo + L - 1) * 2 * M + idx] = (S) XA;
 }
 
 /* L = pois2_L(nz) (or larger): the partitioned solve with chunks of L

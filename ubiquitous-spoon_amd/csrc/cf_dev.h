@@ -159,9 +159,10 @@ static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlin
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % p;
-            C *dstl = buf + line * ld + (i - k) * R + k;
+            C *dstl = buf + line * ld;
+            const int d0 = (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) dstl[q * p] = y[it][q];
+            for (int q = 0; q < R; q++) dstl[d0 + q * p] = y[it][q];
         }
     }
     __syncthreads();
@@ -244,9 +245,10 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
         if (b < nb) {
             const int line = b / T, i = b - line * T;
             const int k = i % P;
-            C *dstl = buf + line * ld + (i - k) * R + k;
+            C *dstl = buf + line * ld;
+            const int d0 = (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) dstl[q * P] = y[it][q];
+            for (int q = 0; q < R; q++) dstl[d0 + q * P] = y[it][q];
         }
     }
     __syncthreads();

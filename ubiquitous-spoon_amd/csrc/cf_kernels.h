@@ -127,6 +127,10 @@ struct PoisTab {
     double db_neu, db_dir, db_top;
 };
 
+/* LDS of the FFT kernels: lines of n elements at stride fft_ld(n), then
+ * the n twiddles of the passes, staged once per workgroup */
+static inline __host__ __device__ int fft_ld(int n) { return n + 1; }
+
 /* order-2 Poisson: the partitioned solve (at most 16 z-chunks of L rows
  * per mode, 8 modes per 256-thread workgroup), L a compiled chunk length
  * >= nz / 16, covers nz <= 512; beyond that one thread per mode (256

@@ -638,6 +638,28 @@ template <typename C> static __device__ __forceinline__ const C *twsel(const dou
 template <> __device__ __forceinline__ const double2 *twsel<double2>(const double2 *d, const float2 *) { return d; }
 template <> __device__ __forceinline__ const float2 *twsel<float2>(const double2 *, const float2 *f) { return f; }
 
+/* twiddles of an n-point pass into LDS after the line data (tws): the
+ * first one per thread is read into a register with the data gather
+ * (tw_first) and stored after it (tw_rest stores it and copies any
+ * others), so the passes read twiddles from LDS without an extra round
+ * trip to memory */
+template <typename C>
+static __device__ __forceinline__ C tw_first(const C *__restrict__ tw, int n)
+{
+    const int t = (int) threadIdx.x;
+    return tw[t < n ? t : n - 1];
+}
+template <typename C>
+static __device__ __forceinline__ void tw_rest(C *tws, C first, const C *__restrict__ tw, int n)
+{
+    const int t = (int) threadIdx.x;
+    if (t < n) tws[t] = first;
+    for (int q = t + (int) blockDim.x; q < n; q += (int) blockDim.x) tws[q] = tw[q];
+}
+
+#define CF_FX_UNR 3            /* elements per thread gathered together by the divergence + x-FFT */
+#define CF_FXI_UNR 5           /* ... and by the inverse x-FFT (nl (nx/2 + 1) <= 1152 for nx >= 16) */
+
 /* unpack two packed real-line spectra and store kx-major: for each kx the
  * block's 2 nl lines are consecutive ky positions (coalesced) */
 template <typename C>
@@ -676,48 +698,82 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
-    const int ld = nx + 1;
+    const int ld = fft_ld(nx);
     const int nl = R.fx_pairs;
-    const long NL = (long) ny * nz;
+    const int NL = ny * nz;
     const size_t P = (size_t) nx * ny;
     const double *rdz = R.rdz + CF_NGT;
     const T m = mptr ? (T) *mptr : (T) 0;
     const T rdx = (T) R.rdx, rdy = (T) R.rdy;
-    const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
-    auto divg = [&](long line, int i) -> T {
-        if (line >= NL) return (T) 0;
-        int k = (int) (line / ny), j = (int) (line - (long) k * ny);
-        int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
-        size_t q = (size_t) k * P + (size_t) j * nx;
-        T hm = h1 ? (T) h1[k + 1] * m : (T) 0;
-        T u1 = us[q + i] + hm, u0 = us[q + im] + hm;
-        T v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
-        T w1 = k < nz - 1 ? ws[q + i] : (T) 0;
-        T w0 = k > 0 ? ws[q - P + i] : (T) 0;
-        if (ORD == 2) return (u1 - u0) * rdx + (v1 - v0) * rdy + (w1 - w0) * (T) rdz[k];
-        const T a1 = (T) A1_, a2 = (T) A2_;
-        const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
-        const int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
-        T u2 = us[q + ip] + hm, um = us[q + im2] + hm;
-        T v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
-        /* w at faces k+1 and k-2: walls at -1 and nz-1 are 0, ghosts odd */
-        T w2 = k + 1 <= nz - 2 ? ws[q + P + i] : (k + 1 == nz - 1 ? (T) 0 : -ws[q - P + i]);
-        T wm = k - 2 >= 0 ? ws[q - 2 * P + i] : (k - 2 == -1 ? (T) 0 : -ws[q + i]);
-        return (a1 * (u1 - u0) + a2 * (u2 - um)) * rdx + (a1 * (v1 - v0) + a2 * (v2 - vm)) * rdy
-             + (a1 * (w1 - w0) + a2 * (w2 - wm)) * (T) rdz[k];
+    const int pair0 = (int) xcd_block(b.x, b.nx) * nl;
+    /* every read is at a valid (clamped) address and the boundary values
+     * are applied by selects, so the reads of a group of elements issue
+     * together without branches */
+    auto divg = [&](int line, int i) -> T {
+        const bool ok = line < NL;
+        const int lc = ok ? line : NL - 1;
+        const int k = lc / ny, j = lc - k * ny;
+        const int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
+        const size_t q = (size_t) k * P + (size_t) j * nx;
+        const T hm = h1 ? (T) h1[k + 1] * m : (T) 0;
+        const T u1 = us[q + i] + hm, u0 = us[q + im] + hm;
+        const T v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
+        const T w1r = ws[q + i], w0r = ws[(k > 0 ? q - P : q) + i];
+        const T w1 = k < nz - 1 ? w1r : (T) 0;
+        const T w0 = k > 0 ? w0r : (T) 0;
+        T d;
+        if (ORD == 2) {
+            d = (u1 - u0) * rdx + (v1 - v0) * rdy + (w1 - w0) * (T) rdz[k];
+        } else {
+            const T a1 = (T) A1_, a2 = (T) A2_;
+            const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
+            const int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
+            const T u2 = us[q + ip] + hm, um = us[q + im2] + hm;
+            const T v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
+            /* w at faces k+1 and k-2: walls at -1 and nz-1 are 0, ghosts odd */
+            const T w2r = ws[(k <= nz - 3 ? q + P : (k == nz - 2 ? q : q - P)) + i];
+            const T w2 = k <= nz - 3 ? w2r : (k == nz - 2 ? (T) 0 : -w2r);
+            const T wmr = ws[(k >= 2 ? q - 2 * P : q) + i];
+            const T wm = k >= 2 ? wmr : (k == 1 ? (T) 0 : -wmr);
+            d = (a1 * (u1 - u0) + a2 * (u2 - um)) * rdx + (a1 * (v1 - v0) + a2 * (v2 - vm)) * rdy
+              + (a1 * (w1 - w0) + a2 * (w2 - wm)) * (T) rdz[k];
+        }
+        return ok ? d : (T) 0;
     };
-    for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
-        int l = e / nx, i = e - l * nx;
-        long pr = pair0 + l;
-        sb[l * ld + i] = mkc<C>(divg(2 * pr, i), divg(2 * pr + 1, i));
+    /* groups of CF_FX_UNR elements per thread: all reads of a group are
+     * in flight before the first LDS write (blockDim.x = 256) */
+    constexpr int UNR = ORD == 2 ? CF_FX_UNR : 2;     /* order 4 reads twice as many values */
+    const C *twg = twsel<C>(R.twx, R.twx32);
+    C *tws = sb + nl * ld;
+    const C tw0 = tw_first(twg, nx);
+    const int tot = nl * nx;
+    for (int e0 = 0; e0 < tot; e0 += UNR * 256) {
+        C res[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            const int ec = e < tot ? e : tot - 1;
+            const int l = ec / nx, i = ec - l * nx;
+            const int pr = pair0 + l;
+            res[u] = mkc<C>(divg(2 * pr, i), divg(2 * pr + 1, i));
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            if (e < tot) {
+                const int l = e / nx, i = e - l * nx;
+                sb[l * ld + i] = res[u];
+            }
+        }
     }
+    tw_rest(tws, tw0, twg, nx);
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, twsel<C>(R.twx, R.twx32), 1.0);
-    store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, (const C *) tws, 1.0);
+    store_half_spectra(sb, ld, nl, nx, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) nx, X);
 }
 
 template <int NX, int ORD, typename T>
-__global__ __launch_bounds__(256) void k_div_fftx(const RunDev R, const T *__restrict__ us,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_div_fftx(const RunDev R, const T *__restrict__ us,
     const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
     const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X)
 {
@@ -733,7 +789,7 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
     double2 *sb = (double2 *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
-    const int ld = nx + 1;
+    const int ld = fft_ld(nx);
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
     const long pair0 = (long) blockIdx.x * nl;
@@ -759,16 +815,36 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
     C *sb = (C *) smem_;
     const int ny = NY > 0 ? NY : R.g.ny;
     const long nlines = (long) R.g.nz * R.g.nkx;
-    const int TK = R.fy_cols, ld = ny + 1;
+    const int TK = R.fy_cols, ld = fft_ld(ny);
     const long L0 = (long) b.x * TK;
     const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
     C *base = X + (size_t) L0 * ny;
-    for (int e = threadIdx.x; e < TK * ny; e += blockDim.x) {
-        int t = e / ny, j = e - t * ny;
-        sb[t * ld + j] = t < nl ? base[e] : mkc<C>((T) 0, (T) 0);
+    /* all reads of a thread in flight before its first LDS write
+     * (TK ny <= 8 x 256, blockDim.x = 256) */
+    const C *twg = twsel<C>(R.twy, R.twy32);
+    C *tws = sb + TK * ld;
+    const C tw0 = tw_first(twg, ny);
+    const int tot = TK * ny, totv = nl * ny;
+    for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
+        C v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            const C x = base[e < totv ? e : totv - 1];
+            v[u] = e < totv ? x : mkc<C>((T) 0, (T) 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            if (e < tot) {
+                const int t = e / ny, j = e - t * ny;
+                sb[t * ld + j] = v[u];
+            }
+        }
     }
+    tw_rest(tws, tw0, twg, ny);
     __syncthreads();
-    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, twsel<C>(R.twy, R.twy32), fwd ? 1.0 : -1.0);
+    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, (const C *) tws, fwd ? 1.0 : -1.0);
     const T sc = fwd ? (T) (1.0 / (double) ny) : (T) 1;
     for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
         int t = e / ny, j = e - t * ny;
@@ -795,29 +871,47 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
     C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
-    const int ld = nx + 1;
+    const int ld = fft_ld(nx);
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
     const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
     const int half = nx / 2;
-    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
-        const int kx = e / nl, l = e - kx * nl;
-        const long la = 2 * (pair0 + l), lb = la + 1;
-        C A = mkc<C>((T) 0, (T) 0), B = A;
-        if (la < NL) {
-            const long k = la / ny, j = la - k * ny;
-            A = X[((size_t) k * nkx + kx) * ny + j];
+    /* groups of CF_FXI_UNR elements per thread, reads at clamped indices,
+     * all in flight before the first LDS write (blockDim.x = 256) */
+    const C *twg = twsel<C>(R.twx, R.twx32);
+    C *tws = sb + nl * ld;
+    const C tw0 = tw_first(twg, nx);
+    const int tot = nl * nkx, NLi = (int) NL;
+    for (int e0 = 0; e0 < tot; e0 += CF_FXI_UNR * 256) {
+        C A[CF_FXI_UNR], B[CF_FXI_UNR];
+#pragma unroll
+        for (int u = 0; u < CF_FXI_UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            const int ec = e < tot ? e : tot - 1;
+            const int kx = ec / nl, l = ec - kx * nl;
+            const int la = 2 * ((int) pair0 + l), lb = la + 1;
+            const int lac = la < NLi ? la : NLi - 1, lbc = lb < NLi ? lb : NLi - 1;
+            const int ka = lac / ny, ja = lac - ka * ny, kb = lbc / ny, jb = lbc - kb * ny;
+            const C a = X[((size_t) ka * nkx + kx) * ny + ja];
+            const C bq = X[((size_t) kb * nkx + kx) * ny + jb];
+            A[u] = la < NLi ? a : mkc<C>((T) 0, (T) 0);
+            B[u] = lb < NLi ? bq : mkc<C>((T) 0, (T) 0);
         }
-        if (lb < NL) {
-            const long k = lb / ny, j = lb - k * ny;
-            B = X[((size_t) k * nkx + kx) * ny + j];
+#pragma unroll
+        for (int u = 0; u < CF_FXI_UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            if (e < tot) {
+                const int kx = e / nl, l = e - kx * nl;
+                C a = A[u], bq = B[u];
+                if (kx == 0 || (nx % 2 == 0 && kx == half)) { a.y = (T) 0; bq.y = (T) 0; }
+                sb[l * ld + kx] = mkc<C>(a.x - bq.y, a.y + bq.x);
+                if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = mkc<C>(a.x + bq.y, bq.x - a.y);
+            }
         }
-        if (kx == 0 || (nx % 2 == 0 && kx == half)) { A.y = (T) 0; B.y = (T) 0; }
-        sb[l * ld + kx] = mkc<C>(A.x - B.y, A.y + B.x);
-        if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = mkc<C>(A.x + B.y, B.x - A.y);
     }
+    tw_rest(tws, tw0, twg, nx);
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, twsel<C>(R.twx, R.twx32), -1.0);
+    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, (const C *) tws, -1.0);
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long la = 2 * (pair0 + l), lb = la + 1;
@@ -1289,7 +1383,7 @@ template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const
  * plan since the runs' lengths differ. */
 
 template <int ORD, typename T>
-__global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
                                                int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {

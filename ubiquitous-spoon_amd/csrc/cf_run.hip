@@ -853,8 +853,14 @@ static cf_status ev_end(cf_run *r)
 
 static dim3 cell_grid(const cf_run *r) { return dim3((r->g.nx + 63) / 64, (r->g.ny + 3) / 4, r->g.nz); }
 
-static size_t fx_lds(const cf_run *r) { return 2 * r->esz * (size_t) r->rd.fx_pairs * (r->g.nx + 1); }
-static size_t fy_lds(const cf_run *r) { return 2 * r->esz * (size_t) r->rd.fy_cols * (r->g.ny + 1); }
+static size_t fx_lds(const cf_run *r)
+{
+    return 2 * r->esz * ((size_t) r->rd.fx_pairs * fft_ld(r->g.nx) + (size_t) r->g.nx);
+}
+static size_t fy_lds(const cf_run *r)
+{
+    return 2 * r->esz * ((size_t) r->rd.fy_cols * fft_ld(r->g.ny) + (size_t) r->g.ny);
+}
 
 /* the kernel sets and typed views of the run's buffers for precision T */
 template <typename T> struct Prec;

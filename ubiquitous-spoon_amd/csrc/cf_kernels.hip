@@ -67,10 +67,11 @@ template __global__ void k_dev_to_io<float>(const float *, double *, int, int, i
  * previous substep's projection (project2, pressure.c:120-152, and the
  * mass-flux correction u += h1 mdpdx, chanfast.c:519-522) applied on the
  * fly, so the projected velocity is never stored.  A workgroup owns a
- * 64 x 4 column tile and marches levels [k0, k1); a three-level LDS ring
+ * 64 x 4 column tile and marches levels [k0, k1); a four-slot LDS ring
  * holds the projected u, v, w of the tile plus a one-cell halo (periodic
- * wrap in x, y; wall ghosts 2 bc - q_1 for u, v and 0 for w).  Level k + 2
- * is fetched into registers before level k is computed.  Substep 0 also
+ * wrap in x, y; wall ghosts 2 bc - q_1 for u, v and 0 for w).  Level k + 3
+ * is fetched into registers before level k is computed and level k + 2 is
+ * built into the ring right after it, with one barrier per level.  Substep 0 also
  * reduces the CFL number of the projected state (get_cfl, get.c:390-417).
  * ------------------------------------------------------------------- */
 #define RT_X 64
@@ -89,9 +90,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
     double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
     double *__restrict__ cflpart, double cfldt, int kc)
 {
-    __shared__ double SU[3][RT_HY][RT_HX], SV[3][RT_HY][RT_HX], SW[3][RT_HY][RT_HX];
+    __shared__ double SU[4][RT_HY][RT_HX], SV[4][RT_HY][RT_HX], SW[4][RT_HY][RT_HX];
     __shared__ double SP[2][2][RT_PY][RT_PX];     /* [buffer][p(L), p(L+1)] */
-    __shared__ double red[256];
+    double *red = &SU[0][0][0];                   /* CFL reduction after the last level */
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int ntx = (nx + RT_X - 1) / RT_X, nty = (ny + RT_Y - 1) / RT_Y;
@@ -130,41 +131,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
         oa[t] = e < RT_NA ? colo(e, RT_HX) : 0;
         op[t] = e < RT_NP ? colo(e, RT_PX) : 0;
     }
-    double fu_[2], fv_[2], fw_[2], pa_[2], pb_[2];
-    int last_wok_level = -100;             /* p(L + 1) of the previous fetch is p(L) of this one */
-    auto fetch = [&](int L) {
+    /* a register set holds one fetched level L of the tile: u*, v*, w* on
+     * the halo tile and p(L + 1) on the pressure tile (0 where w has no
+     * face); p(L) is the previous level's p(L + 1), or read directly where
+     * that level has no face (first level of the chunk, walls) */
+    struct RSet { double u[2], v[2], w[2], pn[2]; };
+    auto wok_of = [&](int L) { return L >= 0 && L <= nz - 2; };
+    auto fetch = [&](int L, RSet &X) {
         const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
-        const bool wok = L >= 0 && L <= nz - 2;
-        const bool reuse = last_wok_level == L - 1;
+        const bool wok = wok_of(L);
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NA) {
-                fu_[t] = S.us[(size_t) Lc * P + oa[t]];
-                fv_[t] = S.vs[(size_t) Lc * P + oa[t]];
-                fw_[t] = wok ? S.ws[(size_t) L * P + oa[t]] : 0.0;
+                X.u[t] = S.us[(size_t) Lc * P + oa[t]];
+                X.v[t] = S.vs[(size_t) Lc * P + oa[t]];
+                X.w[t] = wok ? S.ws[(size_t) L * P + oa[t]] : 0.0;
             }
-            if (e < RT_NP) {
-                pa_[t] = reuse ? pb_[t] : S.p[(size_t) Lc * P + op[t]];
-                pb_[t] = wok ? S.p[(size_t) (L + 1) * P + op[t]] : 0.0;
-            }
+            if (e < RT_NP) X.pn[t] = wok ? S.p[(size_t) (L + 1) * P + op[t]] : 0.0;
         }
-        last_wok_level = wok ? L : -100;
     };
-    auto stage_p = [&](int b) {
+    /* p(L) and p(L + 1) into pressure buffer (L - k0 + 1) & 1 */
+    auto stage_p = [&](int L, const RSet &X, const RSet *Xprev) {
+        const int b = (L - k0 + 1) & 1;
+        const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
+        const bool direct = Xprev == nullptr || !wok_of(L - 1);
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NP) {
                 const int r = e / RT_PX, c = e - r * RT_PX;
-                SP[b][0][r][c] = pa_[t];
-                SP[b][1][r][c] = pb_[t];
+                SP[b][0][r][c] = direct ? S.p[(size_t) Lc * P + op[t]] : Xprev->pn[t];
+                SP[b][1][r][c] = X.pn[t];
             }
         }
     };
-    auto build = [&](int L, int sl, int b) {
+    /* projected u, v, w of level L into ring slot (L - k0 + 1) & 3 */
+    auto build = [&](int L, const RSet &X) {
+        const int sl = (L - k0 + 1) & 3, b = (L - k0 + 1) & 1;
         const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
-        const bool wok = L >= 0 && L <= nz - 2;
+        const bool wok = wok_of(L);
         const bool ghost = L < 0 || L >= nz;
         const double hm = S.h1 ? S.h1[Lc + 1] * m : 0.0;
         const double cz = wok ? S.dtp * rdzh[L] : 0.0;
@@ -174,25 +180,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
             if (e < RT_NA) {
                 const int r = e / RT_HX, c = e - r * RT_HX;
                 const double p0 = SP[b][0][r][c];
-                const double uc = fu_[t] + hm;
+                const double uc = X.u[t] + hm;
                 double up = uc - (SP[b][0][r][c + 1] - p0) * cx;
-                double vp = fv_[t] - (SP[b][0][r + 1][c] - p0) * cy;
+                double vp = X.v[t] - (SP[b][0][r + 1][c] - p0) * cy;
                 if (ghost) { up = 2.0 * ub - up; vp = 2.0 * vb - vp; }
                 SU[sl][r][c] = up;
                 SV[sl][r][c] = vp;
-                SW[sl][r][c] = wok ? fw_[t] - (SP[b][1][r][c] - p0) * cz : 0.0;
+                SW[sl][r][c] = wok ? X.w[t] - (SP[b][1][r][c] - p0) * cz : 0.0;
             }
         }
     };
 
-    /* prologue: levels k0 - 1, k0, k0 + 1 into ring slots 0, 1, 2 */
-    for (int L = k0 - 1; L <= k0 + 1; L++) {
-        const int b = (L - k0 + 1) & 1;
-        fetch(L);
-        stage_p(b);              /* buffer b was last read two builds ago */
-        __syncthreads();
-        build(L, L - k0 + 1, b);
-    }
+    /* prologue: levels k0 - 1, k0, k0 + 1 built into the ring, the
+     * pressure of k0 + 2 staged and its fields in register set B */
+    RSet A, B;
+    fetch(k0 - 1, A);
+    fetch(k0, B);
+    stage_p(k0 - 1, A, nullptr);
+    stage_p(k0, B, &A);
+    __syncthreads();
+    build(k0 - 1, A);
+    build(k0, B);
+    fetch(k0 + 1, A);
+    __syncthreads();                              /* buffer of k0 + 1 was read by build(k0 - 1) */
+    stage_p(k0 + 1, A, &B);
+    __syncthreads();
+    build(k0 + 1, A);
+    fetch(k0 + 2, B);
+    stage_p(k0 + 2, B, &A);                       /* its buffer was last read by build(k0) */
     __syncthreads();
 
     const SubCoef C = R.sc[s];
@@ -208,16 +223,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
     double gnu = 0.0, gnv = 0.0, gnw = 0.0;
     if (act) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
 
-    for (int k = k0; k < k1; k++) {
-        const bool more = k + 2 <= k1;
+    /* level k: register set Y holds level k + 2 (its pressure staged), X
+     * receives level k + 3.  Four ring slots and two pressure buffers let
+     * build(k + 2) and stage(k + 3) run right after the stencil of level k
+     * without a barrier between them: one barrier per level. */
+    auto level = [&](const int k, RSet &X, RSet &Y) {
+        const bool more = k + 2 <= k1, more3 = k + 3 <= k1;
         const double gcu = gnu, gcv = gnv, gcw = gnw;
         if (act && k + 1 < k1) {
             gnu = gu[(size_t) (k + 1) * P + qc];
             gnv = gv[(size_t) (k + 1) * P + qc];
             gnw = gw[(size_t) (k + 1) * P + qc];
         }
-        if (more) fetch(k + 2);
-        const int sm = (k - k0) % 3, s0 = (k - k0 + 1) % 3, sp = (k - k0 + 2) % 3;
+        if (more3) fetch(k + 3, X);
+        const int sm = (k - k0) & 3, s0 = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
         if (act) {
             const size_t q = (size_t) k * P + (size_t) j * nx + i;
             const double dzk = dz[k], dzkp = dz[k + 1];
@@ -310,15 +329,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
 #undef LV
 #undef LW
         }
-        if (more) {
-            /* pressure buffer of level k + 2 alternates with that of k + 1, so
-             * staging it needs no barrier; the ring slot of level k - 1 does */
-            const int b = (k + 2 - k0 + 1) & 1;
-            stage_p(b);
-            __syncthreads();
-            build(k + 2, sm, b);
-            __syncthreads();
-        }
+        if (more) build(k + 2, Y);
+        if (more3) stage_p(k + 3, X, &Y);
+        __syncthreads();
+    };
+    for (int k = k0; k < k1; k += 2) {
+        level(k, A, B);
+        if (k + 1 < k1) level(k + 1, B, A);
     }
     if (cflpart) {
         red[tid] = cflm;
@@ -439,70 +456,72 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     const int type = f == 2 ? 2 : (f == 3 ? 1 : 0);
     const SpikeTab ST = R.sp[s][type];
     const int lane = threadIdx.x & 63;
-    /* wave-uniform chunk index, so the table reads below are scalar loads */
+    /* wave-uniform chunk index, so the table reads below are scalar loads;
+     * the tables hold CF_SPIKE_LMAX rows per chunk, identity past its end,
+     * and every read of the column is at a valid (clamped) row, so the
+     * sweeps are straight-line code whose table reads can all be hoisted */
     const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool live = chunk < ST.S;
     const size_t col = (size_t) b.x * 64 + lane;
-    const bool active = col < P && chunk < ST.S;
+    const bool act = col < P;
     T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
     const int lo = chunk * ST.L;
-    const int hi = min(lo + ST.L, ST.n);
-    T *xc = x + col;
+    const int Lc = live ? min(ST.L, ST.n - lo) : 0;     /* rows of this chunk */
+    T *xc = x + (act ? col : P - 1);
+    const size_t tb = (size_t) chunk * CF_SPIKE_LMAX;
+    const double *ta = ST.a + tb, *tib = ST.ib + tb, *tgam = ST.gam + tb, *tvl = ST.vl + tb, *twr = ST.wr + tb;
     T y[CF_SPIKE_LMAX];
+    if (live) {                                   /* wave-uniform */
 #pragma unroll
-    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
-        const int j = lo + t;
-        y[t] = (active && j < hi) ? xc[(size_t) j * P] : (T) 0;
-    }
-    T d = 0;
+        for (int t = 0; t < CF_SPIKE_LMAX; t++) y[t] = xc[(size_t) (lo + min(t, Lc - 1)) * P];
 #pragma unroll
-    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
-        const int j = lo + t;
-        if (j < hi) {
-            d = (y[t] - (T) ST.a[j] * d) * (T) ST.ib[j];
+        for (int t = 0; t < CF_SPIKE_LMAX; t++) y[t] = (act && t < Lc) ? y[t] : (T) 0;
+        T d = 0;
+#pragma unroll
+        for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+            d = (y[t] - (T) ta[t] * d) * (T) tib[t];
             y[t] = d;
         }
-    }
-    T yl = 0;
+        T yl = 0;
+        d = 0;
 #pragma unroll
-    for (int t = CF_SPIKE_LMAX - 1; t >= 0; t--) {
-        const int j = lo + t;
-        if (j < hi) {
-            if (j < hi - 1) d = y[t] - (T) ST.gam[j + 1] * d;
-            else { d = y[t]; yl = d; }
+        for (int t = CF_SPIKE_LMAX - 1; t >= 0; t--) {
+            d = t < CF_SPIKE_LMAX - 1 ? y[t] - (T) tgam[t + 1] * d : y[t];
+            yl = t == Lc - 1 ? d : yl;
             y[t] = d;
         }
-    }
-    if (chunk < ST.S) {
         itf[2 * chunk][lane] = y[0];
         itf[2 * chunk + 1][lane] = yl;
     }
     __syncthreads();
-    if (chunk >= ST.S) return;                    /* wave-uniform */
+    if (!live) return;                            /* wave-uniform */
     const bool sums = plane != nullptr && f == 0;
-    if (!active && !sums) return;
+    if (!act && !sums) return;
     const int m = 2 * ST.S;
+    constexpr int MP = 2 * CF_SPIKE_SMAX;
     T E = 0, F = 0;
-    if (active && chunk > 0) {
-        const double *row = ST.minv + (size_t) (2 * (chunk - 1) + 1) * m;
-        for (int q = 0; q < m; q++) E += (T) row[q] * itf[q][lane];
+    if (chunk > 0) {
+        const double *row = ST.minv + (size_t) (2 * (chunk - 1) + 1) * MP;
+#pragma unroll
+        for (int q = 0; q < MP; q++) E += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
-    if (active && chunk < ST.S - 1) {
-        const double *row = ST.minv + (size_t) (2 * (chunk + 1)) * m;
-        for (int q = 0; q < m; q++) F += (T) row[q] * itf[q][lane];
+    if (chunk < ST.S - 1) {
+        const double *row = ST.minv + (size_t) (2 * (chunk + 1)) * MP;
+#pragma unroll
+        for (int q = 0; q < MP; q++) F += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
 #pragma unroll
     for (int t = 0; t < CF_SPIKE_LMAX; t++) {
-        const int j = lo + t;
-        if (j < hi) {
-            const T val = y[t] - E * (T) ST.vl[j] - F * (T) ST.wr[j];
-            if (active) xc[(size_t) j * P] = val;
+        if (t < Lc) {                             /* wave-uniform */
+            const T val = y[t] - E * (T) tvl[t] - F * (T) twr[t];
+            if (act) xc[(size_t) (lo + t) * P] = val;
             if (sums) {
-                const double tot = wave_sum(active ? (double) val : 0.0);
-                if (lane == 0) plane[(size_t) j * b.nx + b.x] = tot;
+                const double tot = wave_sum(act ? (double) val : 0.0);
+                if (lane == 0) plane[(size_t) (lo + t) * b.nx + b.x] = tot;
             }
         }
     }
-    if (active && f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
+    if (act && f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
 }
 
 template <typename T>

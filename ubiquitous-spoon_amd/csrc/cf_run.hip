@@ -271,7 +271,25 @@ static int spike_build(const std::vector<double> &a, const std::vector<double> &
         if (s < S - 1) { M[(size_t) rf * m + 2 * (s + 1)] += w[lo]; M[(size_t) re * m + 2 * (s + 1)] += w[hi - 1]; }
     }
     if (invert_dense(M, m)) return -3;
-    T.minv = M;
+    /* device layout: chunk-major blocks of CF_SPIKE_LMAX rows, identity
+     * rows (a = 0, ib = 1, gam = vl = wr = 0) past a chunk's end, so the
+     * kernel's sweeps run the same straight-line code for every chunk;
+     * minv rows padded to 2 CF_SPIKE_SMAX columns */
+    auto padv = [&](const std::vector<double> &v, double fill) {
+        std::vector<double> o((size_t) S * CF_SPIKE_LMAX, fill);
+        for (int c = 0; c < S; c++)
+            for (int t = 0; t < L && c * L + t < n; t++) o[(size_t) c * CF_SPIKE_LMAX + t] = v[(size_t) c * L + t];
+        return o;
+    };
+    T.a = padv(T.a, 0.0);
+    T.ib = padv(T.ib, 1.0);
+    T.gam = padv(T.gam, 0.0);
+    T.vl = padv(T.vl, 0.0);
+    T.wr = padv(T.wr, 0.0);
+    const int mp = 2 * CF_SPIKE_SMAX;
+    T.minv.assign((size_t) m * mp, 0.0);
+    for (int r = 0; r < m; r++)
+        for (int q = 0; q < m; q++) T.minv[(size_t) r * mp + q] = M[(size_t) r * m + q];
     return 0;
 }
 
@@ -680,9 +698,9 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
         for (int t = 0; t < 3; t++) {
             SpikeHost &S = sph[s][t];
             if (S.n == 0) { spike_ok = false; continue; }
-            o_sp[s][t][0] = put(S.a.data(), S.n); o_sp[s][t][1] = put(S.ib.data(), S.n);
-            o_sp[s][t][2] = put(S.gam.data(), S.n); o_sp[s][t][3] = put(S.vl.data(), S.n);
-            o_sp[s][t][4] = put(S.wr.data(), S.n); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
+            o_sp[s][t][0] = put(S.a.data(), S.a.size()); o_sp[s][t][1] = put(S.ib.data(), S.ib.size());
+            o_sp[s][t][2] = put(S.gam.data(), S.gam.size()); o_sp[s][t][3] = put(S.vl.data(), S.vl.size());
+            o_sp[s][t][4] = put(S.wr.data(), S.wr.size()); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
         }
     size_t o_lift = put(liftg.data(), liftg.size());
     size_t o_hp[3][3] = {{0}}, o_p4[4] = {0};

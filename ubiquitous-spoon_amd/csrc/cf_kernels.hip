@@ -761,7 +761,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     };
     /* groups of CF_FX_UNR elements per thread: all reads of a group are
      * in flight before the first LDS write (blockDim.x = 256) */
-    constexpr int UNR = ORD == 2 ? CF_FX_UNR : 2;     /* order 4 reads twice as many values */
+    constexpr int UNR = ORD == 2 ? (sizeof(T) == 4 ? CF_FX_UNR : 2) : (sizeof(T) == 4 ? 2 : 1);   /* fit 128 VGPRs */
     const C *twg = twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
     const C tw0 = tw_first(twg, nx);
@@ -1402,7 +1402,7 @@ template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const
  * plan since the runs' lengths differ. */
 
 template <int ORD, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
+__global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
                                                int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {

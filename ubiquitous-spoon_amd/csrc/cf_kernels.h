@@ -146,6 +146,8 @@ static inline int pois2_L_of(int need)
 }
 /* chunk length for nz (0: one thread per mode) */
 static inline int pois2_L(int nz) { return pois2_L_of((nz + 15) / 16); }
+/* dynamic LDS of the partitioned solve: its six z tables */
+static inline size_t pois2_lds(int L, int nz) { return L > 0 ? 6 * sizeof(double) * (size_t) nz : 0; }
 static inline unsigned pois2_blocks(int L, size_t M)
 {
     return (unsigned) (L > 0 ? (M + 7) / 8 : (M + 255) / 256);

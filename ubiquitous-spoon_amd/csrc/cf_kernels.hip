@@ -1128,41 +1128,56 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
  * round-off. */
 #define CF_P2S_NC 16
 #define CF_P2S_MW 8
+/* occupancy target by chunk length: the largest without register spills */
+#define CF_P2S_WAVES(L) ((L) == 0 ? 2 : ((L) <= 13 ? 4 : ((L) <= 20 ? 3 : 1)))
 
 template <int L, typename A, typename T>
 static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Real<typename Cplx<T>::type>::type *__restrict__ yr,
                                                   size_t M, int nz, int nch, size_t idx, bool act,
                                                   double lxd, double lyd, bool zero, double (*sh)[CF_P2S_NC][16],
-                                                  double (*xs)[16])
+                                                  double (*xs)[16], double *tl)
 {
     typedef typename Real<typename Cplx<T>::type>::type S;
     const int g = threadIdx.x & 15, chunk = threadIdx.x >> 4;
     const A lx = (A) lxd, ly = (A) lyd;
     const bool live = chunk < nch;
     const int lo = chunk * L;
-    /* row coefficients, identity beyond the top wall; every read is at a
-     * clamped (valid) level and the padding is applied by selects, so the
-     * loads issue without branches */
+    const bool ld = live && act;
+    const size_t stride = 2 * M;
+    S *__restrict__ col = yr + idx;
     const int ktop = nz - 1;
+    /* the column's rows (clamped, valid reads), then the z tables into LDS
+     * (tl: a, c, bz, ex, ey, dvol, nz each) with the same round trip */
+    S raw[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) raw[t] = col[(size_t) min(lo + t, ktop) * stride];
+    {
+        const double *src[6] = { T_.a, T_.c, T_.bz, T_.ex, T_.ey, T_.dvol };
+#pragma unroll
+        for (int w = 0; w < 6; w++)
+            for (int k = (int) threadIdx.x; k < nz; k += (int) blockDim.x) tl[w * nz + k] = src[w][k];
+    }
+    __syncthreads();
+    const double *ta = tl, *tc = tl + nz, *tbz = tl + 2 * nz, *tex = tl + 3 * nz, *tey = tl + 4 * nz,
+                 *tdv = tl + 5 * nz;
+    /* row coefficients, identity beyond the top wall; every read is at a
+     * clamped (valid) level and the padding is applied by selects */
     const A dbot = (A) (zero ? T_.db_dir : T_.db_neu), dtop = (A) T_.db_top;
     auto cb = [&](int k) {
         const int kc = min(k, ktop);
-        A b = (A) T_.bz[kc] - lx * (A) T_.ex[kc] - ly * (A) T_.ey[kc];
+        A b = (A) tbz[kc] - lx * (A) tex[kc] - ly * (A) tey[kc];
         b -= k == 0 ? dbot : (A) 0;
         b -= k == ktop ? dtop : (A) 0;
         return k > ktop ? (A) 1 : b;
     };
-    auto ca = [&](int k) { const A v = (A) T_.a[min(k, ktop)]; return (k > 0 && k <= ktop) ? v : (A) 0; };
-    auto cc = [&](int k) { const A v = (A) T_.c[min(k, ktop)]; return k < ktop ? v : (A) 0; };
+    auto ca = [&](int k) { const A v = (A) ta[min(k, ktop)]; return (k > 0 && k <= ktop) ? v : (A) 0; };
+    auto cc = [&](int k) { const A v = (A) tc[min(k, ktop)]; return k < ktop ? v : (A) 0; };
     A d[L - 1], ph[L - 1], gm[L - 1];
     A ds = 0;
-    const bool ld = live && act;
-    const size_t stride = 2 * M;
-    S *__restrict__ col = yr + idx;
 #pragma unroll
     for (int t = 0; t < L; t++) {
-        const int k = lo + t, kc = min(k, ktop);
-        const A v = (A) col[(size_t) kc * stride] * (A) T_.dvol[kc];
+        const int k = lo + t;
+        const A v = (A) raw[t] * (A) tdv[min(k, ktop)];
         const A r = (ld && k <= ktop) ? v : (A) 0;
         if (t < L - 1) d[t] = r;
         else ds = r;
@@ -1273,13 +1288,15 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
             }
         }
         typename Real<typename Cplx<T>::type>::type *yr = (typename Real<typename Cplx<T>::type>::type *) Y;
+        extern __shared__ __align__(16) unsigned char smem_[];
+        double *tl = (double *) smem_;            /* 6 nz doubles (pois2_lds) */
         const size_t idx = 2 * mc + (threadIdx.x & 1);
         const double lx = R.lamx[kx], ly = R.lamy[ky];
         constexpr int LL = L > 1 ? L : 2;
         if (sizeof(T) == sizeof(double) || low)
-            pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs);
+            pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
         else
-            pois2s_run<LL, T, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs);
+            pois2s_run<LL, T, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
         return;
     }
     const size_t qm = (size_t) b.x * blockDim.x + threadIdx.x;
@@ -1294,7 +1311,7 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(256) void k_poisson2(const RunDev R, const PoisTab T_,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_P2S_WAVES(L), 8))) void k_poisson2(const RunDev R, const PoisTab T_,
                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
 {
     pois2_body<T, L>(R, T_, Y, G, blk_native());

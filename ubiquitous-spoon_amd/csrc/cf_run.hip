@@ -927,7 +927,7 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
         KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], X, r->S4);
     else
         KLAUNCH("poisson", pick_poisson2<T>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
-                dim3(256), 0, R, r->pt[tq], X, r->G);
+                dim3(256), pois2_lds(pois2_L(r->g.nz), r->g.nz), R, r->pt[tq], X, r->G);
     KLAUNCH("ffty_inv", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 0);
     KLAUNCH("fftx_inv", Prec<T>::fftxi(r), dim3(gx), dim3(256), fx_lds(r), R, (const C *) X, tv<T>(r->p));
     return CF_OK;
@@ -1446,7 +1446,7 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
                            r->rd, r->pt4[3], r->X, r->S4);
     else
         hipLaunchKernelGGL(pick_poisson2<double>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
-                           dim3(256), 0, r->stream,
+                           dim3(256), pois2_lds(pois2_L(r->g.nz), r->g.nz), r->stream,
                            r->rd, r->pt[3], r->X, r->G);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
@@ -1628,7 +1628,10 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
                 const unsigned tg = batch_total(b, row);
                 const unsigned grid = b->kgrid[k] ? std::min(b->kgrid[k], tg) : tg;
                 if (grid == 0) continue;
-                hipLaunchKernelGGL(pick_batched_poisson<T>(b->p2L[g]), dim3(grid), block, 0, b->stream,
+                size_t lp = 0;
+                for (cf_run *rr : b->runs)
+                    if (pois2_L(rr->g.nz) == b->p2L[g]) lp = std::max(lp, pois2_lds(b->p2L[g], rr->g.nz));
+                hipLaunchKernelGGL(pick_batched_poisson<T>(b->p2L[g]), dim3(grid), block, lp, b->stream,
                                    (const GArgs<T> *) b->args, (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)),
                                    b->n, s);
                 HIPCHK(hipGetLastError());

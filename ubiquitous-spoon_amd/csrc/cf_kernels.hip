@@ -132,13 +132,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
         op[t] = e < RT_NP ? colo(e, RT_PX) : 0;
     }
     /* a register set holds one fetched level L of the tile: u*, v*, w* on
-     * the halo tile and p(L + 1) on the pressure tile (0 where w has no
-     * face); p(L) is the previous level's p(L + 1), or read directly where
-     * that level has no face (first level of the chunk, walls) */
+     * the halo tile and, two levels ahead, p(L + 2) on the pressure tile
+     * (clamped to the walls), so the pressure a level stages was read a
+     * full level earlier */
     struct RSet { double u[2], v[2], w[2], pn[2]; };
     auto wok_of = [&](int L) { return L >= 0 && L <= nz - 2; };
+    auto clampz = [&](int L) { return L < 0 ? 0 : (L >= nz ? nz - 1 : L); };
     auto fetch = [&](int L, RSet &X) {
-        const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
+        const int Lc = clampz(L), L2 = clampz(L + 2);
         const bool wok = wok_of(L);
 #pragma unroll
         for (int t = 0; t < 2; t++) {
@@ -148,23 +149,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
                 X.v[t] = S.vs[(size_t) Lc * P + oa[t]];
                 X.w[t] = wok ? S.ws[(size_t) L * P + oa[t]] : 0.0;
             }
-            if (e < RT_NP) X.pn[t] = wok ? S.p[(size_t) (L + 1) * P + op[t]] : 0.0;
+            if (e < RT_NP) X.pn[t] = S.p[(size_t) L2 * P + op[t]];
         }
     };
-    /* p(L) and p(L + 1) into pressure buffer (L - k0 + 1) & 1 */
-    auto stage_p = [&](int L, const RSet &X, const RSet *Xprev) {
+    /* pressure of level L into buffer (L - k0 + 1) & 1: p(clamp(L)) and,
+     * where w has a face at L, p(L + 1) */
+    auto stage_p = [&](int L, const double *pa, const double *pb) {
         const int b = (L - k0 + 1) & 1;
-        const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
-        const bool direct = Xprev == nullptr || !wok_of(L - 1);
+        const bool wok = wok_of(L);
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NP) {
                 const int r = e / RT_PX, c = e - r * RT_PX;
-                SP[b][0][r][c] = direct ? S.p[(size_t) Lc * P + op[t]] : Xprev->pn[t];
-                SP[b][1][r][c] = X.pn[t];
+                SP[b][0][r][c] = pa[t];
+                SP[b][1][r][c] = wok ? pb[t] : 0.0;
             }
         }
+    };
+    auto pdirect = [&](int L, double *out) {
+        const int Lc = clampz(L);
+#pragma unroll
+        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? S.p[(size_t) Lc * P + op[t]] : 0.0;
     };
     /* projected u, v, w of level L into ring slot (L - k0 + 1) & 3 */
     auto build = [&](int L, const RSet &X) {
@@ -192,22 +198,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
     };
 
     /* prologue: levels k0 - 1, k0, k0 + 1 built into the ring, the
-     * pressure of k0 + 2 staged and its fields in register set B */
+     * pressure of k0 + 2 staged, set A holding level k0 + 1 (p(k0 + 3))
+     * and set B level k0 + 2 (p(k0 + 4)) */
     RSet A, B;
+    double carry[2], pd0[2], pd1[2];
     fetch(k0 - 1, A);
     fetch(k0, B);
-    stage_p(k0 - 1, A, nullptr);
-    stage_p(k0, B, &A);
+    pdirect(k0 - 1, pd0);
+    pdirect(k0, pd1);
+    stage_p(k0 - 1, pd0, pd1);
+    stage_p(k0, pd1, A.pn);                       /* A.pn = p(k0 + 1) */
     __syncthreads();
     build(k0 - 1, A);
     build(k0, B);
+    carry[0] = A.pn[0]; carry[1] = A.pn[1];       /* p(k0 + 1) */
     fetch(k0 + 1, A);
     __syncthreads();                              /* buffer of k0 + 1 was read by build(k0 - 1) */
-    stage_p(k0 + 1, A, &B);
+    stage_p(k0 + 1, carry, B.pn);                 /* B.pn = p(k0 + 2) */
     __syncthreads();
     build(k0 + 1, A);
+    carry[0] = B.pn[0]; carry[1] = B.pn[1];       /* p(k0 + 2) */
     fetch(k0 + 2, B);
-    stage_p(k0 + 2, B, &A);                       /* its buffer was last read by build(k0) */
+    stage_p(k0 + 2, carry, A.pn);                 /* its buffer was last read by build(k0); A.pn = p(k0 + 3) */
     __syncthreads();
 
     const SubCoef C = R.sc[s];
@@ -224,11 +236,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
     if (act) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
 
     /* level k: register set Y holds level k + 2 (its pressure staged), X
-     * receives level k + 3.  Four ring slots and two pressure buffers let
-     * build(k + 2) and stage(k + 3) run right after the stencil of level k
-     * without a barrier between them: one barrier per level. */
+     * (level k + 1, holding p(k + 3)) receives level k + 3.  Four ring slots
+     * and two pressure buffers let build(k + 2) and stage(k + 3) run right
+     * after the stencil of level k without a barrier between them: one
+     * barrier per level. */
     auto level = [&](const int k, RSet &X, RSet &Y) {
         const bool more = k + 2 <= k1, more3 = k + 3 <= k1;
+        double pk3[2] = { X.pn[0], X.pn[1] };     /* p(k + 3) */
         const double gcu = gnu, gcv = gnv, gcw = gnw;
         if (act && k + 1 < k1) {
             gnu = gu[(size_t) (k + 1) * P + qc];
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
 #undef LW
         }
         if (more) build(k + 2, Y);
-        if (more3) stage_p(k + 3, X, &Y);
+        if (more3) stage_p(k + 3, pk3, Y.pn);     /* Y.pn = p(k + 4) */
         __syncthreads();
     };
     for (int k = k0; k < k1; k += 2) {

@@ -110,7 +110,7 @@ static __device__ __forceinline__ Blk blk_native()
 }
 
 /* launch stages of a batched step; each run's grid per stage is dims[stage] */
-enum { ST_CELL = 0, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE, ST_FFTX, ST_FFTY, ST_MODE, CF_NST };
+enum { ST_CELL = 0, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE, ST_FFTX, ST_FFTY, ST_MODE, ST_RHS2, ST_LEVEL, CF_NST };
 #define CF_BATCH_MAX 256        /* runs per batched launch */
 
 /* Where a substep's projected velocity comes from: the previous
@@ -161,10 +161,11 @@ template <typename T> struct GArgs {
     RunDev R;
     T *u, *v, *w, *c, *ru, *rv, *rw, *rc, *fa[4], *fb[4], *p;
     typename Cplx<T>::type *X;
-    double *G, *S4, *scal, *logbuf, *part, *cflpart;
+    double *G, *S4, *scal, *logbuf, *part, *cflpart, *lsum;
     double dt;                  /* step */
     double dt_adhoc;            /* dt of a stand-alone projection (substep index 3) */
-    int nb_plane, gblocks;
+    int nb_plane, gblocks;      /* gblocks: CFL partials of the run's RHS launch */
+    int rhs_kc;                 /* levels per z-chunk of the fast-path RHS */
     PoisTab pt[4];              /* substeps 0..2, stand-alone projection */
     Pois4Tab pt4[4];
     unsigned dims[CF_NST][3];
@@ -274,6 +275,11 @@ template <typename T> __global__ void kb_plane(const GArgs<T> *a, const unsigned
 template <typename T> __global__ void kb_mass(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_cfl(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_step_end(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+/* batched order-2 FP64 fast path */
+__global__ void kb_rhs2t(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
+__global__ void kb_spike_sums(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
+__global__ void kb_level_sum(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
+__global__ void kb_mass_fast(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
 template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);

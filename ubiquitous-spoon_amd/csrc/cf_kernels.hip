@@ -83,12 +83,13 @@ template __global__ void k_dev_to_io<float>(const float *, double *, int, int, i
 #define RT_NA (RT_HX * RT_HY)      /* 396 halo-tile cells */
 #define RT_NP (RT_PX * RT_PY)      /* 469 pressure cells (one more in +x, +y) */
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVES, 8))) void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
+/* lb: the tile's block index within the run (ntx nty nchunk of them) */
+static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, const ProjSrc &S,
     const double *__restrict__ gu, const double *__restrict__ gv, const double *__restrict__ gw,
     const double *__restrict__ scal,
     double *__restrict__ fu, double *__restrict__ fv, double *__restrict__ fw,
     double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
-    double *__restrict__ cflpart, double cfldt, int kc)
+    double *__restrict__ cflpart, double cfldt, int kc, const unsigned lb)
 {
     __shared__ double SU[4][RT_HY][RT_HX], SV[4][RT_HY][RT_HX], SW[4][RT_HY][RT_HX];
     __shared__ double SP[2][2][RT_PY][RT_PX];     /* [buffer][p(L), p(L+1)] */
@@ -96,7 +97,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int ntx = (nx + RT_X - 1) / RT_X, nty = (ny + RT_Y - 1) / RT_Y;
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const int tx = (int) (lb % (unsigned) ntx);
     const unsigned rest = lb / (unsigned) ntx;
     const int ty = (int) (rest % (unsigned) nty);
@@ -358,8 +358,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVE
             if (tid < w) red[tid] = fmax(red[tid], red[tid + w]);
             __syncthreads();
         }
-        if (tid == 0) cflpart[blockIdx.x] = red[0];
+        if (tid == 0) cflpart[lb] = red[0];
     }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVES, 8))) void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
+    const double *__restrict__ gu, const double *__restrict__ gv, const double *__restrict__ gw,
+    const double *__restrict__ scal,
+    double *__restrict__ fu, double *__restrict__ fv, double *__restrict__ fw,
+    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
+    double *__restrict__ cflpart, double cfldt, int kc)
+{
+    rhs2t_body(R, s, S, gu, gv, gw, scal, fu, fv, fw, ru, rv, rw, cflpart, cfldt, kc, xcd_block(blockIdx.x, gridDim.x));
 }
 
 /* ---------------------------------------------------------------------
@@ -554,11 +564,10 @@ __global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s,
 
 /* second stage of the plane sums: level blockIdx.x, nbx partials in a
  * fixed tree */
-__global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ plane, int nbx,
-                                                   double *__restrict__ lsum)
+static __device__ __forceinline__ void level_sum_body(const double *__restrict__ plane, int nbx,
+                                                      double *__restrict__ lsum, const int l)
 {
     __shared__ double red[256];
-    const int l = blockIdx.x;
     double acc = 0.0;
     for (int q = threadIdx.x; q < nbx; q += 256) acc += plane[(size_t) l * nbx + q];
     red[threadIdx.x] = acc;
@@ -568,6 +577,12 @@ __global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ pl
         __syncthreads();
     }
     if (threadIdx.x == 0) lsum[l] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ plane, int nbx,
+                                                   double *__restrict__ lsum)
+{
+    level_sum_body(plane, nbx, lsum, blockIdx.x);
 }
 
 template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
@@ -584,8 +599,8 @@ template __global__ void k_spike<float>(const RunDev, const int, float *, float 
  * Writes the forcing into scal[slot]; on the last substep writes the
  * pstat record.  One block.
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int s, const double *__restrict__ lsum,
-                                                       double *__restrict__ scal, double *__restrict__ logbuf, int kind)
+static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const int s, const double *__restrict__ lsum,
+                                                          double *__restrict__ scal, double *__restrict__ logbuf, int kind)
 {
     extern __shared__ __align__(16) unsigned char smem_[];
     double *avg = (double *) smem_;               /* nz + 2 */
@@ -618,6 +633,12 @@ __global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int
         double *lg = logbuf + (size_t) idx * 13 + (kind == 0 ? 3 : 8);
         lg[0] = m; lg[1] = bot; lg[2] = top; lg[3] = vol; lg[4] = cl;
     }
+}
+
+__global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int s, const double *__restrict__ lsum,
+                                                       double *__restrict__ scal, double *__restrict__ logbuf, int kind)
+{
+    mass_finalize_body(R, s, lsum, scal, logbuf, kind);
 }
 
 /* ---------------------------------------------------------------------
@@ -1527,6 +1548,52 @@ __global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *a_, const uns
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         (void) lb;
         if (threadIdx.x == 0) a_[r].scal[SC_STEP] += 1.0;
+    });
+}
+
+/* ---- batched order-2 FP64 fast path: the single-run kernels' bodies over
+ * the runs of a batch.  The argument table in use maps u, v, w to the
+ * substep's input state set and ru, rv, rw to its output set. */
+__global__ __launch_bounds__(256) void kb_rhs2t(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<double> &a = a_[r];
+        const int ps = (s + 2) % 3;               /* the substep whose projection is applied */
+        ProjSrc S;
+        S.us = a.u; S.vs = a.v; S.ws = a.w; S.p = a.p;
+        S.h1 = a.R.h1 + (size_t) (ps * 2) * (a.R.g.nz + 2);
+        S.m = a.scal + SC_M0 + ps;
+        S.dtp = a.R.sc[ps].abdt;
+        double *const *fnew = (s % 2 == 0) ? a.fa : a.fb;
+        double *const *gold = (s % 2 == 0) ? a.fb : a.fa;
+        rhs2t_body(a.R, s, S, gold[0], gold[1], gold[2], a.scal, fnew[0], fnew[1], fnew[2], a.ru, a.rv, a.rw,
+                   s == 0 ? a.cflpart : nullptr, a.dt, a.rhs_kc, lb);
+    });
+}
+
+__global__ __launch_bounds__(512) void kb_spike_sums(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<double> &a = a_[r];
+        spike_body<double>(a.R, s, a.ru, a.rv, a.rw, nullptr, blk_of(a, ST_SOLVE, lb), a.part);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_level_sum(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+{
+    (void) s;
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        const GArgs<double> &a = a_[r];
+        level_sum_body(a.part, (int) a.dims[ST_SOLVE][0], a.lsum, (int) lb);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+{
+    batch_loop(vs, nrun, [&](int r, unsigned lb) {
+        (void) lb;
+        const GArgs<double> &a = a_[r];
+        mass_finalize_body(a.R, s, a.lsum, a.scal, a.logbuf, 0);
     });
 }
 

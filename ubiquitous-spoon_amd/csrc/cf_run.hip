@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <map>
 #include <string>
 #include <vector>
@@ -1469,11 +1470,13 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
  */
 
 enum { KB_RHS_U = 0, KB_RHS_V, KB_RHS_W, KB_RHS_C, KB_SOLVE, KB_PLANE, KB_MASS, KB_CFL, KB_DIVX, KB_FFTY,
-       KB_POIS, KB_FFTXI, KB_PROJ, KB_STEPEND, KB_N };
+       KB_POIS, KB_FFTXI, KB_PROJ, KB_STEPEND, KB_FRHS, KB_FSPIKE, KB_FLEVEL, KB_FMASS, KB_N };
 static const int KB_STAGE[KB_N] = { ST_CELL, ST_CELL, ST_CELL, ST_CELL, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE,
-                                    ST_FFTX, ST_FFTY, ST_MODE, ST_FFTX, ST_CELL, ST_ONE };
+                                    ST_FFTX, ST_FFTY, ST_MODE, ST_FFTX, ST_CELL, ST_ONE,
+                                    ST_RHS2, ST_SOLVE, ST_LEVEL, ST_ONE };
 static const char *KB_NAME[KB_N] = { "rhs", "rhs", "rhs", "rhs", "solve", "plane_sum", "mass", "cfl_finish",
-                                     "div_fftx", "ffty", "poisson", "fftx_inv", "project", "step_end" };
+                                     "div_fftx", "ffty", "poisson", "fftx_inv", "project", "step_end",
+                                     "rhs", "spike", "plane_sum", "mass" };
 enum { SOLVE_HEPT = 0, SOLVE_SPIKE = 1, SOLVE_THOMAS = 2 };
 
 struct cf_batch {
@@ -1490,6 +1493,13 @@ struct cf_batch {
     size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    /* order-2 FP64 runs without the scalar step on the fast path's fused
+     * kernels: two argument tables (state sets swapped), one graph per
+     * starting set */
+    bool fast = false;
+    int cur = 0, tab = 0;
+    hipGraph_t fgraph[2] = { nullptr, nullptr };
+    hipGraphExec_t fexec[2] = { nullptr, nullptr };
     int steps_done = 0;
     double dt_adhoc = -1.0;
     bool profile = false;
@@ -1528,10 +1538,12 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         a.p = tv<T>(r->p);
         a.X = tx<T>(r->X);
         a.G = r->G; a.S4 = r->S4; a.scal = r->scal; a.logbuf = r->logbuf; a.part = r->part; a.cflpart = r->cflpart;
+        a.lsum = r->lsum;
         a.dt = r->cfg.dt;
         a.dt_adhoc = r->pt_dt;
         a.nb_plane = r->nb_plane;
-        a.gblocks = r->gblocks;
+        a.gblocks = b->fast ? r->rhs_blocks : r->gblocks;
+        a.rhs_kc = r->rhs_kc;
         for (int t = 0; t < 4; t++) { a.pt[t] = r->pt[t]; a.pt4[t] = r->pt4[t]; }
         const unsigned nx = r->g.nx, ny = r->g.ny, nz = r->g.nz;
         const unsigned P = (unsigned) r->P, nf = (unsigned) r->nfield, kinds = r->cfg.scalar ? 2u : 1u;
@@ -1546,6 +1558,8 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         set(ST_FFTX, (unsigned) ((npairs + r->rd.fx_pairs - 1) / r->rd.fx_pairs), 1, 1);
         set(ST_FFTY, fy_blocks(r), 1, 1);
         set(ST_MODE, b->order == 2 ? pois2_blocks(pois2_L(r->g.nz), r->M) : (unsigned) ((r->M + 255) / 256), 1, 1);
+        set(ST_RHS2, (unsigned) std::max(r->rhs_blocks, 1), 1, 1);
+        set(ST_LEVEL, nz, 1, 1);
     }
 }
 
@@ -1568,6 +1582,15 @@ static cf_status batch_upload_tables(cf_batch *b)
         b->hv[(size_t) st * (b->n + 1) + b->n] = acc;
     }
     HIPCHK(hipMemcpy(b->args, A.data(), sizeof(GArgs<T>) * A.size(), hipMemcpyHostToDevice));
+    if (b->fast) {
+        /* table 1: the other state set is the substep's input */
+        for (GArgs<T> &a : A) {
+            std::swap(a.u, a.ru);
+            std::swap(a.v, a.rv);
+            std::swap(a.w, a.rw);
+        }
+        HIPCHK(hipMemcpy((GArgs<T> *) b->args + b->n, A.data(), sizeof(GArgs<T>) * A.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(b->vstart, b->hv.data(), sizeof(unsigned) * b->hv.size(), hipMemcpyHostToDevice));
     return CF_OK;
 }
@@ -1632,14 +1655,28 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
                 for (cf_run *rr : b->runs)
                     if (pois2_L(rr->g.nz) == b->p2L[g]) lp = std::max(lp, pois2_lds(b->p2L[g], rr->g.nz));
                 hipLaunchKernelGGL(pick_batched_poisson<T>(b->p2L[g]), dim3(grid), block, lp, b->stream,
-                                   (const GArgs<T> *) b->args, (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)),
-                                   b->n, s);
+                                   (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
+                                   (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)), b->n, s);
                 HIPCHK(hipGetLastError());
             }
             return bev_end(b);
         }
     case KB_FFTXI: fn = kb_fftxi<T>; lds = b->lds_fx; break;
     case KB_STEPEND: fn = kb_step_end<T>; break;
+    case KB_FRHS:
+        if constexpr (std::is_same<T, double>::value) fn = kb_rhs2t;
+        break;
+    case KB_FSPIKE:
+        if constexpr (std::is_same<T, double>::value) fn = kb_spike_sums;
+        block = dim3(64 * b->spike_waves);
+        break;
+    case KB_FLEVEL:
+        if constexpr (std::is_same<T, double>::value) fn = kb_level_sum;
+        break;
+    case KB_FMASS:
+        if constexpr (std::is_same<T, double>::value) fn = kb_mass_fast;
+        lds = b->lds_mass;
+        break;
     default: cf_set_error("bad batch kernel slot %d", k); return CF_ERR_ARG;
     }
     if (!fn) return CF_OK;
@@ -1649,7 +1686,7 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     if (grid == 0) return CF_OK;
     cf_status e = bev_begin(b, KB_NAME[k]);
     if (e != CF_OK) return e;
-    hipLaunchKernelGGL(fn, dim3(grid), block, lds, b->stream, (const GArgs<T> *) b->args,
+    hipLaunchKernelGGL(fn, dim3(grid), block, lds, b->stream, (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
                        (const unsigned *) (b->vstart + (size_t) st * (b->n + 1)), b->n, s);
     HIPCHK(hipGetLastError());
     return bev_end(b);
@@ -1683,6 +1720,24 @@ static cf_status batch_enqueue_step(cf_batch *b)
         if ((st = batch_launch<T>(b, KB_PROJ, s)) != CF_OK) return st;
     }
     return batch_launch<T>(b, KB_STEPEND, 0);
+}
+
+/* one RK3 step of a fast batch starting from state set `from`: the fast
+ * path's sequence (enqueue_step) with every launch covering all runs */
+static cf_status fbatch_enqueue_step(cf_batch *b, int from)
+{
+    cf_status st;
+    for (int s = 0; s < 3; s++) {
+        b->tab = (from + s) & 1;
+        if ((st = batch_launch<double>(b, KB_FRHS, s)) != CF_OK) return st;
+        if (s == 0 && (st = batch_launch<double>(b, KB_CFL, s)) != CF_OK) return st;
+        if ((st = batch_launch<double>(b, KB_FSPIKE, s)) != CF_OK) return st;
+        if ((st = batch_launch<double>(b, KB_FLEVEL, s)) != CF_OK) return st;
+        if ((st = batch_launch<double>(b, KB_FMASS, s)) != CF_OK) return st;
+        if ((st = batch_enqueue_pressure<double>(b, s)) != CF_OK) return st;
+    }
+    b->tab = 0;
+    return batch_launch<double>(b, KB_STEPEND, 0);
 }
 
 static cf_status batch_collect(cf_batch *b)
@@ -1734,7 +1789,10 @@ static cf_status batch_autotune(cf_batch *b)
 {
     cf_status st;
     b->tuning.clear();
-    if (b->order == 2) {
+    if (b->fast) {
+        b->solve_kind = SOLVE_SPIKE;
+        b->tuning.push_back("fast path: fused RHS, spike with plane sums, FFT Poisson");
+    } else if (b->order == 2) {
         float best = 1e30f;
         int bk = SOLVE_THOMAS;
         bool spike_ok = true;
@@ -1756,8 +1814,10 @@ static cf_status batch_autotune(cf_batch *b)
     }
     if ((st = batch_upload_tables<T>(b)) != CF_OK) return st;
     /* slots whose repeated launch is harmless on a zero state and that do real work */
-    const int slots[] = { KB_RHS_U, KB_RHS_V, KB_RHS_W, KB_RHS_C, KB_SOLVE, KB_PLANE, KB_DIVX, KB_FFTY, KB_POIS,
-                          KB_FFTXI, KB_PROJ };
+    const int gslots[] = { KB_RHS_U, KB_RHS_V, KB_RHS_W, KB_RHS_C, KB_SOLVE, KB_PLANE, KB_DIVX, KB_FFTY, KB_POIS,
+                           KB_FFTXI, KB_PROJ };
+    const int fslots[] = { KB_FRHS, KB_FSPIKE, KB_DIVX, KB_FFTY, KB_POIS, KB_FFTXI };
+    std::vector<int> slots = b->fast ? std::vector<int>(fslots, fslots + 6) : std::vector<int>(gslots, gslots + 11);
     for (int k : slots) {
         if (k == KB_RHS_C && !b->scalar) continue;
         const unsigned total = batch_total(b, KB_STAGE[k]);
@@ -1784,7 +1844,7 @@ static cf_status batch_autotune(cf_batch *b)
     for (cf_run *r : b->runs) {
         HIPCHK(hipMemset(r->scal, 0, sizeof(double) * SC_N));
         HIPCHK(hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N));
-        double *zb[] = { r->u, r->v, r->w, r->c, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
+        double *zb[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                          r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p };
         for (double *z : zb)
             if (z) HIPCHK(hipMemset(z, 0, r->esz * r->N));
@@ -1798,6 +1858,10 @@ static void free_batch(cf_batch *b)
     if (!b) return;
     if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
     if (b->graph) (void) hipGraphDestroy(b->graph);
+    for (int q = 0; q < 2; q++) {
+        if (b->fexec[q]) (void) hipGraphExecDestroy(b->fexec[q]);
+        if (b->fgraph[q]) (void) hipGraphDestroy(b->fgraph[q]);
+    }
     if (b->args) (void) hipFree(b->args);
     if (b->vstart) (void) hipFree(b->vstart);
     for (auto &pe : b->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
@@ -1824,9 +1888,24 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     b->order = cfgs[0].order;
     b->scalar = cfgs[0].scalar;
     b->device = cfgs[0].device;
+    /* order 2, FP64, no scalar, z-solves the spike covers: the fast path */
+    b->fast = b->order == 2 && b->prec == CF_FP64 && !b->scalar;
+    for (int q = 0; q < nrun && b->fast; q++)
+        b->fast = cfgs[q].nz <= CF_SPIKE_LMAX * CF_SPIKE_SMAX && cfgs[q].cflmode == 0;
     for (int q = 0; q < nrun; q++) {
         cf_run *r = nullptr;
-        cf_status st = create_run(&cfgs[q], true, &r);
+        cf_status st = create_run(&cfgs[q], !b->fast, &r);
+        if (st == CF_OK && b->fast && (r->general || !r->use_spike)) {
+            /* a system the spike cannot take: every run goes general */
+            free_run(r);
+            for (cf_run *o : b->runs) free_run(o);
+            b->runs.clear();
+            b->fast = false;
+            b->spike_waves = 1;
+            b->lds_fx = b->lds_fy = b->lds_mass = 0;
+            q = -1;
+            continue;
+        }
         if (st != CF_OK) {
             char msg[512];
             snprintf(msg, sizeof msg, "run %d: %s", q, cf_last_error());
@@ -1846,7 +1925,7 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
         b->ncu = prop.multiProcessorCount;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
-    if (hipMalloc(&b->args, asz * nrun) != hipSuccess
+    if (hipMalloc(&b->args, asz * nrun * (b->fast ? 2 : 1)) != hipSuccess
         || hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG) * (nrun + 1)) != hipSuccess
         || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         free_batch(b);
@@ -1892,6 +1971,14 @@ extern "C" cf_status cf_batch_project(cf_batch *b, double dt)
 {
     CHECK_BATCH(b, 0);
     if (!(dt > 0)) { cf_set_error("cf_batch_project: dt must be positive"); return CF_ERR_ARG; }
+    if (b->fast) {
+        /* the fast path's own projection, run by run (once per batch) */
+        for (cf_run *r : b->runs) {
+            cf_status st = cf_run_project(r, dt);
+            if (st != CF_OK) return st;
+        }
+        return CF_OK;
+    }
     for (cf_run *r : b->runs) {
         cf_status st = set_adhoc_poisson(r, dt);
         if (st != CF_OK) return st;
@@ -1916,6 +2003,34 @@ extern "C" cf_status cf_batch_step(cf_batch *b, int nsteps)
     if (nsteps < 0) { cf_set_error("negative step count"); return CF_ERR_ARG; }
     if (nsteps == 0) return CF_OK;
     cf_status st;
+    if (b->fast) {
+        for (int n = 0; n < nsteps; n++) {
+            const int from = b->cur;
+            if (b->profile) {
+                if ((st = fbatch_enqueue_step(b, from)) != CF_OK) return st;
+            } else {
+                if (!b->fexec[from]) {
+                    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                    st = fbatch_enqueue_step(b, from);
+                    hipGraph_t gr = nullptr;
+                    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+                    if (st != CF_OK) { if (gr) (void) hipGraphDestroy(gr); return st; }
+                    HIPCHK(e);
+                    b->fgraph[from] = gr;
+                    HIPCHK(hipGraphInstantiate(&b->fexec[from], b->fgraph[from], nullptr, nullptr, 0));
+                }
+                HIPCHK(hipGraphLaunch(b->fexec[from], b->stream));
+            }
+            b->cur ^= 1;
+        }
+        b->steps_done += nsteps;
+        for (cf_run *r : b->runs) {
+            r->steps_done += nsteps;
+            r->cur = b->cur;
+            r->proj_pending = true;
+        }
+        return b->profile ? batch_collect(b) : CF_OK;
+    }
     if (b->profile) {
         for (int n = 0; n < nsteps; n++) {
             st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);

@@ -127,6 +127,19 @@ struct PoisTab {
     double db_neu, db_dir, db_top;
 };
 
+/* z-solve chunk lengths compiled for the spike kernels: a run's tables
+ * hold L rows per chunk, L the smallest of these with CF_SPIKE_SMAX L >= nz
+ * (0: nz beyond CF_SPIKE_SMAX CF_SPIKE_LMAX, Thomas instead) */
+#define CF_SPIKE_LSET(X) X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
+static inline int spike_L(int nz)
+{
+    static const int set[] = { 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32 };
+    const int need = (nz + CF_SPIKE_SMAX - 1) / CF_SPIKE_SMAX;
+    for (int l : set)
+        if (l >= need) return l;
+    return 0;
+}
+
 /* LDS of the FFT kernels: lines of n elements at stride fft_ld(n), then
  * the n twiddles of the passes, staged once per workgroup */
 static inline __host__ __device__ int fft_ld(int n) { return n + 1; }
@@ -216,8 +229,10 @@ __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
                         double *fu, double *fv, double *fw, double *ru, double *rv, double *rw,
                         double *cflpart, double cfldt, int kc);
 template <typename T> __global__ void k_thomas(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
-template <typename T> __global__ void k_spike(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
-__global__ void k_spike_sums(const RunDev R, const int s, double *ru, double *rv, double *rw, double *plane);
+template <typename T> using k_spike_t = void (*)(const RunDev, const int, T *, T *, T *, T *);
+template <typename T> k_spike_t<T> pick_spike(int L);
+typedef void (*k_spike_sums_t)(const RunDev, const int, double *, double *, double *, double *);
+k_spike_sums_t pick_spike_sums(int L);
 __global__ void k_level_sum(const double *plane, int nbx, double *lsum);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *lsum, double *scal, double *logbuf,
                                 int kind);
@@ -269,7 +284,7 @@ template <typename T> BSel<T> pick_batched(int order, int scalar);
 template <typename T> typename BSel<T>::divx_t_ pick_batched_divx(int order);
 template <typename T> __global__ void kb_poisson4(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_solve4(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
-template <typename T> __global__ void kb_spike(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> typename BSel<T>::kb_t pick_batched_spike(int L);
 template <typename T> __global__ void kb_thomas(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_plane(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_mass(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
@@ -277,7 +292,7 @@ template <typename T> __global__ void kb_cfl(const GArgs<T> *a, const unsigned *
 template <typename T> __global__ void kb_step_end(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 /* batched order-2 FP64 fast path */
 __global__ void kb_rhs2t(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
-__global__ void kb_spike_sums(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
+BSel<double>::kb_t pick_batched_spike_sums(int L);
 __global__ void kb_level_sum(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
 __global__ void kb_mass_fast(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);

@@ -467,7 +467,7 @@ static __device__ __forceinline__ double wave_sum(double v)
 /* plane != null (field 0 only): also write, per level, the sum of the
  * solved u* over the workgroup's 64 columns to plane[level * b.nx + b.x],
  * the first stage of get_xy_avg (get.c:281-307) */
-template <typename T>
+template <typename T, int LM>
 static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc, const Blk b,
     double *__restrict__ plane = nullptr)
@@ -481,7 +481,7 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     const SpikeTab ST = R.sp[s][type];
     const int lane = threadIdx.x & 63;
     /* wave-uniform chunk index, so the table reads below are scalar loads;
-     * the tables hold CF_SPIKE_LMAX rows per chunk, identity past its end,
+     * the tables hold LM rows per chunk, identity past its end,
      * and every read of the column is at a valid (clamped) row, so the
      * sweeps are straight-line code whose table reads can all be hoisted */
     const int chunk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -492,25 +492,25 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     const int lo = chunk * ST.L;
     const int Lc = live ? min(ST.L, ST.n - lo) : 0;     /* rows of this chunk */
     T *xc = x + (act ? col : P - 1);
-    const size_t tb = (size_t) chunk * CF_SPIKE_LMAX;
+    const size_t tb = (size_t) chunk * LM;
     const double *ta = ST.a + tb, *tib = ST.ib + tb, *tgam = ST.gam + tb, *tvl = ST.vl + tb, *twr = ST.wr + tb;
-    T y[CF_SPIKE_LMAX];
+    T y[LM];
     if (live) {                                   /* wave-uniform */
 #pragma unroll
-        for (int t = 0; t < CF_SPIKE_LMAX; t++) y[t] = xc[(size_t) (lo + min(t, Lc - 1)) * P];
+        for (int t = 0; t < LM; t++) y[t] = xc[(size_t) (lo + min(t, Lc - 1)) * P];
 #pragma unroll
-        for (int t = 0; t < CF_SPIKE_LMAX; t++) y[t] = (act && t < Lc) ? y[t] : (T) 0;
+        for (int t = 0; t < LM; t++) y[t] = (act && t < Lc) ? y[t] : (T) 0;
         T d = 0;
 #pragma unroll
-        for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+        for (int t = 0; t < LM; t++) {
             d = (y[t] - (T) ta[t] * d) * (T) tib[t];
             y[t] = d;
         }
         T yl = 0;
         d = 0;
 #pragma unroll
-        for (int t = CF_SPIKE_LMAX - 1; t >= 0; t--) {
-            d = t < CF_SPIKE_LMAX - 1 ? y[t] - (T) tgam[t + 1] * d : y[t];
+        for (int t = LM - 1; t >= 0; t--) {
+            d = t < LM - 1 ? y[t] - (T) tgam[t + 1] * d : y[t];
             yl = t == Lc - 1 ? d : yl;
             y[t] = d;
         }
@@ -535,7 +535,7 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
         for (int q = 0; q < MP; q++) F += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
 #pragma unroll
-    for (int t = 0; t < CF_SPIKE_LMAX; t++) {
+    for (int t = 0; t < LM; t++) {
         if (t < Lc) {                             /* wave-uniform */
             const T val = y[t] - E * (T) tvl[t] - F * (T) twr[t];
             if (act) xc[(size_t) (lo + t) * P] = val;
@@ -548,18 +548,42 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     if (act && f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
 }
 
-template <typename T>
+template <typename T, int LM>
 __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
     T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw, T *__restrict__ rc)
 {
-    spike_body<T>(R, s, ru, rv, rw, rc, blk_native());
+    spike_body<T, LM>(R, s, ru, rv, rw, rc, blk_native());
 }
 
 /* the fast path's solve: k_spike with the plane sums of u* fused in */
+template <int LM>
 __global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s, double *__restrict__ ru,
     double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ plane)
 {
-    spike_body<double>(R, s, ru, rv, rw, nullptr, blk_native(), plane);
+    spike_body<double, LM>(R, s, ru, rv, rw, nullptr, blk_native(), plane);
+}
+
+template <typename T>
+k_spike_t<T> pick_spike(int L)
+{
+    switch (L) {
+#define X_(N) case N: return k_spike<T, N>;
+    CF_SPIKE_LSET(X_)
+#undef X_
+    default: return nullptr;
+    }
+}
+template k_spike_t<double> pick_spike<double>(int);
+template k_spike_t<float> pick_spike<float>(int);
+
+k_spike_sums_t pick_spike_sums(int L)
+{
+    switch (L) {
+#define X_(N) case N: return k_spike_sums<N>;
+    CF_SPIKE_LSET(X_)
+#undef X_
+    default: return nullptr;
+    }
 }
 
 /* second stage of the plane sums: level blockIdx.x, nbx partials in a
@@ -585,8 +609,7 @@ __global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ pl
     level_sum_body(plane, nbx, lsum, blockIdx.x);
 }
 
-template __global__ void k_spike<double>(const RunDev, const int, double *, double *, double *, double *);
-template __global__ void k_spike<float>(const RunDev, const int, float *, float *, float *, float *);
+
 
 
 /* ---------------------------------------------------------------------
@@ -1513,13 +1536,24 @@ typename BSel<T>::kb_t pick_batched_poisson(int L)
     }
 }
 
-template <typename T>
+template <typename T, int LM>
 __global__ __launch_bounds__(512) void kb_spike(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
-        spike_body<T>(a.R, s, a.ru, a.rv, a.rw, a.rc, blk_of(a, ST_SOLVE, lb));
+        spike_body<T, LM>(a.R, s, a.ru, a.rv, a.rw, a.rc, blk_of(a, ST_SOLVE, lb));
     });
+}
+
+template <typename T>
+typename BSel<T>::kb_t pick_batched_spike(int L)
+{
+    switch (L) {
+#define X_(N) case N: return kb_spike<T, N>;
+    CF_SPIKE_LSET(X_)
+#undef X_
+    default: return nullptr;
+    }
 }
 
 template <typename T>
@@ -1571,12 +1605,23 @@ __global__ __launch_bounds__(256) void kb_rhs2t(const GArgs<double> *a_, const u
     });
 }
 
+template <int LM>
 __global__ __launch_bounds__(512) void kb_spike_sums(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<double> &a = a_[r];
-        spike_body<double>(a.R, s, a.ru, a.rv, a.rw, nullptr, blk_of(a, ST_SOLVE, lb), a.part);
+        spike_body<double, LM>(a.R, s, a.ru, a.rv, a.rw, nullptr, blk_of(a, ST_SOLVE, lb), a.part);
     });
+}
+
+BSel<double>::kb_t pick_batched_spike_sums(int L)
+{
+    switch (L) {
+#define X_(N) case N: return kb_spike_sums<N>;
+    CF_SPIKE_LSET(X_)
+#undef X_
+    default: return nullptr;
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_level_sum(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
@@ -1601,7 +1646,7 @@ __global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<double> *a_, con
     template __global__ void kb_ffty<T>(const GArgs<T> *, const unsigned *, int, int);                            \
     template __global__ void kb_fftxi<T>(const GArgs<T> *, const unsigned *, int, int);                           \
     template typename BSel<T>::kb_t pick_batched_poisson<T>(int);                                                  \
-    template __global__ void kb_spike<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template typename BSel<T>::kb_t pick_batched_spike<T>(int);                                                    \
     template __global__ void kb_thomas<T>(const GArgs<T> *, const unsigned *, int, int);                          \
     template __global__ void kb_cfl<T>(const GArgs<T> *, const unsigned *, int, int);                             \
     template __global__ void kb_step_end<T>(const GArgs<T> *, const unsigned *, int, int);                        \

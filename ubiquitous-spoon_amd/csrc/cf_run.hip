@@ -106,7 +106,7 @@ struct cf_run {
     int rhs_blocks = 0, rhs_kc = 1;
     int steps_done = 0;
     bool use_spike = false;
-    int spike_waves = 1;
+    int spike_waves = 1, sp_L = 0;              /* chunks and rows per chunk of the z solves */
     KSel<double>::div_t kdiv = nullptr;
     k_fftx_plain_t kfxp = nullptr;
     KSel<double>::ffty_t kfy = nullptr;
@@ -235,14 +235,17 @@ static int invert_dense(std::vector<double> &M, int m)
     return 0;
 }
 
+/* L rows per chunk (spike_L of the run's nz, shared by its u, v and w
+ * systems so one kernel instantiation serves all three) */
 static int spike_build(const std::vector<double> &a, const std::vector<double> &b,
-                       const std::vector<double> &c, SpikeHost &T)
+                       const std::vector<double> &c, int Lrun, SpikeHost &T)
 {
     const int n = (int) b.size();
     T.n = n;
-    T.S = (n + CF_SPIKE_LMAX - 1) / CF_SPIKE_LMAX;
+    if (Lrun <= 0) return -1;
+    T.L = Lrun;
+    T.S = (n + T.L - 1) / T.L;
     if (T.S > CF_SPIKE_SMAX) return -1;
-    T.L = (n + T.S - 1) / T.S;
     T.a = a; T.ib.assign(n, 0.0); T.gam.assign(n, 0.0); T.vl.assign(n, 0.0); T.wr.assign(n, 0.0);
     const int S = T.S, L = T.L, m = 2 * S;
     std::vector<double> M((size_t) m * m, 0.0);
@@ -272,14 +275,14 @@ static int spike_build(const std::vector<double> &a, const std::vector<double> &
         if (s < S - 1) { M[(size_t) rf * m + 2 * (s + 1)] += w[lo]; M[(size_t) re * m + 2 * (s + 1)] += w[hi - 1]; }
     }
     if (invert_dense(M, m)) return -3;
-    /* device layout: chunk-major blocks of CF_SPIKE_LMAX rows, identity
-     * rows (a = 0, ib = 1, gam = vl = wr = 0) past a chunk's end, so the
-     * kernel's sweeps run the same straight-line code for every chunk;
-     * minv rows padded to 2 CF_SPIKE_SMAX columns */
+    /* device layout: chunk-major blocks of L rows, identity rows (a = 0,
+     * ib = 1, gam = vl = wr = 0) past the last chunk's end, so the kernel's
+     * sweeps run the same straight-line code for every chunk; minv rows
+     * padded to 2 CF_SPIKE_SMAX columns */
     auto padv = [&](const std::vector<double> &v, double fill) {
-        std::vector<double> o((size_t) S * CF_SPIKE_LMAX, fill);
+        std::vector<double> o((size_t) S * L, fill);
         for (int c = 0; c < S; c++)
-            for (int t = 0; t < L && c * L + t < n; t++) o[(size_t) c * CF_SPIKE_LMAX + t] = v[(size_t) c * L + t];
+            for (int t = 0; t < L && c * L + t < n; t++) o[(size_t) c * L + t] = v[(size_t) c * L + t];
         return o;
     };
     T.a = padv(T.a, 0.0);
@@ -509,7 +512,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
                 delete r;
                 return CF_ERR_NUMERIC;
             }
-            if (spike_build(a, b, c, sph[s][type])) sph[s][type].n = 0;   /* falls back to Thomas */
+            if (spike_build(a, b, c, spike_L(nz), sph[s][type])) sph[s][type].n = 0;   /* falls back to Thomas */
             if (type < 2) {
                 /* unit-forcing profile h1: h = gam dt, solve (chanfast.c:498-500) */
                 std::vector<double> rr(nz), x;
@@ -793,6 +796,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.rdy = 1.0 / r->g.dy;
     R.rdxdy = 1.0 / (r->g.dx * r->g.dy);
     r->use_spike = spike_ok;
+    r->sp_L = spike_ok ? spike_L(nz) : 0;
     for (int s = 0; s < 3; s++)
         for (int t = 0; t < 3; t++) {
             SpikeTab &ST = R.sp[s][t];
@@ -814,7 +818,8 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.lamx = r->tab + o_lx;
     R.lamy = r->tab + o_ly;
     R.fx_pairs = MAXE_LINE / nx > 0 ? MAXE_LINE / nx : 1;
-    R.fy_cols = MAXE_LINE / ny >= 8 ? 8 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
+    /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
+    R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
     for (int q = 0; q < 4; q++) {
         PoisTab &T = r->pt[q];
         T.a = r->tab + o_p[q][0]; T.c = r->tab + o_p[q][1]; T.bz = r->tab + o_p[q][2];
@@ -984,7 +989,7 @@ static cf_status enqueue_step(cf_run *r, int from)
         const int nbx = (int) ((r->P + 63) / 64);
         if (r->use_spike) {
             /* the solve with the first stage of the plane sums of u* fused in */
-            KLAUNCH("spike", k_spike_sums, dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
+            KLAUNCH("spike", pick_spike_sums(r->sp_L), dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
                     ou, ov, ow, r->part);
             KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, nbx,
                     r->lsum);
@@ -1038,7 +1043,7 @@ static cf_status enqueue_step_general_t(cf_run *r)
         if (r->g.order == 4)
             KLAUNCH("solve", k_gsolve4<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
         else if (r->use_spike)
-            KLAUNCH("solve", k_spike<T>, dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
+            KLAUNCH("solve", pick_spike<T>(r->sp_L), dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
                     ru, rv, rw, rc);
         else
             KLAUNCH("solve", k_thomas<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
@@ -1484,10 +1489,11 @@ struct cf_batch {
     int n = 0, prec = CF_FP64, order = 2, scalar = 0, device = 0;
     hipStream_t stream = nullptr;
     void *args = nullptr;                 /* device GArgs<T>[n] */
-    unsigned *vstart = nullptr;           /* device [CF_NST + CF_P2_NG][n + 1] */
+    unsigned *vstart = nullptr;           /* device [CF_NST + CF_P2_NG + CF_SP_NG][n + 1] */
     std::vector<unsigned> hv;
     int solve_kind = SOLVE_HEPT;
     std::vector<int> p2L;                 /* distinct order-2 Poisson chunk lengths, one launch each */
+    std::vector<int> spL, spW;            /* distinct spike chunk lengths (one launch each) and their waves */
     unsigned kgrid[KB_N] = {0};
     int spike_waves = 1, ncu = 256;
     size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
@@ -1513,6 +1519,17 @@ struct cf_batch {
  * solved with its own partition, bit for bit as when stepped alone; the
  * prefix table of group g is row CF_NST + g of vstart. */
 #define CF_P2_NG 20
+#define CF_SP_NG 11                        /* spike launches: rows CF_NST + CF_P2_NG + g */
+static void batch_spike_groups(cf_batch *b)
+{
+    b->spL.clear();
+    b->spW.clear();
+    for (const cf_run *r : b->runs) {
+        const auto it = std::find(b->spL.begin(), b->spL.end(), r->sp_L);
+        if (it == b->spL.end()) { b->spL.push_back(r->sp_L); b->spW.push_back(r->spike_waves); }
+        else { int &w = b->spW[it - b->spL.begin()]; w = std::max(w, r->spike_waves); }
+    }
+}
 static std::vector<int> batch_p2L(const std::vector<cf_run *> &runs)
 {
     std::vector<int> ls;
@@ -1570,13 +1587,26 @@ static cf_status batch_upload_tables(cf_batch *b)
     batch_fill_args<T>(b, A);
     const int ng = (int) b->p2L.size();
     if (ng > CF_P2_NG) { cf_set_error("too many Poisson chunk lengths in one batch"); return CF_ERR_UNSUPPORTED; }
-    b->hv.assign((size_t) (CF_NST + ng) * (b->n + 1), 0u);
-    for (int st = 0; st < CF_NST + ng; st++) {
+    batch_spike_groups(b);
+    const int nsg = (int) b->spL.size();
+    if (nsg > CF_SP_NG) { cf_set_error("too many spike chunk lengths in one batch"); return CF_ERR_UNSUPPORTED; }
+    const int nrows = CF_NST + CF_P2_NG + CF_SP_NG;
+    b->hv.assign((size_t) nrows * (b->n + 1), 0u);
+    for (int st = 0; st < nrows; st++) {
         unsigned acc = 0;
         for (int q = 0; q < b->n; q++) {
             b->hv[(size_t) st * (b->n + 1) + q] = acc;
-            const int sd = st < CF_NST ? st : ST_MODE;
-            const bool in = st < CF_NST || pois2_L(b->runs[q]->g.nz) == b->p2L[st - CF_NST];
+            int sd = st;
+            bool in = true;
+            if (st >= CF_NST + CF_P2_NG) {        /* spike group */
+                const int g = st - CF_NST - CF_P2_NG;
+                sd = ST_SOLVE;
+                in = g < nsg && b->runs[q]->sp_L == b->spL[g];
+            } else if (st >= CF_NST) {            /* Poisson group */
+                const int g = st - CF_NST;
+                sd = ST_MODE;
+                in = g < ng && pois2_L(b->runs[q]->g.nz) == b->p2L[g];
+            }
             if (in) acc += A[q].dims[sd][0] * A[q].dims[sd][1] * A[q].dims[sd][2];
         }
         b->hv[(size_t) st * (b->n + 1) + b->n] = acc;
@@ -1616,6 +1646,33 @@ static cf_status bev_end(cf_batch *b)
 }
 
 /* launch kernel slot k of the batch for substep s (or FFT direction) */
+/* the spike solve of a batch: one launch per chunk length, each over its
+ * own prefix table and as many waves per workgroup as its runs need */
+template <typename T>
+static cf_status batch_launch_spike(cf_batch *b, int k, int s, bool sums)
+{
+    cf_status e = bev_begin(b, KB_NAME[k]);
+    if (e != CF_OK) return e;
+    for (size_t g = 0; g < b->spL.size(); g++) {
+        typename BSel<T>::kb_t fn = nullptr;
+        if (sums) {
+            if constexpr (std::is_same<T, double>::value) fn = pick_batched_spike_sums(b->spL[g]);
+        } else {
+            fn = pick_batched_spike<T>(b->spL[g]);
+        }
+        if (!fn) { cf_set_error("no spike kernel for %d rows per chunk", b->spL[g]); return CF_ERR_UNSUPPORTED; }
+        const int row = CF_NST + CF_P2_NG + (int) g;
+        const unsigned tg = batch_total(b, row);
+        const unsigned grid = b->kgrid[k] ? std::min(b->kgrid[k], tg) : tg;
+        if (grid == 0) continue;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * b->spW[g]), 0, b->stream,
+                           (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
+                           (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)), b->n, s);
+        HIPCHK(hipGetLastError());
+    }
+    return bev_end(b);
+}
+
 template <typename T>
 static cf_status batch_launch(cf_batch *b, int k, int s)
 {
@@ -1632,7 +1689,7 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     case KB_PROJ: fn = S.proj; block = dim3(64, 4); break;
     case KB_SOLVE:
         if (b->solve_kind == SOLVE_HEPT) fn = kb_solve4<T>;
-        else if (b->solve_kind == SOLVE_SPIKE) { fn = kb_spike<T>; block = dim3(64 * b->spike_waves); }
+        else if (b->solve_kind == SOLVE_SPIKE) return batch_launch_spike<T>(b, k, s, false);
         else fn = kb_thomas<T>;
         break;
     case KB_PLANE: fn = kb_plane<T>; break;
@@ -1667,9 +1724,7 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
         if constexpr (std::is_same<T, double>::value) fn = kb_rhs2t;
         break;
     case KB_FSPIKE:
-        if constexpr (std::is_same<T, double>::value) fn = kb_spike_sums;
-        block = dim3(64 * b->spike_waves);
-        break;
+        return batch_launch_spike<T>(b, k, s, true);
     case KB_FLEVEL:
         if constexpr (std::is_same<T, double>::value) fn = kb_level_sum;
         break;
@@ -1891,7 +1946,7 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     /* order 2, FP64, no scalar, z-solves the spike covers: the fast path */
     b->fast = b->order == 2 && b->prec == CF_FP64 && !b->scalar;
     for (int q = 0; q < nrun && b->fast; q++)
-        b->fast = cfgs[q].nz <= CF_SPIKE_LMAX * CF_SPIKE_SMAX && cfgs[q].cflmode == 0;
+        b->fast = spike_L(cfgs[q].nz) > 0 && cfgs[q].cflmode == 0;
     for (int q = 0; q < nrun; q++) {
         cf_run *r = nullptr;
         cf_status st = create_run(&cfgs[q], !b->fast, &r);
@@ -1926,7 +1981,7 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
         b->ncu = prop.multiProcessorCount;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     if (hipMalloc(&b->args, asz * nrun * (b->fast ? 2 : 1)) != hipSuccess
-        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG) * (nrun + 1)) != hipSuccess
+        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG + CF_SP_NG) * (nrun + 1)) != hipSuccess
         || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         free_batch(b);
         cf_set_error("batch allocation failed");
@@ -2263,7 +2318,7 @@ static cf_status srv_regroup(cf_server *s, int k)
     for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     HIPCHK(hipMalloc(&b->args, asz * b->n));
-    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG) * (b->n + 1)));
+    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG + CF_SP_NG) * (b->n + 1)));
     return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
 }
 

@@ -282,7 +282,7 @@ static __device__ __forceinline__ void rhsu_body(const RunDev &R, const int s,
         if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[R.m_prev_slot[s]]);
         else h += (T) C.gamdt * f;
         h -= (T) R.liftg[((size_t) s * 4 + 0) * nz + k];
-        fo[q] = f;
+        if (s < 2) fo[q] = f;                 /* the last substep's tendency is never read */
         ro[q] = h * dzk;                      /* the solve's right-hand side is h dvol (diffuseu.c:202) */
         if (cflpart) {                        /* get_cfl, get.c:390-417 */
             const double dzph = 0.5 * (dz[k] + dz[k + 1]);
@@ -397,7 +397,7 @@ static __device__ __forceinline__ void rhsv_body(const RunDev &R, const int s,
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 1) * nz + k];
-    fo[q] = f;
+    if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
     ro[q] = h * dzk;
 }
 
@@ -425,7 +425,7 @@ static __device__ __forceinline__ void rhsw_body(const RunDev &R, const int s,
     const size_t P = (size_t) nx * ny;
     const size_t cc = (size_t) j * nx + i, q = (size_t) k * P + cc;
     if (k == nz - 1) {                    /* top wall face: f = 0 (advect.c:301), no unknown */
-        fo[q] = 0.0;
+        if (s < 2) fo[q] = 0.0;
         ro[q] = 0.0;
         return;
     }
@@ -509,7 +509,7 @@ static __device__ __forceinline__ void rhsw_body(const RunDev &R, const int s,
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 2) * nz + k];
-    fo[q] = f;
+    if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
     ro[q] = h * ((T) 0.5 * (DZ(k + 1) + DZ(k)));   /* times the face volume (diffusew.c:205) */
 }
 
@@ -572,7 +572,7 @@ static __device__ __forceinline__ void rhsc_body(const RunDev &R, const int s,
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[SC_MC0 + (s + 2) % 3]);
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 3) * nz + k];
-    fo[q] = f;
+    if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
     ro[q] = h * DZ(k);
 }
 

@@ -232,8 +232,12 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     double cflm = 0.0;
     /* previous tendencies of this point, one level ahead */
     const size_t qc = (size_t) j * nx + i;
+    /* gu == nullptr: the substep's zeta is 0 (no previous tendency read);
+     * fu == nullptr: this substep's tendency is never read (the next
+     * step's first substep has zeta = 0), so it is not stored */
+    const bool useg = gu != nullptr, keepf = fu != nullptr;
     double gnu = 0.0, gnv = 0.0, gnw = 0.0;
-    if (act) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
+    if (act && useg) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
 
     /* level k: register set Y holds level k + 2 (its pressure staged), X
      * (level k + 1, holding p(k + 3)) receives level k + 3.  Four ring slots
@@ -244,7 +248,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         const bool more = k + 2 <= k1, more3 = k + 3 <= k1;
         double pk3[2] = { X.pn[0], X.pn[1] };     /* p(k + 3) */
         const double gcu = gnu, gcv = gnv, gcw = gnw;
-        if (act && k + 1 < k1) {
+        if (act && useg && k + 1 < k1) {
             gnu = gu[(size_t) (k + 1) * P + qc];
             gnv = gv[(size_t) (k + 1) * P + qc];
             gnw = gw[(size_t) (k + 1) * P + qc];
@@ -287,7 +291,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 h += C.gamdt * f + C.zetdt * g;
                 if (k == 0) h -= R.lift[s][0];
                 if (k == nz - 1) h -= R.lift[s][1];
-                fu[q] = f;
+                if (keepf) fu[q] = f;
                 ru[q] = h * dzk;
             }
             SEQ();
@@ -310,7 +314,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 h += C.gamdt * f + C.zetdt * gcv;
                 if (k == 0) h -= R.lift[s][2];
                 if (k == nz - 1) h -= R.lift[s][3];
-                fv[q] = f;
+                if (keepf) fv[q] = f;
                 rv[q] = h * dzk;
             }
             SEQ();
@@ -332,10 +336,10 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 double f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
                 double h = w0 + C.nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
                 h += C.gamdt * f + C.zetdt * gcw;
-                fw[q] = f;
+                if (keepf) fw[q] = f;
                 rw[q] = h * dzp1h;
             } else {
-                fw[q] = 0.0;
+                if (keepf) fw[q] = 0.0;
                 rw[q] = 0.0;
             }
 #undef SEQ
@@ -1600,7 +1604,9 @@ __global__ __launch_bounds__(256) void kb_rhs2t(const GArgs<double> *a_, const u
         S.dtp = a.R.sc[ps].abdt;
         double *const *fnew = (s % 2 == 0) ? a.fa : a.fb;
         double *const *gold = (s % 2 == 0) ? a.fb : a.fa;
-        rhs2t_body(a.R, s, S, gold[0], gold[1], gold[2], a.scal, fnew[0], fnew[1], fnew[2], a.ru, a.rv, a.rw,
+        const bool useg = a.R.sc[s].zetdt != 0.0, keepf = s < 2;
+        rhs2t_body(a.R, s, S, useg ? gold[0] : nullptr, gold[1], gold[2], a.scal,
+                   keepf ? fnew[0] : nullptr, fnew[1], fnew[2], a.ru, a.rv, a.rw,
                    s == 0 ? a.cflpart : nullptr, a.dt, a.rhs_kc, lb);
     });
 }

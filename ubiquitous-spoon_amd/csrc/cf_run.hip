@@ -979,9 +979,12 @@ static cf_status enqueue_step(cf_run *r, int from)
         S.h1 = R.h1 + (size_t) (ps * 2 + 0) * (nz + 2);
         S.m = r->scal + SC_M0 + ps;
         S.dtp = R.sc[ps].abdt;
+        /* zeta = 0 (first substep): no previous tendency; the last substep's
+         * tendency is never read (the next step starts with zeta = 0) */
+        const bool useg = RK_ZET[s] != 0.0, keepf = s < 2;
         KLAUNCH("rhs", k_rhs2t, dim3(r->rhs_blocks), dim3(256), 0, R, s, S,
-                (const double *) gold[0], (const double *) gold[1], (const double *) gold[2],
-                (const double *) r->scal, fnew[0], fnew[1], fnew[2], ou, ov, ow,
+                (const double *) (useg ? gold[0] : nullptr), (const double *) gold[1], (const double *) gold[2],
+                (const double *) r->scal, keepf ? fnew[0] : nullptr, fnew[1], fnew[2], ou, ov, ow,
                 s == 0 ? r->cflpart : (double *) nullptr, dt, r->rhs_kc);
         if (s == 0)
             KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart,

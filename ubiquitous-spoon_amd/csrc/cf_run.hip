@@ -2184,6 +2184,7 @@ struct cf_server {
         int nsteps = 0, done = 0, state = 0;       /* 0 queued, 1 active, 2 done (retained), 3 released */
         bool retain = false;
         bool solo = false;                         /* stepped through its own graph, not a group batch */
+        int gkey = 0;                              /* group of a batched run (group_key, + 8 on the fast path) */
         size_t bytes = 0;
         cf_run *run = nullptr;
         double t_submit = 0, t_admit = -1, t_done = -1;
@@ -2201,6 +2202,26 @@ static double srv_now(const cf_server *s)
 }
 
 static int group_key(const cf_config &c) { return (c.order == 4 ? 4 : 0) + (c.scalar ? 2 : 0) + (c.precision == CF_FP32 ? 1 : 0); }
+/* order-2 FP64 runs without the scalar whose z-solves the spike covers
+ * are batched on the fast path */
+static bool fast_eligible(const cf_config &c)
+{
+    return c.order == 2 && c.precision == CF_FP64 && !c.scalar && c.cflmode == 0 && spike_L(c.nz) > 0;
+}
+/* relabel a fast-path run's two state sets (host pointers only), so that
+ * it joins a batch stepping from the other set; its single-run graphs,
+ * captured with the old labels, are dropped */
+static void swap_state_sets(cf_run *r)
+{
+    std::swap(r->u, r->ru);
+    std::swap(r->v, r->rv);
+    std::swap(r->w, r->rw);
+    r->cur ^= 1;
+    for (int q = 0; q < 2; q++) {
+        if (r->gexec[q]) { (void) hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+        if (r->graph[q]) { (void) hipGraphDestroy(r->graph[q]); r->graph[q] = nullptr; }
+    }
+}
 
 extern "C" cf_status cf_server_set_solo_points(cf_server *s, size_t points)
 {
@@ -2278,7 +2299,7 @@ static cf_status srv_regroup(cf_server *s, int k)
 {
     std::vector<cf_run *> runs;
     for (auto &r : s->reqs)
-        if (r.state == 1 && !r.solo && group_key(r.cfg) == k) runs.push_back(r.run);
+        if (r.state == 1 && !r.solo && r.gkey == k) runs.push_back(r.run);
     auto it = s->groups.find(k);
     if (runs.empty()) {
         if (it != s->groups.end()) { srv_drop_group(it->second); s->groups.erase(it); }
@@ -2306,6 +2327,12 @@ static cf_status srv_regroup(cf_server *s, int k)
     b->runs = runs;
     b->n = (int) runs.size();
     if (b->n > CF_BATCH_MAX) { cf_set_error("more than %d active runs in one group", CF_BATCH_MAX); return CF_ERR_UNSUPPORTED; }
+    if (k >= 8) {
+        /* fast group: every run steps from the batch's state set */
+        if (!b->fast) { b->fast = true; b->cur = runs[0]->cur; }
+        for (cf_run *r : runs)
+            if (r->cur != b->cur) swap_state_sets(r);
+    }
     b->spike_waves = 1;
     b->lds_fx = b->lds_fy = b->lds_mass = 0;
     bool spike_ok = true;
@@ -2320,7 +2347,7 @@ static cf_status srv_regroup(cf_server *s, int k)
     b->p2L = batch_p2L(runs);
     for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
-    HIPCHK(hipMalloc(&b->args, asz * b->n));
+    HIPCHK(hipMalloc(&b->args, asz * b->n * (b->fast ? 2 : 1)));
     HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG + CF_SP_NG) * (b->n + 1)));
     return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
 }
@@ -2342,7 +2369,14 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
          * stream; small ones join their group's batched launches.  The
          * choice depends on the request alone, never on the mix. */
         r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
-        if ((st = create_run(&r.cfg, !r.solo, &run)) != CF_OK) return st;
+        const bool fast = !r.solo && fast_eligible(r.cfg);
+        if ((st = create_run(&r.cfg, !(r.solo || fast), &run)) != CF_OK) return st;
+        r.gkey = group_key(r.cfg) + (fast && !run->general && run->use_spike ? 8 : 0);
+        if (fast && r.gkey < 8) {                  /* the spike cannot take it: the general batch */
+            free_run(run);
+            run = nullptr;
+            if ((st = create_run(&r.cfg, true, &run)) != CF_OK) return st;
+        }
         for (int f = 0; f < 4; f++) {
             if (r.init[f].empty()) continue;
             if ((st = cf_run_upload(run, f, r.init[f].data())) != CF_OK) { free_run(run); return st; }
@@ -2353,7 +2387,7 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         r.state = 1;
         r.t_admit = srv_now(s);
         s->used += r.bytes;
-        if (!r.solo) changed.push_back(group_key(r.cfg));
+        if (!r.solo) changed.push_back(r.gkey);
         s->queue.erase(s->queue.begin());
     }
     for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
@@ -2364,11 +2398,21 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         cf_batch *b = g.second;
         int k = s->quantum;
         for (auto &r : s->reqs)
-            if (r.state == 1 && !r.solo && group_key(r.cfg) == g.first) k = std::min(k, r.nsteps - r.done);
+            if (r.state == 1 && !r.solo && r.gkey == g.first) k = std::min(k, r.nsteps - r.done);
         for (int n = 0; n < k; n++) {
-            st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            if (b->fast) {
+                st = fbatch_enqueue_step(b, b->cur);
+                b->cur ^= 1;
+            } else {
+                st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            }
             if (st != CF_OK) return st;
         }
+        if (b->fast)
+            for (cf_run *r : b->runs) {
+                r->cur = b->cur;
+                if (k > 0) r->proj_pending = true;
+            }
         adv[g.first] = k;
     }
     std::vector<int> solo_adv(s->reqs.size(), 0);
@@ -2386,7 +2430,7 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
     for (size_t q = 0; q < s->reqs.size(); q++) {
         cf_server::Req &r = s->reqs[q];
         if (r.state != 1) continue;
-        const int k = r.solo ? solo_adv[q] : adv[group_key(r.cfg)];
+        const int k = r.solo ? solo_adv[q] : adv[r.gkey];
         r.done += k;
         if (!r.solo) r.run->steps_done += k;
         s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
@@ -2398,7 +2442,7 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
                 r.run = nullptr;
                 s->used -= r.bytes;
             }
-            if (!r.solo) retire.push_back(group_key(r.cfg));
+            if (!r.solo) retire.push_back(r.gkey);
         }
     }
     for (int k : retire) if ((st = srv_regroup(s, k)) != CF_OK) return st;

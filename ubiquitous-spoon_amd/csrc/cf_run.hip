@@ -2357,9 +2357,43 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
     if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
     HIPCHK(hipSetDevice(s->device));
     cf_status st;
+    /* 1. one quantum per group and per solo run, all in flight together on
+     * their own streams */
+    std::vector<int> stepped(s->reqs.size(), 0);
+    for (auto &g : s->groups) {
+        cf_batch *b = g.second;
+        int k = s->quantum;
+        for (auto &r : s->reqs)
+            if (r.state == 1 && !r.solo && r.gkey == g.first) k = std::min(k, r.nsteps - r.done);
+        for (int n = 0; n < k; n++) {
+            if (b->fast) {
+                st = fbatch_enqueue_step(b, b->cur);
+                b->cur ^= 1;
+            } else {
+                st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
+            }
+            if (st != CF_OK) return st;
+        }
+        if (b->fast)
+            for (cf_run *r : b->runs) {
+                r->cur = b->cur;
+                if (k > 0) r->proj_pending = true;
+            }
+        for (size_t q = 0; q < s->reqs.size(); q++)
+            if (s->reqs[q].state == 1 && !s->reqs[q].solo && s->reqs[q].gkey == g.first) stepped[q] = k;
+    }
+    for (size_t q = 0; q < s->reqs.size(); q++) {
+        cf_server::Req &r = s->reqs[q];
+        if (r.state != 1 || !r.solo) continue;
+        stepped[q] = std::min(s->quantum, r.nsteps - r.done);
+        if ((st = cf_run_step(r.run, stepped[q])) != CF_OK) return st;   /* counts its own steps */
+    }
+    /* 2. admit in arrival order while the budget allows (head-of-line: a
+     * large request waits for memory rather than being overtaken
+     * indefinitely); the host tables, allocations, uploads and projections
+     * of new runs overlap the steps in flight, and new runs step from the
+     * next tick on */
     std::vector<int> changed;
-    /* admit in arrival order while the budget allows (head-of-line: a large
-     * request waits for memory rather than being overtaken indefinitely) */
     while (!s->queue.empty()) {
         cf_server::Req &r = s->reqs[s->queue.front()];
         if (s->used + r.bytes > s->budget) break;
@@ -2390,47 +2424,16 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         if (!r.solo) changed.push_back(r.gkey);
         s->queue.erase(s->queue.begin());
     }
-    for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
-    /* one quantum per group and per solo run, all in flight together on
-     * their own streams */
-    std::map<int, int> adv;
-    for (auto &g : s->groups) {
-        cf_batch *b = g.second;
-        int k = s->quantum;
-        for (auto &r : s->reqs)
-            if (r.state == 1 && !r.solo && r.gkey == g.first) k = std::min(k, r.nsteps - r.done);
-        for (int n = 0; n < k; n++) {
-            if (b->fast) {
-                st = fbatch_enqueue_step(b, b->cur);
-                b->cur ^= 1;
-            } else {
-                st = b->prec == CF_FP32 ? batch_enqueue_step<float>(b) : batch_enqueue_step<double>(b);
-            }
-            if (st != CF_OK) return st;
-        }
-        if (b->fast)
-            for (cf_run *r : b->runs) {
-                r->cur = b->cur;
-                if (k > 0) r->proj_pending = true;
-            }
-        adv[g.first] = k;
-    }
-    std::vector<int> solo_adv(s->reqs.size(), 0);
-    for (size_t q = 0; q < s->reqs.size(); q++) {
-        cf_server::Req &r = s->reqs[q];
-        if (r.state != 1 || !r.solo) continue;
-        solo_adv[q] = std::min(s->quantum, r.nsteps - r.done);
-        if ((st = cf_run_step(r.run, solo_adv[q])) != CF_OK) return st;   /* counts its own steps */
-    }
+    /* 3. wait for the steps */
     for (auto &g : s->groups) HIPCHK(hipStreamSynchronize(g.second->stream));
-    for (auto &r : s->reqs)
-        if (r.state == 1 && r.solo) HIPCHK(hipStreamSynchronize(r.run->stream));
-    std::vector<int> retire;
+    for (size_t q = 0; q < s->reqs.size(); q++)
+        if (stepped[q] > 0 && s->reqs[q].solo) HIPCHK(hipStreamSynchronize(s->reqs[q].run->stream));
+    /* 4. account and retire */
     const double now = srv_now(s);
     for (size_t q = 0; q < s->reqs.size(); q++) {
         cf_server::Req &r = s->reqs[q];
-        if (r.state != 1) continue;
-        const int k = r.solo ? solo_adv[q] : adv[r.gkey];
+        if (r.state != 1 || stepped[q] == 0) continue;
+        const int k = stepped[q];
         r.done += k;
         if (!r.solo) r.run->steps_done += k;
         s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
@@ -2442,10 +2445,13 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
                 r.run = nullptr;
                 s->used -= r.bytes;
             }
-            if (!r.solo) retire.push_back(r.gkey);
+            if (!r.solo) changed.push_back(r.gkey);
         }
     }
-    for (int k : retire) if ((st = srv_regroup(s, k)) != CF_OK) return st;
+    /* 5. rebuild the batches whose membership changed */
+    std::sort(changed.begin(), changed.end());
+    changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
+    for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
     int na = 0, nd = 0;
     for (auto &r : s->reqs) { na += r.state == 1; nd += r.state >= 2; }
     if (n_active) *n_active = na;

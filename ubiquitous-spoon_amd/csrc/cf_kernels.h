@@ -22,6 +22,10 @@ enum {
 #ifndef CF_RHS_WAVES
 #define CF_RHS_WAVES 3          /* occupancy target (waves per SIMD) of k_rhs2t */
 #endif
+/* xy tile of the fast-path RHS (one thread per column, 256 threads; 32 x 8 measured faster
+ * than 64 x 4 and 16 x 16 at 256x193x256) */
+#define CF_RT_X 32
+#define CF_RT_Y 8
 #ifndef CF_RHS_KC
 #define CF_RHS_KC 24             /* target levels per z-chunk of k_rhs2t */
 #endif

@@ -67,15 +67,15 @@ template __global__ void k_dev_to_io<float>(const float *, double *, int, int, i
  * previous substep's projection (project2, pressure.c:120-152, and the
  * mass-flux correction u += h1 mdpdx, chanfast.c:519-522) applied on the
  * fly, so the projected velocity is never stored.  A workgroup owns a
- * 64 x 4 column tile and marches levels [k0, k1); a four-slot LDS ring
+ * 32 x 8 column tile and marches levels [k0, k1); a four-slot LDS ring
  * holds the projected u, v, w of the tile plus a one-cell halo (periodic
  * wrap in x, y; wall ghosts 2 bc - q_1 for u, v and 0 for w).  Level k + 3
  * is fetched into registers before level k is computed and level k + 2 is
  * built into the ring right after it, with one barrier per level.  Substep 0 also
  * reduces the CFL number of the projected state (get_cfl, get.c:390-417).
  * ------------------------------------------------------------------- */
-#define RT_X 64
-#define RT_Y 4
+#define RT_X CF_RT_X
+#define RT_Y CF_RT_Y
 #define RT_HX (RT_X + 2)
 #define RT_HY (RT_Y + 2)
 #define RT_PX (RT_X + 3)
@@ -106,7 +106,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     if (k0 >= nz) return;                         /* uniform over the block */
     const int k1 = min(k0 + kc, nz);
     const int tid = threadIdx.x;
-    const int lx = tid & 63, ly = tid >> 6;
+    const int lx = tid % RT_X, ly = tid / RT_X;
     const int i = i0 + lx, j = j0 + ly;
     const bool act = i < nx && j < ny;
 

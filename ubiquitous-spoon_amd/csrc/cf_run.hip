@@ -741,7 +741,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
         int nchunk = (nz + CF_RHS_KC - 1) / CF_RHS_KC;
         r->rhs_kc = (nz + nchunk - 1) / nchunk;
         nchunk = (nz + r->rhs_kc - 1) / r->rhs_kc;
-        r->rhs_blocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nchunk;
+        r->rhs_blocks = ((nx + CF_RT_X - 1) / CF_RT_X) * ((ny + CF_RT_Y - 1) / CF_RT_Y) * nchunk;
     }
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
     ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));

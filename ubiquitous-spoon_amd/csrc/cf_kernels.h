@@ -120,10 +120,12 @@ enum { ST_CELL = 0, ST_SOLVE, ST_PLANE, ST_MASS, ST_ONE, ST_FFTX, ST_FFTY, ST_MO
 /* Where a substep's projected velocity comes from: the previous
  * substep's solve output, its pressure, mass-flux profile and forcing
  * (h1, m may be null: no correction) and its projection dt. */
-struct ProjSrc {
-    const double *us, *vs, *ws, *p, *h1, *m;
+template <typename T> struct ProjSrcT {
+    const T *us, *vs, *ws, *p;               /* state of storage type T */
+    const double *h1, *m;
     double dtp;
 };
+typedef ProjSrcT<double> ProjSrc;
 
 /* z-tables of the Poisson operator for one dt (pressure.c:318-339) */
 struct PoisTab {
@@ -228,15 +230,15 @@ static __device__ __forceinline__ Blk blk_of(const GArgs<T> &a, int stage, unsig
 
 template <typename T> __global__ void k_io_to_dev(const double *io, T *dev, int nx, int ny, int nz);
 template <typename T> __global__ void k_dev_to_io(const T *dev, double *io, int nx, int ny, int nz);
-__global__ void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
-                        const double *gu, const double *gv, const double *gw, const double *scal,
-                        double *fu, double *fv, double *fw, double *ru, double *rv, double *rw,
-                        double *cflpart, double cfldt, int kc);
+template <typename T>
+__global__ void k_rhs2t(const RunDev R, const int s, const ProjSrcT<T> S, const T *gu, const T *gv, const T *gw,
+                        const double *scal, T *fu, T *fv, T *fw, T *ru, T *rv, T *rw, double *cflpart, double cfldt,
+                        int kc);
 template <typename T> __global__ void k_thomas(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> using k_spike_t = void (*)(const RunDev, const int, T *, T *, T *, T *);
 template <typename T> k_spike_t<T> pick_spike(int L);
-typedef void (*k_spike_sums_t)(const RunDev, const int, double *, double *, double *, double *);
-k_spike_sums_t pick_spike_sums(int L);
+template <typename T> using k_spike_sums_t = void (*)(const RunDev, const int, T *, T *, T *, double *);
+template <typename T> k_spike_sums_t<T> pick_spike_sums(int L);
 __global__ void k_level_sum(const double *plane, int nbx, double *lsum);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *lsum, double *scal, double *logbuf,
                                 int kind);
@@ -257,9 +259,9 @@ k_fftx_plain_t pick_fftx_plain(int nx);
 template <typename T> typename KSel<T>::ffty_t pick_ffty(int ny);
 template <typename T> typename KSel<T>::fftxi_t pick_fftx_inv(int nx);
 template <typename T> typename KSel<T>::pois2_t pick_poisson2(int lmax);
-__global__ void k_project2(const RunDev R, const double *us, const double *vs, const double *ws,
-                           const double *h1, const double *mptr, const double *p, double dtp,
-                           double *u, double *v, double *w);
+template <typename T>
+__global__ void k_project2(const RunDev R, const T *us, const T *vs, const T *ws, const double *h1, const double *mptr,
+                           const T *p, double dtp, T *u, T *v, T *w);
 template <typename T>
 __global__ void k_maxdiv_part(const RunDev R, const T *u, const T *v, const T *w, double *part, int nb);
 

@@ -84,16 +84,17 @@ template __global__ void k_dev_to_io<float>(const float *, double *, int, int, i
 #define RT_NP (RT_PX * RT_PY)      /* 469 pressure cells (one more in +x, +y) */
 
 /* lb: the tile's block index within the run (ntx nty nchunk of them) */
-static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, const ProjSrc &S,
-    const double *__restrict__ gu, const double *__restrict__ gv, const double *__restrict__ gw,
+template <typename T>
+static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, const ProjSrcT<T> &S,
+    const T *__restrict__ gu, const T *__restrict__ gv, const T *__restrict__ gw,
     const double *__restrict__ scal,
-    double *__restrict__ fu, double *__restrict__ fv, double *__restrict__ fw,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
+    T *__restrict__ fu, T *__restrict__ fv, T *__restrict__ fw,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw,
     double *__restrict__ cflpart, double cfldt, int kc, const unsigned lb)
 {
-    __shared__ double SU[4][RT_HY][RT_HX], SV[4][RT_HY][RT_HX], SW[4][RT_HY][RT_HX];
-    __shared__ double SP[2][2][RT_PY][RT_PX];     /* [buffer][p(L), p(L+1)] */
-    double *red = &SU[0][0][0];                   /* CFL reduction after the last level */
+    __shared__ __align__(16) T SU[4][RT_HY][RT_HX], SV[4][RT_HY][RT_HX], SW[4][RT_HY][RT_HX];
+    __shared__ T SP[2][2][RT_PY][RT_PX];          /* [buffer][p(L), p(L+1)] */
+    double *red = (double *) &SU[0][0][0];        /* CFL reduction after the last level (>= 256 doubles) */
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int ntx = (nx + RT_X - 1) / RT_X, nty = (ny + RT_Y - 1) / RT_Y;
@@ -110,11 +111,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     const int i = i0 + lx, j = j0 + ly;
     const bool act = i < nx && j < ny;
 
-    const double m = S.m ? *S.m : 0.0;
-    const double cx = S.dtp * R.rdx, cy = S.dtp * R.rdy;
+    const T m = S.m ? (T) *S.m : (T) 0;
+    const T cx = (T) (S.dtp * R.rdx), cy = (T) (S.dtp * R.rdy);
     const double *dz = R.dzt + CF_NGT;
     const double *rdz = R.rdz + CF_NGT, *rdzh = R.rdzh + CF_NGT;
-    const double ub = R.ubc, vb = R.vbc;
+    const T ub = (T) R.ubc, vb = (T) R.vbc;
 
     /* element e of the halo tile / pressure tile -> global column offset */
     auto colo = [&](int e, int w) -> size_t {
@@ -135,7 +136,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
      * the halo tile and, two levels ahead, p(L + 2) on the pressure tile
      * (clamped to the walls), so the pressure a level stages was read a
      * full level earlier */
-    struct RSet { double u[2], v[2], w[2], pn[2]; };
+    struct RSet { T u[2], v[2], w[2], pn[2]; };
     auto wok_of = [&](int L) { return L >= 0 && L <= nz - 2; };
     auto clampz = [&](int L) { return L < 0 ? 0 : (L >= nz ? nz - 1 : L); };
     auto fetch = [&](int L, RSet &X) {
@@ -147,14 +148,14 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
             if (e < RT_NA) {
                 X.u[t] = S.us[(size_t) Lc * P + oa[t]];
                 X.v[t] = S.vs[(size_t) Lc * P + oa[t]];
-                X.w[t] = wok ? S.ws[(size_t) L * P + oa[t]] : 0.0;
+                X.w[t] = wok ? S.ws[(size_t) L * P + oa[t]] : (T) 0;
             }
             if (e < RT_NP) X.pn[t] = S.p[(size_t) L2 * P + op[t]];
         }
     };
     /* pressure of level L into buffer (L - k0 + 1) & 1: p(clamp(L)) and,
      * where w has a face at L, p(L + 1) */
-    auto stage_p = [&](int L, const double *pa, const double *pb) {
+    auto stage_p = [&](int L, const T *pa, const T *pb) {
         const int b = (L - k0 + 1) & 1;
         const bool wok = wok_of(L);
 #pragma unroll
@@ -163,14 +164,14 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
             if (e < RT_NP) {
                 const int r = e / RT_PX, c = e - r * RT_PX;
                 SP[b][0][r][c] = pa[t];
-                SP[b][1][r][c] = wok ? pb[t] : 0.0;
+                SP[b][1][r][c] = wok ? pb[t] : (T) 0;
             }
         }
     };
-    auto pdirect = [&](int L, double *out) {
+    auto pdirect = [&](int L, T *out) {
         const int Lc = clampz(L);
 #pragma unroll
-        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? S.p[(size_t) Lc * P + op[t]] : 0.0;
+        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? S.p[(size_t) Lc * P + op[t]] : (T) 0;
     };
     /* projected u, v, w of level L into ring slot (L - k0 + 1) & 3 */
     auto build = [&](int L, const RSet &X) {
@@ -178,21 +179,21 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
         const bool wok = wok_of(L);
         const bool ghost = L < 0 || L >= nz;
-        const double hm = S.h1 ? S.h1[Lc + 1] * m : 0.0;
-        const double cz = wok ? S.dtp * rdzh[L] : 0.0;
+        const T hm = S.h1 ? (T) S.h1[Lc + 1] * m : (T) 0;
+        const T cz = wok ? (T) (S.dtp * rdzh[L]) : (T) 0;
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NA) {
                 const int r = e / RT_HX, c = e - r * RT_HX;
-                const double p0 = SP[b][0][r][c];
-                const double uc = X.u[t] + hm;
-                double up = uc - (SP[b][0][r][c + 1] - p0) * cx;
-                double vp = X.v[t] - (SP[b][0][r + 1][c] - p0) * cy;
-                if (ghost) { up = 2.0 * ub - up; vp = 2.0 * vb - vp; }
+                const T p0 = SP[b][0][r][c];
+                const T uc = X.u[t] + hm;
+                T up = uc - (SP[b][0][r][c + 1] - p0) * cx;
+                T vp = X.v[t] - (SP[b][0][r + 1][c] - p0) * cy;
+                if (ghost) { up = (T) 2.0 * ub - up; vp = (T) 2.0 * vb - vp; }
                 SU[sl][r][c] = up;
                 SV[sl][r][c] = vp;
-                SW[sl][r][c] = wok ? X.w[t] - (SP[b][1][r][c] - p0) * cz : 0.0;
+                SW[sl][r][c] = wok ? X.w[t] - (SP[b][1][r][c] - p0) * cz : (T) 0;
             }
         }
     };
@@ -201,7 +202,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
      * pressure of k0 + 2 staged, set A holding level k0 + 1 (p(k0 + 3))
      * and set B level k0 + 2 (p(k0 + 4)) */
     RSet A, B;
-    double carry[2], pd0[2], pd1[2];
+    T carry[2], pd0[2], pd1[2];
     fetch(k0 - 1, A);
     fetch(k0, B);
     pdirect(k0 - 1, pd0);
@@ -223,11 +224,12 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     __syncthreads();
 
     const SubCoef C = R.sc[s];
-    const double nu = R.nu;
-    const double dx = R.g.dx, dy = R.g.dy;
-    const double rdx = R.rdx, rdy = R.rdy, rdxdy = R.rdxdy;
-    const double rdx2 = rdx * rdx, rdy2 = rdy * rdy;
-    const double gmprev = scal[R.m_prev_slot[s]];
+    const T nu = (T) R.nu;
+    const T dx = (T) R.g.dx, dy = (T) R.g.dy;
+    const T rdx = (T) R.rdx, rdy = (T) R.rdy, rdxdy = (T) R.rdxdy;
+    const T rdx2 = rdx * rdx, rdy2 = rdy * rdy;
+    const T gmprev = (T) scal[R.m_prev_slot[s]];
+    const T nuadt = (T) C.nuadt, gamdt = (T) C.gamdt, zetdt = (T) C.zetdt;
     const int r0 = ly + 1, c0 = lx + 1;
     double cflm = 0.0;
     /* previous tendencies of this point, one level ahead */
@@ -236,7 +238,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
      * fu == nullptr: this substep's tendency is never read (the next
      * step's first substep has zeta = 0), so it is not stored */
     const bool useg = gu != nullptr, keepf = fu != nullptr;
-    double gnu = 0.0, gnv = 0.0, gnw = 0.0;
+    T gnu = 0, gnv = 0, gnw = 0;
     if (act && useg) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
 
     /* level k: register set Y holds level k + 2 (its pressure staged), X
@@ -246,8 +248,8 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
      * barrier per level. */
     auto level = [&](const int k, RSet &X, RSet &Y) {
         const bool more = k + 2 <= k1, more3 = k + 3 <= k1;
-        double pk3[2] = { X.pn[0], X.pn[1] };     /* p(k + 3) */
-        const double gcu = gnu, gcv = gnv, gcw = gnw;
+        T pk3[2] = { X.pn[0], X.pn[1] };          /* p(k + 3) */
+        const T gcu = gnu, gcv = gnv, gcw = gnw;
         if (act && useg && k + 1 < k1) {
             gnu = gu[(size_t) (k + 1) * P + qc];
             gnv = gv[(size_t) (k + 1) * P + qc];
@@ -257,9 +259,9 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         const int sm = (k - k0) & 3, s0 = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
         if (act) {
             const size_t q = (size_t) k * P + (size_t) j * nx + i;
-            const double dzk = dz[k], dzkp = dz[k + 1];
-            const double rzk = rdz[k], rzkp = rdz[k + 1];
-            const double rzp = rdzh[k], rzm = rdzh[k - 1];
+            const T dzk = (T) dz[k], dzkp = (T) dz[k + 1];
+            const T rzk = (T) rdz[k], rzkp = (T) rdz[k + 1];
+            const T rzp = (T) rdzh[k], rzm = (T) rdzh[k - 1];
 #define LU(sl, dr, dc) SU[sl][r0 + (dr)][c0 + (dc)]
 #define LV(sl, dr, dc) SV[sl][r0 + (dr)][c0 + (dc)]
 #define LW(sl, dr, dc) SW[sl][r0 + (dr)][c0 + (dc)]
@@ -272,75 +274,75 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 if (cflm < (c = fabs(LW(s0, 0, 0)) * cfldt * rzp)) cflm = c;
             }
             {   /* u */
-                const double u0 = LU(s0, 0, 0), uip = LU(s0, 0, 1), uim = LU(s0, 0, -1);
-                const double ujp = LU(s0, 1, 0), ujm = LU(s0, -1, 0), ukp = LU(sp, 0, 0), ukm = LU(sm, 0, 0);
-                const double v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vjm = LV(s0, -1, 0), vipjm = LV(s0, -1, 1);
-                const double w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wkm = LW(sm, 0, 0), wipkm = LW(sm, 0, 1);
-                double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * uip);
-                double mxm = 0.5 * (dy * dzk * uim + dy * dzk * u0);
-                double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vip);
-                double mym = 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
-                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wip);
-                double mzm = 0.5 * (dx * dy * wkm + dx * dy * wipkm);
-                double adv = -((mxp * 0.5 * (u0 + uip) - mxm * 0.5 * (u0 + uim))
-                             + (myp * 0.5 * (u0 + ujp) - mym * 0.5 * (u0 + ujm))
-                             + (mzp * 0.5 * (u0 + ukp) - mzm * 0.5 * (u0 + ukm))) * (rdxdy * rzk);
-                double f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
-                double h = u0 + C.nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
-                double g = gcu + gmprev;
-                h += C.gamdt * f + C.zetdt * g;
-                if (k == 0) h -= R.lift[s][0];
-                if (k == nz - 1) h -= R.lift[s][1];
+                const T u0 = LU(s0, 0, 0), uip = LU(s0, 0, 1), uim = LU(s0, 0, -1);
+                const T ujp = LU(s0, 1, 0), ujm = LU(s0, -1, 0), ukp = LU(sp, 0, 0), ukm = LU(sm, 0, 0);
+                const T v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vjm = LV(s0, -1, 0), vipjm = LV(s0, -1, 1);
+                const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wkm = LW(sm, 0, 0), wipkm = LW(sm, 0, 1);
+                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzk * uip);
+                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzk * u0);
+                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vip);
+                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
+                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wip);
+                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * wipkm);
+                T adv = -((mxp * (T) 0.5 * (u0 + uip) - mxm * (T) 0.5 * (u0 + uim))
+                             + (myp * (T) 0.5 * (u0 + ujp) - mym * (T) 0.5 * (u0 + ujm))
+                             + (mzp * (T) 0.5 * (u0 + ukp) - mzm * (T) 0.5 * (u0 + ukm))) * (rdxdy * rzk);
+                T f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
+                T h = u0 + nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
+                T g = gcu + gmprev;
+                h += gamdt * f + zetdt * g;
+                if (k == 0) h -= (T) R.lift[s][0];
+                if (k == nz - 1) h -= (T) R.lift[s][1];
                 if (keepf) fu[q] = f;
                 ru[q] = h * dzk;
             }
             SEQ();
             {   /* v */
-                const double u0 = LU(s0, 0, 0), ujp = LU(s0, 1, 0), uim = LU(s0, 0, -1), uimjp = LU(s0, 1, -1);
-                const double v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vim = LV(s0, 0, -1);
-                const double vjp = LV(s0, 1, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vkm = LV(sm, 0, 0);
-                const double w0 = LW(s0, 0, 0), wjp = LW(s0, 1, 0), wkm = LW(sm, 0, 0), wjpkm = LW(sm, 1, 0);
-                double mxp = 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
-                double mxm = 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
-                double myp = 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
-                double mym = 0.5 * (dx * dzk * vjm + dx * dzk * v0);
-                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wjp);
-                double mzm = 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
-                double adv = -((mxp * 0.5 * (v0 + vip) - mxm * 0.5 * (v0 + vim))
-                             + (myp * 0.5 * (v0 + vjp) - mym * 0.5 * (v0 + vjm))
-                             + (mzp * 0.5 * (v0 + vkp) - mzm * 0.5 * (v0 + vkm))) * (rdxdy * rzk);
-                double f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
-                double h = v0 + C.nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
-                h += C.gamdt * f + C.zetdt * gcv;
-                if (k == 0) h -= R.lift[s][2];
-                if (k == nz - 1) h -= R.lift[s][3];
+                const T u0 = LU(s0, 0, 0), ujp = LU(s0, 1, 0), uim = LU(s0, 0, -1), uimjp = LU(s0, 1, -1);
+                const T v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vim = LV(s0, 0, -1);
+                const T vjp = LV(s0, 1, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vkm = LV(sm, 0, 0);
+                const T w0 = LW(s0, 0, 0), wjp = LW(s0, 1, 0), wkm = LW(sm, 0, 0), wjpkm = LW(sm, 1, 0);
+                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
+                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
+                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
+                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * v0);
+                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wjp);
+                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
+                T adv = -((mxp * (T) 0.5 * (v0 + vip) - mxm * (T) 0.5 * (v0 + vim))
+                             + (myp * (T) 0.5 * (v0 + vjp) - mym * (T) 0.5 * (v0 + vjm))
+                             + (mzp * (T) 0.5 * (v0 + vkp) - mzm * (T) 0.5 * (v0 + vkm))) * (rdxdy * rzk);
+                T f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
+                T h = v0 + nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
+                h += gamdt * f + zetdt * gcv;
+                if (k == 0) h -= (T) R.lift[s][2];
+                if (k == nz - 1) h -= (T) R.lift[s][3];
                 if (keepf) fv[q] = f;
                 rv[q] = h * dzk;
             }
             SEQ();
             if (k < nz - 1) {   /* w on the interior faces */
-                const double u0 = LU(s0, 0, 0), uim = LU(s0, 0, -1), ukp = LU(sp, 0, 0), uimkp = LU(sp, 0, -1);
-                const double v0 = LV(s0, 0, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vjmkp = LV(sp, -1, 0);
-                const double w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wim = LW(s0, 0, -1);
-                const double wjp = LW(s0, 1, 0), wjm = LW(s0, -1, 0), wkp = LW(sp, 0, 0), wkm = LW(sm, 0, 0);
-                const double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
-                double mxp = 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
-                double mxm = 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
-                double myp = 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
-                double mym = 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
-                double mzp = 0.5 * (dx * dy * w0 + dx * dy * wkp);
-                double mzm = 0.5 * (dx * dy * wkm + dx * dy * w0);
-                double adv = -((mxp * 0.5 * (w0 + wip) - mxm * 0.5 * (w0 + wim))
-                             + (myp * 0.5 * (w0 + wjp) - mym * 0.5 * (w0 + wjm))
-                             + (mzp * 0.5 * (w0 + wkp) - mzm * 0.5 * (w0 + wkm))) * (rdxdy * rzp);
-                double f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
-                double h = w0 + C.nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
-                h += C.gamdt * f + C.zetdt * gcw;
+                const T u0 = LU(s0, 0, 0), uim = LU(s0, 0, -1), ukp = LU(sp, 0, 0), uimkp = LU(sp, 0, -1);
+                const T v0 = LV(s0, 0, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vjmkp = LV(sp, -1, 0);
+                const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wim = LW(s0, 0, -1);
+                const T wjp = LW(s0, 1, 0), wjm = LW(s0, -1, 0), wkp = LW(sp, 0, 0), wkm = LW(sm, 0, 0);
+                const T dzp1h = (T) 0.5 * (dz[k + 1] + dz[k]);
+                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
+                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
+                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
+                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
+                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wkp);
+                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * w0);
+                T adv = -((mxp * (T) 0.5 * (w0 + wip) - mxm * (T) 0.5 * (w0 + wim))
+                             + (myp * (T) 0.5 * (w0 + wjp) - mym * (T) 0.5 * (w0 + wjm))
+                             + (mzp * (T) 0.5 * (w0 + wkp) - mzm * (T) 0.5 * (w0 + wkm))) * (rdxdy * rzp);
+                T f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
+                T h = w0 + nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
+                h += gamdt * f + zetdt * gcw;
                 if (keepf) fw[q] = f;
                 rw[q] = h * dzp1h;
             } else {
-                if (keepf) fw[q] = 0.0;
-                rw[q] = 0.0;
+                if (keepf) fw[q] = (T) 0;
+                rw[q] = (T) 0;
             }
 #undef SEQ
 #undef LU
@@ -366,15 +368,20 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVES, 8))) void k_rhs2t(const RunDev R, const int s, const ProjSrc S,
-    const double *__restrict__ gu, const double *__restrict__ gv, const double *__restrict__ gw,
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_RHS_WAVES, 8))) void k_rhs2t(const RunDev R, const int s, const ProjSrcT<T> S,
+    const T *__restrict__ gu, const T *__restrict__ gv, const T *__restrict__ gw,
     const double *__restrict__ scal,
-    double *__restrict__ fu, double *__restrict__ fv, double *__restrict__ fw,
-    double *__restrict__ ru, double *__restrict__ rv, double *__restrict__ rw,
+    T *__restrict__ fu, T *__restrict__ fv, T *__restrict__ fw,
+    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw,
     double *__restrict__ cflpart, double cfldt, int kc)
 {
-    rhs2t_body(R, s, S, gu, gv, gw, scal, fu, fv, fw, ru, rv, rw, cflpart, cfldt, kc, xcd_block(blockIdx.x, gridDim.x));
+    rhs2t_body<T>(R, s, S, gu, gv, gw, scal, fu, fv, fw, ru, rv, rw, cflpart, cfldt, kc, xcd_block(blockIdx.x, gridDim.x));
 }
+template __global__ void k_rhs2t<double>(const RunDev, const int, const ProjSrcT<double>, const double *, const double *,
+    const double *, const double *, double *, double *, double *, double *, double *, double *, double *, double, int);
+template __global__ void k_rhs2t<float>(const RunDev, const int, const ProjSrcT<float>, const float *, const float *,
+    const float *, const double *, float *, float *, float *, float *, float *, float *, double *, double, int);
 
 /* ---------------------------------------------------------------------
  * Batched constant-coefficient Thomas solve, one thread per (column,
@@ -559,12 +566,13 @@ __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
     spike_body<T, LM>(R, s, ru, rv, rw, rc, blk_native());
 }
 
-/* the fast path's solve: k_spike with the plane sums of u* fused in */
-template <int LM>
-__global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s, double *__restrict__ ru,
-    double *__restrict__ rv, double *__restrict__ rw, double *__restrict__ plane)
+/* the fast path's solve: k_spike with the plane sums of u* fused in
+ * (accumulated in FP64 whatever the storage) */
+template <typename T, int LM>
+__global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s, T *__restrict__ ru,
+    T *__restrict__ rv, T *__restrict__ rw, double *__restrict__ plane)
 {
-    spike_body<double, LM>(R, s, ru, rv, rw, nullptr, blk_native(), plane);
+    spike_body<T, LM>(R, s, ru, rv, rw, nullptr, blk_native(), plane);
 }
 
 template <typename T>
@@ -580,15 +588,18 @@ k_spike_t<T> pick_spike(int L)
 template k_spike_t<double> pick_spike<double>(int);
 template k_spike_t<float> pick_spike<float>(int);
 
-k_spike_sums_t pick_spike_sums(int L)
+template <typename T>
+k_spike_sums_t<T> pick_spike_sums(int L)
 {
     switch (L) {
-#define X_(N) case N: return k_spike_sums<N>;
+#define X_(N) case N: return k_spike_sums<T, N>;
     CF_SPIKE_LSET(X_)
 #undef X_
     default: return nullptr;
     }
 }
+template k_spike_sums_t<double> pick_spike_sums<double>(int);
+template k_spike_sums_t<float> pick_spike_sums<float>(int);
 
 /* second stage of the plane sums: level blockIdx.x, nbx partials in a
  * fixed tree */
@@ -1398,10 +1409,11 @@ template KSel<float>::pois2_t pick_poisson2<float>(int);
  * top-wall face is left at its Dirichlet value 0 (the reference subtracts
  * the Neumann-ghost difference there, which is round-off).
  * ------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_project2(const RunDev R, const double *__restrict__ us,
-    const double *__restrict__ vs, const double *__restrict__ ws, const double *__restrict__ h1,
-    const double *__restrict__ mptr, const double *__restrict__ p, double dtp,
-    double *__restrict__ u, double *__restrict__ v, double *__restrict__ w)
+template <typename T>
+__global__ __launch_bounds__(256) void k_project2(const RunDev R, const T *__restrict__ us,
+    const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
+    const double *__restrict__ mptr, const T *__restrict__ p, double dtp,
+    T *__restrict__ u, T *__restrict__ v, T *__restrict__ w)
 {
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const int i = blockIdx.x * 64 + threadIdx.x;
@@ -1412,17 +1424,21 @@ __global__ __launch_bounds__(256) void k_project2(const RunDev R, const double *
     const int ip = i + 1 < nx ? i + 1 : 0, jp = j + 1 < ny ? j + 1 : 0;
     const size_t q = (size_t) k * P + (size_t) j * nx + i;
     const double *rdzh = R.rdzh + CF_NGT;
-    const double m = mptr ? *mptr : 0.0;
-    const double p0 = p[q];
-    double uc = us[q] + (h1 ? h1[k + 1] * m : 0.0);
-    u[q] = uc - (p[(size_t) k * P + (size_t) j * nx + ip] - p0) * (dtp * R.rdx);
-    v[q] = vs[q] - (p[(size_t) k * P + (size_t) jp * nx + i] - p0) * (dtp * R.rdy);
+    const T m = mptr ? (T) *mptr : (T) 0;
+    const T p0 = p[q];
+    T uc = us[q] + (h1 ? (T) h1[k + 1] * m : (T) 0);
+    u[q] = uc - (p[(size_t) k * P + (size_t) j * nx + ip] - p0) * (T) (dtp * R.rdx);
+    v[q] = vs[q] - (p[(size_t) k * P + (size_t) jp * nx + i] - p0) * (T) (dtp * R.rdy);
     if (k < nz - 1) {
-        w[q] = ws[q] - (p[q + P] - p0) * (dtp * rdzh[k]);
+        w[q] = ws[q] - (p[q + P] - p0) * (T) (dtp * rdzh[k]);
     } else {
-        w[q] = 0.0;
+        w[q] = (T) 0;
     }
 }
+template __global__ void k_project2<double>(const RunDev, const double *, const double *, const double *, const double *,
+    const double *, const double *, double, double *, double *, double *);
+template __global__ void k_project2<float>(const RunDev, const float *, const float *, const float *, const double *,
+    const double *, const float *, double, float *, float *, float *);
 
 /* max |div| over interior cells (case1 check, cases.c:473-486), the
  * divergence of the order in force, evaluated in FP64 whatever the

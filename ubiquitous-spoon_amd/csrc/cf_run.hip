@@ -554,7 +554,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     const int order = cfg->order;
     r->prec = cfg->precision;
     r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
-    r->general = force_general || order == 4 || cfg->scalar || cfg->precision == CF_FP32;
+    r->general = force_general || order == 4 || cfg->scalar;
     r->nfield = cfg->scalar ? 4 : 3;
     Clos clo;
     if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
@@ -941,11 +941,14 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
 
 /* the projection parameters in force for the current state: the last
  * substep's h1, forcing slot and (alpha + beta) dt */
-static ProjSrc current_projection(cf_run *r)
+template <typename T>
+static ProjSrcT<T> current_projection(cf_run *r)
 {
-    ProjSrc S;
-    state_set(r, r->cur, (double **) &S.us, (double **) &S.vs, (double **) &S.ws);
-    S.p = r->p;
+    ProjSrcT<T> S;
+    double *su, *sv, *sw;
+    state_set(r, r->cur, &su, &sv, &sw);
+    S.us = tv<T>(su); S.vs = tv<T>(sv); S.ws = tv<T>(sw);
+    S.p = tv<T>(r->p);
     S.h1 = r->rd.h1 + (size_t) (2 * 2 + 0) * (r->g.nz + 2);
     S.m = r->scal + SC_M0 + 2;
     S.dtp = r->rd.sc[2].abdt;
@@ -953,16 +956,25 @@ static ProjSrc current_projection(cf_run *r)
 }
 
 /* materialise the projected velocity of the current state into (u, v, w) */
-static cf_status enqueue_materialise(cf_run *r, double *u, double *v, double *w)
+template <typename T>
+static cf_status enqueue_materialise_t(cf_run *r, double *u, double *v, double *w)
 {
-    ProjSrc S = current_projection(r);
-    KLAUNCH("materialise", k_project2, cell_grid(r), dim3(64, 4), 0, r->rd, S.us, S.vs, S.ws, S.h1, S.m,
-            S.p, S.dtp, u, v, w);
+    ProjSrcT<T> S = current_projection<T>(r);
+    KLAUNCH("materialise", k_project2<T>, cell_grid(r), dim3(64, 4), 0, r->rd, S.us, S.vs, S.ws, S.h1, S.m,
+            S.p, S.dtp, tv<T>(u), tv<T>(v), tv<T>(w));
     return CF_OK;
 }
 
+/* the projected velocity of the fast path's current state into (u, v, w),
+ * buffers of the run's storage type */
+static cf_status enqueue_materialise(cf_run *r, double *u, double *v, double *w)
+{
+    return r->prec == CF_FP32 ? enqueue_materialise_t<float>(r, u, v, w) : enqueue_materialise_t<double>(r, u, v, w);
+}
+
 /* one RK3 step from state set `from` to the other set */
-static cf_status enqueue_step(cf_run *r, int from)
+template <typename T>
+static cf_status enqueue_step_t(cf_run *r, int from)
 {
     const RunDev &R = r->rd;
     const int nz = r->g.nz;
@@ -970,21 +982,22 @@ static cf_status enqueue_step(cf_run *r, int from)
     for (int s = 0; s < 3; s++) {
         double **fnew = (s % 2 == 0) ? r->fa : r->fb;
         double **gold = (s % 2 == 0) ? r->fb : r->fa;
-        double *iu, *iv, *iw, *ou, *ov, *ow;
-        state_set(r, (from + s) % 2, &iu, &iv, &iw);
-        state_set(r, (from + s + 1) % 2, &ou, &ov, &ow);
+        double *iu_, *iv_, *iw_, *ou_, *ov_, *ow_;
+        state_set(r, (from + s) % 2, &iu_, &iv_, &iw_);
+        state_set(r, (from + s + 1) % 2, &ou_, &ov_, &ow_);
+        T *iu = tv<T>(iu_), *iv = tv<T>(iv_), *iw = tv<T>(iw_), *ou = tv<T>(ou_), *ov = tv<T>(ov_), *ow = tv<T>(ow_);
         const int ps = (s + 2) % 3;
-        ProjSrc S;
-        S.us = iu; S.vs = iv; S.ws = iw; S.p = r->p;
+        ProjSrcT<T> S;
+        S.us = iu; S.vs = iv; S.ws = iw; S.p = tv<T>(r->p);
         S.h1 = R.h1 + (size_t) (ps * 2 + 0) * (nz + 2);
         S.m = r->scal + SC_M0 + ps;
         S.dtp = R.sc[ps].abdt;
         /* zeta = 0 (first substep): no previous tendency; the last substep's
          * tendency is never read (the next step starts with zeta = 0) */
         const bool useg = RK_ZET[s] != 0.0, keepf = s < 2;
-        KLAUNCH("rhs", k_rhs2t, dim3(r->rhs_blocks), dim3(256), 0, R, s, S,
-                (const double *) (useg ? gold[0] : nullptr), (const double *) gold[1], (const double *) gold[2],
-                (const double *) r->scal, keepf ? fnew[0] : nullptr, fnew[1], fnew[2], ou, ov, ow,
+        KLAUNCH("rhs", k_rhs2t<T>, dim3(r->rhs_blocks), dim3(256), 0, R, s, S,
+                (const T *) (useg ? tv<T>(gold[0]) : nullptr), (const T *) tv<T>(gold[1]), (const T *) tv<T>(gold[2]),
+                (const double *) r->scal, keepf ? tv<T>(fnew[0]) : nullptr, tv<T>(fnew[1]), tv<T>(fnew[2]), ou, ov, ow,
                 s == 0 ? r->cflpart : (double *) nullptr, dt, r->rhs_kc);
         if (s == 0)
             KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart,
@@ -992,26 +1005,31 @@ static cf_status enqueue_step(cf_run *r, int from)
         const int nbx = (int) ((r->P + 63) / 64);
         if (r->use_spike) {
             /* the solve with the first stage of the plane sums of u* fused in */
-            KLAUNCH("spike", pick_spike_sums(r->sp_L), dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
+            KLAUNCH("spike", pick_spike_sums<T>(r->sp_L), dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
                     ou, ov, ow, r->part);
             KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, nbx,
                     r->lsum);
         } else {
-            KLAUNCH("thomas", k_thomas<double>, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
-                    (double *) nullptr);
-            KLAUNCH("plane_sum", k_gplane<double>, dim3(r->nb_plane, nz, 1), dim3(256), 0, R, (const double *) ou,
-                    (const double *) nullptr, r->part, r->nb_plane);
+            KLAUNCH("thomas", k_thomas<T>, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
+                    (T *) nullptr);
+            KLAUNCH("plane_sum", k_gplane<T>, dim3(r->nb_plane, nz, 1), dim3(256), 0, R, (const T *) ou,
+                    (const T *) nullptr, r->part, r->nb_plane);
             KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, r->nb_plane,
                     r->lsum);
         }
         KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
                 (const double *) r->lsum, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
-        cf_status st = enqueue_pressure<double>(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
+        cf_status st = enqueue_pressure<T>(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
     }
     KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
     return CF_OK;
+}
+
+static cf_status enqueue_step(cf_run *r, int from)
+{
+    return r->prec == CF_FP32 ? enqueue_step_t<float>(r, from) : enqueue_step_t<double>(r, from);
 }
 
 /* one RK3 step of the general path (order 4, the scalar, or FP32 storage):
@@ -1108,10 +1126,10 @@ static cf_status reset_projection(cf_run *r)
     state_set(r, r->cur, &su, &sv, &sw);
     cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
-    HIPCHK(hipMemcpyAsync(su, r->fa[0], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
-    HIPCHK(hipMemcpyAsync(sv, r->fa[1], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
-    HIPCHK(hipMemcpyAsync(sw, r->fa[2], sizeof(double) * r->N, hipMemcpyDeviceToDevice, r->stream));
-    HIPCHK(hipMemsetAsync(r->p, 0, sizeof(double) * r->N, r->stream));
+    HIPCHK(hipMemcpyAsync(su, r->fa[0], r->esz * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(sv, r->fa[1], r->esz * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(sw, r->fa[2], r->esz * r->N, hipMemcpyDeviceToDevice, r->stream));
+    HIPCHK(hipMemsetAsync(r->p, 0, r->esz * r->N, r->stream));
     HIPCHK(hipMemsetAsync(r->scal + SC_M0, 0, sizeof(double) * 6, r->stream));
     r->proj_pending = false;
     return CF_OK;
@@ -1205,6 +1223,24 @@ static cf_status set_adhoc_poisson(cf_run *r, double dt)
     return CF_OK;
 }
 
+/* the initial projection on the fast path: materialise, divergence ->
+ * Poisson(dt) -> projection into the current state set */
+template <typename T>
+static cf_status project_fast(cf_run *r, double dt)
+{
+    cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+    if (st != CF_OK) return st;
+    T *a = tv<T>(r->fa[0]), *b = tv<T>(r->fa[1]), *c = tv<T>(r->fa[2]);
+    st = enqueue_pressure<T>(r, a, b, c, nullptr, nullptr, 3);
+    if (st != CF_OK) return st;
+    double *su, *sv, *sw;
+    state_set(r, r->cur, &su, &sv, &sw);
+    KLAUNCH("project", k_project2<T>, cell_grid(r), dim3(64, 4), 0, r->rd, (const T *) a, (const T *) b,
+            (const T *) c, (const double *) nullptr, (const double *) nullptr, (const T *) tv<T>(r->p), dt,
+            tv<T>(su), tv<T>(sv), tv<T>(sw));
+    return CF_OK;
+}
+
 extern "C" cf_status cf_run_project(cf_run *r, double dt)
 {
     CHECK_RUN(r);
@@ -1217,16 +1253,9 @@ extern "C" cf_status cf_run_project(cf_run *r, double dt)
         HIPCHK(hipStreamSynchronize(r->stream));
         return collect_profile(r);
     }
-    st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+    st = r->prec == CF_FP32 ? project_fast<float>(r, dt) : project_fast<double>(r, dt);
     if (st != CF_OK) return st;
-    st = enqueue_pressure<double>(r, r->fa[0], r->fa[1], r->fa[2], nullptr, nullptr, 3);
-    if (st != CF_OK) return st;
-    double *su, *sv, *sw;
-    state_set(r, r->cur, &su, &sv, &sw);
-    KLAUNCH("project", k_project2, cell_grid(r), dim3(64, 4), 0, r->rd, (const double *) r->fa[0],
-            (const double *) r->fa[1], (const double *) r->fa[2], (const double *) nullptr,
-            (const double *) nullptr, (const double *) r->p, dt, su, sv, sw);
-    HIPCHK(hipMemsetAsync(r->p, 0, sizeof(double) * r->N, r->stream));
+    HIPCHK(hipMemsetAsync(r->p, 0, r->esz * r->N, r->stream));
     HIPCHK(hipMemsetAsync(r->scal + SC_M0, 0, sizeof(double) * 6, r->stream));
     r->proj_pending = false;
     HIPCHK(hipStreamSynchronize(r->stream));

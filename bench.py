@@ -97,7 +97,7 @@ def time_form(name, cfg, steps, warmup, names="uvw", bytes_per_pt=None):
         run.step(2)
         run.sync()
         kt = {}
-        for k in ("rhs", "cfl_finish", "solve", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
+        for k in ("rhs", "cfl_finish", "solve", "spike", "plane_sum", "mass", "div_fftx", "ffty_fwd", "poisson",
                   "ffty_inv", "fftx_inv", "project"):
             ms, n = run.kernel_time(k)
             if n:

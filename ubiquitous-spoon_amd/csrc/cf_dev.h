@@ -37,6 +37,20 @@ struct Thomas {
 };
 
 /* real <-> complex type pairs of the two storage precisions */
+/* A pointer into global memory, typed so.  Pointers read out of memory
+ * (a batch's argument table, a run's parameter block) are generic, and
+ * accesses through them compile to flat instructions, which also hold the
+ * LDS wait counter and never take the scalar path; held as gp<T> every use
+ * compiles to global (or, when uniform and read-only, scalar) loads. */
+template <typename P>
+using gp = __attribute__((address_space(1))) P *;
+/* read-only tables (written by the host before the launch, never by a
+ * kernel): the constant address space lets uniform reads compile to scalar
+ * loads even inside a persistent loop whose stores the compiler cannot
+ * prove disjoint from them */
+template <typename P>
+using cp = __attribute__((address_space(4))) const P *;
+
 template <typename T> struct Cplx;
 template <> struct Cplx<double> { typedef double2 type; };
 template <> struct Cplx<float> { typedef float2 type; };
@@ -50,6 +64,18 @@ template <typename C> static __device__ __forceinline__ C mkc(typename Real<C>::
     c.x = x;
     c.y = y;
     return c;
+}
+/* a complex element as clang's native two-lane vector: loads and stores of
+ * it go through address-space-typed pointers (gp, cp), which the HIP vector
+ * types' constructors and assignments (generic references) cannot */
+template <typename C> using cv = typename Real<C>::type __attribute__((ext_vector_type(2)));
+template <typename C> static __device__ __forceinline__ C cx_of(const cv<C> v) { return mkc<C>(v.x, v.y); }
+template <typename C> static __device__ __forceinline__ cv<C> cv_of(const C c)
+{
+    cv<C> v;
+    v.x = c.x;
+    v.y = c.y;
+    return v;
 }
 template <typename C> static __device__ __forceinline__ C cadd(C a, C b) { return mkc<C>(a.x + b.x, a.y + b.y); }
 template <typename C> static __device__ __forceinline__ C csub(C a, C b) { return mkc<C>(a.x - b.x, a.y - b.y); }
@@ -169,7 +195,7 @@ static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlin
 }
 
 template <int MAXE, typename C>
-static __device__ void lds_fft(C *buf, int n, int ld, int nlines, const int *rad, int nrad,
+static __device__ void lds_fft(C *buf, int n, int ld, int nlines, cp<int> rad, int nrad,
                                const C *__restrict__ tw, double sg)
 {
     int p = 1;
@@ -267,7 +293,7 @@ static __device__ __forceinline__ void lds_fft_ct(C *buf, int ld, int nlines,
 
 /* N > 0: compile-time plan; N == 0: runtime plan (any 2^a 3^b 5^c) */
 template <int N, int MAXE, typename C>
-static __device__ __forceinline__ void fft_lines(C *buf, int n, int ld, int nlines, const int *rad,
+static __device__ __forceinline__ void fft_lines(C *buf, int n, int ld, int nlines, cp<int> rad,
                                                  int nrad, const C *__restrict__ tw, double sg)
 {
     if constexpr (N > 0) lds_fft_ct<N, MAXE>(buf, ld, nlines, tw, sg);

@@ -976,7 +976,7 @@ __global__ __launch_bounds__(256) void kb_proj(const GArgs<T> *__restrict__ A, c
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_solve4(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_solve4(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(256) void kb_solve4(const GArgs<T> *a_, const unsig
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_plane(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_plane(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
@@ -995,7 +995,7 @@ __global__ __launch_bounds__(256) void kb_plane(const GArgs<T> *a_, const unsign
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_mass(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_mass(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -1004,7 +1004,7 @@ __global__ __launch_bounds__(256) void kb_mass(const GArgs<T> *a_, const unsigne
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_poisson4(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_poisson4(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];

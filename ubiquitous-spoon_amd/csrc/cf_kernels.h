@@ -203,14 +203,19 @@ static __device__ __forceinline__ void batch_loop(const unsigned *__restrict__ v
     const int nt = blockDim.x * blockDim.y;
     for (int t = tid; t <= nrun; t += nt) lvs[t] = vstart[t];
     __syncthreads();
-    const unsigned total = lvs[nrun];
+    /* every thread searches for the same virtual block, so the run and the
+     * local block are workgroup-uniform: readfirstlane keeps them (and the
+     * argument-table loads they index) in scalar registers */
+    const unsigned total = (unsigned) __builtin_amdgcn_readfirstlane((int) lvs[nrun]);
     for (unsigned vb = blockIdx.x; vb < total; vb += gridDim.x) {
         int lo = 0, hi = nrun - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (lvs[mid] <= vb) lo = mid; else hi = mid - 1;
+            const unsigned v = (unsigned) __builtin_amdgcn_readfirstlane((int) lvs[mid]);
+            if (v <= vb) lo = mid; else hi = mid - 1;
         }
-        fn(lo, vb - lvs[lo]);
+        const unsigned lb = vb - (unsigned) __builtin_amdgcn_readfirstlane((int) lvs[lo]);
+        fn(lo, lb);
         __syncthreads();
     }
 }
@@ -296,11 +301,11 @@ template <typename T> __global__ void kb_plane(const GArgs<T> *a, const unsigned
 template <typename T> __global__ void kb_mass(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_cfl(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_step_end(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
-/* batched order-2 FP64 fast path */
-__global__ void kb_rhs2t(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
-BSel<double>::kb_t pick_batched_spike_sums(int L);
-__global__ void kb_level_sum(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
-__global__ void kb_mass_fast(const GArgs<double> *a, const unsigned *vs, int nrun, int s);
+/* batched order-2 fast path (FP64 or FP32 storage) */
+template <typename T> __global__ void kb_rhs2t(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> typename BSel<T>::kb_t pick_batched_spike_sums(int L);
+template <typename T> __global__ void kb_level_sum(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+template <typename T> __global__ void kb_mass_fast(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
 template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);

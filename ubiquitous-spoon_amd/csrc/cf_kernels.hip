@@ -86,12 +86,20 @@ template __global__ void k_dev_to_io<float>(const float *, double *, int, int, i
 /* lb: the tile's block index within the run (ntx nty nchunk of them) */
 template <typename T>
 static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, const ProjSrcT<T> &S,
-    const T *__restrict__ gu, const T *__restrict__ gv, const T *__restrict__ gw,
-    const double *__restrict__ scal,
-    T *__restrict__ fu, T *__restrict__ fv, T *__restrict__ fw,
-    T *__restrict__ ru, T *__restrict__ rv, T *__restrict__ rw,
-    double *__restrict__ cflpart, double cfldt, int kc, const unsigned lb)
+    const T *__restrict__ gu_, const T *__restrict__ gv_, const T *__restrict__ gw_,
+    const double *__restrict__ scal_,
+    T *__restrict__ fu_, T *__restrict__ fv_, T *__restrict__ fw_,
+    T *__restrict__ ru_, T *__restrict__ rv_, T *__restrict__ rw_,
+    double *__restrict__ cflpart_, double cfldt, int kc, const unsigned lb)
 {
+    const gp<const T> gu = (gp<const T>) gu_, gv = (gp<const T>) gv_, gw = (gp<const T>) gw_;
+    const gp<T> fu = (gp<T>) fu_, fv = (gp<T>) fv_, fw = (gp<T>) fw_;
+    const gp<T> ru = (gp<T>) ru_, rv = (gp<T>) rv_, rw = (gp<T>) rw_;
+    const gp<double> cflpart = (gp<double>) cflpart_;
+    const cp<double> scal = (cp<double>) scal_;
+    const gp<const T> Sus = (gp<const T>) S.us, Svs = (gp<const T>) S.vs, Sws = (gp<const T>) S.ws;
+    const gp<const T> Sp = (gp<const T>) S.p;
+    const cp<double> Sh1 = (cp<double>) S.h1, Sm = (cp<double>) S.m;
     __shared__ __align__(16) T SU[4][RT_HY][RT_HX], SV[4][RT_HY][RT_HX], SW[4][RT_HY][RT_HX];
     __shared__ T SP[2][2][RT_PY][RT_PX];          /* [buffer][p(L), p(L+1)] */
     double *red = (double *) &SU[0][0][0];        /* CFL reduction after the last level (>= 256 doubles) */
@@ -111,10 +119,10 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     const int i = i0 + lx, j = j0 + ly;
     const bool act = i < nx && j < ny;
 
-    const T m = S.m ? (T) *S.m : (T) 0;
+    const T m = Sm ? (T) *Sm : (T) 0;
     const T cx = (T) (S.dtp * R.rdx), cy = (T) (S.dtp * R.rdy);
-    const double *dz = R.dzt + CF_NGT;
-    const double *rdz = R.rdz + CF_NGT, *rdzh = R.rdzh + CF_NGT;
+    const cp<double> dz = (cp<double>) R.dzt + CF_NGT;
+    const cp<double> rdz = (cp<double>) R.rdz + CF_NGT, rdzh = (cp<double>) R.rdzh + CF_NGT;
     const T ub = (T) R.ubc, vb = (T) R.vbc;
 
     /* element e of the halo tile / pressure tile -> global column offset */
@@ -146,11 +154,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NA) {
-                X.u[t] = S.us[(size_t) Lc * P + oa[t]];
-                X.v[t] = S.vs[(size_t) Lc * P + oa[t]];
-                X.w[t] = wok ? S.ws[(size_t) L * P + oa[t]] : (T) 0;
+                X.u[t] = Sus[(size_t) Lc * P + oa[t]];
+                X.v[t] = Svs[(size_t) Lc * P + oa[t]];
+                X.w[t] = wok ? Sws[(size_t) L * P + oa[t]] : (T) 0;
             }
-            if (e < RT_NP) X.pn[t] = S.p[(size_t) L2 * P + op[t]];
+            if (e < RT_NP) X.pn[t] = Sp[(size_t) L2 * P + op[t]];
         }
     };
     /* pressure of level L into buffer (L - k0 + 1) & 1: p(clamp(L)) and,
@@ -171,7 +179,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     auto pdirect = [&](int L, T *out) {
         const int Lc = clampz(L);
 #pragma unroll
-        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? S.p[(size_t) Lc * P + op[t]] : (T) 0;
+        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? Sp[(size_t) Lc * P + op[t]] : (T) 0;
     };
     /* projected u, v, w of level L into ring slot (L - k0 + 1) & 3 */
     auto build = [&](int L, const RSet &X) {
@@ -179,7 +187,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         const int Lc = L < 0 ? 0 : (L >= nz ? nz - 1 : L);
         const bool wok = wok_of(L);
         const bool ghost = L < 0 || L >= nz;
-        const T hm = S.h1 ? (T) S.h1[Lc + 1] * m : (T) 0;
+        const T hm = Sh1 ? (T) Sh1[Lc + 1] * m : (T) 0;
         const T cz = wok ? (T) (S.dtp * rdzh[L]) : (T) 0;
 #pragma unroll
         for (int t = 0; t < 2; t++) {
@@ -499,12 +507,15 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     const bool live = chunk < ST.S;
     const size_t col = (size_t) b.x * 64 + lane;
     const bool act = col < P;
-    T *x = f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc));
+    const gp<T> x = (gp<T>) (f == 0 ? ru : (f == 1 ? rv : (f == 2 ? rw : rc)));
+    const gp<double> pl = (gp<double>) plane;
     const int lo = chunk * ST.L;
     const int Lc = live ? min(ST.L, ST.n - lo) : 0;     /* rows of this chunk */
-    T *xc = x + (act ? col : P - 1);
+    const gp<T> xc = x + (act ? col : P - 1);
     const size_t tb = (size_t) chunk * LM;
-    const double *ta = ST.a + tb, *tib = ST.ib + tb, *tgam = ST.gam + tb, *tvl = ST.vl + tb, *twr = ST.wr + tb;
+    const cp<double> ta = (cp<double>) ST.a + tb, tib = (cp<double>) ST.ib + tb;
+    const cp<double> tgam = (cp<double>) ST.gam + tb, tvl = (cp<double>) ST.vl + tb;
+    const cp<double> twr = (cp<double>) ST.wr + tb, tminv = (cp<double>) ST.minv;
     T y[LM];
     if (live) {                                   /* wave-uniform */
 #pragma unroll
@@ -536,12 +547,12 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
     constexpr int MP = 2 * CF_SPIKE_SMAX;
     T E = 0, F = 0;
     if (chunk > 0) {
-        const double *row = ST.minv + (size_t) (2 * (chunk - 1) + 1) * MP;
+        const cp<double> row = tminv + (size_t) (2 * (chunk - 1) + 1) * MP;
 #pragma unroll
         for (int q = 0; q < MP; q++) E += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
     if (chunk < ST.S - 1) {
-        const double *row = ST.minv + (size_t) (2 * (chunk + 1)) * MP;
+        const cp<double> row = tminv + (size_t) (2 * (chunk + 1)) * MP;
 #pragma unroll
         for (int q = 0; q < MP; q++) F += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
@@ -552,7 +563,7 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
             if (act) xc[(size_t) (lo + t) * P] = val;
             if (sums) {
                 const double tot = wave_sum(act ? (double) val : 0.0);
-                if (lane == 0) plane[(size_t) (lo + t) * b.nx + b.x] = tot;
+                if (lane == 0) pl[(size_t) (lo + t) * b.nx + b.x] = tot;
             }
         }
     }
@@ -635,7 +646,7 @@ __global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ pl
  * per-level plane sums of the solved field (k_spike_sums, k_level_sum);
  * the wall-ghost averages are 2 bc - the adjacent level (apply_bc_u2).
  * Writes the forcing into scal[slot]; on the last substep writes the
- * pstat record.  One block.
+ * pstat record.  One block of 256 threads.
  * ------------------------------------------------------------------- */
 static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const int s, const double *__restrict__ lsum,
                                                           double *__restrict__ scal, double *__restrict__ logbuf, int kind)
@@ -645,16 +656,29 @@ static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const double *dz = R.dzt + CF_NGT;
     const double scale = 1.0 / ((double) nx * (double) ny);
-    for (int l = threadIdx.x; l < nz; l += blockDim.x) avg[l + 1] = lsum[l] * scale;
+    /* volume integral and height: per-thread partials over k = t, t + 256,
+     * ..., then a fixed tree (deterministic, the same in every launch) */
+    __shared__ double rv[256], rl[256];
+    double pv = 0.0, pl = 0.0;
+    for (int l = threadIdx.x; l < nz; l += 256) {
+        const double a = lsum[l] * scale;
+        avg[l + 1] = a;
+        pv += a * dz[l];
+        pl += dz[l];
+    }
+    rv[threadIdx.x] = pv;
+    rl[threadIdx.x] = pl;
     __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { rv[threadIdx.x] += rv[threadIdx.x + w]; rl[threadIdx.x] += rl[threadIdx.x + w]; }
+        __syncthreads();
+    }
     if (threadIdx.x != 0) return;
     double *A = avg + 1;                          /* A[-1 .. nz] */
     const double bc = kind == 0 ? R.ubc : R.cbot;
     A[-1] = 2.0 * bc - A[0];                      /* apply_bc_u2 ghosts of the solved field */
     A[nz] = 2.0 * (kind == 0 ? R.ubc : R.ctop) - A[nz - 1];
-    double vol = 0.0, lz = 0.0;
-    for (int k = 0; k < nz; k++) { vol += A[k] * dz[k]; lz += dz[k]; }
-    vol = vol / lz;
+    double vol = rv[0] / rl[0];
     const double kap = kind == 0 ? R.nu : R.nu / R.prandtl;
     double bot = kap * (A[0] - A[-1]) / (0.5 * (dz[0] + dz[-1]));
     double top = kap * (A[nz] - A[nz - 1]) / (0.5 * (dz[nz] + dz[nz - 1]));
@@ -736,17 +760,17 @@ template <> __device__ __forceinline__ const float2 *twsel<float2>(const double2
  * others), so the passes read twiddles from LDS without an extra round
  * trip to memory */
 template <typename C>
-static __device__ __forceinline__ C tw_first(const C *__restrict__ tw, int n)
+static __device__ __forceinline__ C tw_first(cp<cv<C>> tw, int n)
 {
     const int t = (int) threadIdx.x;
-    return tw[t < n ? t : n - 1];
+    return cx_of<C>(tw[t < n ? t : n - 1]);
 }
 template <typename C>
-static __device__ __forceinline__ void tw_rest(C *tws, C first, const C *__restrict__ tw, int n)
+static __device__ __forceinline__ void tw_rest(C *tws, C first, cp<cv<C>> tw, int n)
 {
     const int t = (int) threadIdx.x;
     if (t < n) tws[t] = first;
-    for (int q = t + (int) blockDim.x; q < n; q += (int) blockDim.x) tws[q] = tw[q];
+    for (int q = t + (int) blockDim.x; q < n; q += (int) blockDim.x) tws[q] = cx_of<C>(tw[q]);
 }
 
 #define CF_FX_UNR 3            /* elements per thread gathered together by the divergence + x-FFT */
@@ -756,8 +780,7 @@ static __device__ __forceinline__ void tw_rest(C *tws, C first, const C *__restr
  * block's 2 nl lines are consecutive ky positions (coalesced) */
 template <typename C>
 static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, int nl, int nx, int nkx,
-                                                          int ny, long NL, long pair0, double scd,
-                                                          C *__restrict__ X)
+                                                          int ny, long NL, long pair0, double scd, gp<cv<C>> X)
 {
     typedef typename Real<C>::type F;
     const F sc = (F) scd;
@@ -767,11 +790,10 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
         if (la >= NL) continue;
         const C Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
         const long k = la / ny, j = la - k * ny;
-        C *dst = X + ((size_t) k * nkx + kx) * ny + j;
-        dst[0] = mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
+        X[((size_t) k * nkx + kx) * ny + j] = cv_of(mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc));
         if (lb < NL) {
             const long kb = lb / ny, jb = lb - kb * ny;
-            X[((size_t) kb * nkx + kx) * ny + jb] = mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc);
+            X[((size_t) kb * nkx + kx) * ny + jb] = cv_of(mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc));
         }
     }
 }
@@ -781,11 +803,14 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
  * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
  * mirrored oddly about the (zero) wall faces, as divergence4 sets them. */
 template <int NX, int ORD, typename T>
-static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__restrict__ us,
-    const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
-    const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X, const Blk b)
+static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__restrict__ us_,
+    const T *__restrict__ vs_, const T *__restrict__ ws_, const double *__restrict__ h1_,
+    const double *__restrict__ mptr_, typename Cplx<T>::type *__restrict__ X_, const Blk b)
 {
     typedef typename Cplx<T>::type C;
+    const gp<const T> us = (gp<const T>) us_, vs = (gp<const T>) vs_, ws = (gp<const T>) ws_;
+    const cp<double> h1 = (cp<double>) h1_, mptr = (cp<double>) mptr_;
+    const gp<cv<C>> X = (gp<cv<C>>) X_;
     extern __shared__ __align__(16) unsigned char smem_[];
     C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
@@ -794,7 +819,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     const int nl = R.fx_pairs;
     const int NL = ny * nz;
     const size_t P = (size_t) nx * ny;
-    const double *rdz = R.rdz + CF_NGT;
+    const cp<double> rdz = (cp<double>) R.rdz + CF_NGT;
     const T m = mptr ? (T) *mptr : (T) 0;
     const T rdx = (T) R.rdx, rdy = (T) R.rdy;
     const int pair0 = (int) xcd_block(b.x, b.nx) * nl;
@@ -835,9 +860,9 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     /* groups of CF_FX_UNR elements per thread: all reads of a group are
      * in flight before the first LDS write (blockDim.x = 256) */
     constexpr int UNR = ORD == 2 ? (sizeof(T) == 4 ? CF_FX_UNR : 2) : (sizeof(T) == 4 ? 2 : 1);   /* fit 128 VGPRs */
-    const C *twg = twsel<C>(R.twx, R.twx32);
+    const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
-    const C tw0 = tw_first(twg, nx);
+    const C tw0 = tw_first<C>(twg, nx);
     const int tot = nl * nx;
     for (int e0 = 0; e0 < tot; e0 += UNR * 256) {
         C res[UNR];
@@ -858,10 +883,10 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
             }
         }
     }
-    tw_rest(tws, tw0, twg, nx);
+    tw_rest<C>(tws, tw0, twg, nx);
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, (const C *) tws, 1.0);
-    store_half_spectra(sb, ld, nl, nx, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) nx, X);
+    fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, (const C *) tws, 1.0);
+    store_half_spectra<C>(sb, ld, nl, nx, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) nx, X);
 }
 
 template <int NX, int ORD, typename T>
@@ -893,16 +918,17 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
         sb[l * ld + i] = make_double2(va, vb);
     }
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, R.twx, 1.0);
-    store_half_spectra(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, X);
+    fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, R.twx, 1.0);
+    store_half_spectra<double2>(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, (gp<cv<double2>>) X);
 }
 
 /* complex y FFT in place, forward scaled 1/ny (fft.c:186-202) or unscaled
  * inverse (fft.c:205-208); a block transforms TK contiguous lines */
 template <int NY, typename T>
-static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<T>::type *__restrict__ X, int fwd, const Blk b)
+static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<T>::type *__restrict__ X_, int fwd, const Blk b)
 {
     typedef typename Cplx<T>::type C;
+    const gp<cv<C>> X = (gp<cv<C>>) X_;
     extern __shared__ __align__(16) unsigned char smem_[];
     C *sb = (C *) smem_;
     const int ny = NY > 0 ? NY : R.g.ny;
@@ -910,19 +936,19 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
     const int TK = R.fy_cols, ld = fft_ld(ny);
     const long L0 = (long) b.x * TK;
     const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
-    C *base = X + (size_t) L0 * ny;
+    const gp<cv<C>> base = X + (size_t) L0 * ny;
     /* all reads of a thread in flight before its first LDS write
      * (TK ny <= 8 x 256, blockDim.x = 256) */
-    const C *twg = twsel<C>(R.twy, R.twy32);
+    const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twy, R.twy32);
     C *tws = sb + TK * ld;
-    const C tw0 = tw_first(twg, ny);
+    const C tw0 = tw_first<C>(twg, ny);
     const int tot = TK * ny, totv = nl * ny;
     for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
         C v[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int e = e0 + (int) threadIdx.x + u * 256;
-            const C x = base[e < totv ? e : totv - 1];
+            const C x = cx_of<C>(base[e < totv ? e : totv - 1]);
             v[u] = e < totv ? x : mkc<C>((T) 0, (T) 0);
         }
 #pragma unroll
@@ -934,14 +960,14 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
             }
         }
     }
-    tw_rest(tws, tw0, twg, ny);
+    tw_rest<C>(tws, tw0, twg, ny);
     __syncthreads();
-    fft_lines<NY, 8>(sb, ny, ld, TK, R.rady, R.nrady, (const C *) tws, fwd ? 1.0 : -1.0);
+    fft_lines<NY, 8>(sb, ny, ld, TK, (cp<int>) R.rady, R.nrady, (const C *) tws, fwd ? 1.0 : -1.0);
     const T sc = fwd ? (T) (1.0 / (double) ny) : (T) 1;
     for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
         int t = e / ny, j = e - t * ny;
         C val = sb[t * ld + j];
-        base[e] = fwd ? cscale(val, sc) : val;
+        base[e] = cv_of(fwd ? cscale(val, sc) : val);
     }
 }
 
@@ -955,10 +981,12 @@ __global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::
  * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two real
  * lines per complex transform, writing p */
 template <int NX, typename T>
-static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typename Cplx<T>::type *__restrict__ X,
-                                                  T *__restrict__ p, const Blk b)
+static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typename Cplx<T>::type *__restrict__ X_,
+                                                  T *__restrict__ p_, const Blk b)
 {
     typedef typename Cplx<T>::type C;
+    const gp<const cv<C>> X = (gp<const cv<C>>) X_;
+    const gp<T> p = (gp<T>) p_;
     extern __shared__ __align__(16) unsigned char smem_[];
     C *sb = (C *) smem_;
     const int nx = NX > 0 ? NX : R.g.nx;
@@ -970,9 +998,9 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
     const int half = nx / 2;
     /* groups of CF_FXI_UNR elements per thread, reads at clamped indices,
      * all in flight before the first LDS write (blockDim.x = 256) */
-    const C *twg = twsel<C>(R.twx, R.twx32);
+    const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
-    const C tw0 = tw_first(twg, nx);
+    const C tw0 = tw_first<C>(twg, nx);
     const int tot = nl * nkx, NLi = (int) NL;
     for (int e0 = 0; e0 < tot; e0 += CF_FXI_UNR * 256) {
         C A[CF_FXI_UNR], B[CF_FXI_UNR];
@@ -984,8 +1012,8 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
             const int la = 2 * ((int) pair0 + l), lb = la + 1;
             const int lac = la < NLi ? la : NLi - 1, lbc = lb < NLi ? lb : NLi - 1;
             const int ka = lac / ny, ja = lac - ka * ny, kb = lbc / ny, jb = lbc - kb * ny;
-            const C a = X[((size_t) ka * nkx + kx) * ny + ja];
-            const C bq = X[((size_t) kb * nkx + kx) * ny + jb];
+            const C a = cx_of<C>(X[((size_t) ka * nkx + kx) * ny + ja]);
+            const C bq = cx_of<C>(X[((size_t) kb * nkx + kx) * ny + jb]);
             A[u] = la < NLi ? a : mkc<C>((T) 0, (T) 0);
             B[u] = lb < NLi ? bq : mkc<C>((T) 0, (T) 0);
         }
@@ -1001,9 +1029,9 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
             }
         }
     }
-    tw_rest(tws, tw0, twg, nx);
+    tw_rest<C>(tws, tw0, twg, nx);
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, R.radx, R.nradx, (const C *) tws, -1.0);
+    fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, (const C *) tws, -1.0);
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long la = 2 * (pair0 + l), lb = la + 1;
@@ -1217,7 +1245,7 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
     const int lo = chunk * L;
     const bool ld = live && act;
     const size_t stride = 2 * M;
-    S *__restrict__ col = yr + idx;
+    const gp<S> col = (gp<S>) yr + idx;
     const int ktop = nz - 1;
     /* the column's rows (clamped, valid reads), then the z tables into LDS
      * (tl: a, c, bz, ex, ey, dvol, nz each) with the same round trip */
@@ -1225,7 +1253,8 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
 #pragma unroll
     for (int t = 0; t < L; t++) raw[t] = col[(size_t) min(lo + t, ktop) * stride];
     {
-        const double *src[6] = { T_.a, T_.c, T_.bz, T_.ex, T_.ey, T_.dvol };
+        const cp<double> src[6] = { (cp<double>) T_.a, (cp<double>) T_.c, (cp<double>) T_.bz,
+                                    (cp<double>) T_.ex, (cp<double>) T_.ey, (cp<double>) T_.dvol };
 #pragma unroll
         for (int w = 0; w < 6; w++)
             for (int k = (int) threadIdx.x; k < nz; k += (int) blockDim.x) tl[w * nz + k] = src[w][k];
@@ -1320,7 +1349,7 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
     if (!ld) return;
     const A XB = chunk > 0 ? (A) xs[chunk - 1][g] : (A) 0;
     const A XA = (A) xs[chunk][g];
-    S *__restrict__ out = col + (size_t) lo * stride;
+    const gp<S> out = col + (size_t) lo * stride;
     if (lo + L - 1 <= ktop) {                     /* a full chunk */
 #pragma unroll
         for (int t = 0; t < L - 1; t++) out[(size_t) t * stride] = (S) (d[t] + ph[t] * XB + gm[t] * XA);
@@ -1364,7 +1393,7 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
         extern __shared__ __align__(16) unsigned char smem_[];
         double *tl = (double *) smem_;            /* 6 nz doubles (pois2_lds) */
         const size_t idx = 2 * mc + (threadIdx.x & 1);
-        const double lx = R.lamx[kx], ly = R.lamy[ky];
+        const double lx = ((cp<double>) R.lamx)[kx], ly = ((cp<double>) R.lamy)[ky];
         constexpr int LL = L > 1 ? L : 2;
         if (sizeof(T) == sizeof(double) || low)
             pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
@@ -1518,7 +1547,7 @@ typename BSel<T>::divx_t_ pick_batched_divx(int order)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_ffty(const GArgs<T> *a_, const unsigned *vs, int nrun, int fwd)
+__global__ __launch_bounds__(256) void kb_ffty(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int fwd)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -1527,7 +1556,7 @@ __global__ __launch_bounds__(256) void kb_ffty(const GArgs<T> *a_, const unsigne
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
@@ -1537,7 +1566,7 @@ __global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *a_, const unsign
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(256) void kb_poisson(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_poisson(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -1557,7 +1586,7 @@ typename BSel<T>::kb_t pick_batched_poisson(int L)
 }
 
 template <typename T, int LM>
-__global__ __launch_bounds__(512) void kb_spike(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(512) void kb_spike(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -1577,7 +1606,7 @@ typename BSel<T>::kb_t pick_batched_spike(int L)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_thomas(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_thomas(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
@@ -1586,7 +1615,7 @@ __global__ __launch_bounds__(256) void kb_thomas(const GArgs<T> *a_, const unsig
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_cfl(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_cfl(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
@@ -1596,7 +1625,7 @@ __global__ __launch_bounds__(256) void kb_cfl(const GArgs<T> *a_, const unsigned
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *a_, const unsigned *vs, int nrun, int s)
+__global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
@@ -1605,61 +1634,66 @@ __global__ __launch_bounds__(256) void kb_step_end(const GArgs<T> *a_, const uns
     });
 }
 
-/* ---- batched order-2 FP64 fast path: the single-run kernels' bodies over
- * the runs of a batch.  The argument table in use maps u, v, w to the
- * substep's input state set and ru, rv, rw to its output set. */
-__global__ __launch_bounds__(256) void kb_rhs2t(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+/* ---- batched order-2 fast path (FP64 or FP32 storage): the single-run
+ * kernels' bodies over the runs of a batch.  The argument table in use maps
+ * u, v, w to the substep's input state set and ru, rv, rw to its output
+ * set. */
+template <typename T>
+__global__ __launch_bounds__(256) void kb_rhs2t(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
-        const GArgs<double> &a = a_[r];
+        const GArgs<T> &a = a_[r];
         const int ps = (s + 2) % 3;               /* the substep whose projection is applied */
-        ProjSrc S;
+        ProjSrcT<T> S;
         S.us = a.u; S.vs = a.v; S.ws = a.w; S.p = a.p;
         S.h1 = a.R.h1 + (size_t) (ps * 2) * (a.R.g.nz + 2);
         S.m = a.scal + SC_M0 + ps;
         S.dtp = a.R.sc[ps].abdt;
-        double *const *fnew = (s % 2 == 0) ? a.fa : a.fb;
-        double *const *gold = (s % 2 == 0) ? a.fb : a.fa;
+        T *const *fnew = (s % 2 == 0) ? a.fa : a.fb;
+        T *const *gold = (s % 2 == 0) ? a.fb : a.fa;
         const bool useg = a.R.sc[s].zetdt != 0.0, keepf = s < 2;
-        rhs2t_body(a.R, s, S, useg ? gold[0] : nullptr, gold[1], gold[2], a.scal,
-                   keepf ? fnew[0] : nullptr, fnew[1], fnew[2], a.ru, a.rv, a.rw,
-                   s == 0 ? a.cflpart : nullptr, a.dt, a.rhs_kc, lb);
+        rhs2t_body<T>(a.R, s, S, useg ? gold[0] : nullptr, gold[1], gold[2], a.scal,
+                      keepf ? fnew[0] : nullptr, fnew[1], fnew[2], a.ru, a.rv, a.rw,
+                      s == 0 ? a.cflpart : nullptr, a.dt, a.rhs_kc, lb);
     });
 }
 
-template <int LM>
-__global__ __launch_bounds__(512) void kb_spike_sums(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+template <typename T, int LM>
+__global__ __launch_bounds__(512) void kb_spike_sums(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
-        const GArgs<double> &a = a_[r];
-        spike_body<double, LM>(a.R, s, a.ru, a.rv, a.rw, nullptr, blk_of(a, ST_SOLVE, lb), a.part);
+        const GArgs<T> &a = a_[r];
+        spike_body<T, LM>(a.R, s, a.ru, a.rv, a.rw, nullptr, blk_of(a, ST_SOLVE, lb), a.part);
     });
 }
 
-BSel<double>::kb_t pick_batched_spike_sums(int L)
+template <typename T>
+typename BSel<T>::kb_t pick_batched_spike_sums(int L)
 {
     switch (L) {
-#define X_(N) case N: return kb_spike_sums<N>;
+#define X_(N) case N: return kb_spike_sums<T, N>;
     CF_SPIKE_LSET(X_)
 #undef X_
     default: return nullptr;
     }
 }
 
-__global__ __launch_bounds__(256) void kb_level_sum(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+template <typename T>
+__global__ __launch_bounds__(256) void kb_level_sum(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
-        const GArgs<double> &a = a_[r];
+        const GArgs<T> &a = a_[r];
         level_sum_body(a.part, (int) a.dims[ST_SOLVE][0], a.lsum, (int) lb);
     });
 }
 
-__global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<double> *a_, const unsigned *vs, int nrun, int s)
+template <typename T>
+__global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         (void) lb;
-        const GArgs<double> &a = a_[r];
+        const GArgs<T> &a = a_[r];
         mass_finalize_body(a.R, s, a.lsum, a.scal, a.logbuf, 0);
     });
 }
@@ -1672,7 +1706,11 @@ __global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<double> *a_, con
     template __global__ void kb_thomas<T>(const GArgs<T> *, const unsigned *, int, int);                          \
     template __global__ void kb_cfl<T>(const GArgs<T> *, const unsigned *, int, int);                             \
     template __global__ void kb_step_end<T>(const GArgs<T> *, const unsigned *, int, int);                        \
-    template typename BSel<T>::divx_t_ pick_batched_divx<T>(int);
+    template typename BSel<T>::divx_t_ pick_batched_divx<T>(int);                                                  \
+    template __global__ void kb_rhs2t<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template typename BSel<T>::kb_t pick_batched_spike_sums<T>(int);                                               \
+    template __global__ void kb_level_sum<T>(const GArgs<T> *, const unsigned *, int, int);                       \
+    template __global__ void kb_mass_fast<T>(const GArgs<T> *, const unsigned *, int, int);
 CF_INST_B(double)
 CF_INST_B(float)
 #undef CF_INST_B

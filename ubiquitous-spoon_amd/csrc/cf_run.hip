@@ -1531,7 +1531,7 @@ struct cf_batch {
     size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    /* order-2 FP64 runs without the scalar step on the fast path's fused
+    /* order-2 runs (FP64 or FP32) without the scalar step on the fast path's fused
      * kernels: two argument tables (state sets swapped), one graph per
      * starting set */
     bool fast = false;
@@ -1687,11 +1687,7 @@ static cf_status batch_launch_spike(cf_batch *b, int k, int s, bool sums)
     if (e != CF_OK) return e;
     for (size_t g = 0; g < b->spL.size(); g++) {
         typename BSel<T>::kb_t fn = nullptr;
-        if (sums) {
-            if constexpr (std::is_same<T, double>::value) fn = pick_batched_spike_sums(b->spL[g]);
-        } else {
-            fn = pick_batched_spike<T>(b->spL[g]);
-        }
+        fn = sums ? pick_batched_spike_sums<T>(b->spL[g]) : pick_batched_spike<T>(b->spL[g]);
         if (!fn) { cf_set_error("no spike kernel for %d rows per chunk", b->spL[g]); return CF_ERR_UNSUPPORTED; }
         const int row = CF_NST + CF_P2_NG + (int) g;
         const unsigned tg = batch_total(b, row);
@@ -1752,18 +1748,11 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
         }
     case KB_FFTXI: fn = kb_fftxi<T>; lds = b->lds_fx; break;
     case KB_STEPEND: fn = kb_step_end<T>; break;
-    case KB_FRHS:
-        if constexpr (std::is_same<T, double>::value) fn = kb_rhs2t;
-        break;
+    case KB_FRHS: fn = kb_rhs2t<T>; break;
     case KB_FSPIKE:
         return batch_launch_spike<T>(b, k, s, true);
-    case KB_FLEVEL:
-        if constexpr (std::is_same<T, double>::value) fn = kb_level_sum;
-        break;
-    case KB_FMASS:
-        if constexpr (std::is_same<T, double>::value) fn = kb_mass_fast;
-        lds = b->lds_mass;
-        break;
+    case KB_FLEVEL: fn = kb_level_sum<T>; break;
+    case KB_FMASS: fn = kb_mass_fast<T>; lds = b->lds_mass; break;
     default: cf_set_error("bad batch kernel slot %d", k); return CF_ERR_ARG;
     }
     if (!fn) return CF_OK;
@@ -1811,20 +1800,26 @@ static cf_status batch_enqueue_step(cf_batch *b)
 
 /* one RK3 step of a fast batch starting from state set `from`: the fast
  * path's sequence (enqueue_step) with every launch covering all runs */
-static cf_status fbatch_enqueue_step(cf_batch *b, int from)
+template <typename T>
+static cf_status fbatch_enqueue_step_t(cf_batch *b, int from)
 {
     cf_status st;
     for (int s = 0; s < 3; s++) {
         b->tab = (from + s) & 1;
-        if ((st = batch_launch<double>(b, KB_FRHS, s)) != CF_OK) return st;
-        if (s == 0 && (st = batch_launch<double>(b, KB_CFL, s)) != CF_OK) return st;
-        if ((st = batch_launch<double>(b, KB_FSPIKE, s)) != CF_OK) return st;
-        if ((st = batch_launch<double>(b, KB_FLEVEL, s)) != CF_OK) return st;
-        if ((st = batch_launch<double>(b, KB_FMASS, s)) != CF_OK) return st;
-        if ((st = batch_enqueue_pressure<double>(b, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_FRHS, s)) != CF_OK) return st;
+        if (s == 0 && (st = batch_launch<T>(b, KB_CFL, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_FSPIKE, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_FLEVEL, s)) != CF_OK) return st;
+        if ((st = batch_launch<T>(b, KB_FMASS, s)) != CF_OK) return st;
+        if ((st = batch_enqueue_pressure<T>(b, s)) != CF_OK) return st;
     }
     b->tab = 0;
-    return batch_launch<double>(b, KB_STEPEND, 0);
+    return batch_launch<T>(b, KB_STEPEND, 0);
+}
+
+static cf_status fbatch_enqueue_step(cf_batch *b, int from)
+{
+    return b->prec == CF_FP32 ? fbatch_enqueue_step_t<float>(b, from) : fbatch_enqueue_step_t<double>(b, from);
 }
 
 static cf_status batch_collect(cf_batch *b)
@@ -1975,8 +1970,8 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     b->order = cfgs[0].order;
     b->scalar = cfgs[0].scalar;
     b->device = cfgs[0].device;
-    /* order 2, FP64, no scalar, z-solves the spike covers: the fast path */
-    b->fast = b->order == 2 && b->prec == CF_FP64 && !b->scalar;
+    /* order 2, no scalar, z-solves the spike covers: the fast path */
+    b->fast = b->order == 2 && !b->scalar;
     for (int q = 0; q < nrun && b->fast; q++)
         b->fast = spike_L(cfgs[q].nz) > 0 && cfgs[q].cflmode == 0;
     for (int q = 0; q < nrun; q++) {
@@ -2231,11 +2226,11 @@ static double srv_now(const cf_server *s)
 }
 
 static int group_key(const cf_config &c) { return (c.order == 4 ? 4 : 0) + (c.scalar ? 2 : 0) + (c.precision == CF_FP32 ? 1 : 0); }
-/* order-2 FP64 runs without the scalar whose z-solves the spike covers
- * are batched on the fast path */
+/* order-2 runs without the scalar whose z-solves the spike covers are
+ * batched on the fast path (FP64 and FP32 in groups of their own) */
 static bool fast_eligible(const cf_config &c)
 {
-    return c.order == 2 && c.precision == CF_FP64 && !c.scalar && c.cflmode == 0 && spike_L(c.nz) > 0;
+    return c.order == 2 && !c.scalar && c.cflmode == 0 && spike_L(c.nz) > 0;
 }
 /* relabel a fast-path run's two state sets (host pointers only), so that
  * it joins a batch stepping from the other set; its single-run graphs,
@@ -2428,7 +2423,7 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         if (s->used + r.bytes > s->budget) break;
         cf_run *run = nullptr;
         /* large runs fill the GPU on their own: they keep the single-run
-         * graph (the order-2 FP64 fast path where it applies) on their own
+         * graph (the order-2 fast path where it applies) on their own
          * stream; small ones join their group's batched launches.  The
          * choice depends on the request alone, never on the mix. */
         r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;

@@ -167,9 +167,29 @@ static inline int pois2_L_of(int need)
 static inline int pois2_L(int nz) { return pois2_L_of((nz + 15) / 16); }
 /* dynamic LDS of the partitioned solve: its six z tables */
 static inline size_t pois2_lds(int L, int nz) { return L > 0 ? 6 * sizeof(double) * (size_t) nz : 0; }
+/* workgroup shape: CF_P2S_NC z-chunks x CF_P2S_MW modes x 2 parts; a
+ * z-level of the group is 2 CF_P2S_MW contiguous reals (128 B in FP64;
+ * 16 modes per workgroup measured slower at 256x193x256) */
+#define CF_P2S_NC 16
+#define CF_P2S_MW 8
+#define CF_P2S_GW (2 * CF_P2S_MW)
+#define CF_P2S_THREADS (CF_P2S_NC * CF_P2S_GW)
+/* modes per workgroup: CF_P2S_MW partitioned, or one per thread (L = 0) */
+static inline size_t pois2_modes_per_block(int L) { return L > 0 ? CF_P2S_MW : CF_P2S_THREADS; }
 static inline unsigned pois2_blocks(int L, size_t M)
 {
-    return (unsigned) (L > 0 ? (M + 7) / 8 : (M + 255) / 256);
+    const size_t per = pois2_modes_per_block(L);
+    return (unsigned) ((M + per - 1) / per);
+}
+/* FP32 storage: the leading workgroups of the order-2 Poisson grid that
+ * can hold an FP64 island mode (kx <= CF_F64_KLOW: modes below
+ * (CF_F64_KLOW + 1) ny in the kx-major order) */
+static inline unsigned pois2_island_blocks(int L, int ny, size_t M)
+{
+    const size_t per = pois2_modes_per_block(L);
+    const size_t n = ((size_t) (CF_F64_KLOW + 1) * (size_t) ny + per - 1) / per;
+    const unsigned all = pois2_blocks(L, M);
+    return n < all ? (unsigned) n : all;
 }
 
 /* One run of a batched launch: its parameter block, buffers (storage
@@ -264,6 +284,7 @@ k_fftx_plain_t pick_fftx_plain(int nx);
 template <typename T> typename KSel<T>::ffty_t pick_ffty(int ny);
 template <typename T> typename KSel<T>::fftxi_t pick_fftx_inv(int nx);
 template <typename T> typename KSel<T>::pois2_t pick_poisson2(int lmax);
+KSel<float>::pois2_t pick_poisson2_islands(int lmax);
 template <typename T>
 __global__ void k_project2(const RunDev R, const T *us, const T *vs, const T *ws, const double *h1, const double *mptr,
                            const T *p, double dtp, T *u, T *v, T *w);
@@ -309,6 +330,7 @@ template <typename T> __global__ void kb_mass_fast(const GArgs<T> *a, const unsi
 template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
 template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);
+BSel<float>::kb_t pick_batched_poisson_islands(int L);
 template <typename T> __global__ void k_gsolve4(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> __global__ void k_gplane(const RunDev R, const T *q0, const T *q1, double *part, int nb);
 __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);

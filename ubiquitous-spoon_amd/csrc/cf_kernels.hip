@@ -1213,9 +1213,10 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
 }
 
 /* Partitioned form of the same solve, for nz <= 16 * 32.  A workgroup of
- * 256 threads owns CF_P2S_MW modes; thread t takes part t & 1 (real or
- * imaginary: both parts share the matrix), mode (t >> 1) & 7 of the group
- * and z-chunk t >> 4 of L rows (L a compile-time choice >= nz / 16).  Rows
+ * CF_P2S_THREADS threads owns CF_P2S_MW modes; thread t takes part t & 1
+ * (real or imaginary: both parts share the matrix), mode (t >> 1) mod
+ * CF_P2S_MW of the group and z-chunk t / (2 CF_P2S_MW) of L rows (L a
+ * compile-time choice >= nz / 16).  Rows
  * above nz are identity rows (x = 0), so every chunk has L - 1 interior
  * rows and a separator (its last row).  Each thread holds its chunk in
  * registers and solves the interior as a Dirichlet problem in the two
@@ -1227,19 +1228,17 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
  * latency of the level recurrences, which bounds the one-thread-per-mode
  * sweep at these mode counts.  Differs from sequential Thomas by
  * round-off. */
-#define CF_P2S_NC 16
-#define CF_P2S_MW 8
 /* occupancy target by chunk length: the largest without register spills */
 #define CF_P2S_WAVES(L) ((L) == 0 ? 2 : ((L) <= 13 ? 4 : ((L) <= 20 ? 3 : 1)))
 
 template <int L, typename A, typename T>
 static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Real<typename Cplx<T>::type>::type *__restrict__ yr,
                                                   size_t M, int nz, int nch, size_t idx, bool act,
-                                                  double lxd, double lyd, bool zero, double (*sh)[CF_P2S_NC][16],
-                                                  double (*xs)[16], double *tl)
+                                                  double lxd, double lyd, bool zero, double (*sh)[CF_P2S_NC][CF_P2S_GW],
+                                                  double (*xs)[CF_P2S_GW], double *tl)
 {
     typedef typename Real<typename Cplx<T>::type>::type S;
-    const int g = threadIdx.x & 15, chunk = threadIdx.x >> 4;
+    const int g = threadIdx.x & (CF_P2S_GW - 1), chunk = threadIdx.x / CF_P2S_GW;
     const A lx = (A) lxd, ly = (A) lyd;
     const bool live = chunk < nch;
     const int lo = chunk * L;
@@ -1362,16 +1361,21 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
 }
 
 /* L = pois2_L(nz) (or larger): the partitioned solve with chunks of L
- * rows; L = 0: one thread per mode */
-template <typename T, int L>
+ * rows; L = 0: one thread per mode.  FP32 storage solves the modes with
+ * |kx|, |ky| <= CF_F64_KLOW in FP64 (the islands); ISL selects what a
+ * launch covers: 0 the storage-precision modes only (island workgroups
+ * return at once), 1 the islands only, 2 both.  Splitting them keeps the
+ * FP64 path's registers out of the FP32 kernel. */
+template <typename T, int L, int ISL = 2>
 static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab T_,
                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G, const Blk b)
 {
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const size_t M = (size_t) ny * nkx;
+    constexpr bool f64 = sizeof(T) == sizeof(double);
     if (L > 0) {
-        __shared__ double sh[7][CF_P2S_NC][16];
-        __shared__ double xs[CF_P2S_NC][16];
+        __shared__ double sh[7][CF_P2S_NC][CF_P2S_GW];
+        __shared__ double xs[CF_P2S_NC][CF_P2S_GW];
         const int nch = (nz + L - 1) / (L > 0 ? L : 1);
         const size_t m0 = (size_t) b.x * CF_P2S_MW;
         const size_t mode = m0 + ((threadIdx.x >> 1) & (CF_P2S_MW - 1));
@@ -1382,23 +1386,30 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
         /* FP32 storage: workgroups holding any mode with |kx|, |ky| <=
          * CF_F64_KLOW solve in FP64 (a workgroup-uniform choice) */
         bool low = false;
-        if (sizeof(T) != sizeof(double)) {
+        if (!f64) {
             const size_t m1 = min(m0 + CF_P2S_MW, M) - 1;
             for (size_t q = m0; q <= m1; q++) {
                 const int qx = (int) (q / ny), qy = (int) (q % ny);
                 low = low || (qx <= CF_F64_KLOW && (qy <= CF_F64_KLOW || ny - qy <= CF_F64_KLOW));
             }
         }
+        if (!f64 && ((ISL == 0 && low) || (ISL == 1 && !low))) return;   /* workgroup-uniform */
         typename Real<typename Cplx<T>::type>::type *yr = (typename Real<typename Cplx<T>::type>::type *) Y;
         extern __shared__ __align__(16) unsigned char smem_[];
         double *tl = (double *) smem_;            /* 6 nz doubles (pois2_lds) */
         const size_t idx = 2 * mc + (threadIdx.x & 1);
         const double lx = ((cp<double>) R.lamx)[kx], ly = ((cp<double>) R.lamy)[ky];
         constexpr int LL = L > 1 ? L : 2;
-        if (sizeof(T) == sizeof(double) || low)
+        if constexpr (f64 || ISL == 1) {
             pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
-        else
+        } else if constexpr (ISL == 0) {
             pois2s_run<LL, T, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
+        } else {
+            if (low)
+                pois2s_run<LL, double, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
+            else
+                pois2s_run<LL, T, T>(T_, yr, M, nz, nch, idx, act, lx, ly, zero, sh, xs, tl);
+        }
         return;
     }
     const size_t qm = (size_t) b.x * blockDim.x + threadIdx.x;
@@ -1406,31 +1417,45 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
     const int kx = (int) (qm / ny), ky = (int) (qm % ny);     /* layout [kx][ky] per level */
     const bool zero = kx == 0 && ky == 0;
     const bool low = kx <= CF_F64_KLOW && (ky <= CF_F64_KLOW || ny - ky <= CF_F64_KLOW);
-    if (sizeof(T) == sizeof(double) || low)
+    if (!f64 && ((ISL == 0 && low) || (ISL == 1 && !low))) return;
+    if (f64 || low)
         pois2_mode<double>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
     else
         pois2_mode<T>(T_, Y + qm, G + qm, M, nz, R.lamx[kx], R.lamy[ky], zero);
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_P2S_WAVES(L), 8))) void k_poisson2(const RunDev R, const PoisTab T_,
+template <typename T, int L, int ISL>
+__global__ __launch_bounds__(CF_P2S_THREADS) __attribute__((amdgpu_waves_per_eu(CF_P2S_WAVES(L), 8))) void k_poisson2(const RunDev R, const PoisTab T_,
                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ G)
 {
-    pois2_body<T, L>(R, T_, Y, G, blk_native());
+    pois2_body<T, L, ISL>(R, T_, Y, G, blk_native());
 }
 
+/* FP64: the one solve; FP32: the storage-precision modes (the islands are
+ * a launch of their own, pick_poisson2_islands) */
 template <typename T>
 typename KSel<T>::pois2_t pick_poisson2(int L)
 {
+    constexpr int I = sizeof(T) == sizeof(double) ? 2 : 0;
     switch (L) {
-#define X_(N) case N: return k_poisson2<T, N>;
+#define X_(N) case N: return k_poisson2<T, N, I>;
     CF_P2S_LSET(X_)
 #undef X_
-    default: return k_poisson2<T, 0>;
+    default: return k_poisson2<T, 0, I>;
     }
 }
 template KSel<double>::pois2_t pick_poisson2<double>(int);
 template KSel<float>::pois2_t pick_poisson2<float>(int);
+
+KSel<float>::pois2_t pick_poisson2_islands(int L)
+{
+    switch (L) {
+#define X_(N) case N: return k_poisson2<float, N, 1>;
+    CF_P2S_LSET(X_)
+#undef X_
+    default: return k_poisson2<float, 0, 1>;
+    }
+}
 
 /* ---------------------------------------------------------------------
  * project2 (pressure.c:120-152) fused with the mass-flux correction:
@@ -1565,23 +1590,34 @@ __global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *__restrict__ a_,
     });
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(256) void kb_poisson(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
+template <typename T, int L, int ISL>
+__global__ __launch_bounds__(CF_P2S_THREADS) void kb_poisson(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
-        pois2_body<T, L>(a.R, a.pt[s], a.X, a.G, blk_of(a, ST_MODE, lb));
+        pois2_body<T, L, ISL>(a.R, a.pt[s], a.X, a.G, blk_of(a, ST_MODE, lb));
     });
 }
 
 template <typename T>
 typename BSel<T>::kb_t pick_batched_poisson(int L)
 {
+    constexpr int I = sizeof(T) == sizeof(double) ? 2 : 0;
     switch (L) {
-#define X_(N) case N: return kb_poisson<T, N>;
+#define X_(N) case N: return kb_poisson<T, N, I>;
     CF_P2S_LSET(X_)
 #undef X_
-    default: return kb_poisson<T, 0>;
+    default: return kb_poisson<T, 0, I>;
+    }
+}
+
+BSel<float>::kb_t pick_batched_poisson_islands(int L)
+{
+    switch (L) {
+#define X_(N) case N: return kb_poisson<float, N, 1>;
+    CF_P2S_LSET(X_)
+#undef X_
+    default: return kb_poisson<float, 0, 1>;
     }
 }
 

@@ -932,8 +932,14 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
     if (r->g.order == 4)
         KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], X, r->S4);
     else
-        KLAUNCH("poisson", pick_poisson2<T>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
-                dim3(256), pois2_lds(pois2_L(r->g.nz), r->g.nz), R, r->pt[tq], X, r->G);
+    {
+        const int L = pois2_L(r->g.nz);
+        KLAUNCH("poisson", pick_poisson2<T>(L), dim3(pois2_blocks(L, r->M)), dim3(CF_P2S_THREADS), pois2_lds(L, r->g.nz),
+                R, r->pt[tq], X, r->G);
+        if constexpr (std::is_same<T, float>::value)   /* the FP64 islands */
+            KLAUNCH("poisson", pick_poisson2_islands(L), dim3(pois2_island_blocks(L, r->g.ny, r->M)), dim3(CF_P2S_THREADS),
+                    pois2_lds(L, r->g.nz), R, r->pt[tq], (typename Cplx<float>::type *) X, r->G);
+    }
     KLAUNCH("ffty_inv", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 0);
     KLAUNCH("fftx_inv", Prec<T>::fftxi(r), dim3(gx), dim3(256), fx_lds(r), R, (const C *) X, tv<T>(r->p));
     return CF_OK;
@@ -1484,7 +1490,7 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
                            r->rd, r->pt4[3], r->X, r->S4);
     else
         hipLaunchKernelGGL(pick_poisson2<double>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
-                           dim3(256), pois2_lds(pois2_L(r->g.nz), r->g.nz), r->stream,
+                           dim3(CF_P2S_THREADS), pois2_lds(pois2_L(r->g.nz), r->g.nz), r->stream,
                            r->rd, r->pt[3], r->X, r->G);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(d.data(), r->X, sizeof(double2) * d.size(), hipMemcpyDeviceToHost, r->stream));
@@ -1521,7 +1527,7 @@ struct cf_batch {
     int n = 0, prec = CF_FP64, order = 2, scalar = 0, device = 0;
     hipStream_t stream = nullptr;
     void *args = nullptr;                 /* device GArgs<T>[n] */
-    unsigned *vstart = nullptr;           /* device [CF_NST + CF_P2_NG + CF_SP_NG][n + 1] */
+    unsigned *vstart = nullptr;           /* device [CF_VS_ROWS][n + 1] */
     std::vector<unsigned> hv;
     int solve_kind = SOLVE_HEPT;
     std::vector<int> p2L;                 /* distinct order-2 Poisson chunk lengths, one launch each */
@@ -1552,6 +1558,8 @@ struct cf_batch {
  * prefix table of group g is row CF_NST + g of vstart. */
 #define CF_P2_NG 20
 #define CF_SP_NG 11                        /* spike launches: rows CF_NST + CF_P2_NG + g */
+#define CF_P2I_ROW0 (CF_NST + CF_P2_NG + CF_SP_NG)  /* FP32 Poisson islands: rows CF_P2I_ROW0 + g */
+#define CF_VS_ROWS (CF_P2I_ROW0 + CF_P2_NG)
 static void batch_spike_groups(cf_batch *b)
 {
     b->spL.clear();
@@ -1622,7 +1630,7 @@ static cf_status batch_upload_tables(cf_batch *b)
     batch_spike_groups(b);
     const int nsg = (int) b->spL.size();
     if (nsg > CF_SP_NG) { cf_set_error("too many spike chunk lengths in one batch"); return CF_ERR_UNSUPPORTED; }
-    const int nrows = CF_NST + CF_P2_NG + CF_SP_NG;
+    const int nrows = CF_VS_ROWS;
     b->hv.assign((size_t) nrows * (b->n + 1), 0u);
     for (int st = 0; st < nrows; st++) {
         unsigned acc = 0;
@@ -1630,6 +1638,13 @@ static cf_status batch_upload_tables(cf_batch *b)
             b->hv[(size_t) st * (b->n + 1) + q] = acc;
             int sd = st;
             bool in = true;
+            if (st >= CF_P2I_ROW0) {              /* FP32 Poisson island group */
+                const int g = st - CF_P2I_ROW0;
+                const cf_run *rq = b->runs[q];
+                if (g < ng && pois2_L(rq->g.nz) == b->p2L[g])
+                    acc += pois2_island_blocks(b->p2L[g], rq->g.ny, rq->M);
+                continue;
+            }
             if (st >= CF_NST + CF_P2_NG) {        /* spike group */
                 const int g = st - CF_NST - CF_P2_NG;
                 sd = ST_SOLVE;
@@ -1729,6 +1744,7 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
         if (b->order == 4) { fn = kb_poisson4<T>; break; }
         {
             /* one launch per chunk-length group, each over its own prefix table */
+            block = dim3(CF_P2S_THREADS);
             cf_status e = bev_begin(b, KB_NAME[k]);
             if (e != CF_OK) return e;
             for (size_t g = 0; g < b->p2L.size(); g++) {
@@ -1743,6 +1759,16 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
                                    (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
                                    (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)), b->n, s);
                 HIPCHK(hipGetLastError());
+                if constexpr (std::is_same<T, float>::value) {    /* the FP64 islands */
+                    const int irow = CF_P2I_ROW0 + (int) g;
+                    const unsigned ti = batch_total(b, irow);
+                    const unsigned gi = b->kgrid[k] ? std::min(b->kgrid[k], ti) : ti;
+                    if (gi == 0) continue;
+                    hipLaunchKernelGGL(pick_batched_poisson_islands(b->p2L[g]), dim3(gi), block, lp, b->stream,
+                                       (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
+                                       (const unsigned *) (b->vstart + (size_t) irow * (b->n + 1)), b->n, s);
+                    HIPCHK(hipGetLastError());
+                }
             }
             return bev_end(b);
         }
@@ -2008,7 +2034,7 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
         b->ncu = prop.multiProcessorCount;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     if (hipMalloc(&b->args, asz * nrun * (b->fast ? 2 : 1)) != hipSuccess
-        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG + CF_SP_NG) * (nrun + 1)) != hipSuccess
+        || hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_VS_ROWS * (nrun + 1)) != hipSuccess
         || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         free_batch(b);
         cf_set_error("batch allocation failed");
@@ -2372,7 +2398,7 @@ static cf_status srv_regroup(cf_server *s, int k)
     for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
     size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
     HIPCHK(hipMalloc(&b->args, asz * b->n * (b->fast ? 2 : 1)));
-    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * (CF_NST + CF_P2_NG + CF_SP_NG) * (b->n + 1)));
+    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_VS_ROWS * (b->n + 1)));
     return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
 }
 

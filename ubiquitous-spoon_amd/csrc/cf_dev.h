@@ -147,6 +147,20 @@ static __device__ __forceinline__ void dft_small(C *a, double sgd)
     }
 }
 
+/* LDS slot of element i of a transform line whose length is a multiple of
+ * 64 (on): within each 64-element block, bits 0-1 and 2-3 of i are XORed
+ * with bits 4-5.  With 16-byte elements the strided writes of the first
+ * two radix-4 passes (stride 4, and blocks of 4 at stride 16) then fall on
+ * 16 distinct bank groups per 16 lanes instead of 4, while contiguous runs
+ * stay conflict-free.  A permutation within each block, so any length
+ * that is a multiple of 64 stays inside its line; other lengths keep the
+ * plain order. */
+static __device__ __forceinline__ int fsw(int i, bool on)
+{
+    return on ? i ^ (((i >> 4) & 3) * 5) : i;
+}
+static __host__ __device__ __forceinline__ bool fsw_on(int n) { return (n & 63) == 0; }
+
 template <int R, int MAXE, typename C>
 static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlines, int p,
                                                  const C *__restrict__ tw, double sg)
@@ -157,6 +171,7 @@ static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlin
     const int T = n / R;
     const int nb = nlines * T;
     const int tstride = n / (p * R);
+    const bool on = fsw_on(n);
     C y[NB][R];
 #pragma unroll
     for (int it = 0; it < NB; it++) {
@@ -167,7 +182,7 @@ static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlin
             const C *src = buf + line * ld;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                C x = src[i + m * T];
+                C x = src[fsw(i + m * T, on)];
                 if (m > 0 && k > 0) {
                     C w = tw[(k * m * tstride) % n];
                     w.y *= (F) sg;
@@ -188,7 +203,7 @@ static __device__ __forceinline__ void fft_stage(C *buf, int n, int ld, int nlin
             C *dstl = buf + line * ld;
             const int d0 = (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) dstl[d0 + q * p] = y[it][q];
+            for (int q = 0; q < R; q++) dstl[fsw(d0 + q * p, on)] = y[it][q];
         }
     }
     __syncthreads();
@@ -241,6 +256,7 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
     constexpr int NB = (MAXE + R - 1) / R;
     constexpr int T = N / R;
     constexpr int TS = N / (P * R);
+    constexpr bool on = (N & 63) == 0;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nb = nlines * T;
     C y[NB][R];
@@ -253,7 +269,7 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
             const C *src = buf + line * ld;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                C x = src[i + m * T];
+                C x = src[fsw(i + m * T, on)];
                 if (m > 0 && P > 1) {
                     C w = tw[k * m * TS];
                     w.y *= (F) sg;
@@ -274,7 +290,7 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
             C *dstl = buf + line * ld;
             const int d0 = (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) dstl[d0 + q * P] = y[it][q];
+            for (int q = 0; q < R; q++) dstl[fsw(d0 + q * P, on)] = y[it][q];
         }
     }
     __syncthreads();

@@ -788,7 +788,8 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
         const int kx = e / nl, l = e - kx * nl;
         const long la = 2 * (pair0 + l), lb = la + 1;
         if (la >= NL) continue;
-        const C Z = sb[l * ld + kx], Wc = sb[l * ld + (kx == 0 ? 0 : nx - kx)];
+        const bool on = fsw_on(nx);
+        const C Z = sb[l * ld + fsw(kx, on)], Wc = sb[l * ld + fsw(kx == 0 ? 0 : nx - kx, on)];
         const long k = la / ny, j = la - k * ny;
         X[((size_t) k * nkx + kx) * ny + j] = cv_of(mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc));
         if (lb < NL) {
@@ -879,7 +880,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
             const int e = e0 + (int) threadIdx.x + u * 256;
             if (e < tot) {
                 const int l = e / nx, i = e - l * nx;
-                sb[l * ld + i] = res[u];
+                sb[l * ld + fsw(i, fsw_on(nx))] = res[u];
             }
         }
     }
@@ -915,7 +916,7 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
         long la = 2 * (pair0 + l), lb = la + 1;
         double va = la < NL ? a[(size_t) la * nx + i] : 0.0;
         double vb = lb < NL ? a[(size_t) lb * nx + i] : 0.0;
-        sb[l * ld + i] = make_double2(va, vb);
+        sb[l * ld + fsw(i, fsw_on(nx))] = make_double2(va, vb);
     }
     __syncthreads();
     fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, R.twx, 1.0);
@@ -956,7 +957,7 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
             const int e = e0 + (int) threadIdx.x + u * 256;
             if (e < tot) {
                 const int t = e / ny, j = e - t * ny;
-                sb[t * ld + j] = v[u];
+                sb[t * ld + fsw(j, fsw_on(ny))] = v[u];
             }
         }
     }
@@ -966,7 +967,7 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
     const T sc = fwd ? (T) (1.0 / (double) ny) : (T) 1;
     for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
         int t = e / ny, j = e - t * ny;
-        C val = sb[t * ld + j];
+        C val = sb[t * ld + fsw(j, fsw_on(ny))];
         base[e] = cv_of(fwd ? cscale(val, sc) : val);
     }
 }
@@ -1024,8 +1025,8 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
                 const int kx = e / nl, l = e - kx * nl;
                 C a = A[u], bq = B[u];
                 if (kx == 0 || (nx % 2 == 0 && kx == half)) { a.y = (T) 0; bq.y = (T) 0; }
-                sb[l * ld + kx] = mkc<C>(a.x - bq.y, a.y + bq.x);
-                if (kx > 0 && kx < nx - kx) sb[l * ld + nx - kx] = mkc<C>(a.x + bq.y, bq.x - a.y);
+                sb[l * ld + fsw(kx, fsw_on(nx))] = mkc<C>(a.x - bq.y, a.y + bq.x);
+                if (kx > 0 && kx < nx - kx) sb[l * ld + fsw(nx - kx, fsw_on(nx))] = mkc<C>(a.x + bq.y, bq.x - a.y);
             }
         }
     }
@@ -1035,7 +1036,7 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long la = 2 * (pair0 + l), lb = la + 1;
-        C z = sb[l * ld + i];
+        C z = sb[l * ld + fsw(i, fsw_on(nx))];
         if (la < NL) p[(size_t) la * nx + i] = z.x;
         if (lb < NL) p[(size_t) lb * nx + i] = z.y;
     }

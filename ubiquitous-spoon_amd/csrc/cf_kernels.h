@@ -27,7 +27,10 @@ enum {
 #define CF_RT_X 32
 #define CF_RT_Y 8
 #ifndef CF_RHS_KC
-#define CF_RHS_KC 24             /* target levels per z-chunk of k_rhs2t */
+#define CF_RHS_KC 16             /* target levels per z-chunk of k_rhs2t (16: 264 us, 24: 270 us, 32: 266 us at 256x193x256) */
+#endif
+#ifndef CF_RHS_KC_BATCH
+#define CF_RHS_KC_BATCH 24       /* ... of kb_rhs2t, whose launches cover many runs */
 #endif
 #define CF_SPIKE_LMAX 32        /* rows per chunk held in registers */
 #define CF_SPIKE_SMAX 8         /* chunks (waves) per workgroup */

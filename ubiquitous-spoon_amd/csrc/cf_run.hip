@@ -442,6 +442,16 @@ static void free_run(cf_run *r)
 
 /* ------------------------------------------------------------ create */
 
+/* balanced z-chunks of the fused RHS for a target length: levels per
+ * chunk and the launch's workgroup count (also its CFL partials) */
+static void rhs_chunks(int nx, int ny, int nz, int target, int *kc, int *blocks)
+{
+    int nchunk = (nz + target - 1) / target;
+    *kc = (nz + nchunk - 1) / nchunk;
+    nchunk = (nz + *kc - 1) / *kc;
+    *blocks = ((nx + CF_RT_X - 1) / CF_RT_X) * ((ny + CF_RT_Y - 1) / CF_RT_Y) * nchunk;
+}
+
 static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out);
 
 extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
@@ -737,12 +747,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     size_t npart = std::max((size_t) r->nb_plane * (2 * nz + 2), (size_t) ((r->P + 63) / 64) * nz);
     if (npart < (size_t) r->nb_red) npart = r->nb_red;
     ok = ok && dalloc(&r->part, npart) && dalloc(&r->lsum, (size_t) nz + 2);
-    {
-        int nchunk = (nz + CF_RHS_KC - 1) / CF_RHS_KC;
-        r->rhs_kc = (nz + nchunk - 1) / nchunk;
-        nchunk = (nz + r->rhs_kc - 1) / r->rhs_kc;
-        r->rhs_blocks = ((nx + CF_RT_X - 1) / CF_RT_X) * ((ny + CF_RT_Y - 1) / CF_RT_Y) * nchunk;
-    }
+    rhs_chunks(nx, ny, nz, CF_RHS_KC, &r->rhs_kc, &r->rhs_blocks);
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
     ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
@@ -1599,8 +1604,12 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         a.dt = r->cfg.dt;
         a.dt_adhoc = r->pt_dt;
         a.nb_plane = r->nb_plane;
-        a.gblocks = b->fast ? r->rhs_blocks : r->gblocks;
-        a.rhs_kc = r->rhs_kc;
+        /* a batch has workgroups to spare: longer RHS z-chunks (fewer halo
+         * re-reads) than a run alone; the chunking does not change results */
+        int bkc = 1, bblocks = 0;
+        rhs_chunks((int) r->g.nx, (int) r->g.ny, (int) r->g.nz, CF_RHS_KC_BATCH, &bkc, &bblocks);
+        a.gblocks = b->fast ? bblocks : r->gblocks;
+        a.rhs_kc = bkc;
         for (int t = 0; t < 4; t++) { a.pt[t] = r->pt[t]; a.pt4[t] = r->pt4[t]; }
         const unsigned nx = r->g.nx, ny = r->g.ny, nz = r->g.nz;
         const unsigned P = (unsigned) r->P, nf = (unsigned) r->nfield, kinds = r->cfg.scalar ? 2u : 1u;
@@ -1615,7 +1624,7 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         set(ST_FFTX, (unsigned) ((npairs + r->rd.fx_pairs - 1) / r->rd.fx_pairs), 1, 1);
         set(ST_FFTY, fy_blocks(r), 1, 1);
         set(ST_MODE, b->order == 2 ? pois2_blocks(pois2_L(r->g.nz), r->M) : (unsigned) ((r->M + 255) / 256), 1, 1);
-        set(ST_RHS2, (unsigned) std::max(r->rhs_blocks, 1), 1, 1);
+        set(ST_RHS2, (unsigned) std::max(bblocks, 1), 1, 1);
         set(ST_LEVEL, nz, 1, 1);
     }
 }

@@ -57,6 +57,9 @@ static const double RK_ZET[3] = { 0.0, -17.0 / 60.0, -5.0 / 12.0 };
 
 static const int LOGCAP = 4096;
 static const int MAXE_LINE = 2048;      /* elements per FFT block (8 per thread x 256) */
+#ifndef CF_FX_ELEMS
+#define CF_FX_ELEMS 2048                /* elements per x-FFT workgroup: fx_pairs = CF_FX_ELEMS / nx */
+#endif
 
 /* ------------------------------------------------------------ run */
 
@@ -822,7 +825,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.twy32 = (const float2 *) (r->tab + o_twy32);
     R.lamx = r->tab + o_lx;
     R.lamy = r->tab + o_ly;
-    R.fx_pairs = MAXE_LINE / nx > 0 ? MAXE_LINE / nx : 1;
+    R.fx_pairs = CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1;
     /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
     R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
     for (int q = 0; q < 4; q++) {

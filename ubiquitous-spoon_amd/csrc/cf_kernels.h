@@ -330,8 +330,21 @@ template <typename T> __global__ void kb_rhs2t(const GArgs<T> *a, const unsigned
 template <typename T> typename BSel<T>::kb_t pick_batched_spike_sums(int L);
 template <typename T> __global__ void kb_level_sum(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
 template <typename T> __global__ void kb_mass_fast(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
-template <typename T> __global__ void kb_ffty(const GArgs<T> *a, const unsigned *vs, int nrun, int fwd);
-template <typename T> __global__ void kb_fftxi(const GArgs<T> *a, const unsigned *vs, int nrun, int s);
+/* batched FFT passes: one launch per group of runs of one length; lengths
+ * in CF_BFFT_SIZES have their plan compiled in, the others share a launch
+ * with the runtime plan (key 0) */
+#define CF_BFFT_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+static inline int bfft_key(int n)
+{
+    switch (n) {
+#define X_(N) case N: return N;
+    CF_BFFT_SIZES(X_)
+#undef X_
+    default: return 0;
+    }
+}
+template <typename T> typename BSel<T>::kb_t pick_batched_ffty(int n);
+template <typename T> typename BSel<T>::kb_t pick_batched_fftxi(int n);
 template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);
 BSel<float>::kb_t pick_batched_poisson_islands(int L);
 template <typename T> __global__ void k_gsolve4(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);

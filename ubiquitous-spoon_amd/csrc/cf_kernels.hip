@@ -1551,7 +1551,10 @@ template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const
  * the per-step bookkeeping over all runs of a batch; FFTs use the runtime
  * plan since the runs' lengths differ. */
 
-template <int ORD, typename T>
+/* NX / NY > 0: a launch over the runs of one FFT length with its plan
+ * compiled in; 0: any lengths, runtime plan.  Where it fits without
+ * spills, the occupancy target caps them at 128 VGPRs (4 waves per SIMD). */
+template <int NX, int ORD, typename T>
 __global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
                                                int nrun, int s)
 {
@@ -1559,36 +1562,61 @@ __global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, c
         const GArgs<T> &a = A[r];
         const Blk b = blk_of(a, ST_FFTX, lb);
         if (s < 3)
-            divx_body<0, ORD, T>(a.R, a.ru, a.rv, a.rw, a.R.h1 + (size_t) (s * 2) * (a.R.g.nz + 2),
-                                 a.scal + SC_M0 + s, a.X, b);
+            divx_body<NX, ORD, T>(a.R, a.ru, a.rv, a.rw, a.R.h1 + (size_t) (s * 2) * (a.R.g.nz + 2),
+                                  a.scal + SC_M0 + s, a.X, b);
         else
-            divx_body<0, ORD, T>(a.R, a.u, a.v, a.w, nullptr, nullptr, a.X, b);
+            divx_body<NX, ORD, T>(a.R, a.u, a.v, a.w, nullptr, nullptr, a.X, b);
     });
 }
 
 template <typename T>
 typename BSel<T>::divx_t_ pick_batched_divx(int order)
 {
-    return order == 4 ? kb_divx<4, T> : kb_divx<2, T>;
+    return order == 4 ? kb_divx<0, 4, T> : kb_divx<0, 2, T>;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void kb_ffty(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int fwd)
+template <int NY, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 0 ? 4 : 1, 8))) void kb_ffty(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int fwd)
 {
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
-        ffty_body<0, T>(a.R, a.X, fwd, blk_of(a, ST_FFTY, lb));
+        ffty_body<NY, T>(a.R, a.X, fwd, blk_of(a, ST_FFTY, lb));
     });
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void kb_fftxi(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
+template <int NX, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && NX <= 128) ? 4 : 1, 8))) void kb_fftxi(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = a_[r];
-        fftxi_body<0, T>(a.R, a.X, a.p, blk_of(a, ST_FFTX, lb));
+        fftxi_body<NX, T>(a.R, a.X, a.p, blk_of(a, ST_FFTX, lb));
     });
+}
+
+/* per-length batched FFT launches (n = bfft_key of the group's length);
+ * the divergence + x-FFT keeps one runtime-plan launch over all runs
+ * (measured as fast or faster: its compiled-length forms need more
+ * registers inside the batch loop) */
+template <typename T>
+typename BSel<T>::kb_t pick_batched_ffty(int n)
+{
+    switch (n) {
+#define X_(N) case N: return kb_ffty<N, T>;
+    CF_BFFT_SIZES(X_)
+#undef X_
+    default: return kb_ffty<0, T>;
+    }
+}
+template <typename T>
+typename BSel<T>::kb_t pick_batched_fftxi(int n)
+{
+    switch (n) {
+#define X_(N) case N: return kb_fftxi<N, T>;
+    CF_BFFT_SIZES(X_)
+#undef X_
+    default: return kb_fftxi<0, T>;
+    }
 }
 
 template <typename T, int L, int ISL>
@@ -1736,8 +1764,8 @@ __global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<T> *__restrict__
 }
 
 #define CF_INST_B(T)                                                                                              \
-    template __global__ void kb_ffty<T>(const GArgs<T> *, const unsigned *, int, int);                            \
-    template __global__ void kb_fftxi<T>(const GArgs<T> *, const unsigned *, int, int);                           \
+    template typename BSel<T>::kb_t pick_batched_ffty<T>(int);                                                     \
+    template typename BSel<T>::kb_t pick_batched_fftxi<T>(int);                                                    \
     template typename BSel<T>::kb_t pick_batched_poisson<T>(int);                                                  \
     template typename BSel<T>::kb_t pick_batched_spike<T>(int);                                                    \
     template __global__ void kb_thomas<T>(const GArgs<T> *, const unsigned *, int, int);                          \

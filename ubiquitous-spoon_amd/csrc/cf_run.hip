@@ -1540,6 +1540,7 @@ struct cf_batch {
     int solve_kind = SOLVE_HEPT;
     std::vector<int> p2L;                 /* distinct order-2 Poisson chunk lengths, one launch each */
     std::vector<int> spL, spW;            /* distinct spike chunk lengths (one launch each) and their waves */
+    std::vector<int> fxK, fyK;            /* distinct x / y FFT plan keys (bfft_key), one launch each */
     unsigned kgrid[KB_N] = {0};
     int spike_waves = 1, ncu = 256;
     size_t lds_fx = 0, lds_fy = 0, lds_mass = 0;
@@ -1567,7 +1568,10 @@ struct cf_batch {
 #define CF_P2_NG 20
 #define CF_SP_NG 11                        /* spike launches: rows CF_NST + CF_P2_NG + g */
 #define CF_P2I_ROW0 (CF_NST + CF_P2_NG + CF_SP_NG)  /* FP32 Poisson islands: rows CF_P2I_ROW0 + g */
-#define CF_VS_ROWS (CF_P2I_ROW0 + CF_P2_NG)
+#define CF_FFT_NG 8                        /* FFT length groups: bfft_key values (7 sizes and 0) */
+#define CF_FX_ROW0 (CF_P2I_ROW0 + CF_P2_NG)      /* x-FFT groups (by nx): rows CF_FX_ROW0 + g */
+#define CF_FY_ROW0 (CF_FX_ROW0 + CF_FFT_NG)      /* y-FFT groups (by ny) */
+#define CF_VS_ROWS (CF_FY_ROW0 + CF_FFT_NG)
 static void batch_spike_groups(cf_batch *b)
 {
     b->spL.clear();
@@ -1587,6 +1591,17 @@ static std::vector<int> batch_p2L(const std::vector<cf_run *> &runs)
     }
     std::sort(ls.begin(), ls.end());
     return ls;
+}
+
+static void batch_fft_groups(cf_batch *b)
+{
+    b->fxK.clear();
+    b->fyK.clear();
+    for (const cf_run *r : b->runs) {
+        const int kx = bfft_key((int) r->g.nx), ky = bfft_key((int) r->g.ny);
+        if (std::find(b->fxK.begin(), b->fxK.end(), kx) == b->fxK.end()) b->fxK.push_back(kx);
+        if (std::find(b->fyK.begin(), b->fyK.end(), ky) == b->fyK.end()) b->fyK.push_back(ky);
+    }
 }
 
 template <typename T>
@@ -1642,6 +1657,7 @@ static cf_status batch_upload_tables(cf_batch *b)
     batch_spike_groups(b);
     const int nsg = (int) b->spL.size();
     if (nsg > CF_SP_NG) { cf_set_error("too many spike chunk lengths in one batch"); return CF_ERR_UNSUPPORTED; }
+    batch_fft_groups(b);                  /* at most CF_FFT_NG keys each, by construction */
     const int nrows = CF_VS_ROWS;
     b->hv.assign((size_t) nrows * (b->n + 1), 0u);
     for (int st = 0; st < nrows; st++) {
@@ -1650,6 +1666,17 @@ static cf_status batch_upload_tables(cf_batch *b)
             b->hv[(size_t) st * (b->n + 1) + q] = acc;
             int sd = st;
             bool in = true;
+            if (st >= CF_FX_ROW0) {               /* FFT length group */
+                const bool y = st >= CF_FY_ROW0;
+                const int g = st - (y ? CF_FY_ROW0 : CF_FX_ROW0);
+                const std::vector<int> &K = y ? b->fyK : b->fxK;
+                const cf_run *rq = b->runs[q];
+                if (g < (int) K.size() && bfft_key((int) (y ? rq->g.ny : rq->g.nx)) == K[g]) {
+                    const int sd2 = y ? ST_FFTY : ST_FFTX;
+                    acc += A[q].dims[sd2][0] * A[q].dims[sd2][1] * A[q].dims[sd2][2];
+                }
+                continue;
+            }
             if (st >= CF_P2I_ROW0) {              /* FP32 Poisson island group */
                 const int g = st - CF_P2I_ROW0;
                 const cf_run *rq = b->runs[q];
@@ -1728,6 +1755,31 @@ static cf_status batch_launch_spike(cf_batch *b, int k, int s, bool sums)
     return bev_end(b);
 }
 
+/* the y-FFTs and the inverse x-FFT of a batch: one launch per length
+ * group, each over its own prefix table, with the group's plan and LDS */
+template <typename T>
+static cf_status batch_launch_fft(cf_batch *b, int k, int s)
+{
+    cf_status e = bev_begin(b, KB_NAME[k]);
+    if (e != CF_OK) return e;
+    const bool y = k == KB_FFTY;
+    const std::vector<int> &K = y ? b->fyK : b->fxK;
+    for (size_t g = 0; g < K.size(); g++) {
+        typename BSel<T>::kb_t fn = y ? pick_batched_ffty<T>(K[g]) : pick_batched_fftxi<T>(K[g]);
+        size_t lds = 0;
+        for (const cf_run *r : b->runs)
+            if (bfft_key((int) (y ? r->g.ny : r->g.nx)) == K[g]) lds = std::max(lds, y ? fy_lds(r) : fx_lds(r));
+        const int row = (y ? CF_FY_ROW0 : CF_FX_ROW0) + (int) g;
+        const unsigned tg = batch_total(b, row);
+        const unsigned grid = b->kgrid[k] ? std::min(b->kgrid[k], tg) : tg;
+        if (grid == 0) continue;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, b->stream, (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
+                           (const unsigned *) (b->vstart + (size_t) row * (b->n + 1)), b->n, s);
+        HIPCHK(hipGetLastError());
+    }
+    return bev_end(b);
+}
+
 template <typename T>
 static cf_status batch_launch(cf_batch *b, int k, int s)
 {
@@ -1751,7 +1803,9 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     case KB_MASS: fn = kb_mass<T>; lds = b->lds_mass; break;
     case KB_CFL: fn = kb_cfl<T>; break;
     case KB_DIVX: fn = S.divx; lds = b->lds_fx; break;
-    case KB_FFTY: fn = kb_ffty<T>; lds = b->lds_fy; break;
+    case KB_FFTY:
+    case KB_FFTXI:
+        return batch_launch_fft<T>(b, k, s);
     case KB_POIS:
         if (b->order == 4) { fn = kb_poisson4<T>; break; }
         {
@@ -1784,7 +1838,6 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
             }
             return bev_end(b);
         }
-    case KB_FFTXI: fn = kb_fftxi<T>; lds = b->lds_fx; break;
     case KB_STEPEND: fn = kb_step_end<T>; break;
     case KB_FRHS: fn = kb_rhs2t<T>; break;
     case KB_FSPIKE:

@@ -57,6 +57,9 @@ static const double RK_ZET[3] = { 0.0, -17.0 / 60.0, -5.0 / 12.0 };
 
 static const int LOGCAP = 4096;
 static const int MAXE_LINE = 2048;      /* elements per FFT block (8 per thread x 256) */
+#ifndef CF_P4_THREADS
+#define CF_P4_THREADS 64                /* order-4 Poisson: modes (threads) per workgroup of a single run */
+#endif
 #ifndef CF_FX_ELEMS
 #define CF_FX_ELEMS 2048                /* elements per x-FFT workgroup: fx_pairs = CF_FX_ELEMS / nx */
 #endif
@@ -938,7 +941,8 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
     dim3 gy(fy_blocks(r));
     KLAUNCH("ffty_fwd", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 1);
     if (r->g.order == 4)
-        KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, R, r->pt4[tq], X, r->S4);
+        KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                dim3(CF_P4_THREADS), 0, R, r->pt4[tq], X, r->S4);
     else
     {
         const int L = pois2_L(r->g.nz);
@@ -1494,7 +1498,8 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
     spec_host_to_dev(r, spec, d);
     HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
     if (r->g.order == 4)
-        hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + 255) / 256)), dim3(256), 0, r->stream,
+        hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                           dim3(CF_P4_THREADS), 0, r->stream,
                            r->rd, r->pt4[3], r->X, r->S4);
     else
         hipLaunchKernelGGL(pick_poisson2<double>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),

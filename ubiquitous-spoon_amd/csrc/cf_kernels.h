@@ -343,6 +343,7 @@ static inline int bfft_key(int n)
     default: return 0;
     }
 }
+template <typename T> typename BSel<T>::kb_t pick_batched_divx_n(int order, int n);
 template <typename T> typename BSel<T>::kb_t pick_batched_ffty(int n);
 template <typename T> typename BSel<T>::kb_t pick_batched_fftxi(int n);
 template <typename T> typename BSel<T>::kb_t pick_batched_poisson(int L);

@@ -803,7 +803,7 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
  * velocity (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x
  * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
  * mirrored oddly about the (zero) wall faces, as divergence4 sets them. */
-template <int NX, int ORD, typename T>
+template <int NX, int ORD, typename T, int UNRX = 0>
 static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__restrict__ us_,
     const T *__restrict__ vs_, const T *__restrict__ ws_, const double *__restrict__ h1_,
     const double *__restrict__ mptr_, typename Cplx<T>::type *__restrict__ X_, const Blk b)
@@ -860,7 +860,8 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     };
     /* groups of CF_FX_UNR elements per thread: all reads of a group are
      * in flight before the first LDS write (blockDim.x = 256) */
-    constexpr int UNR = ORD == 2 ? (sizeof(T) == 4 ? CF_FX_UNR : 2) : (sizeof(T) == 4 ? 2 : 1);   /* fit 128 VGPRs */
+    constexpr int UNR = UNRX > 0 ? UNRX
+                      : (ORD == 2 ? (sizeof(T) == 4 ? CF_FX_UNR : 2) : (sizeof(T) == 4 ? 2 : 1));   /* fit 128 VGPRs */
     const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
     const C tw0 = tw_first<C>(twg, nx);
@@ -1554,18 +1555,21 @@ template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const
 /* NX / NY > 0: a launch over the runs of one FFT length with its plan
  * compiled in; 0: any lengths, runtime plan.  Where it fits without
  * spills, the occupancy target caps them at 128 VGPRs (4 waves per SIMD). */
+/* compiled lengths gather 2 elements per thread in FP32 (not 3) so that
+ * the batch loop's extra registers still fit 4 waves per SIMD */
 template <int NX, int ORD, typename T>
-__global__ __launch_bounds__(256) void kb_divx(const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs,
-                                               int nrun, int s)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && ORD == 2) ? 4 : 1, 8))) void kb_divx(
+    const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs, int nrun, int s)
 {
+    constexpr int U = (NX > 0 && ORD == 2 && sizeof(T) == 4) ? 2 : 0;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {
         const GArgs<T> &a = A[r];
         const Blk b = blk_of(a, ST_FFTX, lb);
         if (s < 3)
-            divx_body<NX, ORD, T>(a.R, a.ru, a.rv, a.rw, a.R.h1 + (size_t) (s * 2) * (a.R.g.nz + 2),
-                                  a.scal + SC_M0 + s, a.X, b);
+            divx_body<NX, ORD, T, U>(a.R, a.ru, a.rv, a.rw, a.R.h1 + (size_t) (s * 2) * (a.R.g.nz + 2),
+                                     a.scal + SC_M0 + s, a.X, b);
         else
-            divx_body<NX, ORD, T>(a.R, a.u, a.v, a.w, nullptr, nullptr, a.X, b);
+            divx_body<NX, ORD, T, U>(a.R, a.u, a.v, a.w, nullptr, nullptr, a.X, b);
     });
 }
 
@@ -1594,10 +1598,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && 
     });
 }
 
-/* per-length batched FFT launches (n = bfft_key of the group's length);
- * the divergence + x-FFT keeps one runtime-plan launch over all runs
- * (measured as fast or faster: its compiled-length forms need more
- * registers inside the batch loop) */
+/* per-length batched FFT launches (n = bfft_key of the group's length).
+ * The divergence + x-FFT has its length compiled in only where that fits
+ * 4 waves per SIMD without spills inside the batch loop (order 2, n <= 128
+ * in FP32, n <= 64 in FP64); other groups take the runtime plan. */
+template <typename T>
+typename BSel<T>::kb_t pick_batched_divx_n(int order, int n)
+{
+    constexpr int lim = sizeof(T) == 4 ? 128 : 64;
+    if (order == 2) {
+        switch (n) {
+        case 32: return kb_divx<32, 2, T>;
+        case 48: return kb_divx<48, 2, T>;
+        case 64: return kb_divx<64, 2, T>;
+        case 96: if constexpr (lim >= 96) return kb_divx<96, 2, T>; break;
+        case 128: if constexpr (lim >= 128) return kb_divx<128, 2, T>; break;
+        default: break;
+        }
+    }
+    return pick_batched_divx<T>(order);
+}
 template <typename T>
 typename BSel<T>::kb_t pick_batched_ffty(int n)
 {
@@ -1764,6 +1784,7 @@ __global__ __launch_bounds__(256) void kb_mass_fast(const GArgs<T> *__restrict__
 }
 
 #define CF_INST_B(T)                                                                                              \
+    template typename BSel<T>::kb_t pick_batched_divx_n<T>(int, int);                                             \
     template typename BSel<T>::kb_t pick_batched_ffty<T>(int);                                                     \
     template typename BSel<T>::kb_t pick_batched_fftxi<T>(int);                                                    \
     template typename BSel<T>::kb_t pick_batched_poisson<T>(int);                                                  \

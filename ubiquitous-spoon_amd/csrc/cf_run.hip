@@ -1760,8 +1760,8 @@ static cf_status batch_launch_spike(cf_batch *b, int k, int s, bool sums)
     return bev_end(b);
 }
 
-/* the y-FFTs and the inverse x-FFT of a batch: one launch per length
- * group, each over its own prefix table, with the group's plan and LDS */
+/* the FFT passes of a batch: one launch per length group, each over its
+ * own prefix table, with the group's plan and LDS */
 template <typename T>
 static cf_status batch_launch_fft(cf_batch *b, int k, int s)
 {
@@ -1770,7 +1770,8 @@ static cf_status batch_launch_fft(cf_batch *b, int k, int s)
     const bool y = k == KB_FFTY;
     const std::vector<int> &K = y ? b->fyK : b->fxK;
     for (size_t g = 0; g < K.size(); g++) {
-        typename BSel<T>::kb_t fn = y ? pick_batched_ffty<T>(K[g]) : pick_batched_fftxi<T>(K[g]);
+        typename BSel<T>::kb_t fn = y ? pick_batched_ffty<T>(K[g])
+                                  : (k == KB_DIVX ? pick_batched_divx_n<T>(b->order, K[g]) : pick_batched_fftxi<T>(K[g]));
         size_t lds = 0;
         for (const cf_run *r : b->runs)
             if (bfft_key((int) (y ? r->g.ny : r->g.nx)) == K[g]) lds = std::max(lds, y ? fy_lds(r) : fx_lds(r));
@@ -1807,7 +1808,11 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     case KB_PLANE: fn = kb_plane<T>; break;
     case KB_MASS: fn = kb_mass<T>; lds = b->lds_mass; break;
     case KB_CFL: fn = kb_cfl<T>; break;
-    case KB_DIVX: fn = S.divx; lds = b->lds_fx; break;
+    case KB_DIVX:
+        /* FP64: one runtime-plan launch over all runs (its compiled forms
+         * spill beyond length 64, and split launches measured slower) */
+        if constexpr (std::is_same<T, double>::value) { fn = S.divx; lds = b->lds_fx; break; }
+        else return batch_launch_fft<T>(b, k, s);
     case KB_FFTY:
     case KB_FFTXI:
         return batch_launch_fft<T>(b, k, s);

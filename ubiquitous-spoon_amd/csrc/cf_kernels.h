@@ -143,6 +143,14 @@ struct PoisTab {
 static inline int spike_L(int nz)
 {
     static const int set[] = { 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32 };
+    /* six chunks where a compiled length gives exactly six (measured faster
+     * than seven at nz = 65 and 129; nz = 97 would round to five, slower) */
+    const int need6 = (nz + 5) / 6;
+    for (int l : set)
+        if (l >= need6) {
+            if ((nz + l - 1) / l == 6) return l;
+            break;
+        }
     const int need = (nz + CF_SPIKE_SMAX - 1) / CF_SPIKE_SMAX;
     for (int l : set)
         if (l >= need) return l;

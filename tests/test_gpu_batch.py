@@ -102,11 +102,10 @@ def test_same_alone_and_in_the_mix(prec):
 def test_fp32_batch_matches_fp32_single_runs():
     """FP32 order-2 runs of a batch take the batched fast path; each matches
     the same run stepped alone on the single-run FP32 fast path to 1e-5
-    relative L2 after 15 steps.  Not bit for bit: a single run's FFT passes
-    are instantiated per length (radix plan folded at compile time) and a
-    batch's take the length at run time, and the two round differently in
-    FP32 (about 1e-6 after 15 steps; the FP64 instantiations agree to
-    1e-12)."""
+    relative L2 after 15 steps.  Not bit for bit in general: the two may use
+    different FFT instantiations (a batch compiles fewer lengths in; nx = 16
+    here takes the runtime plan), which round differently in FP32 (about
+    1e-6 after 15 steps; the FP64 instantiations agree to 1e-12)."""
     cfgs = [cf.Config(nx=32, ny=24, nz=33, stretch="tanh", nu=1.0 / 2800.0, dt=1e-3, precision=cf.CF_FP32),
             cf.Config(nx=48, ny=32, nz=49, stretch="tanh", nu=1.0 / 2000.0, dt=1e-3, precision=cf.CF_FP32),
             cf.Config(nx=16, ny=12, nz=17, stretch="tanh", nu=1.0 / 4000.0, dt=1e-3, precision=cf.CF_FP32)]

@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# ORACLE_LIB_PATH selects another build of the same sources (e.g. one with a
+# deliberately planted mistake, to check that the pinning tests catch it)
+ORACLE_LIB = os.environ.get("ORACLE_LIB_PATH") or os.path.join(ORACLE_DIR, "liboracle.so")
 LOG_N = 13
 
 
@@ -88,8 +90,13 @@ def lib():
             "orc_fft3d_fwd": [dp, dp, G],
             "orc_fft3d_bwd": [dp, dp, G],
             "orc_xy_avg": [dp, dp, G],
+            "orc_flux_quick": [dp, dp, dp, dp, dp, dp, dp, G],
+            "orc_advect_scalar": [dp, dp, dp, dp, G],
+            "orc_add_buoyancy": [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, G, C.c_int],
         }.items():
             getattr(L, name).argtypes = args
+        L.orc_cfl.argtypes = [dp, dp, dp, G, C.c_double]
+        L.orc_cfl.restype = C.c_double
         L.orc_z_avg.argtypes = [dp, C.c_int, C.c_int, dp]
         L.orc_z_avg.restype = C.c_double
         _L = L

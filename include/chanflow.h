@@ -148,6 +148,25 @@ cf_status cf_run_footprint(const cf_config *cfg, size_t *bytes);
 cf_status cf_run_set_profiling(cf_run *run, int on);
 cf_status cf_run_kernel_time(cf_run *run, const char *kernel, double *avg_ms, int *launches);
 
+/* ---- plane statistics of the current state (the istat block of
+ *      chanfast.c:658-861) ---- */
+
+enum { CF_STAT_AVG = 0, CF_STAT_CORR = 1, CF_STAT_MOMENT = 2, CF_STAT_SPEC = 3 };
+
+/* Statistics over the x-y plane at each of the nz interior levels of the
+ * fields CF_U, CF_V, CF_W, CF_C (scalar runs) or CF_P:
+ *   CF_STAT_AVG     get_xy_avg (get.c:281-307) of field_a: out[nz]
+ *   CF_STAT_CORR    get_xy_corr (get.c:136-174), mean of the product of the
+ *                   deviations of field_a and field_b: out[nz]
+ *   CF_STAT_MOMENT  get_xy_moment (get.c:100-133), mean of
+ *                   (field_a - mean)^en: out[nz]
+ *   CF_STAT_SPEC    get_xy_spec (get.c:5-97) of the FFT3D spectra (1/nx,
+ *                   1/ny scaled) of field_a and field_b: out = specx
+ *                   [nx/2+1][nz][re, im], then specy [ny/2+1][nz][re, im]
+ * The reference's profiles also list its ghost levels, which hold solver
+ * work values rather than flow statistics; those are not computed. */
+cf_status cf_run_xy_stats(cf_run *run, int kind, int field_a, int field_b, double en, double *out);
+
 /* ---- operator-level entry points (parity tests of single stages) ---- */
 
 /* Layout transpose io [nx][ny][nz] <-> device [nz][ny][nx] through the
@@ -197,16 +216,22 @@ cf_status cf_batch_tuning(cf_batch *batch, char *buf, size_t len);
 /* ---- serving: continuous batching of independent runs ----
  *
  * No counterpart in the reference (one run per process).  Requests are
- * queued by cf_server_submit; each cf_server_tick admits queued requests in
- * arrival order while their footprint (cf_run_footprint) fits the HBM
- * budget, projects them once (dt = 1, as cf_run_project), steps every group
- * of active runs sharing order / scalar / precision in batched launches
- * (groups on separate streams), steps each run of at least the solo size
- * (cf_server_set_solo_points, default 4 Mi cells) through its own
- * single-run graph on its own stream, all in flight together, and retires
- * runs that reached their step count.  Times are seconds since
- * cf_server_create.  A run's result does not depend on which other runs
- * share its ticks. */
+ * queued by cf_server_submit.  Device work goes out in quanta of a few
+ * steps: one per group of active runs sharing order / scalar / precision
+ * (at most CF_BATCH_MAX = 256 runs, batched launches on the group's own
+ * stream) and one per run of at least the solo size
+ * (cf_server_set_solo_points, default 4 Mi cells; its own single-run graph
+ * on its own stream).  Each cf_server_tick takes in the quanta that have
+ * completed (crediting steps, retiring runs that reached their count),
+ * admits queued requests in arrival order while their footprint
+ * (cf_run_footprint) fits the HBM budget and projects them once (dt = 1,
+ * as cf_run_project), and enqueues the next quantum of every idle group and
+ * solo run; if nothing moved it first waits for one quantum to complete.
+ * Groups never wait for each other.  A request whose group is full waits
+ * while later ones pass; a request that cannot be set up is marked failed
+ * (cf_server_request_state) and dropped, and the server goes on.  Times
+ * are seconds since cf_server_create.  A run's result does not depend on
+ * which other runs share the device. */
 typedef struct cf_server cf_server;
 
 cf_status cf_server_create(size_t hbm_budget_bytes, int device, int quantum_steps, cf_server **out);
@@ -224,6 +249,9 @@ cf_status cf_server_log(cf_server *server, int request_id, double *out, int max_
 cf_status cf_server_release(cf_server *server, int request_id);
 cf_status cf_server_request_info(cf_server *server, int request_id, double *t_submit, double *t_admit,
                                  double *t_done, int *steps_done);
+/* 0 queued, 1 active, 2 done (retained), 3 done (released), 4 failed;
+ * msg (may be NULL) receives why a failed request failed */
+cf_status cf_server_request_state(cf_server *server, int request_id, int *state, char *msg, size_t len);
 cf_status cf_server_stats(cf_server *server, double *seconds, double *cell_steps, size_t *hbm_in_use);
 
 #ifdef __cplusplus

@@ -148,6 +148,11 @@ void orc_poisson(double *cout, const orc_grid *g, int order, double dt);
 void orc_fft3d_fwd(const double *a, double *cout, const orc_grid *g);
 void orc_fft3d_bwd(const double *cout, double *a, const orc_grid *g);
 void orc_xy_avg(const double *u, double *avg, const orc_grid *g);        /* get.c:281-307 */
+void orc_xy_corr(const double *u, const double *v, double *corr, const orc_grid *g); /* get.c:136-174 */
+void orc_xy_moment(const double *u, double *mom, double en, const orc_grid *g);     /* get.c:100-133 */
+/* get.c:5-97; uhat, vhat [(nx/2+1)][ny][n2][2]; specx [(nx/2+1)][n2][2], specy [(ny/2+1)][n2][2] */
+void orc_xy_spec(const double *uhat, const double *vhat, double *specx, double *specy,
+                 int nx, int ny, int n2);
 double orc_cfl(const double *u, const double *v, const double *w,
                const orc_grid *g, double dt);                            /* get.c:390-417 */
 

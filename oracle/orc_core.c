@@ -1420,6 +1420,79 @@ void orc_xy_avg(const double *u, double *avg, const orc_grid *g)
     for (int k = 0; k < n2; k++) avg[k] *= scale;
 }
 
+void orc_xy_corr(const double *u, const double *v, double *corr, const orc_grid *g)
+{
+    /* get_xy_corr, get.c:136-174: per level (all n2), the plane mean of the
+     * product of the deviations from the plane means */
+    int ng = g->ng, n2 = g->n2;
+    double *ua = (double *) malloc(sizeof(double) * (size_t) n2);
+    double *va = (double *) malloc(sizeof(double) * (size_t) n2);
+    if (!ua || !va) { free(ua); free(va); return; }
+    orc_xy_avg(u, ua, g);
+    orc_xy_avg(v, va, g);
+    for (int k = 0; k < n2; k++) corr[k] = 0.0;
+    for (int i = ng; i < g->n0 - ng; i++)
+        for (int j = ng; j < g->n1 - ng; j++)
+            for (int k = 0; k < n2; k++) {
+                size_t o = ID(g, i, j, k);
+                corr[k] += (u[o] - ua[k]) * (v[o] - va[k]);
+            }
+    double scale = 1.0 / ((double) g->nx * (double) g->ny);
+    for (int k = 0; k < n2; k++) corr[k] *= scale;
+    free(ua);
+    free(va);
+}
+
+void orc_xy_moment(const double *u, double *mom, double en, const orc_grid *g)
+{
+    /* get_xy_moment, get.c:100-133: plane mean of (u - mean)^en per level */
+    int ng = g->ng, n2 = g->n2;
+    double *ua = (double *) malloc(sizeof(double) * (size_t) n2);
+    if (!ua) return;
+    orc_xy_avg(u, ua, g);
+    for (int k = 0; k < n2; k++) mom[k] = 0.0;
+    for (int i = ng; i < g->n0 - ng; i++)
+        for (int j = ng; j < g->n1 - ng; j++)
+            for (int k = 0; k < n2; k++) mom[k] += pow(u[ID(g, i, j, k)] - ua[k], en);
+    double scale = 1.0 / ((double) g->nx * (double) g->ny);
+    for (int k = 0; k < n2; k++) mom[k] *= scale;
+    free(ua);
+}
+
+void orc_xy_spec(const double *uhat, const double *vhat, double *specx, double *specy,
+                 int nx, int ny, int n2)
+{
+    /* get_xy_spec, get.c:5-97 on one rank: the cross-spectrum
+     * conj(uhat) vhat ... as re = Re(u)Re(v) + Im(u)Im(v), im = Im(u)Re(v) -
+     * Re(u)Im(v); specx[ix][k] sums it over ky (the (0,0) mode left out),
+     * specy[iy][k] over kx for 0 <= iy <= ny/2, folding the modes above ny/2
+     * onto ny - iy with the imaginary part negated (the kx = 0 column only
+     * for 0 < iy <= ny/2, the kx = nx/2 column of an even nx once) */
+    int nkx = nx / 2 + 1;
+    for (int q = 0; q < nkx * n2 * 2; q++) specx[q] = 0.0;
+    for (int q = 0; q < (ny / 2 + 1) * n2 * 2; q++) specy[q] = 0.0;
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int k = 0; k < n2; k++) {
+                size_t o = ((size_t) (ix * ny + iy) * n2 + k) * 2;
+                double ru = uhat[o], iu = uhat[o + 1], rv = vhat[o], iv = vhat[o + 1];
+                double re = ru * rv + iu * iv;
+                double im = iu * rv - ru * iv;
+                int niy = (ny - iy) % ny;
+                size_t xo = ((size_t) ix * n2 + k) * 2, yo = ((size_t) iy * n2 + k) * 2;
+                size_t no = ((size_t) niy * n2 + k) * 2;
+                if (!(ix == 0 && iy == 0)) { specx[xo] += re; specx[xo + 1] += im; }
+                if (ix == 0) {
+                    if (0 < iy && iy <= ny / 2) { specy[yo] += re; specy[yo + 1] += im; }
+                } else if (ix == nx / 2 && nx % 2 == 0) {
+                    if (iy <= ny / 2) { specy[yo] += re; specy[yo + 1] += im; }
+                } else {
+                    if (iy <= ny / 2) { specy[yo] += re; specy[yo + 1] += im; }
+                    if (niy <= ny / 2) { specy[no] += re; specy[no + 1] += -im; }
+                }
+            }
+}
+
 double orc_cfl(const double *u, const double *v, const double *w, const orc_grid *g, double dt)
 {
     /* get_cfl, get.c:390-417 */

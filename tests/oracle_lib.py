@@ -92,6 +92,9 @@ def lib():
             "orc_xy_avg": [dp, dp, G],
             "orc_flux_quick": [dp, dp, dp, dp, dp, dp, dp, G],
             "orc_advect_scalar": [dp, dp, dp, dp, G],
+            "orc_xy_corr": [dp, dp, dp, G],
+            "orc_xy_moment": [dp, dp, C.c_double, G],
+            "orc_xy_spec": [dp, dp, dp, dp, C.c_int, C.c_int, C.c_int],
             "orc_add_buoyancy": [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, G, C.c_int],
         }.items():
             getattr(L, name).argtypes = args

@@ -97,8 +97,10 @@ def test_projection_matches_oracle():
         assert run.max_divergence() <= 1e-12
 
 
-@pytest.mark.parametrize("shape,steps", [((16, 12, 17), 3), ((32, 24, 33), 10)])
+@pytest.mark.parametrize("shape,steps", [((16, 12, 17), 3), ((32, 24, 33), 10), ((32, 24, 129), 5)])
 def test_step_parity_small(shape, steps):
+    """Includes nz = 129, whose z solve runs as six spike chunks of 24 rows
+    with a short last chunk."""
     cfg, dz, orc, init, f = projected_case(*shape)
     olog = orc.steps(steps)
     with cf.Run(cfg) as run:

@@ -648,3 +648,64 @@ def test_pstat_log_on_poiseuille_profile(order):
     if order == 2:
         assert min(errw[0] / errw[1], errw[1] / errw[2]) >= 1.8
         assert errw[-1] <= 0.01 * 3 * nu
+
+
+# ------------------------------------------------------------ statistics
+
+def stat_fields(nx=12, ny=10, nz=7, seed=51):
+    g = grid(nx=nx, ny=ny, nz=nz)
+    rng = np.random.default_rng(seed)
+    u, v = rng.uniform(size=g.shape), rng.uniform(size=g.shape)
+    for a in (u, v):
+        L.orc_periodic(dp(a), g.ref)
+    return g, u, v
+
+
+def test_moments_and_correlation_case15():
+    """case15 (cases.c:4293-4424): the first central moment vanishes and the
+    second equals the autocorrelation, per level (<= 1e-13); both also equal
+    numpy's plane statistics over the interior points."""
+    g, u, v = stat_fields()
+    n2 = g.g.n2
+    corr, m1, m2, m3, m4, cuv = (np.zeros(n2) for _ in range(6))
+    L.orc_xy_corr(dp(u), dp(u), dp(corr), g.ref)
+    L.orc_xy_corr(dp(u), dp(v), dp(cuv), g.ref)
+    for en, m in ((1.0, m1), (2.0, m2), (3.0, m3), (4.0, m4)):
+        L.orc_xy_moment(dp(u), dp(m), en, g.ref)
+    assert np.abs(m1).max() < 1e-13 and np.abs(m2 - corr).max() < 1e-13
+    I = u[3:-3, 3:-3, :]
+    J = v[3:-3, 3:-3, :]
+    du, dv = I - I.mean(axis=(0, 1)), J - J.mean(axis=(0, 1))
+    assert np.allclose(cuv, (du * dv).mean(axis=(0, 1)), rtol=0, atol=1e-15)
+    assert np.allclose(m3, (du ** 3).mean(axis=(0, 1)), rtol=0, atol=1e-15)
+    assert np.allclose(m4, (du ** 4).mean(axis=(0, 1)), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("nx,ny", [(12, 10), (9, 10), (12, 7)])
+def test_spectra_parseval_case16(nx, ny):
+    """case16 (cases.c:4427-4732): the one-dimensional cross-spectra sum to
+    the cross-correlation at every level -- the cospectrum counted twice
+    except at kx = 0 and the Nyquist kx of an even length, the quadrature
+    spectrum summing to zero (<= 1e-15 relative)."""
+    g, u, v = stat_fields(nx=nx, ny=ny, nz=5, seed=52)
+    n2 = g.g.n2
+
+    def fft3d(a):   # FFT3D_FWD scaling (fft.c:161-202): 1/nx then 1/ny
+        s = np.fft.fft(np.fft.rfft(a[3:-3, 3:-3, :], axis=0) / nx, axis=1) / ny
+        return np.ascontiguousarray(np.stack([s.real, s.imag], -1))
+    uh, vh = fft3d(u), fft3d(v)
+    sx = np.zeros((nx // 2 + 1, n2, 2))
+    sy = np.zeros((ny // 2 + 1, n2, 2))
+    L.orc_xy_spec(dp(uh), dp(vh), dp(sx), dp(sy), nx, ny, n2)
+    corr = np.zeros(n2)
+    L.orc_xy_corr(dp(u), dp(v), dp(corr), g.ref)
+
+    def total(spec, n):
+        w = np.full(spec.shape[0], 2.0)
+        w[0] = 1.0
+        if n % 2 == 0:
+            w[-1] = 1.0
+        return (w[:, None] * spec[:, :, 0]).sum(0)
+    scale = np.abs(corr).max()
+    assert np.abs(total(sx, nx) - corr).max() <= 1e-14 * scale
+    assert np.abs(total(sy, ny) - corr).max() <= 1e-14 * scale

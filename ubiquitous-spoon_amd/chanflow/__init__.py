@@ -138,6 +138,7 @@ def lib():
         "cf_run_footprint": (C.c_int, [C.POINTER(_CConfig), C.POINTER(C.c_size_t)]),
         "cf_run_set_profiling": (C.c_int, [P, C.c_int]),
         "cf_run_kernel_time": (C.c_int, [P, C.c_char_p, dp, C.POINTER(C.c_int)]),
+        "cf_run_xy_stats": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, dp]),
         "cf_op_transpose_roundtrip": (C.c_int, [P, dp, dp, dp]),
         "cf_op_fft3d_fwd": (C.c_int, [P, dp, dp]),
         "cf_op_fft3d_bwd": (C.c_int, [P, dp, dp]),
@@ -282,6 +283,27 @@ class Run:
         n = C.c_int(0)
         _check(lib().cf_run_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    STATS = {"avg": 0, "corr": 1, "moment": 2, "spec": 3}
+
+    def xy_stats(self, kind, a, b=None, en=0.0):
+        """Plane statistics of the current state at the interior levels
+        (cf_run_xy_stats): "avg" / "moment" (power en) of field a, "corr" of a
+        and b -> array [nz]; "spec" of a and b -> (specx [nx/2+1][nz],
+        specy [ny/2+1][nz]) complex."""
+        k = self.STATS[kind]
+        fa, fb = FIELDS[a], FIELDS[b if b is not None else a]
+        nz = self.cfg.nz
+        if k == 3:
+            nsx, nsy = (self.cfg.nx // 2 + 1) * nz * 2, (self.cfg.ny // 2 + 1) * nz * 2
+            out = np.zeros(nsx + nsy)
+            _check(lib().cf_run_xy_stats(self._h, k, fa, fb, float(en), _dptr(out)))
+            sx = out[:nsx].reshape(-1, nz, 2)
+            sy = out[nsx:].reshape(-1, nz, 2)
+            return sx[..., 0] + 1j * sx[..., 1], sy[..., 0] + 1j * sy[..., 1]
+        out = np.zeros(nz)
+        _check(lib().cf_run_xy_stats(self._h, k, fa, fb, float(en), _dptr(out)))
+        return out
 
     # operator-level entry points
     def transpose_roundtrip(self, a):

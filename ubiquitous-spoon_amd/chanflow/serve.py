@@ -31,6 +31,7 @@ def _bind():
         "cf_server_release": (C.c_int, [P, C.c_int]),
         "cf_server_request_info": (C.c_int, [P, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]),
         "cf_server_stats": (C.c_int, [P, dp, dp, C.POINTER(C.c_size_t)]),
+        "cf_server_request_state": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -106,6 +107,15 @@ class Server:
         n = C.c_int(0)
         _check(_bind().cf_server_request_info(self._h, rid, C.byref(ts), C.byref(ta), C.byref(td), C.byref(n)))
         return ts.value, ta.value, td.value, n.value
+
+    STATES = ("queued", "active", "done", "released", "failed")
+
+    def state(self, rid):
+        """(state name, failure message) of a request."""
+        st = C.c_int(0)
+        msg = C.create_string_buffer(512)
+        _check(_bind().cf_server_request_state(self._h, rid, C.byref(st), msg, 512))
+        return self.STATES[st.value], msg.value.decode()
 
     def stats(self):
         sec, cs = C.c_double(), C.c_double()

@@ -362,4 +362,14 @@ __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb,
 template <typename T>
 __global__ void k_poisson4(const RunDev R, const Pois4Tab T_, typename Cplx<T>::type *Y, double *S);
 
+/* plane statistics (cf_stats.hip) */
+template <typename T, int MODE>
+__global__ void k_stat_plane(const T *a, const T *b, const double *ma, const double *mb, double en, int P, int nbl,
+                             double *part);
+__global__ void k_stat_finish(const double *part, int nbl, int nz, double scale, double *out);
+__global__ void k_stat_spec(const double2 *A, const double2 *B, int nx, int ny, int nkx, int nz, double *specx,
+                            double *specy);
+template <typename T> __global__ void k_stat_widen(const T *in, double *out, size_t n);
+#define CF_STAT_NBL 64          /* workgroups (partials) per level of a plane statistic */
+
 #endif

@@ -760,17 +760,19 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
         free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
     }
-    if (hipMemcpy(r->tab, pack.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice) != hipSuccess) {
+    /* on the run's own stream, so that setting up a run does not wait for
+     * other runs' work in flight on the device */
+    if (hipMemcpyAsync(r->tab, pack.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice, r->stream) != hipSuccess) {
         free_run(r); cf_set_error("table upload failed"); return CF_ERR_DEVICE;
     }
     /* zero state, tendencies and scalars (finite values: g meets zeta_0 = 0) */
     std::vector<double> zeros(SC_N, 0.0);
-    bool zok = hipMemcpy(r->scal, zeros.data(), sizeof(double) * SC_N, hipMemcpyHostToDevice) == hipSuccess;
+    bool zok = hipMemcpyAsync(r->scal, zeros.data(), sizeof(double) * SC_N, hipMemcpyHostToDevice, r->stream) == hipSuccess;
     double *zb[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                      r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p };
-    for (double *b : zb) if (b) zok = zok && hipMemset(b, 0, r->esz * r->N) == hipSuccess;
-    zok = zok && hipMemset(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N) == hipSuccess;
-    if (!zok || hipDeviceSynchronize() != hipSuccess) { free_run(r); cf_set_error("device init failed"); return CF_ERR_DEVICE; }
+    for (double *b : zb) if (b) zok = zok && hipMemsetAsync(b, 0, r->esz * r->N, r->stream) == hipSuccess;
+    zok = zok && hipMemsetAsync(r->logbuf, 0, sizeof(double) * (size_t) LOGCAP * CF_LOG_N, r->stream) == hipSuccess;
+    if (!zok || hipStreamSynchronize(r->stream) != hipSuccess) { free_run(r); cf_set_error("device init failed"); return CF_ERR_DEVICE; }
 
     /* ---- kernel parameter block ---- */
     RunDev &R = r->rd;
@@ -1373,6 +1375,172 @@ extern "C" cf_status cf_run_max_divergence(cf_run *r, double *out)
     return CF_OK;
 }
 
+/* ------------------------------------------------------------ statistics */
+
+/* the device buffer of a field of the current state (storage precision of
+ * the run); the fast path's velocity is materialised into the forcing
+ * slots first, which are free between steps */
+static cf_status stat_source(cf_run *r, int field, const double **src, bool *mat)
+{
+    if (field == CF_P) { *src = r->p; return CF_OK; }
+    if (field == CF_C) {
+        if (!r->c) { cf_set_error("statistics: field c needs a scalar run"); return CF_ERR_ARG; }
+        *src = r->c;
+        return CF_OK;
+    }
+    if (field < CF_U || field > CF_W) { cf_set_error("statistics: bad field %d", field); return CF_ERR_ARG; }
+    if (r->general) {
+        *src = field == CF_U ? r->u : field == CF_V ? r->v : r->w;
+        return CF_OK;
+    }
+    if (!*mat) {
+        cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+        if (st != CF_OK) return st;
+        *mat = true;
+    }
+    *src = r->fa[field];
+    return CF_OK;
+}
+
+/* a plane statistic of kind AVG, CORR or MOMENT into dev_out[nz], with
+ * part (CF_STAT_NBL nz doubles) and means (2 nz doubles) as device work
+ * space; enqueued on the run's stream */
+template <typename T>
+static cf_status stat_plane_t(cf_run *r, int kind, const T *a, const T *b, double en, double *part, double *means,
+                              double *dev_out)
+{
+    const int nz = r->g.nz, P = (int) r->P, nbl = CF_STAT_NBL;
+    const double scale = 1.0 / ((double) r->g.nx * (double) r->g.ny);
+    const dim3 grid(nbl, nz), fin((nz + 255) / 256);
+    double *ma = means, *mb = means + nz;
+    const double *nul = nullptr;
+    hipLaunchKernelGGL((k_stat_plane<T, 0>), grid, dim3(256), 0, r->stream, a, (const T *) nullptr, nul, nul, 0.0, P,
+                       nbl, part);
+    hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale,
+                       kind == CF_STAT_AVG ? dev_out : ma);
+    if (kind == CF_STAT_CORR) {
+        hipLaunchKernelGGL((k_stat_plane<T, 0>), grid, dim3(256), 0, r->stream, b, (const T *) nullptr, nul, nul, 0.0,
+                           P, nbl, part);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, mb);
+        hipLaunchKernelGGL((k_stat_plane<T, 1>), grid, dim3(256), 0, r->stream, a, b, (const double *) ma,
+                           (const double *) mb, 0.0, P, nbl, part);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out);
+    } else if (kind == CF_STAT_MOMENT) {
+        hipLaunchKernelGGL((k_stat_plane<T, 2>), grid, dim3(256), 0, r->stream, a, (const T *) nullptr,
+                           (const double *) ma, nul, en, P, nbl, part);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out);
+    }
+    LAUNCHCHK();
+    return CF_OK;
+}
+
+static cf_status stat_plane(cf_run *r, int kind, int fa, int fb, double en, double *part, double *means,
+                            double *dev_out)
+{
+    bool mat = false;
+    const double *a = nullptr, *b = nullptr;
+    cf_status st = stat_source(r, fa, &a, &mat);
+    if (st == CF_OK && kind == CF_STAT_CORR) st = stat_source(r, fb, &b, &mat);
+    if (st != CF_OK) return st;
+    if (r->prec == CF_FP32)
+        return stat_plane_t<float>(r, kind, (const float *) a, (const float *) b, en, part, means, dev_out);
+    return stat_plane_t<double>(r, kind, a, b, en, part, means, dev_out);
+}
+
+/* FFT3D_FWD (1/nx, 1/ny scaled) of an FP64 field in the device layout into
+ * dst [nz][nkx][ny]: the run path's x and y FFT kernels */
+static cf_status stat_fft(cf_run *r, const double *src, double2 *dst)
+{
+    const RunDev &R = r->rd;
+    const long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
+    const unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
+    const size_t lx = 2 * sizeof(double) * ((size_t) R.fx_pairs * fft_ld(r->g.nx) + (size_t) r->g.nx);
+    const size_t ly = 2 * sizeof(double) * ((size_t) R.fy_cols * fft_ld(r->g.ny) + (size_t) r->g.ny);
+    hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), lx, r->stream, R, src, dst);
+    hipLaunchKernelGGL(r->kfy, dim3(fy_blocks(r)), dim3(256), ly, r->stream, R, dst, 1);
+    LAUNCHCHK();
+    return CF_OK;
+}
+
+extern "C" cf_status cf_run_xy_stats(cf_run *r, int kind, int fa, int fb, double en, double *out)
+{
+    CHECK_RUN(r);
+    if (!out || kind < CF_STAT_AVG || kind > CF_STAT_SPEC) { cf_set_error("cf_run_xy_stats: bad kind or output"); return CF_ERR_ARG; }
+    const int nz = r->g.nz;
+    cf_status st;
+    if (kind != CF_STAT_SPEC) {
+        double *work = nullptr;
+        HIPCHK(hipMalloc((void **) &work, sizeof(double) * ((size_t) CF_STAT_NBL * nz + 3 * (size_t) nz)));
+        st = stat_plane(r, kind, fa, fb, en, work, work + (size_t) CF_STAT_NBL * nz,
+                        work + (size_t) CF_STAT_NBL * nz + 2 * nz);
+        if (st == CF_OK && hipMemcpyAsync(out, work + (size_t) CF_STAT_NBL * nz + 2 * nz, sizeof(double) * nz,
+                                          hipMemcpyDeviceToHost, r->stream) != hipSuccess) {
+            cf_set_error("cf_run_xy_stats: copy failed");
+            st = CF_ERR_DEVICE;
+        }
+        (void) hipStreamSynchronize(r->stream);
+        (void) hipFree(work);
+        return st;
+    }
+    /* spectra: both fields widened to FP64 (FP32 runs), transformed, the
+     * level sums of the cross-spectrum */
+    const int nkx = r->g.nkx, ny = r->g.ny;
+    const size_t M = r->M * (size_t) nz, nsx = (size_t) nkx * nz * 2, nsy = (size_t) (ny / 2 + 1) * nz * 2;
+    double2 *X2 = nullptr;
+    double *spec = nullptr, *wide = nullptr;
+    bool ok = hipMalloc((void **) &X2, sizeof(double2) * 2 * M) == hipSuccess
+           && hipMalloc((void **) &spec, sizeof(double) * (nsx + nsy)) == hipSuccess
+           && hipMalloc((void **) &wide, sizeof(double) * r->N) == hipSuccess;
+    st = ok ? CF_OK : CF_ERR_NOMEM;
+    if (!ok) cf_set_error("cf_run_xy_stats: device allocation failed");
+    bool mat = false;
+    for (int q = 0; q < 2 && st == CF_OK; q++) {
+        const double *src = nullptr;
+        st = stat_source(r, q ? fb : fa, &src, &mat);
+        if (st != CF_OK) break;
+        if (r->prec == CF_FP32) {
+            hipLaunchKernelGGL(k_stat_widen<float>, dim3((unsigned) ((r->N + 255) / 256)), dim3(256), 0, r->stream,
+                               (const float *) src, wide, r->N);
+            src = wide;
+        }
+        st = stat_fft(r, src, X2 + q * M);
+    }
+    if (st == CF_OK) {
+        hipLaunchKernelGGL(k_stat_spec, dim3(nz), dim3(256), 0, r->stream, (const double2 *) X2,
+                           (const double2 *) (X2 + M), r->g.nx, ny, nkx, nz, spec, spec + nsx);
+        if (hipGetLastError() != hipSuccess
+            || hipMemcpyAsync(out, spec, sizeof(double) * (nsx + nsy), hipMemcpyDeviceToHost, r->stream) != hipSuccess) {
+            cf_set_error("cf_run_xy_stats: spectra failed");
+            st = CF_ERR_DEVICE;
+        }
+    }
+    (void) hipStreamSynchronize(r->stream);
+    if (X2) (void) hipFree(X2);
+    if (spec) (void) hipFree(spec);
+    if (wide) (void) hipFree(wide);
+    return st;
+}
+
+/* for entry points in other files: a plane statistic of kind AVG, CORR or
+ * MOMENT into res[nz] with the caller's work space (part: CF_STAT_NBL nz
+ * doubles, means: 2 nz), enqueued on the run's stream; nz, the stream and
+ * the device come back for the caller */
+extern "C" cf_status cf_run_stat_plane_into(cf_run *r, int kind, int fa, int fb, double en, double *part,
+                                           double *means, double *res, int *nz, void **stream, int *device)
+{
+    CHECK_RUN(r);
+    if (nz) *nz = r->g.nz;
+    if (stream) *stream = (void *) r->stream;
+    if (device) *device = r->cfg.device;
+    if (!part) return CF_OK;                      /* shape query only */
+    /* profiling would create events: off for this call */
+    const bool prof = r->profile;
+    r->profile = false;
+    const cf_status st = stat_plane(r, kind, fa, fb, en, part, means, res);
+    r->profile = prof;
+    return st;
+}
+
 extern "C" cf_status cf_run_info(cf_run *r, double *dt, int *steps_done)
 {
     if (!r) { cf_set_error("null run handle"); return CF_ERR_ARG; }
@@ -1576,12 +1744,18 @@ struct cf_batch {
 #define CF_FFT_NG 8                        /* FFT length groups: bfft_key values (7 sizes and 0) */
 #define CF_FX_ROW0 (CF_P2I_ROW0 + CF_P2_NG)      /* x-FFT groups (by nx): rows CF_FX_ROW0 + g */
 #define CF_FY_ROW0 (CF_FX_ROW0 + CF_FFT_NG)      /* y-FFT groups (by ny) */
-#define CF_VS_ROWS (CF_FY_ROW0 + CF_FFT_NG)
+#define CF_TH_ROW (CF_FY_ROW0 + CF_FFT_NG)       /* order-2 runs without spike tables: Thomas */
+#define CF_VS_ROWS (CF_TH_ROW + 1)
+/* an order-2 run solves its z systems as it does alone: the spike where
+ * its tables exist, Thomas otherwise (the two round differently) */
+static bool run_uses_spike(const cf_batch *b, const cf_run *r) { return b->order == 2 && r->use_spike; }
+
 static void batch_spike_groups(cf_batch *b)
 {
     b->spL.clear();
     b->spW.clear();
     for (const cf_run *r : b->runs) {
+        if (!run_uses_spike(b, r)) continue;
         const auto it = std::find(b->spL.begin(), b->spL.end(), r->sp_L);
         if (it == b->spL.end()) { b->spL.push_back(r->sp_L); b->spW.push_back(r->spike_waves); }
         else { int &w = b->spW[it - b->spL.begin()]; w = std::max(w, r->spike_waves); }
@@ -1638,7 +1812,7 @@ static void batch_fill_args(cf_batch *b, std::vector<GArgs<T>> &A)
         const unsigned P = (unsigned) r->P, nf = (unsigned) r->nfield, kinds = r->cfg.scalar ? 2u : 1u;
         auto set = [&](int st, unsigned x, unsigned y, unsigned z) { a.dims[st][0] = x; a.dims[st][1] = y; a.dims[st][2] = z; };
         set(ST_CELL, (nx + 63) / 64, (ny + 3) / 4, nz);
-        if (b->solve_kind == SOLVE_SPIKE) set(ST_SOLVE, (P + 63) / 64, nf, 1);
+        if (run_uses_spike(b, r)) set(ST_SOLVE, (P + 63) / 64, nf, 1);
         else set(ST_SOLVE, (P + 255) / 256, nf, 1);
         set(ST_PLANE, (unsigned) r->nb_plane, nz, kinds);
         set(ST_MASS, kinds, 1, 1);
@@ -1671,6 +1845,11 @@ static cf_status batch_upload_tables(cf_batch *b)
             b->hv[(size_t) st * (b->n + 1) + q] = acc;
             int sd = st;
             bool in = true;
+            if (st == CF_TH_ROW) {                /* Thomas runs of an order-2 batch */
+                if (b->order == 2 && !run_uses_spike(b, b->runs[q]))
+                    acc += A[q].dims[ST_SOLVE][0] * A[q].dims[ST_SOLVE][1] * A[q].dims[ST_SOLVE][2];
+                continue;
+            }
             if (st >= CF_FX_ROW0) {               /* FFT length group */
                 const bool y = st >= CF_FY_ROW0;
                 const int g = st - (y ? CF_FY_ROW0 : CF_FX_ROW0);
@@ -1801,10 +1980,22 @@ static cf_status batch_launch(cf_batch *b, int k, int s)
     case KB_RHS_C: fn = S.c; block = dim3(64, 4); break;
     case KB_PROJ: fn = S.proj; block = dim3(64, 4); break;
     case KB_SOLVE:
-        if (b->solve_kind == SOLVE_HEPT) fn = kb_solve4<T>;
-        else if (b->solve_kind == SOLVE_SPIKE) return batch_launch_spike<T>(b, k, s, false);
-        else fn = kb_thomas<T>;
-        break;
+        if (b->solve_kind == SOLVE_HEPT) { fn = kb_solve4<T>; break; }
+        {
+            /* order 2: the spike launches (one per chunk length), then
+             * Thomas over the runs without spike tables */
+            cf_status e = batch_launch_spike<T>(b, k, s, false);
+            if (e != CF_OK) return e;
+            const unsigned tt = batch_total(b, CF_TH_ROW);
+            const unsigned gt = b->kgrid[k] ? std::min(b->kgrid[k], tt) : tt;
+            if (gt == 0) return CF_OK;
+            if ((e = bev_begin(b, KB_NAME[k])) != CF_OK) return e;
+            hipLaunchKernelGGL(kb_thomas<T>, dim3(gt), dim3(256), 0, b->stream,
+                               (const GArgs<T> *) b->args + (size_t) b->tab * b->n,
+                               (const unsigned *) (b->vstart + (size_t) CF_TH_ROW * (b->n + 1)), b->n, s);
+            HIPCHK(hipGetLastError());
+            return bev_end(b);
+        }
     case KB_PLANE: fn = kb_plane<T>; break;
     case KB_MASS: fn = kb_mass<T>; lds = b->lds_mass; break;
     case KB_CFL: fn = kb_cfl<T>; break;
@@ -1976,22 +2167,10 @@ static cf_status batch_autotune(cf_batch *b)
         b->solve_kind = SOLVE_SPIKE;
         b->tuning.push_back("fast path: fused RHS, spike with plane sums, FFT Poisson");
     } else if (b->order == 2) {
-        float best = 1e30f;
-        int bk = SOLVE_THOMAS;
-        bool spike_ok = true;
-        for (cf_run *r : b->runs) spike_ok = spike_ok && r->use_spike;
-        for (int kind : { SOLVE_SPIKE, SOLVE_THOMAS }) {
-            if (kind == SOLVE_SPIKE && !spike_ok) continue;
-            b->solve_kind = kind;
-            if ((st = batch_upload_tables<T>(b)) != CF_OK) return st;
-            b->kgrid[KB_SOLVE] = 0;
-            float ms;
-            if ((st = batch_time_slot<T>(b, KB_SOLVE, &ms)) != CF_OK) return st;
-            if (ms < best) { best = ms; bk = kind; }
-        }
-        b->solve_kind = bk;
-        b->tuning.push_back(std::string("solve: ") + (bk == SOLVE_SPIKE ? "spike (chunked, several wavefronts per system)"
-                                                                        : "thomas (one lane per system)"));
+        /* the solve is fixed per run, never tuned: the spike where the run
+         * has tables, Thomas otherwise, as the run does alone */
+        b->solve_kind = SOLVE_SPIKE;
+        b->tuning.push_back("solve: per run, spike where its tables exist (as alone), else Thomas");
     } else {
         b->solve_kind = SOLVE_HEPT;
     }
@@ -2285,19 +2464,35 @@ extern "C" cf_status cf_batch_tuning(cf_batch *b, char *buf, size_t len)
 /* ------------------------------------------------------------ serving
  *
  * Continuous batching of independent channel runs.  Requests queue on
- * submit; every tick admits queued requests in arrival order while their
- * device footprint fits the HBM budget (fields stay resident until the
- * run retires), then steps each group of active runs that share order,
- * scalar setting and precision with one batched launch per stage, on the
- * group's own stream so groups overlap.  A tick advances a group by the
- * smallest remaining step count of its runs (at most the quantum), so runs
- * retire exactly at their step count and new arrivals join at the next
- * tick.  Per-run arithmetic does not depend on the other runs of the
- * group (per-run reductions, fixed solve shape), so a run's result is the
- * same alone and in any mix.
+ * submit.  The unit of device work is a quantum: a few RK3 steps of one
+ * group (the active runs sharing order, scalar setting and precision, in
+ * one batch whose every stage is one launch) or of one solo run (a run of
+ * at least solo_points cells, which fills the GPU alone and keeps its
+ * single-run graph), each on its own stream and closed by an event.  A
+ * tick
+ *   1. takes in the quanta whose events have completed: credits their
+ *      steps and retires the runs that reached their step count;
+ *   2. admits queued requests in arrival order while their footprint fits
+ *      the HBM budget (a request whose group already holds CF_BATCH_MAX
+ *      runs waits and lets later ones pass; one that fails to set up is
+ *      marked failed and dropped);
+ *   3. rebuilds the batch of every idle group whose membership changed;
+ *   4. enqueues the next quantum of every idle group and solo run;
+ * and, when none of that made progress, waits for the first quantum in
+ * flight to complete and does 1, 3 and 4 once more.  Groups therefore
+ * never wait for each other: a group of small runs re-enqueues as soon as
+ * its own quantum is done.  A quantum is the smallest remaining step count
+ * of its runs, at most the quantum size, so runs retire exactly at their
+ * step count.  Per-run arithmetic does not depend on the other runs of a
+ * group (per-run reductions, the z solve fixed per run, one Poisson launch
+ * per chunk length, FFT instantiations fixed per length), so a run's
+ * result is the same alone and in any mix.
  */
 
 #include <chrono>
+#include <thread>
+
+enum { RQ_QUEUED = 0, RQ_ACTIVE = 1, RQ_DONE = 2, RQ_RELEASED = 3, RQ_FAILED = 4 };
 
 struct cf_server {
     size_t budget = 0, used = 0;
@@ -2306,20 +2501,37 @@ struct cf_server {
     struct Req {
         cf_config cfg;
         std::vector<double> init[4];
-        int nsteps = 0, done = 0, state = 0;       /* 0 queued, 1 active, 2 done (retained), 3 released */
+        int nsteps = 0, done = 0, state = RQ_QUEUED;
         bool retain = false;
         bool solo = false;                         /* stepped through its own graph, not a group batch */
         int gkey = 0;                              /* group of a batched run (group_key, + 8 on the fast path) */
         size_t bytes = 0;
         cf_run *run = nullptr;
         double t_submit = 0, t_admit = -1, t_done = -1;
+        /* a solo run's quantum in flight */
+        hipEvent_t ev = nullptr;
+        int inflight = 0;
+        std::string err;                           /* why a failed request failed */
+    };
+    struct Group {
+        cf_batch *b = nullptr;                     /* non-owning batch of the members */
+        hipEvent_t ev = nullptr;
+        int inflight = 0;                          /* steps of the quantum in flight (0: idle) */
+        bool dirty = false;                        /* membership changed: rebuild before the next quantum */
+        std::vector<int> members;                  /* request ids in b->runs order */
     };
     std::vector<Req> reqs;
     std::vector<int> queue;
-    std::map<int, cf_batch *> groups;              /* key -> non-owning batch of the group's active runs */
+    std::map<int, Group> groups;                   /* key -> the group's batch and quantum state */
+    std::vector<std::pair<cf_run *, size_t>> retired;   /* runs to free once the device is idle */
+    size_t retired_bytes = 0;
     size_t solo_points = (size_t) 4 << 20;         /* runs of at least this many cells step on their own */
     double pt_steps = 0.0;                         /* completed cell-steps */
 };
+
+/* device memory of retired runs is returned while no quantum is in flight
+ * (hipFree waits for the whole device), or at once past this many bytes */
+static const size_t SRV_RETIRE_FLUSH = (size_t) 32 << 30;
 
 static double srv_now(const cf_server *s)
 {
@@ -2372,11 +2584,18 @@ extern "C" cf_status cf_server_create(size_t hbm_budget_bytes, int device, int q
     return CF_OK;
 }
 
-static void srv_drop_group(cf_batch *b)
+static void srv_drop_batch(cf_batch *b)
 {
     if (!b) return;
     b->runs.clear();                               /* runs belong to their requests */
     free_batch(b);
+}
+
+static void srv_flush_retired(cf_server *s)
+{
+    for (auto &pr : s->retired) free_run(pr.first);
+    s->retired.clear();
+    s->retired_bytes = 0;
 }
 
 extern "C" cf_status cf_server_destroy(cf_server *s)
@@ -2384,9 +2603,15 @@ extern "C" cf_status cf_server_destroy(cf_server *s)
     if (!s) return CF_OK;
     (void) hipSetDevice(s->device);
     (void) hipDeviceSynchronize();
-    for (auto &g : s->groups) srv_drop_group(g.second);
-    for (auto &r : s->reqs)
+    for (auto &g : s->groups) {
+        srv_drop_batch(g.second.b);
+        if (g.second.ev) (void) hipEventDestroy(g.second.ev);
+    }
+    for (auto &r : s->reqs) {
         if (r.run) free_run(r.run);
+        if (r.ev) (void) hipEventDestroy(r.ev);
+    }
+    srv_flush_retired(s);
     delete s;
     return CF_OK;
 }
@@ -2419,18 +2644,27 @@ extern "C" cf_status cf_server_submit(cf_server *s, const cf_config *cfg, const 
     return CF_OK;
 }
 
-/* rebuild the group batch of key k from the currently active runs */
-static cf_status srv_regroup(cf_server *s, int k)
+/* rebuild the batch of an idle group from its currently active runs */
+static cf_status srv_regroup(cf_server *s, int k, cf_server::Group &G)
 {
-    std::vector<cf_run *> runs;
-    for (auto &r : s->reqs)
-        if (r.state == 1 && !r.solo && r.gkey == k) runs.push_back(r.run);
-    auto it = s->groups.find(k);
-    if (runs.empty()) {
-        if (it != s->groups.end()) { srv_drop_group(it->second); s->groups.erase(it); }
+    std::vector<int> ids;
+    for (size_t q = 0; q < s->reqs.size(); q++) {
+        const cf_server::Req &r = s->reqs[q];
+        if (r.state == RQ_ACTIVE && !r.solo && r.gkey == k) ids.push_back((int) q);
+    }
+    if (ids.size() > (size_t) CF_BATCH_MAX) {      /* admission keeps groups within the limit */
+        cf_set_error("group %d: %zu active runs, more than %d", k, ids.size(), CF_BATCH_MAX);
+        return CF_ERR_STATE;
+    }
+    G.dirty = false;
+    G.members = ids;
+    if (ids.empty()) {                             /* keep the idle batch: freeing would wait for the device */
+        if (G.b) { G.b->runs.clear(); G.b->n = 0; }
         return CF_OK;
     }
-    cf_batch *b = it != s->groups.end() ? it->second : nullptr;
+    std::vector<cf_run *> runs;
+    for (int q : ids) runs.push_back(s->reqs[q].run);
+    cf_batch *b = G.b;
     if (!b) {
         b = new cf_batch();
         b->prec = runs[0]->cfg.precision;
@@ -2440,18 +2674,19 @@ static cf_status srv_regroup(cf_server *s, int k)
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
             b->ncu = prop.multiProcessorCount;
-        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-            delete b;
-            cf_set_error("stream creation failed");
-            return CF_ERR_DEVICE;
+        /* argument and prefix tables sized for the largest group once */
+        const size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess
+            || hipMalloc(&b->args, asz * CF_BATCH_MAX * 2) != hipSuccess
+            || hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_VS_ROWS * (CF_BATCH_MAX + 1)) != hipSuccess) {
+            free_batch(b);
+            cf_set_error("group batch allocation failed");
+            return CF_ERR_NOMEM;
         }
-        s->groups[k] = b;
+        G.b = b;
     }
-    if (b->args) { (void) hipFree(b->args); b->args = nullptr; }
-    if (b->vstart) { (void) hipFree(b->vstart); b->vstart = nullptr; }
     b->runs = runs;
     b->n = (int) runs.size();
-    if (b->n > CF_BATCH_MAX) { cf_set_error("more than %d active runs in one group", CF_BATCH_MAX); return CF_ERR_UNSUPPORTED; }
     if (k >= 8) {
         /* fast group: every run steps from the batch's state set */
         if (!b->fast) { b->fast = true; b->cur = runs[0]->cur; }
@@ -2460,36 +2695,159 @@ static cf_status srv_regroup(cf_server *s, int k)
     }
     b->spike_waves = 1;
     b->lds_fx = b->lds_fy = b->lds_mass = 0;
-    bool spike_ok = true;
     for (cf_run *r : runs) {
         b->spike_waves = std::max(b->spike_waves, r->spike_waves);
         b->lds_fx = std::max(b->lds_fx, fx_lds(r));
         b->lds_fy = std::max(b->lds_fy, fy_lds(r));
         b->lds_mass = std::max(b->lds_mass, sizeof(double) * (size_t) (r->g.nz + 6));
-        spike_ok = spike_ok && r->use_spike;
     }
-    b->solve_kind = b->order == 4 ? SOLVE_HEPT : (spike_ok ? SOLVE_SPIKE : SOLVE_THOMAS);
+    b->solve_kind = b->order == 4 ? SOLVE_HEPT : SOLVE_SPIKE;   /* per run: spike or Thomas as alone */
     b->p2L = batch_p2L(runs);
     for (int q = 0; q < KB_N; q++) b->kgrid[q] = 0;
-    size_t asz = b->prec == CF_FP32 ? sizeof(GArgs<float>) : sizeof(GArgs<double>);
-    HIPCHK(hipMalloc(&b->args, asz * b->n * (b->fast ? 2 : 1)));
-    HIPCHK(hipMalloc((void **) &b->vstart, sizeof(unsigned) * CF_VS_ROWS * (b->n + 1)));
     return b->prec == CF_FP32 ? batch_upload_tables<float>(b) : batch_upload_tables<double>(b);
 }
 
-extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, int *n_done)
+/* credit the steps of a completed quantum and retire finished runs */
+static void srv_credit(cf_server *s, int q, int k, double now, std::vector<int> &changed)
 {
-    if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
-    HIPCHK(hipSetDevice(s->device));
-    cf_status st;
-    /* 1. one quantum per group and per solo run, all in flight together on
-     * their own streams */
-    std::vector<int> stepped(s->reqs.size(), 0);
+    cf_server::Req &r = s->reqs[q];
+    r.done += k;
+    if (!r.solo) r.run->steps_done += k;
+    s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
+    if (r.done < r.nsteps) return;
+    r.t_done = now;
+    r.state = r.retain ? RQ_DONE : RQ_RELEASED;
+    if (!r.retain) {
+        s->retired.push_back({ r.run, r.bytes });
+        s->retired_bytes += r.bytes;
+        r.run = nullptr;
+        s->used -= r.bytes;
+    }
+    if (!r.solo) changed.push_back(r.gkey);
+}
+
+/* take in every completed quantum; returns the number taken in */
+static cf_status srv_poll(cf_server *s, int *ncompleted)
+{
+    *ncompleted = 0;
+    const double now = srv_now(s);
+    std::vector<int> changed;
     for (auto &g : s->groups) {
-        cf_batch *b = g.second;
+        cf_server::Group &G = g.second;
+        if (G.inflight == 0) continue;
+        hipError_t e = hipEventQuery(G.ev);
+        if (e == hipErrorNotReady) continue;
+        HIPCHK(e);
+        for (int q : G.members) srv_credit(s, q, G.inflight, now, changed);
+        G.inflight = 0;
+        ++*ncompleted;
+    }
+    for (size_t q = 0; q < s->reqs.size(); q++) {
+        cf_server::Req &r = s->reqs[q];
+        if (r.inflight == 0) continue;
+        hipError_t e = hipEventQuery(r.ev);
+        if (e == hipErrorNotReady) continue;
+        HIPCHK(e);
+        const int k = r.inflight;
+        r.inflight = 0;
+        srv_credit(s, (int) q, k, now, changed);
+        ++*ncompleted;
+    }
+    for (int k : changed) s->groups[k].dirty = true;
+    return CF_OK;
+}
+
+static bool srv_busy(const cf_server *s)
+{
+    for (const auto &g : s->groups)
+        if (g.second.inflight) return true;
+    for (const auto &r : s->reqs)
+        if (r.inflight) return true;
+    return false;
+}
+
+/* active batched runs of a request's group family (fast and general
+ * variants of one group_key together) */
+static int srv_group_count(const cf_server *s, const cf_config &c)
+{
+    const int key = group_key(c);
+    int n = 0;
+    for (const auto &r : s->reqs)
+        if (r.state == RQ_ACTIVE && !r.solo && (r.gkey & 7) == key) n++;
+    return n;
+}
+
+/* set up one admitted request's run: create, upload, project */
+static cf_status srv_setup(cf_server *s, cf_server::Req &r)
+{
+    cf_status st;
+    cf_run *run = nullptr;
+    /* large runs fill the GPU on their own: they keep the single-run graph
+     * (the order-2 fast path where it applies) on their own stream; small
+     * ones join their group's batched launches.  The choice depends on the
+     * request alone, never on the mix. */
+    r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
+    const bool fast = !r.solo && fast_eligible(r.cfg);
+    if ((st = create_run(&r.cfg, !(r.solo || fast), &run)) != CF_OK) return st;
+    r.gkey = group_key(r.cfg) + (fast && !run->general && run->use_spike ? 8 : 0);
+    if (fast && r.gkey < 8) {                      /* the spike cannot take it: the general batch */
+        free_run(run);
+        run = nullptr;
+        if ((st = create_run(&r.cfg, true, &run)) != CF_OK) return st;
+    }
+    for (int f = 0; f < 4; f++) {
+        if (r.init[f].empty()) continue;
+        if ((st = cf_run_upload(run, f, r.init[f].data())) != CF_OK) { free_run(run); return st; }
+    }
+    if ((st = cf_run_project(run, 1.0)) != CF_OK) { free_run(run); return st; }
+    r.run = run;
+    return CF_OK;
+}
+
+/* admit queued requests; returns the number admitted or failed */
+static int srv_admit(cf_server *s)
+{
+    int n = 0;
+    size_t qi = 0;
+    while (qi < s->queue.size()) {
+        cf_server::Req &r = s->reqs[s->queue[qi]];
+        const bool solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
+        if (!solo && srv_group_count(s, r.cfg) >= CF_BATCH_MAX) { qi++; continue; }   /* its group is full */
+        /* head-of-line for memory: a large request waits for memory rather
+         * than being overtaken indefinitely */
+        if (s->used + r.bytes > s->budget) break;
+        if (s->used + r.bytes + s->retired_bytes > s->budget + SRV_RETIRE_FLUSH) srv_flush_retired(s);
+        cf_status st = srv_setup(s, r);
+        for (int f = 0; f < 4; f++) std::vector<double>().swap(r.init[f]);
+        s->queue.erase(s->queue.begin() + (long) qi);
+        n++;
+        if (st != CF_OK) {
+            r.state = RQ_FAILED;
+            r.err = cf_last_error();
+            r.t_done = srv_now(s);
+            continue;
+        }
+        r.state = RQ_ACTIVE;
+        r.t_admit = srv_now(s);
+        s->used += r.bytes;
+        if (!r.solo) s->groups[r.gkey].dirty = true;
+    }
+    return n;
+}
+
+/* enqueue the next quantum of every idle group and solo run */
+static cf_status srv_launch(cf_server *s)
+{
+    cf_status st;
+    for (auto &g : s->groups) {
+        cf_server::Group &G = g.second;
+        if (G.inflight) continue;
+        if (G.dirty && (st = srv_regroup(s, g.first, G)) != CF_OK) return st;
+        cf_batch *b = G.b;
+        if (!b || b->n == 0) continue;
         int k = s->quantum;
-        for (auto &r : s->reqs)
-            if (r.state == 1 && !r.solo && r.gkey == g.first) k = std::min(k, r.nsteps - r.done);
+        for (int q : G.members) k = std::min(k, s->reqs[q].nsteps - s->reqs[q].done);
+        if (k <= 0) continue;
         for (int n = 0; n < k; n++) {
             if (b->fast) {
                 st = fbatch_enqueue_step(b, b->cur);
@@ -2502,83 +2860,44 @@ extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, 
         if (b->fast)
             for (cf_run *r : b->runs) {
                 r->cur = b->cur;
-                if (k > 0) r->proj_pending = true;
+                r->proj_pending = true;
             }
-        for (size_t q = 0; q < s->reqs.size(); q++)
-            if (s->reqs[q].state == 1 && !s->reqs[q].solo && s->reqs[q].gkey == g.first) stepped[q] = k;
+        if (!G.ev) HIPCHK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(G.ev, b->stream));
+        G.inflight = k;
     }
     for (size_t q = 0; q < s->reqs.size(); q++) {
         cf_server::Req &r = s->reqs[q];
-        if (r.state != 1 || !r.solo) continue;
-        stepped[q] = std::min(s->quantum, r.nsteps - r.done);
-        if ((st = cf_run_step(r.run, stepped[q])) != CF_OK) return st;   /* counts its own steps */
+        if (r.state != RQ_ACTIVE || !r.solo || r.inflight) continue;
+        const int k = std::min(s->quantum, r.nsteps - r.done);
+        if (k <= 0) continue;
+        if ((st = cf_run_step(r.run, k)) != CF_OK) return st;   /* counts its own steps */
+        if (!r.ev) HIPCHK(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(r.ev, r.run->stream));
+        r.inflight = k;
     }
-    /* 2. admit in arrival order while the budget allows (head-of-line: a
-     * large request waits for memory rather than being overtaken
-     * indefinitely); the host tables, allocations, uploads and projections
-     * of new runs overlap the steps in flight, and new runs step from the
-     * next tick on */
-    std::vector<int> changed;
-    while (!s->queue.empty()) {
-        cf_server::Req &r = s->reqs[s->queue.front()];
-        if (s->used + r.bytes > s->budget) break;
-        cf_run *run = nullptr;
-        /* large runs fill the GPU on their own: they keep the single-run
-         * graph (the order-2 fast path where it applies) on their own
-         * stream; small ones join their group's batched launches.  The
-         * choice depends on the request alone, never on the mix. */
-        r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
-        const bool fast = !r.solo && fast_eligible(r.cfg);
-        if ((st = create_run(&r.cfg, !(r.solo || fast), &run)) != CF_OK) return st;
-        r.gkey = group_key(r.cfg) + (fast && !run->general && run->use_spike ? 8 : 0);
-        if (fast && r.gkey < 8) {                  /* the spike cannot take it: the general batch */
-            free_run(run);
-            run = nullptr;
-            if ((st = create_run(&r.cfg, true, &run)) != CF_OK) return st;
-        }
-        for (int f = 0; f < 4; f++) {
-            if (r.init[f].empty()) continue;
-            if ((st = cf_run_upload(run, f, r.init[f].data())) != CF_OK) { free_run(run); return st; }
-            std::vector<double>().swap(r.init[f]);
-        }
-        if ((st = cf_run_project(run, 1.0)) != CF_OK) { free_run(run); return st; }
-        r.run = run;
-        r.state = 1;
-        r.t_admit = srv_now(s);
-        s->used += r.bytes;
-        if (!r.solo) changed.push_back(r.gkey);
-        s->queue.erase(s->queue.begin());
+    return CF_OK;
+}
+
+extern "C" cf_status cf_server_tick(cf_server *s, int *n_active, int *n_queued, int *n_done)
+{
+    if (!s) { cf_set_error("null server"); return CF_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->device));
+    cf_status st;
+    int ncomp = 0;
+    if ((st = srv_poll(s, &ncomp)) != CF_OK) return st;
+    const int nadm = srv_admit(s);
+    if ((st = srv_launch(s)) != CF_OK) return st;
+    if (ncomp == 0 && nadm == 0 && srv_busy(s)) {
+        /* nothing moved: wait for the first quantum in flight to finish */
+        while ((st = srv_poll(s, &ncomp)) == CF_OK && ncomp == 0)
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (st != CF_OK) return st;
+        if ((st = srv_launch(s)) != CF_OK) return st;
     }
-    /* 3. wait for the steps */
-    for (auto &g : s->groups) HIPCHK(hipStreamSynchronize(g.second->stream));
-    for (size_t q = 0; q < s->reqs.size(); q++)
-        if (stepped[q] > 0 && s->reqs[q].solo) HIPCHK(hipStreamSynchronize(s->reqs[q].run->stream));
-    /* 4. account and retire */
-    const double now = srv_now(s);
-    for (size_t q = 0; q < s->reqs.size(); q++) {
-        cf_server::Req &r = s->reqs[q];
-        if (r.state != 1 || stepped[q] == 0) continue;
-        const int k = stepped[q];
-        r.done += k;
-        if (!r.solo) r.run->steps_done += k;
-        s->pt_steps += (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * k;
-        if (r.done >= r.nsteps) {
-            r.t_done = now;
-            r.state = r.retain ? 2 : 3;
-            if (!r.retain) {
-                free_run(r.run);
-                r.run = nullptr;
-                s->used -= r.bytes;
-            }
-            if (!r.solo) changed.push_back(r.gkey);
-        }
-    }
-    /* 5. rebuild the batches whose membership changed */
-    std::sort(changed.begin(), changed.end());
-    changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
-    for (int k : changed) if ((st = srv_regroup(s, k)) != CF_OK) return st;
+    if (!srv_busy(s) && !s->retired.empty()) srv_flush_retired(s);
     int na = 0, nd = 0;
-    for (auto &r : s->reqs) { na += r.state == 1; nd += r.state >= 2; }
+    for (auto &r : s->reqs) { na += r.state == RQ_ACTIVE; nd += r.state >= RQ_DONE; }
     if (n_active) *n_active = na;
     if (n_queued) *n_queued = (int) s->queue.size();
     if (n_done) *n_done = nd;
@@ -2595,7 +2914,7 @@ extern "C" cf_status cf_server_result(cf_server *s, int request_id, int field, d
 {
     cf_server::Req *r = srv_req(s, request_id);
     if (!r) return CF_ERR_ARG;
-    if (r->state != 2 || !r->run) { cf_set_error("request %d holds no retained result", request_id); return CF_ERR_STATE; }
+    if (r->state != RQ_DONE || !r->run) { cf_set_error("request %d holds no retained result", request_id); return CF_ERR_STATE; }
     return cf_run_download(r->run, field, host_io);
 }
 
@@ -2604,6 +2923,7 @@ extern "C" cf_status cf_server_log(cf_server *s, int request_id, double *out, in
     cf_server::Req *r = srv_req(s, request_id);
     if (!r) return CF_ERR_ARG;
     if (!r->run) { cf_set_error("request %d is not resident", request_id); return CF_ERR_STATE; }
+    if (r->inflight) { cf_set_error("request %d has steps in flight", request_id); return CF_ERR_STATE; }
     return cf_run_log(r->run, out, max_steps, nsteps_out);
 }
 
@@ -2611,11 +2931,13 @@ extern "C" cf_status cf_server_release(cf_server *s, int request_id)
 {
     cf_server::Req *r = srv_req(s, request_id);
     if (!r) return CF_ERR_ARG;
-    if (r->state == 2 && r->run) {
-        free_run(r->run);
+    if (r->state == RQ_DONE && r->run) {
+        s->retired.push_back({ r->run, r->bytes });
+        s->retired_bytes += r->bytes;
         r->run = nullptr;
         s->used -= r->bytes;
-        r->state = 3;
+        r->state = RQ_RELEASED;
+        if (!srv_busy(s)) srv_flush_retired(s);
     }
     return CF_OK;
 }
@@ -2629,6 +2951,15 @@ extern "C" cf_status cf_server_request_info(cf_server *s, int request_id, double
     if (t_admit) *t_admit = r->t_admit;
     if (t_done) *t_done = r->t_done;
     if (steps_done) *steps_done = r->done;
+    return CF_OK;
+}
+
+extern "C" cf_status cf_server_request_state(cf_server *s, int request_id, int *state, char *msg, size_t len)
+{
+    cf_server::Req *r = srv_req(s, request_id);
+    if (!r) return CF_ERR_ARG;
+    if (state) *state = r->state;
+    if (msg && len) snprintf(msg, len, "%s", r->err.c_str());
     return CF_OK;
 }
 

@@ -8,9 +8,10 @@ flux Ub = 1, seeded laminar + 0.1 noise projected to zero divergence.
 
 Prints one JSON line: Mpt.RK3-steps/s (whole job, inputs resident in HBM),
 the dominant kernel's HBM roofline fraction from HIP-event kernel times,
-and the single-thread CPU baseline (the oracle) on a bounded sample.
+and the single-thread CPU baseline (the oracle built -O3 -march=native on
+the host; 1 warm-up step, median of 3) on a bounded sample.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps 1]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps 3]
 """
 import argparse
 import json
@@ -210,22 +211,60 @@ def served_stream(nreq):
     return out
 
 
+def native_oracle():
+    """Build the oracle on this host with the baseline's flags (-O3
+    -march=native, as the reference's Makefile builds for its own host);
+    returns (library path, flags) or (None, why not)."""
+    import subprocess
+    import tempfile
+    flags = ["-O3", "-march=native", "-std=gnu99", "-fPIC", "-shared"]
+    out = os.path.join(tempfile.mkdtemp(prefix="cf_cpu_baseline_"), "liboracle_native.so")
+    src = [os.path.join(ROOT, "oracle", f) for f in ("orc_core.c", "orc_step.c")]
+    try:
+        subprocess.run(["gcc", *flags, "-o", out, *src, "-lm"], check=True, capture_output=True, timeout=300)
+    except (OSError, subprocess.SubprocessError) as e:
+        return None, f"native build failed ({e.__class__.__name__}); prebuilt -O3 -ffp-contract=off oracle"
+    return out, "gcc " + " ".join(flags)
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(cfg, init, nsteps):
-    """The CPU oracle (single thread) on a bounded sample of the same workload."""
+    """The CPU oracle, single-threaded, on a bounded sample of the same
+    workload: one warm-up step, then the median of `nsteps` timed steps."""
+    lib_path, flags = native_oracle()
+    if lib_path:
+        os.environ["ORACLE_LIB_PATH"] = lib_path
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     orc = ol.OracleRun(ol.oracle_cfg(cfg, cf.grid_dz(cfg)))
     for n in "uvw":
         orc.put(n, init[n])
-    t0 = time.perf_counter()
-    orc.steps(nsteps)
-    t1 = time.perf_counter()
+    orc.steps(1)
+    times = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        orc.steps(1)
+        times.append(time.perf_counter() - t0)
     orc.close()
+    med = float(np.median(times))
     npts = cfg.nx * cfg.ny * cfg.nz
-    return {"value": npts / 1e6 * nsteps / (t1 - t0), "unit": "Mpt*RK3-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{nsteps} RK3 step(s) of the same 256x193x256 FP64 workload on the "
-                      f"single-threaded C oracle ({t1 - t0:.1f} s)"}
+    return {"value": npts / 1e6 / med, "unit": "Mpt*RK3-steps/s", "cores": 1, "threads": 1,
+            "kind": "port", "compile": flags, "host": host_cpu(),
+            "step_seconds": times,
+            "sample": f"1 warm-up + median of {nsteps} RK3 steps of the same 256x193x256 FP64 workload on the "
+                      f"single-threaded C oracle ({med:.2f} s per step)"}
 
 
 def main():
@@ -234,7 +273,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps (median; after 1 warm-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=50, help="timed steps of each other form (0: skip)")
     ap.add_argument("--served", type=int, default=24, help="requests of the served stream (0: skip)")

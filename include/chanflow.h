@@ -106,6 +106,15 @@ cf_status cf_initial_fields(const cf_config *cfg, uint64_t seed, double amp, dou
 cf_status cf_io_write(const char *path, const double *buf, size_t n);
 cf_status cf_io_read(const char *path, double *buf, size_t n);
 cf_status cf_io_append_ascii(const char *path, const double *vals, int n);
+/* io_write_ascii (io.c:22-34): n values, one "%+e" per line */
+cf_status cf_io_write_ascii(const char *path, const double *vals, int n);
+
+/* get_z_at_p / get_z_at_w (get.c:458-488) for nghost >= 1 ghost levels
+ * per wall (heights mirrored, ghost.c:31-54): zp, zw of nz + 2 nghost
+ * values each, cell centres and the faces above each level, measured from
+ * the bottom wall (the zp.dat / zw.dat files of chanfast.c:855-856).
+ * Either output may be NULL. */
+cf_status cf_grid_z(const cf_config *cfg, int nghost, double *zp, double *zw);
 
 const char *cf_last_error(void);
 const char *cf_version(void);

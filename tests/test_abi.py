@@ -132,3 +132,32 @@ def test_footprint_scales_with_grid():
     small = cf.footprint(cf.Config(nx=64, ny=64, nz=65))
     big = cf.footprint(cf.Config(nx=256, ny=256, nz=193))
     assert big > 40 * (small - (16 << 20)) and big < 400e9
+
+
+@pytest.mark.parametrize("ng", [1, 3])
+def test_grid_z_closed_form(ng):
+    """zp / zw of get_z_at_p / get_z_at_w (get.c:458-488): on the cosine grid
+    the faces are lz/2 (1 - cos(pi k / nz)) (chanfast.c:184-189), measured
+    from the bottom wall; cell centres sit midway between faces; the
+    mirrored ghost levels are odd about each wall."""
+    nz = 17
+    cfg = cf.Config(nx=8, ny=8, nz=nz, stretch="cosine", lz=2.0)
+    zp, zw = cf.grid_z(cfg, ng)
+    faces = 1.0 - np.cos(np.pi * np.arange(nz + 1) / nz)
+    assert np.allclose(zw[ng - 1:ng + nz], faces, rtol=0, atol=1e-14)
+    assert np.allclose(zp[ng:ng + nz], 0.5 * (faces[:-1] + faces[1:]), rtol=0, atol=1e-14)
+    for m in range(ng):
+        assert abs(zp[ng - 1 - m] + zp[ng + m]) <= 1e-14
+        assert abs((zp[ng + nz + m] - 2.0) + (zp[ng + nz - 1 - m] - 2.0)) <= 1e-14
+    for m in range(1, ng):
+        assert abs(zw[ng - 1 - m] + zw[ng - 1 + m]) <= 1e-14
+    with pytest.raises(cf.ChanflowError):
+        cf.grid_z(cfg, 0)
+
+
+def test_write_ascii_format(tmp_path):
+    """io_write_ascii (io.c:22-34): one "%+e" value per line."""
+    p = str(tmp_path / "prof.dat")
+    vals = np.array([1.5, -2.25e-7, 0.0])
+    assert cf.lib().cf_io_write_ascii(p.encode(), vals.ctypes.data_as(cf.C.POINTER(cf.C.c_double)), 3) == 0
+    assert open(p).read() == "+1.500000e+00\n-2.250000e-07\n+0.000000e+00\n"

@@ -121,6 +121,8 @@ def lib():
     sig = {
         "cf_config_default": (None, [C.POINTER(_CConfig)]),
         "cf_grid_dz": (C.c_int, [C.POINTER(_CConfig), dp]),
+        "cf_grid_z": (C.c_int, [C.POINTER(_CConfig), C.c_int, dp, dp]),
+        "cf_io_write_ascii": (C.c_int, [C.c_char_p, dp, C.c_int]),
         "cf_initial_fields": (C.c_int, [C.POINTER(_CConfig), C.c_uint64, C.c_double, C.c_double, dp, dp, dp, dp]),
         "cf_io_write": (C.c_int, [C.c_char_p, dp, C.c_size_t]),
         "cf_io_read": (C.c_int, [C.c_char_p, dp, C.c_size_t]),
@@ -182,6 +184,17 @@ def grid_dz(cfg: Config) -> np.ndarray:
     out = np.zeros(cfg.nz)
     _check(lib().cf_grid_dz(C.byref(c), _dptr(out)))
     return out
+
+
+def grid_z(cfg: Config, nghost=3):
+    """zp, zw of get_z_at_p / get_z_at_w (get.c:458-488): cell-centre and
+    face heights of the padded grid (nz + 2 nghost levels)."""
+    c, keep = cfg.to_c()
+    n2 = cfg.nz + 2 * nghost
+    zp, zw = np.zeros(n2), np.zeros(n2)
+    _check(lib().cf_grid_z(C.byref(c), int(nghost), _dptr(zp), _dptr(zw)))
+    del keep
+    return zp, zw
 
 
 def initial_fields(cfg: Config, seed=1234, amp=0.1, camp=0.05):

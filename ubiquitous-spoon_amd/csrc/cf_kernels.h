@@ -15,6 +15,7 @@ enum {
     SC_STEP = 6,    /* completed steps */
     SC_CFL = 7,     /* cfl at the start of the current step */
     SC_ZERO = 8,    /* always 0 */
+    SC_CFL0 = 9,    /* cflmode 1: cfl of the state at the start of a step, with the previous dt */
     SC_N = 16
 };
 
@@ -87,6 +88,7 @@ struct RunDev {
     int cmode[2];
     double ctarget[2], cfixed[2];
     int logcap;
+    double cfl_log;             /* > 0: the cfl.dat value of the step (cflmode 1 logs want_cfl) */
     /* FFT plans */
     int radx[CF_MAX_RAD], nradx, rady[CF_MAX_RAD], nrady;
     const double2 *twx, *twy;
@@ -278,6 +280,9 @@ template <typename T> k_spike_sums_t<T> pick_spike_sums(int L);
 __global__ void k_level_sum(const double *plane, int nbx, double *lsum);
 __global__ void k_mass_finalize(const RunDev R, const int s, const double *lsum, double *scal, double *logbuf,
                                 int kind);
+template <typename T>
+__global__ void k_cfl_plain(const RunDev R, const T *u, const T *v, const T *w, double dt, double *part);
+__global__ void k_cfl_reduce(const double *part, int nb, double *scal);
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);

@@ -726,14 +726,66 @@ static __device__ __forceinline__ void cfl_body(const RunDev &R, const double *_
     for (int t = 0; t < 13; t++) lg[t] = 0.0;
     lg[0] = scal[SC_STEP] + 1.0;
     lg[1] = dt;
-    lg[2] = red[0];
-    scal[SC_CFL] = red[0];
+    lg[2] = R.cfl_log > 0.0 ? R.cfl_log : red[0];
+    scal[SC_CFL] = lg[2];
 }
 
 __global__ __launch_bounds__(256) void k_cfl_finish(const RunDev R, const double *__restrict__ part, int nb,
                                                     double dt, double *__restrict__ scal, double *__restrict__ logbuf)
 {
     cfl_body(R, part, nb, dt, scal, logbuf, blk_native());
+}
+
+/* cflmode 1 (chanfast.c:385-391): get_cfl (get.c:390-417) of the plain
+ * fields (u, v, w) with the current dt, before the step that rescales dt;
+ * one workgroup of 64 x 4 per (x tile, y tile, level), its max into
+ * part[block] */
+template <typename T>
+__global__ __launch_bounds__(256) void k_cfl_plain(const RunDev R, const T *__restrict__ u, const T *__restrict__ v,
+                                                   const T *__restrict__ w, double dt, double *__restrict__ part)
+{
+    const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double m = 0.0;
+    if (i < nx && j < ny && k < nz) {
+        const size_t q = (size_t) k * nx * ny + (size_t) j * nx + i;
+        const double *dz = R.dzt + CF_NGT;
+        const double dzph = 0.5 * (dz[k] + dz[k + 1]);
+        double c;
+        if (m < (c = fabs((double) u[q]) * dt / R.g.dx)) m = c;
+        if (m < (c = fabs((double) v[q]) * dt / R.g.dy)) m = c;
+        if (m < (c = fabs((double) w[q]) * dt / dzph)) m = c;
+    }
+    __shared__ double red[256];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    red[tid] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) part[((size_t) blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+template __global__ void k_cfl_plain<double>(const RunDev, const double *, const double *, const double *, double,
+                                             double *);
+template __global__ void k_cfl_plain<float>(const RunDev, const float *, const float *, const float *, double,
+                                            double *);
+
+/* the max of nb partials into scal[SC_CFL0] (a fixed tree) */
+__global__ __launch_bounds__(256) void k_cfl_reduce(const double *__restrict__ part, int nb, double *__restrict__ scal)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[b]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scal[SC_CFL0] = red[0];
 }
 
 __global__ void k_step_end(double *__restrict__ scal)

@@ -314,7 +314,7 @@ Shape shape_of(int local_nx, int local_ny, int nz, int gx, int gy, int gz, int o
     SHIM_CHECK(gx == gy && gy == gz, "nghost_x = nghost_y = nghost_z required (got %d %d %d)", gx, gy, gz);
     SHIM_CHECK(gz >= (order == 2 ? 1 : 3), "order %d needs nghost >= %d (pressure.c:775, :798)", order,
                order == 2 ? 1 : 3);
-    SHIM_CHECK(local_nx >= 1 && local_ny >= 1 && nz >= 2, "bad grid %d x %d x %d", local_nx, local_ny, nz);
+    SHIM_CHECK(local_nx >= 1 && local_ny >= 1 && nz >= 4, "bad grid %d x %d x %d", local_nx, local_ny, nz);
     Shape s;
     s.ng = gz;
     s.n0 = local_nx + 2 * gx;

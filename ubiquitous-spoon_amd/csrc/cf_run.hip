@@ -120,6 +120,12 @@ struct cf_run {
     KSel<float>::div_t kdiv32 = nullptr;
     KSel<float>::ffty_t kfy32 = nullptr;
     KSel<float>::fftxi_t kfxi32 = nullptr;
+    /* cflmode 1: the host copy of the tables being uploaded, and the
+     * projection left pending by the last step, with that step's dt */
+    std::vector<double> tabhost;
+    double *h1prev = nullptr;
+    double dtp_prev = 0.0;
+    bool prev_proj = false;
     /* profiling */
     bool profile = false;
     std::map<std::string, std::pair<double, int>> ktime;
@@ -400,7 +406,8 @@ static cf_status validate(const cf_config *c)
     }
     if (c->scalar && (c->nx < 4 || c->ny < 4)) { cf_set_error("the scalar needs nx, ny >= 4"); return CF_ERR_ARG; }
     if (c->precision != CF_FP64 && c->precision != CF_FP32) { cf_set_error("precision must be CF_FP64 or CF_FP32"); return CF_ERR_ARG; }
-    if (c->cflmode != 0) { cf_set_error("cflmode 1 (adaptive dt) is not built in this library version"); return CF_ERR_UNSUPPORTED; }
+    if (c->cflmode != 0 && c->cflmode != 1) { cf_set_error("cflmode must be 0 or 1"); return CF_ERR_ARG; }
+    if (c->cflmode == 1 && !(c->want_cfl > 0)) { cf_set_error("cflmode 1 needs want_cfl > 0"); return CF_ERR_ARG; }
     int rad[CF_MAX_RAD];
     if (c->nx % 1 || factor_radix(c->nx, rad) < 0 || c->nx > MAXE_LINE) {
         cf_set_error("nx = %d: FFT lengths must be 2^a 3^b 5^c and <= %d", c->nx, MAXE_LINE);
@@ -438,7 +445,7 @@ static void free_run(cf_run *r)
     }
     double *bufs[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                        r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p, r->G, r->S4, r->stage, r->tab, r->scal,
-                       r->logbuf, r->part, r->lsum, r->cflpart };
+                       r->logbuf, r->part, r->lsum, r->cflpart, r->h1prev };
     for (double *b : bufs) if (b) (void) hipFree(b);
     if (r->X) (void) hipFree(r->X);
     for (auto &pe : r->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
@@ -456,6 +463,308 @@ static void rhs_chunks(int nx, int ny, int nz, int target, int *kc, int *blocks)
     *kc = (nz + nchunk - 1) / nchunk;
     nchunk = (nz + *kc - 1) / *kc;
     *blocks = ((nx + CF_RT_X - 1) / CF_RT_X) * ((ny + CF_RT_Y - 1) / CF_RT_Y) * nchunk;
+}
+
+/* Every table of a run that depends on dt (the z-solve factors and spikes,
+ * the unit-forcing profiles h1 and their statistics, the boundary lifts,
+ * the Poisson z-tables) together with the dt-independent ones (metrics,
+ * twiddles, modified wavenumbers), packed in the device layout of r->tab.
+ * The layout depends on the grid alone, so a new dt (cflmode 1) repacks
+ * into the same offsets. */
+struct TabPack {
+    std::vector<double> pack;
+    SubCoef sc[3];
+    double lift[3][4];
+    SpikeHost sph[3][3];
+    PoisHost ph[4];
+    Clos clo;
+    bool spike_ok = true;
+    size_t o_dz = 0, o_th = 0, o_st = 0, o_h1 = 0, o_twx = 0, o_twy = 0, o_twx32 = 0, o_twy32 = 0, o_lx = 0, o_ly = 0;
+    size_t o_rdz = 0, o_rdzh = 0, o_lift = 0;
+    size_t o_p[4][6], o_sp[3][3][6], o_hp[3][3], o_p4[4];
+};
+
+static cf_status build_tables(const cf_config *cfg, const std::vector<double> &dz, double dx, double dy, double dt,
+                              TabPack &T)
+{
+    const double nu = cfg->nu;
+    const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    const double *dzl = dz.data() + CF_NGT;
+
+    /* ---- host tables ---- */
+    const size_t nzt = (size_t) nz;
+    std::vector<double> th(3 * 4 * 3 * nzt, 0.0), stat1(3 * 2 * 4, 0.0), h1(3 * 2 * (nzt + 2), 0.0);
+    SpikeHost (&sph)[3][3] = T.sph;
+    PoisHost (&ph)[4] = T.ph;
+    for (int s = 0; s < 3; s++) {
+        SubCoef &C = T.sc[s];
+        C.nuadt = nu * RK_ALP[s] * dt;
+        C.nubdt = nu * RK_BET[s] * dt;
+        C.abdt = (RK_ALP[s] + RK_BET[s]) * dt;
+        C.gamdt = RK_GAM[s] * dt;
+        C.zetdt = RK_ZET[s] * dt;
+        const double kap = cfg->scalar ? nu / cfg->prandtl : nu;   /* chanfast.c:540-547 */
+        C.kadt = kap * RK_ALP[s] * dt;
+        C.kbdt = kap * RK_BET[s] * dt;
+        for (int type = 0; type < 3; type++) {
+            std::vector<double> a, b, c;
+            if (type == 0) cell2_system(dzl, nz, -C.nubdt, a, b, c);
+            else if (type == 1) cell2_system(dzl, nz, -C.kbdt, a, b, c);
+            else face2_system(dzl, nz, -C.nubdt, a, b, c);
+            double *A = &th[((size_t) (s * 4 + type) * 3 + 0) * nzt];
+            double *IB = &th[((size_t) (s * 4 + type) * 3 + 1) * nzt];
+            double *GM = &th[((size_t) (s * 4 + type) * 3 + 2) * nzt];
+            if (thomas_factors(a, b, c, A, IB, GM)) {
+                cf_set_error("zero pivot in the implicit z system (substep %d)", s);
+                return CF_ERR_NUMERIC;
+            }
+            if (spike_build(a, b, c, spike_L(nz), sph[s][type])) sph[s][type].n = 0;   /* falls back to Thomas */
+            if (type < 2) {
+                /* unit-forcing profile h1: h = gam dt, solve (chanfast.c:498-500) */
+                std::vector<double> rr(nz), x;
+                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
+                tridag_host(a, b, c, rr, x);
+                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
+                for (int k = 0; k < nz; k++) H[k + 1] = x[k];
+                H[0] = (0.5 * x[0] - 0.0) / -0.5;              /* ghost_fill_bot2 */
+                H[nz + 1] = (0.5 * x[nz - 1] - 0.0) / -0.5;    /* ghost_fill_top2 */
+                double kap = type == 0 ? nu : nu / cfg->prandtl;
+                double vol = 0.0, lz = 0.0;
+                for (int k = 0; k < nz; k++) { vol += H[k + 1] * dzl[k]; lz += dzl[k]; }
+                double *S = &stat1[(size_t) (s * 2 + type) * 4];
+                S[0] = vol / lz;
+                S[1] = kap * (H[1] - H[0]) / (0.5 * (dzl[0] + dzl[-1]));
+                S[2] = kap * (H[nz + 1] - H[nz]) / (0.5 * (dzl[nz] + dzl[nz - 1]));
+                S[3] = nz % 2 == 0 ? 0.5 * (H[nz / 2] + H[nz / 2 + 1]) : H[nz / 2 + 1];
+            }
+        }
+        /* boundary lift: h -= H1D(bc-only field; -nu b dt) (chanfast.c:472-477) */
+        double bcs[2] = { 0.0 - cfg->uframe, 0.0 - cfg->vframe };
+        for (int f = 0; f < 2; f++) {
+            double bc = bcs[f];
+            double g = (0.5 * 0.0 - bc) / -0.5;                 /* apply_bc_u2 on a zero field */
+            double nudt = -C.nubdt;
+            double dzp1h = 0.5 * (dzl[1] + dzl[0]), dzm1h = 0.5 * (dzl[-1] + dzl[0]);
+            double bot_next = nz > 1 ? 0.0 : g;
+            T.lift[s][2 * f] = 0.0 + nudt * (((bot_next - 0.0) / dzp1h - (0.0 - g) / dzm1h) / dzl[0]);
+            dzp1h = 0.5 * (dzl[nz] + dzl[nz - 1]);
+            dzm1h = 0.5 * (dzl[nz - 2] + dzl[nz - 1]);
+            T.lift[s][2 * f + 1] = 0.0 + nudt * (((g - 0.0) / dzp1h - (0.0 - 0.0) / dzm1h) / dzl[nz - 1]);
+        }
+        poisson_tables(dzl, nz, dx, dy, C.abdt, ph[s]);
+    }
+    poisson_tables(dzl, nz, dx, dy, 1.0, ph[3]);
+
+    /* ---- general path: closures, boundary lifts, order-4 systems ---- */
+    const int order = cfg->order;
+    Clos &clo = T.clo;
+    if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
+        || cft::closure(cft::CELL, order, 1, cft::DIRICHLET, dzl, nz, clo.cell[1])
+        || cft::closure(cft::FACE, order, 0, cft::DIRICHLET, dzl, nz, clo.face[0])
+        || cft::closure(cft::FACE, order, 1, cft::DIRICHLET, dzl, nz, clo.face[1])) {
+        cf_set_error("singular wall closure rows");
+        return CF_ERR_NUMERIC;
+    }
+    std::vector<double> liftg((size_t) 3 * 4 * nz, 0.0);
+    for (int s = 0; s < 3; s++)
+        for (int f = 0; f < 4; f++) {
+            /* h -= H1D(bc-only field; -kappa b dt) (chanfast.c:472-477, :556-563, :627-632) */
+            const int fam = f == 2 ? cft::FACE : cft::CELL;
+            const int n = fam == cft::CELL ? nz : nz - 1;
+            const double bcb = f == 0 ? 0.0 - cfg->uframe : f == 1 ? 0.0 - cfg->vframe : f == 2 ? 0.0 : -0.5;
+            const double bct = f == 3 ? 0.5 : bcb;
+            const double nudt = f == 3 ? -T.sc[s].kbdt : -T.sc[s].nubdt;
+            const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
+            std::vector<double> col((size_t) n + 6, 0.0), outv((size_t) n, 0.0);
+            for (int m = 0; m < 3; m++) {
+                col[(size_t) (-1 - m + 3)] = cl[0].B[m] * bcb;
+                col[(size_t) (n + m + 3)] = cl[1].B[m] * bct;
+            }
+            cft::helm_apply(fam, order, dzl, nz, nudt, col.data(), outv.data());
+            for (int k = 0; k < n; k++) liftg[((size_t) s * 4 + f) * nz + k] = outv[k];
+        }
+    std::vector<double> hept[3][3], p4[4];
+    if (order == 4) {
+        for (int s = 0; s < 3; s++) {
+            const SubCoef &C = T.sc[s];
+            for (int type = 0; type < 3; type++) {
+                const int fam = type == 2 ? cft::FACE : cft::CELL;
+                const double nudt = type == 1 ? -C.kbdt : -C.nubdt;
+                const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
+                std::vector<cft::Row7> rows;
+                cft::op_rows(fam, 4, dzl, nz, nudt, 1.0, true, rows);
+                cft::eliminate(fam, nz, cl[0], cl[1], rows);
+                if (cft::hept_factor(rows, hept[s][type])) {
+                    cf_set_error("zero pivot in the order-4 implicit z system (substep %d)", s);
+                        return CF_ERR_NUMERIC;
+                }
+                if (type == 2) continue;
+                /* unit-forcing profile (chanfast.c:1422-1424) and its statistics */
+                std::vector<double> rr(nz), x;
+                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
+                cft::hept_solve(hept[s][type], nz, rr, x);
+                std::vector<double> A((size_t) nz + 6, 0.0);
+                for (int k = 0; k < nz; k++) A[k + 3] = x[k];
+                for (int m = 0; m < 3; m++) {
+                    A[2 - m] = cl[0].G[m][0] * x[0] + cl[0].G[m][1] * x[1] + cl[0].G[m][2] * x[2];
+                    A[nz + 3 + m] = cl[1].G[m][0] * x[nz - 1] + cl[1].G[m][1] * x[nz - 2] + cl[1].G[m][2] * x[nz - 3];
+                }
+                const double *Ap = A.data() + 3;
+                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
+                for (int k = -1; k <= nz; k++) H[k + 1] = Ap[k];
+                const double kap = type == 0 ? nu : nu / cfg->prandtl;
+                double vol = 0.0, lz = 0.0;
+                for (int k = 0; k < nz; k++) { vol += Ap[k] * dzl[k]; lz += dzl[k]; }
+                const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
+                double *S = &stat1[(size_t) (s * 2 + type) * 4];
+                S[0] = vol / lz;
+                S[1] = kap * ((A1h * (Ap[0] - Ap[-1]) + A2h * (Ap[1] - Ap[-2])) / (0.5 * (dzl[0] + dzl[-1])));
+                S[2] = kap * ((A1h * (Ap[nz] - Ap[nz - 1]) + A2h * (Ap[nz + 1] - Ap[nz - 2])) / (0.5 * (dzl[nz] + dzl[nz - 1])));
+                S[3] = nz % 2 == 0 ? 0.5 * (Ap[nz / 2 - 1] + Ap[nz / 2]) : Ap[nz / 2];
+            }
+        }
+        for (int q = 0; q < 4; q++) {
+            if (pois4_tables(dzl, nz, dx, dy, q < 3 ? T.sc[q].abdt : 1.0, p4[q])) {
+                cf_set_error("singular order-4 pressure closure rows");
+                return CF_ERR_NUMERIC;
+            }
+        }
+    }
+
+    /* FFT plans: radices, twiddles (host libm), modified wavenumbers */
+    const int nkx = nx / 2 + 1;
+    std::vector<double> twx(2 * nx), twy(2 * ny), lamx(nkx), lamy(ny);
+    for (int j = 0; j < nx; j++) {
+        double th_ = 2.0 * M_PI * (double) j / (double) nx;
+        twx[2 * j] = cos(th_); twx[2 * j + 1] = -sin(th_);
+    }
+    for (int j = 0; j < ny; j++) {
+        double th_ = 2.0 * M_PI * (double) j / (double) ny;
+        twy[2 * j] = cos(th_); twy[2 * j + 1] = -sin(th_);
+    }
+    /* modified wavenumbers: pressure.c:318-320 (order 2), :382-389 (order 4) */
+    auto lam = [&](int i, int n) -> double {
+        if (order == 2) return 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+        const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
+        double c1 = cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+        double c2 = cos(2.0 * M_PI * 2.0 * (double) i / (double) n);
+        double c3 = cos(2.0 * M_PI * 3.0 * (double) i / (double) n);
+        return (2.0 * A2h * A2h + 2.0 * A1h * A1h) + (4.0 * A1h * A2h - 2.0 * A1h * A1h) * c1
+             + (-4.0 * A1h * A2h) * c2 + (-2.0 * A2h * A2h) * c3;
+    };
+    for (int ix = 0; ix < nkx; ix++) lamx[ix] = lam(ix, nx);
+    for (int iy = 0; iy < ny; iy++) lamy[iy] = lam(iy, ny);
+
+    /* pack tables: dz | th | stat1 | h1 | twx | twy | lamx | lamy | pois x4 */
+    std::vector<double> &pack = T.pack;
+    pack.clear();
+    auto put = [&](const double *src, size_t n) { size_t o = pack.size(); pack.insert(pack.end(), src, src + n); return o; };
+    T.o_dz = put(dz.data(), dz.size());
+    T.o_th = put(th.data(), th.size());
+    T.o_st = put(stat1.data(), stat1.size());
+    T.o_h1 = put(h1.data(), h1.size());
+    if (pack.size() % 2) pack.push_back(0.0);       /* 16-byte align the complex twiddles */
+    T.o_twx = put(twx.data(), twx.size());
+    T.o_twy = put(twy.data(), twy.size());
+    /* FP32 copies of the twiddles, two floats per double slot */
+    auto put32 = [&](const std::vector<double> &src) {
+        std::vector<float> f(src.begin(), src.end());
+        if (f.size() % 2) f.push_back(0.f);
+        std::vector<double> slots(f.size() / 2);
+        memcpy(slots.data(), f.data(), sizeof(float) * f.size());
+        return put(slots.data(), slots.size());
+    };
+    T.o_twx32 = put32(twx);
+    T.o_twy32 = put32(twy);
+    T.o_lx = put(lamx.data(), lamx.size());
+    T.o_ly = put(lamy.data(), lamy.size());
+    size_t (&o_p)[4][6] = T.o_p;
+    for (int q = 0; q < 4; q++) {
+        o_p[q][0] = put(ph[q].a.data(), nz); o_p[q][1] = put(ph[q].c.data(), nz);
+        o_p[q][2] = put(ph[q].bz.data(), nz); o_p[q][3] = put(ph[q].ex.data(), nz);
+        o_p[q][4] = put(ph[q].ey.data(), nz); o_p[q][5] = put(ph[q].dvol.data(), nz);
+    }
+    std::vector<double> rdz(nz + 6), rdzh(nz + 6, 0.0);
+    for (int k = -3; k < nz + 3; k++) rdz[k + 3] = 1.0 / dzl[k];
+    for (int k = -3; k < nz + 2; k++) rdzh[k + 3] = 1.0 / (0.5 * (dzl[k] + dzl[k + 1]));
+    T.o_rdz = put(rdz.data(), rdz.size());
+    T.o_rdzh = put(rdzh.data(), rdzh.size());
+    size_t (&o_sp)[3][3][6] = T.o_sp;
+    bool &spike_ok = T.spike_ok;
+    spike_ok = true;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) {
+            SpikeHost &S = sph[s][t];
+            if (S.n == 0) { spike_ok = false; continue; }
+            o_sp[s][t][0] = put(S.a.data(), S.a.size()); o_sp[s][t][1] = put(S.ib.data(), S.ib.size());
+            o_sp[s][t][2] = put(S.gam.data(), S.gam.size()); o_sp[s][t][3] = put(S.vl.data(), S.vl.size());
+            o_sp[s][t][4] = put(S.wr.data(), S.wr.size()); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
+        }
+    T.o_lift = put(liftg.data(), liftg.size());
+    size_t (&o_hp)[3][3] = T.o_hp;
+    size_t (&o_p4)[4] = T.o_p4;
+    memset(o_hp, 0, sizeof(T.o_hp));
+    memset(o_p4, 0, sizeof(T.o_p4));
+    if (order == 4) {
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 3; t++) o_hp[s][t] = put(hept[s][t].data(), hept[s][t].size());
+        for (int q = 0; q < 4; q++) o_p4[q] = put(p4[q].data(), p4[q].size());
+    }
+
+    return CF_OK;
+}
+
+
+/* point the run's parameter block at the tables of TP (already uploaded to
+ * r->tab) and take over its by-value coefficients */
+static void apply_tables(cf_run *r, const TabPack &TP)
+{
+    RunDev &R = r->rd;
+    const int nz = r->g.nz, order = r->g.order;
+    const double *tab = r->tab;
+    R.dzt = tab + TP.o_dz;
+    R.th = tab + TP.o_th;
+    R.rdz = tab + TP.o_rdz;
+    R.rdzh = tab + TP.o_rdzh;
+    R.cl = TP.clo;
+    R.liftg = tab + TP.o_lift;
+    for (int s = 0; s < 3; s++) {
+        R.sc[s] = TP.sc[s];
+        for (int f = 0; f < 4; f++) R.lift[s][f] = TP.lift[s][f];
+    }
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) {
+            R.hp[s][t].n = t == 2 ? nz - 1 : nz;
+            R.hp[s][t].t = order == 4 ? tab + TP.o_hp[s][t] : nullptr;
+            SpikeTab &ST = R.sp[s][t];
+            ST.n = TP.sph[s][t].n; ST.S = TP.sph[s][t].S; ST.L = TP.sph[s][t].L;
+            if (!TP.spike_ok) continue;
+            ST.a = tab + TP.o_sp[s][t][0]; ST.ib = tab + TP.o_sp[s][t][1]; ST.gam = tab + TP.o_sp[s][t][2];
+            ST.vl = tab + TP.o_sp[s][t][3]; ST.wr = tab + TP.o_sp[s][t][4]; ST.minv = tab + TP.o_sp[s][t][5];
+        }
+    R.stat1 = tab + TP.o_st;
+    R.h1 = tab + TP.o_h1;
+    R.twx = (const double2 *) (tab + TP.o_twx);
+    R.twy = (const double2 *) (tab + TP.o_twy);
+    R.twx32 = (const float2 *) (tab + TP.o_twx32);
+    R.twy32 = (const float2 *) (tab + TP.o_twy32);
+    R.lamx = tab + TP.o_lx;
+    R.lamy = tab + TP.o_ly;
+    for (int q = 0; q < 4; q++) {
+        PoisTab &T = r->pt[q];
+        T.a = tab + TP.o_p[q][0]; T.c = tab + TP.o_p[q][1]; T.bz = tab + TP.o_p[q][2];
+        T.ex = tab + TP.o_p[q][3]; T.ey = tab + TP.o_p[q][4]; T.dvol = tab + TP.o_p[q][5];
+        T.db_neu = TP.ph[q].db_neu; T.db_dir = TP.ph[q].db_dir; T.db_top = TP.ph[q].db_top;
+        Pois4Tab &T4 = r->pt4[q];
+        if (order == 4) {
+            const double *b = tab + TP.o_p4[q];
+            T4.band[0] = b; T4.band[1] = b + (size_t) 7 * nz;
+            T4.ex = b + (size_t) 14 * nz; T4.ey = b + (size_t) 15 * nz; T4.dvol = b + (size_t) 16 * nz;
+        }
+    }
+    r->off_pois_adhoc = TP.o_p[3][0];
+    r->off_pois4_adhoc = TP.o_p4[3];
+    r->pt_dt = 1.0;
 }
 
 static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out);
@@ -498,238 +807,21 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     for (int k = 0; k < nz; k++) dzl[k] = dzi[k];
     for (int k = 1; k <= 3; k++) { dzl[-k] = dzl[k - 1]; dzl[nz - 1 + k] = dzl[nz - k]; }
 
-    const double dx = r->g.dx, dy = r->g.dy, dt = cfg->dt, nu = cfg->nu;
-
-    /* ---- host tables ---- */
-    const size_t nzt = (size_t) nz;
-    std::vector<double> th(3 * 4 * 3 * nzt, 0.0), stat1(3 * 2 * 4, 0.0), h1(3 * 2 * (nzt + 2), 0.0);
-    SpikeHost sph[3][3];
-    PoisHost ph[4];
-    for (int s = 0; s < 3; s++) {
-        SubCoef &C = r->rd.sc[s];
-        C.nuadt = nu * RK_ALP[s] * dt;
-        C.nubdt = nu * RK_BET[s] * dt;
-        C.abdt = (RK_ALP[s] + RK_BET[s]) * dt;
-        C.gamdt = RK_GAM[s] * dt;
-        C.zetdt = RK_ZET[s] * dt;
-        const double kap = cfg->scalar ? nu / cfg->prandtl : nu;   /* chanfast.c:540-547 */
-        C.kadt = kap * RK_ALP[s] * dt;
-        C.kbdt = kap * RK_BET[s] * dt;
-        for (int type = 0; type < 3; type++) {
-            std::vector<double> a, b, c;
-            if (type == 0) cell2_system(dzl, nz, -C.nubdt, a, b, c);
-            else if (type == 1) cell2_system(dzl, nz, -C.kbdt, a, b, c);
-            else face2_system(dzl, nz, -C.nubdt, a, b, c);
-            double *A = &th[((size_t) (s * 4 + type) * 3 + 0) * nzt];
-            double *IB = &th[((size_t) (s * 4 + type) * 3 + 1) * nzt];
-            double *GM = &th[((size_t) (s * 4 + type) * 3 + 2) * nzt];
-            if (thomas_factors(a, b, c, A, IB, GM)) {
-                cf_set_error("zero pivot in the implicit z system (substep %d)", s);
-                delete r;
-                return CF_ERR_NUMERIC;
-            }
-            if (spike_build(a, b, c, spike_L(nz), sph[s][type])) sph[s][type].n = 0;   /* falls back to Thomas */
-            if (type < 2) {
-                /* unit-forcing profile h1: h = gam dt, solve (chanfast.c:498-500) */
-                std::vector<double> rr(nz), x;
-                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
-                tridag_host(a, b, c, rr, x);
-                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
-                for (int k = 0; k < nz; k++) H[k + 1] = x[k];
-                H[0] = (0.5 * x[0] - 0.0) / -0.5;              /* ghost_fill_bot2 */
-                H[nz + 1] = (0.5 * x[nz - 1] - 0.0) / -0.5;    /* ghost_fill_top2 */
-                double kap = type == 0 ? nu : nu / cfg->prandtl;
-                double vol = 0.0, lz = 0.0;
-                for (int k = 0; k < nz; k++) { vol += H[k + 1] * dzl[k]; lz += dzl[k]; }
-                double *S = &stat1[(size_t) (s * 2 + type) * 4];
-                S[0] = vol / lz;
-                S[1] = kap * (H[1] - H[0]) / (0.5 * (dzl[0] + dzl[-1]));
-                S[2] = kap * (H[nz + 1] - H[nz]) / (0.5 * (dzl[nz] + dzl[nz - 1]));
-                S[3] = nz % 2 == 0 ? 0.5 * (H[nz / 2] + H[nz / 2 + 1]) : H[nz / 2 + 1];
-            }
-        }
-        /* boundary lift: h -= H1D(bc-only field; -nu b dt) (chanfast.c:472-477) */
-        double bcs[2] = { 0.0 - cfg->uframe, 0.0 - cfg->vframe };
-        for (int f = 0; f < 2; f++) {
-            double bc = bcs[f];
-            double g = (0.5 * 0.0 - bc) / -0.5;                 /* apply_bc_u2 on a zero field */
-            double nudt = -C.nubdt;
-            double dzp1h = 0.5 * (dzl[1] + dzl[0]), dzm1h = 0.5 * (dzl[-1] + dzl[0]);
-            double bot_next = nz > 1 ? 0.0 : g;
-            r->rd.lift[s][2 * f] = 0.0 + nudt * (((bot_next - 0.0) / dzp1h - (0.0 - g) / dzm1h) / dzl[0]);
-            dzp1h = 0.5 * (dzl[nz] + dzl[nz - 1]);
-            dzm1h = 0.5 * (dzl[nz - 2] + dzl[nz - 1]);
-            r->rd.lift[s][2 * f + 1] = 0.0 + nudt * (((g - 0.0) / dzp1h - (0.0 - 0.0) / dzm1h) / dzl[nz - 1]);
-        }
-        poisson_tables(dzl, nz, dx, dy, C.abdt, ph[s]);
-    }
-    poisson_tables(dzl, nz, dx, dy, 1.0, ph[3]);
-    r->pt_dt = 1.0;
-
-    /* ---- general path: closures, boundary lifts, order-4 systems ---- */
+    const double dx = r->g.dx, dy = r->g.dy;
     const int order = cfg->order;
     r->prec = cfg->precision;
     r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
     r->general = force_general || order == 4 || cfg->scalar;
     r->nfield = cfg->scalar ? 4 : 3;
-    Clos clo;
-    if (cft::closure(cft::CELL, order, 0, cft::DIRICHLET, dzl, nz, clo.cell[0])
-        || cft::closure(cft::CELL, order, 1, cft::DIRICHLET, dzl, nz, clo.cell[1])
-        || cft::closure(cft::FACE, order, 0, cft::DIRICHLET, dzl, nz, clo.face[0])
-        || cft::closure(cft::FACE, order, 1, cft::DIRICHLET, dzl, nz, clo.face[1])) {
-        cf_set_error("singular wall closure rows");
-        delete r;
-        return CF_ERR_NUMERIC;
-    }
-    std::vector<double> liftg((size_t) 3 * 4 * nz, 0.0);
-    for (int s = 0; s < 3; s++)
-        for (int f = 0; f < 4; f++) {
-            /* h -= H1D(bc-only field; -kappa b dt) (chanfast.c:472-477, :556-563, :627-632) */
-            const int fam = f == 2 ? cft::FACE : cft::CELL;
-            const int n = fam == cft::CELL ? nz : nz - 1;
-            const double bcb = f == 0 ? 0.0 - cfg->uframe : f == 1 ? 0.0 - cfg->vframe : f == 2 ? 0.0 : -0.5;
-            const double bct = f == 3 ? 0.5 : bcb;
-            const double nudt = f == 3 ? -r->rd.sc[s].kbdt : -r->rd.sc[s].nubdt;
-            const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
-            std::vector<double> col((size_t) n + 6, 0.0), outv((size_t) n, 0.0);
-            for (int m = 0; m < 3; m++) {
-                col[(size_t) (-1 - m + 3)] = cl[0].B[m] * bcb;
-                col[(size_t) (n + m + 3)] = cl[1].B[m] * bct;
-            }
-            cft::helm_apply(fam, order, dzl, nz, nudt, col.data(), outv.data());
-            for (int k = 0; k < n; k++) liftg[((size_t) s * 4 + f) * nz + k] = outv[k];
-        }
-    std::vector<double> hept[3][3], p4[4];
-    if (order == 4) {
-        for (int s = 0; s < 3; s++) {
-            const SubCoef &C = r->rd.sc[s];
-            for (int type = 0; type < 3; type++) {
-                const int fam = type == 2 ? cft::FACE : cft::CELL;
-                const double nudt = type == 1 ? -C.kbdt : -C.nubdt;
-                const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
-                std::vector<cft::Row7> rows;
-                cft::op_rows(fam, 4, dzl, nz, nudt, 1.0, true, rows);
-                cft::eliminate(fam, nz, cl[0], cl[1], rows);
-                if (cft::hept_factor(rows, hept[s][type])) {
-                    cf_set_error("zero pivot in the order-4 implicit z system (substep %d)", s);
-                    delete r;
-                    return CF_ERR_NUMERIC;
-                }
-                if (type == 2) continue;
-                /* unit-forcing profile (chanfast.c:1422-1424) and its statistics */
-                std::vector<double> rr(nz), x;
-                for (int k = 0; k < nz; k++) rr[k] = C.gamdt * dzl[k];
-                cft::hept_solve(hept[s][type], nz, rr, x);
-                std::vector<double> A((size_t) nz + 6, 0.0);
-                for (int k = 0; k < nz; k++) A[k + 3] = x[k];
-                for (int m = 0; m < 3; m++) {
-                    A[2 - m] = cl[0].G[m][0] * x[0] + cl[0].G[m][1] * x[1] + cl[0].G[m][2] * x[2];
-                    A[nz + 3 + m] = cl[1].G[m][0] * x[nz - 1] + cl[1].G[m][1] * x[nz - 2] + cl[1].G[m][2] * x[nz - 3];
-                }
-                const double *Ap = A.data() + 3;
-                double *H = &h1[(size_t) (s * 2 + type) * (nzt + 2)];
-                for (int k = -1; k <= nz; k++) H[k + 1] = Ap[k];
-                const double kap = type == 0 ? nu : nu / cfg->prandtl;
-                double vol = 0.0, lz = 0.0;
-                for (int k = 0; k < nz; k++) { vol += Ap[k] * dzl[k]; lz += dzl[k]; }
-                const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
-                double *S = &stat1[(size_t) (s * 2 + type) * 4];
-                S[0] = vol / lz;
-                S[1] = kap * ((A1h * (Ap[0] - Ap[-1]) + A2h * (Ap[1] - Ap[-2])) / (0.5 * (dzl[0] + dzl[-1])));
-                S[2] = kap * ((A1h * (Ap[nz] - Ap[nz - 1]) + A2h * (Ap[nz + 1] - Ap[nz - 2])) / (0.5 * (dzl[nz] + dzl[nz - 1])));
-                S[3] = nz % 2 == 0 ? 0.5 * (Ap[nz / 2 - 1] + Ap[nz / 2]) : Ap[nz / 2];
-            }
-        }
-        for (int q = 0; q < 4; q++) {
-            if (pois4_tables(dzl, nz, dx, dy, q < 3 ? r->rd.sc[q].abdt : 1.0, p4[q])) {
-                cf_set_error("singular order-4 pressure closure rows");
-                delete r;
-                return CF_ERR_NUMERIC;
-            }
-        }
-    }
     r->rd.m_prev_slot[0] = SC_M0 + 2;
     r->rd.m_prev_slot[1] = SC_M0 + 0;
     r->rd.m_prev_slot[2] = SC_M0 + 1;
-
-    /* FFT plans: radices, twiddles (host libm), modified wavenumbers */
-    int nkx = r->g.nkx;
     r->rd.nradx = factor_radix(nx, r->rd.radx);
     r->rd.nrady = factor_radix(ny, r->rd.rady);
-    std::vector<double> twx(2 * nx), twy(2 * ny), lamx(nkx), lamy(ny);
-    for (int j = 0; j < nx; j++) {
-        double th_ = 2.0 * M_PI * (double) j / (double) nx;
-        twx[2 * j] = cos(th_); twx[2 * j + 1] = -sin(th_);
-    }
-    for (int j = 0; j < ny; j++) {
-        double th_ = 2.0 * M_PI * (double) j / (double) ny;
-        twy[2 * j] = cos(th_); twy[2 * j + 1] = -sin(th_);
-    }
-    /* modified wavenumbers: pressure.c:318-320 (order 2), :382-389 (order 4) */
-    auto lam = [&](int i, int n) -> double {
-        if (order == 2) return 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
-        const double A1h = 1.125 / 1.0, A2h = -0.125 / 3.0;
-        double c1 = cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
-        double c2 = cos(2.0 * M_PI * 2.0 * (double) i / (double) n);
-        double c3 = cos(2.0 * M_PI * 3.0 * (double) i / (double) n);
-        return (2.0 * A2h * A2h + 2.0 * A1h * A1h) + (4.0 * A1h * A2h - 2.0 * A1h * A1h) * c1
-             + (-4.0 * A1h * A2h) * c2 + (-2.0 * A2h * A2h) * c3;
-    };
-    for (int ix = 0; ix < nkx; ix++) lamx[ix] = lam(ix, nx);
-    for (int iy = 0; iy < ny; iy++) lamy[iy] = lam(iy, ny);
-
-    /* pack tables: dz | th | stat1 | h1 | twx | twy | lamx | lamy | pois x4 */
-    std::vector<double> pack;
-    auto put = [&](const double *src, size_t n) { size_t o = pack.size(); pack.insert(pack.end(), src, src + n); return o; };
-    size_t o_dz = put(r->dz.data(), r->dz.size());
-    size_t o_th = put(th.data(), th.size());
-    size_t o_st = put(stat1.data(), stat1.size());
-    size_t o_h1 = put(h1.data(), h1.size());
-    if (pack.size() % 2) pack.push_back(0.0);       /* 16-byte align the complex twiddles */
-    size_t o_twx = put(twx.data(), twx.size());
-    size_t o_twy = put(twy.data(), twy.size());
-    /* FP32 copies of the twiddles, two floats per double slot */
-    auto put32 = [&](const std::vector<double> &src) {
-        std::vector<float> f(src.begin(), src.end());
-        if (f.size() % 2) f.push_back(0.f);
-        std::vector<double> slots(f.size() / 2);
-        memcpy(slots.data(), f.data(), sizeof(float) * f.size());
-        return put(slots.data(), slots.size());
-    };
-    size_t o_twx32 = put32(twx);
-    size_t o_twy32 = put32(twy);
-    size_t o_lx = put(lamx.data(), lamx.size());
-    size_t o_ly = put(lamy.data(), lamy.size());
-    size_t o_p[4][6];
-    for (int q = 0; q < 4; q++) {
-        o_p[q][0] = put(ph[q].a.data(), nz); o_p[q][1] = put(ph[q].c.data(), nz);
-        o_p[q][2] = put(ph[q].bz.data(), nz); o_p[q][3] = put(ph[q].ex.data(), nz);
-        o_p[q][4] = put(ph[q].ey.data(), nz); o_p[q][5] = put(ph[q].dvol.data(), nz);
-    }
-    r->off_pois_adhoc = o_p[3][0];
-    std::vector<double> rdz(nz + 6), rdzh(nz + 6, 0.0);
-    for (int k = -3; k < nz + 3; k++) rdz[k + 3] = 1.0 / dzl[k];
-    for (int k = -3; k < nz + 2; k++) rdzh[k + 3] = 1.0 / (0.5 * (dzl[k] + dzl[k + 1]));
-    size_t o_rdz = put(rdz.data(), rdz.size());
-    size_t o_rdzh = put(rdzh.data(), rdzh.size());
-    size_t o_sp[3][3][6];
-    bool spike_ok = true;
-    for (int s = 0; s < 3; s++)
-        for (int t = 0; t < 3; t++) {
-            SpikeHost &S = sph[s][t];
-            if (S.n == 0) { spike_ok = false; continue; }
-            o_sp[s][t][0] = put(S.a.data(), S.a.size()); o_sp[s][t][1] = put(S.ib.data(), S.ib.size());
-            o_sp[s][t][2] = put(S.gam.data(), S.gam.size()); o_sp[s][t][3] = put(S.vl.data(), S.vl.size());
-            o_sp[s][t][4] = put(S.wr.data(), S.wr.size()); o_sp[s][t][5] = put(S.minv.data(), S.minv.size());
-        }
-    size_t o_lift = put(liftg.data(), liftg.size());
-    size_t o_hp[3][3] = {{0}}, o_p4[4] = {0};
-    if (order == 4) {
-        for (int s = 0; s < 3; s++)
-            for (int t = 0; t < 3; t++) o_hp[s][t] = put(hept[s][t].data(), hept[s][t].size());
-        for (int q = 0; q < 4; q++) o_p4[q] = put(p4[q].data(), p4[q].size());
-    }
-    r->off_pois4_adhoc = o_p4[3];
+    TabPack TP;
+    st = build_tables(cfg, r->dz, dx, dy, cfg->dt, TP);
+    if (st != CF_OK) { delete r; return st; }
+    const std::vector<double> &pack = TP.pack;
     r->tab_n = pack.size();
 
     /* ---- device memory ---- */
@@ -756,6 +848,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     rhs_chunks(nx, ny, nz, CF_RHS_KC, &r->rhs_kc, &r->rhs_blocks);
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
     ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));
+    if (cfg->cflmode == 1) ok = ok && dalloc(&r->h1prev, (size_t) nz + 2);
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
         free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
@@ -777,29 +870,18 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     /* ---- kernel parameter block ---- */
     RunDev &R = r->rd;
     R.g = r->g;
-    R.dzt = r->tab + o_dz;
-    R.nu = nu;
+    R.nu = cfg->nu;
     R.prandtl = cfg->prandtl;
     R.ubc = 0.0 - cfg->uframe;                  /* chanfast.c:59-62 */
     R.vbc = 0.0 - cfg->vframe;
     R.cbot = -0.5; R.ctop = 0.5;                /* chanfast.c:67-68 */
-    R.th = r->tab + o_th;
-    R.rdz = r->tab + o_rdz;
-    R.rdzh = r->tab + o_rdzh;
     r->kdiv = pick_div_fftx<double>(nx, order);
     r->kdiv32 = pick_div_fftx<float>(nx, order);
     r->g64 = pick_general<double>(order, cfg->scalar);
     r->g32 = pick_general<float>(order, cfg->scalar);
-    R.cl = clo;
     R.scalar = cfg->scalar;
     R.betg[0] = cfg->betg_x; R.betg[1] = cfg->betg_y; R.betg[2] = cfg->betg_z;
-    R.kap = cfg->scalar ? nu / cfg->prandtl : nu;
-    R.liftg = r->tab + o_lift;
-    for (int s = 0; s < 3; s++)
-        for (int t = 0; t < 3; t++) {
-            R.hp[s][t].n = t == 2 ? nz - 1 : nz;
-            R.hp[s][t].t = order == 4 ? r->tab + o_hp[s][t] : nullptr;
-        }
+    R.kap = cfg->scalar ? cfg->nu / cfg->prandtl : cfg->nu;
     r->kfxp = pick_fftx_plain(nx);
     r->kfy = pick_ffty<double>(ny);
     r->kfxi = pick_fftx_inv<double>(nx);
@@ -808,43 +890,17 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.rdx = 1.0 / r->g.dx;
     R.rdy = 1.0 / r->g.dy;
     R.rdxdy = 1.0 / (r->g.dx * r->g.dy);
-    r->use_spike = spike_ok;
-    r->sp_L = spike_ok ? spike_L(nz) : 0;
-    for (int s = 0; s < 3; s++)
-        for (int t = 0; t < 3; t++) {
-            SpikeTab &ST = R.sp[s][t];
-            ST.n = sph[s][t].n; ST.S = sph[s][t].S; ST.L = sph[s][t].L;
-            if (!spike_ok) continue;
-            ST.a = r->tab + o_sp[s][t][0]; ST.ib = r->tab + o_sp[s][t][1]; ST.gam = r->tab + o_sp[s][t][2];
-            ST.vl = r->tab + o_sp[s][t][3]; ST.wr = r->tab + o_sp[s][t][4]; ST.minv = r->tab + o_sp[s][t][5];
-            r->spike_waves = std::max(r->spike_waves, ST.S);
-        }
-    R.stat1 = r->tab + o_st;
-    R.h1 = r->tab + o_h1;
     R.cmode[0] = cfg->uvolavgmode; R.ctarget[0] = cfg->want_uvolavg - cfg->uframe; R.cfixed[0] = -cfg->dpdx;
     R.cmode[1] = cfg->cvolavgmode; R.ctarget[1] = cfg->want_cvolavg; R.cfixed[1] = 0.0;
     R.logcap = LOGCAP;
-    R.twx = (const double2 *) (r->tab + o_twx);
-    R.twy = (const double2 *) (r->tab + o_twy);
-    R.twx32 = (const float2 *) (r->tab + o_twx32);
-    R.twy32 = (const float2 *) (r->tab + o_twy32);
-    R.lamx = r->tab + o_lx;
-    R.lamy = r->tab + o_ly;
     R.fx_pairs = CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1;
     /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
     R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
-    for (int q = 0; q < 4; q++) {
-        PoisTab &T = r->pt[q];
-        T.a = r->tab + o_p[q][0]; T.c = r->tab + o_p[q][1]; T.bz = r->tab + o_p[q][2];
-        T.ex = r->tab + o_p[q][3]; T.ey = r->tab + o_p[q][4]; T.dvol = r->tab + o_p[q][5];
-        T.db_neu = ph[q].db_neu; T.db_dir = ph[q].db_dir; T.db_top = ph[q].db_top;
-        Pois4Tab &T4 = r->pt4[q];
-        if (order == 4) {
-            const double *b = r->tab + o_p4[q];
-            T4.band[0] = b; T4.band[1] = b + (size_t) 7 * nz;
-            T4.ex = b + (size_t) 14 * nz; T4.ey = b + (size_t) 15 * nz; T4.dvol = b + (size_t) 16 * nz;
-        }
-    }
+    apply_tables(r, TP);
+    r->use_spike = TP.spike_ok;
+    r->sp_L = TP.spike_ok ? spike_L(nz) : 0;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) r->spike_waves = std::max(r->spike_waves, R.sp[s][t].S);
     *out = r;
     return CF_OK;
 }
@@ -1012,6 +1068,10 @@ static cf_status enqueue_step_t(cf_run *r, int from)
         S.h1 = R.h1 + (size_t) (ps * 2 + 0) * (nz + 2);
         S.m = r->scal + SC_M0 + ps;
         S.dtp = R.sc[ps].abdt;
+        if (s == 0 && r->prev_proj) {       /* the pending projection was made with the previous dt */
+            S.h1 = r->h1prev;
+            S.dtp = r->dtp_prev;
+        }
         /* zeta = 0 (first substep): no previous tendency; the last substep's
          * tendency is never read (the next step starts with zeta = 0) */
         const bool useg = RK_ZET[s] != 0.0, keepf = s < 2;
@@ -1282,11 +1342,92 @@ extern "C" cf_status cf_run_project(cf_run *r, double dt)
     return collect_profile(r);
 }
 
+/* Rebuild every dt-dependent table for a new dt and upload it into the
+ * run's table block (same layout).  The stream must be idle: the kernels
+ * of earlier steps read the old tables. */
+static cf_status set_dt(cf_run *r, double dt)
+{
+    TabPack TP;
+    cf_config c = r->cfg;
+    c.dt = dt;
+    cf_status st = build_tables(&c, r->dz, r->g.dx, r->g.dy, dt, TP);
+    if (st != CF_OK) return st;
+    if (TP.pack.size() != r->tab_n || TP.spike_ok != r->use_spike) {
+        cf_set_error("table layout changed with dt");
+        return CF_ERR_STATE;
+    }
+    r->tabhost.swap(TP.pack);
+    HIPCHK(hipMemcpyAsync(r->tab, r->tabhost.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice, r->stream));
+    apply_tables(r, TP);
+    r->cfg.dt = dt;
+    return CF_OK;
+}
+
+/* One step with cflmode 1 (chanfast.c:383-393): the cfl of the current
+ * state with the current dt, dt *= want_cfl / cfl, the tables for the new
+ * dt, then the step launched directly (no graph: the tables change). */
+static cf_status adaptive_step(cf_run *r)
+{
+    const double *mu = r->u, *mv = r->v, *mw = r->w;
+    if (!r->general) {
+        cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
+        if (st != CF_OK) return st;
+        mu = r->fa[0]; mv = r->fa[1]; mw = r->fa[2];
+    }
+    const dim3 cg = cell_grid(r);
+    if (r->prec == CF_FP32)
+        KLAUNCH("cfl", k_cfl_plain<float>, cg, dim3(64, 4), 0, r->rd, (const float *) mu, (const float *) mv,
+                (const float *) mw, r->cfg.dt, r->cflpart);
+    else
+        KLAUNCH("cfl", k_cfl_plain<double>, cg, dim3(64, 4), 0, r->rd, mu, mv, mw, r->cfg.dt, r->cflpart);
+    KLAUNCH("cfl", k_cfl_reduce, dim3(1), dim3(256), 0, (const double *) r->cflpart, (int) (cg.x * cg.y * cg.z),
+            r->scal);
+    double cfl = 0.0;
+    HIPCHK(hipMemcpyAsync(&cfl, r->scal + SC_CFL0, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    if (!std::isfinite(cfl)) { cf_set_error("non-finite cfl %g", cfl); return CF_ERR_NUMERIC; }
+    const double dt = r->cfg.dt * (cfl > 0.0 ? r->cfg.want_cfl / cfl : 1.0);
+    r->rd.cfl_log = cfl > 0.0 ? r->cfg.want_cfl : 0.0;
+    r->prev_proj = false;
+    if (dt != r->cfg.dt) {
+        if (!r->general && r->proj_pending) {
+            /* the next step projects the current state with the old dt's h1 and (alpha + beta) dt */
+            HIPCHK(hipMemcpyAsync(r->h1prev, r->rd.h1 + (size_t) (2 * 2 + 0) * (r->g.nz + 2),
+                                  sizeof(double) * (r->g.nz + 2), hipMemcpyDeviceToDevice, r->stream));
+            r->dtp_prev = r->rd.sc[2].abdt;
+            r->prev_proj = true;
+        }
+        cf_status st = set_dt(r, dt);
+        if (st != CF_OK) return st;
+    }
+    cf_status st = r->general ? enqueue_step_general(r) : enqueue_step(r, r->cur);
+    r->prev_proj = false;
+    if (st != CF_OK) return st;
+    if (!r->general) {
+        r->cur ^= 1;
+        r->proj_pending = true;
+    }
+    r->steps_done += 1;
+    /* the graphs hold the old tables' coefficients */
+    for (int q = 0; q < 2; q++) {
+        if (r->gexec[q]) { (void) hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
+        if (r->graph[q]) { (void) hipGraphDestroy(r->graph[q]); r->graph[q] = nullptr; }
+    }
+    return CF_OK;
+}
+
 extern "C" cf_status cf_run_step(cf_run *r, int nsteps)
 {
     CHECK_RUN(r);
     if (nsteps < 0) { cf_set_error("negative step count"); return CF_ERR_ARG; }
     if (nsteps == 0) return CF_OK;
+    if (r->cfg.cflmode == 1) {
+        for (int n = 0; n < nsteps; n++) {
+            cf_status st = adaptive_step(r);
+            if (st != CF_OK) return st;
+        }
+        return collect_profile(r);
+    }
     if (r->profile) {
         for (int n = 0; n < nsteps; n++) {
             cf_status st = r->general ? enqueue_step_general(r) : enqueue_step(r, r->cur);
@@ -2237,6 +2378,11 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     if (!out || !cfgs) { cf_set_error("null argument"); return CF_ERR_ARG; }
     *out = nullptr;
     if (nrun < 1 || nrun > CF_BATCH_MAX) { cf_set_error("a batch holds 1..%d runs", CF_BATCH_MAX); return CF_ERR_ARG; }
+    for (int q = 0; q < nrun; q++)
+        if (cfgs[q].cflmode != 0) {
+            cf_set_error("run %d: cflmode 1 changes dt every step; step it as a run of its own", q);
+            return CF_ERR_UNSUPPORTED;
+        }
     for (int q = 1; q < nrun; q++) {
         if (cfgs[q].order != cfgs[0].order || cfgs[q].scalar != cfgs[0].scalar
             || cfgs[q].precision != cfgs[0].precision || cfgs[q].device != cfgs[0].device) {
@@ -2786,7 +2932,7 @@ static cf_status srv_setup(cf_server *s, cf_server::Req &r)
      * (the order-2 fast path where it applies) on their own stream; small
      * ones join their group's batched launches.  The choice depends on the
      * request alone, never on the mix. */
-    r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
+    r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1;
     const bool fast = !r.solo && fast_eligible(r.cfg);
     if ((st = create_run(&r.cfg, !(r.solo || fast), &run)) != CF_OK) return st;
     r.gkey = group_key(r.cfg) + (fast && !run->general && run->use_spike ? 8 : 0);
@@ -2811,7 +2957,7 @@ static int srv_admit(cf_server *s)
     size_t qi = 0;
     while (qi < s->queue.size()) {
         cf_server::Req &r = s->reqs[s->queue[qi]];
-        const bool solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points;
+        const bool solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1;
         if (!solo && srv_group_count(s, r.cfg) >= CF_BATCH_MAX) { qi++; continue; }   /* its group is full */
         /* head-of-line for memory: a large request waits for memory rather
          * than being overtaken indefinitely */

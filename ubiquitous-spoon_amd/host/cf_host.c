@@ -8,6 +8,7 @@
 
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 void cf_config_default(cf_config *cfg)
@@ -150,5 +151,50 @@ cf_status cf_io_append_ascii(const char *path, const double *vals, int n)
     for (int i = 0; i < n; i++) fprintf(fp, i ? " %+e" : "%+e", vals[i]);
     fprintf(fp, "\n");
     fclose(fp);
+    return CF_OK;
+}
+
+cf_status cf_io_write_ascii(const char *path, const double *vals, int n)
+{
+    /* io.c:22-34: one value per line, "%+e" */
+    FILE *fp = fopen(path, "w");
+    if (!fp) { cf_set_error("cf_io_write_ascii: cannot open %s", path); return CF_ERR_ARG; }
+    for (int i = 0; i < n; i++) fprintf(fp, "%+e\n", vals[i]);
+    fclose(fp);
+    return CF_OK;
+}
+
+cf_status cf_grid_z(const cf_config *cfg, int nghost, double *zp, double *zw)
+{
+    /* get_z_at_p / get_z_at_w (get.c:458-488) on the padded dz of
+     * chanfast.c:184-190: ghost heights mirrored about each wall
+     * (ghost_even_z, ghost.c:31-54); zp at cell centres, zw at the faces
+     * above each level, both measured from the bottom wall */
+    if (!cfg || nghost < 1 || cfg->nz < 1 || nghost > cfg->nz || cfg->nz > 8000) {
+        cf_set_error("cf_grid_z: need 1 <= nghost <= nz <= 8000");
+        return CF_ERR_ARG;
+    }
+    const int nz = cfg->nz, n2 = nz + 2 * nghost;
+    double *dz = malloc(sizeof(double) * (size_t) n2);
+    if (!dz) { cf_set_error("cf_grid_z: out of memory"); return CF_ERR_NOMEM; }
+    cf_status st = cf_grid_dz(cfg, dz + nghost);
+    if (st != CF_OK) { free(dz); return st; }
+    for (int k = 0; k < nghost; k++) {
+        dz[nghost - 1 - k] = dz[nghost + k];
+        dz[nghost + nz + k] = dz[nghost + nz - 1 - k];
+    }
+    if (zp) {
+        zp[0] = 0.5 * dz[0];
+        for (int k = 1; k < n2; k++) zp[k] = zp[k - 1] + 0.5 * (dz[k - 1] + dz[k]);
+        const double z0 = 0.5 * (zp[nghost - 1] + zp[nghost]);
+        for (int k = 0; k < n2; k++) zp[k] -= z0;
+    }
+    if (zw) {
+        zw[0] = dz[0];
+        for (int k = 1; k < n2; k++) zw[k] = zw[k - 1] + dz[k];
+        const double z0 = zw[nghost - 1];
+        for (int k = 0; k < n2; k++) zw[k] -= z0;
+    }
+    free(dz);
     return CF_OK;
 }

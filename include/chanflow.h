@@ -70,7 +70,17 @@ typedef struct cf_config {
     double uframe, vframe;   /* wall velocities are -uframe, -vframe */
     double betg_x, betg_y, betg_z;   /* Boussinesq buoyancy coefficients */
     int device;              /* HIP device ordinal */
+    int driver;              /* CF_DRIVER_CHANFAST or CF_DRIVER_CHANNEL */
 } cf_config;
+
+/* The two time-step drivers of the reference: chanfast2/4 (chanfast.c,
+ * x-y diffusion explicit, z diffusion implicit by 1-D banded solves) and
+ * channel2/4 (channel.c:71-1968, every diffusion term implicit: the 3-D
+ * Helmholtz operator on the right-hand side and a spectral solve
+ * FFT3D_FWD -> per (kx, ky) banded z solve -> FFT3D_BWD per field,
+ * diffuseu.c:460-614, diffusew.c:466-622).  CF_DRIVER_CHANNEL runs in
+ * FP64 on single runs (not in batches; served runs step on their own). */
+enum { CF_DRIVER_CHANFAST = 0, CF_DRIVER_CHANNEL = 1 };
 
 /* Per-step log record, the values the reference appends to cfl.dat,
  * pstat.dat and cstat.dat (chanfast.c:392-393, :526-529, :613-616). */

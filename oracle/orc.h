@@ -69,7 +69,11 @@ typedef struct orc_cfg {
     int cvolavgmode;
     double uframe, vframe;
     double betg_x, betg_y, betg_z;
+    int driver;           /* ORC_CHANFAST (x-y diffusion explicit) or ORC_CHANNEL (all implicit) */
 } orc_cfg;
+
+#define ORC_CHANFAST 0    /* chanfast2 / chanfast4, chanfast.c:71-1916 */
+#define ORC_CHANNEL  1    /* channel2 / channel4, channel.c:71-1968 */
 
 typedef struct orc_run orc_run;
 
@@ -142,6 +146,16 @@ void orc_project(double *u, double *v, double *w, const double *p,
                  const orc_grid *g, int order, double dt);               /* pressure.c:120-195 */
 void orc_pressure_laplacian(const double *p, double *l, const orc_grid *g,
                             int order, double dt);                       /* pressure.c:197-291 */
+/* the 3-D Helmholtz operators h = q + nudt lap q (diffuseu.c:362-457,
+ * diffusew.c:366-465) */
+void orc_helmholtz3d_cell(const double *u, double *h, const orc_grid *g, int order, double nudt);
+void orc_helmholtz3d_face(const double *w, double *h, const orc_grid *g, int order, double nudt);
+/* spectral Helmholtz solves on cout[(ix*ny+iy)*n2+k][2] (diffuseu.c:460-614,
+ * diffusew.c:466-622): wall values apply to the real part of mode (0,0) */
+void orc_solve_helmholtz_cell(double *cout, const orc_grid *g, int order, double nudt,
+                              double bcb, double bct, int typb, int typt);
+void orc_solve_helmholtz_face(double *cout, const orc_grid *g, int order, double nudt,
+                              double bcb, double bct);
 /* spectral Poisson on cout[(ix*ny+iy)*n2+k][2] (pressure.c:293-444) */
 void orc_poisson(double *cout, const orc_grid *g, int order, double dt);
 /* FFT3D_FWD / FFT3D_BWD of a padded field over all n2 levels (chanfast.c:21-55, fft.c:161-208) */

@@ -1268,6 +1268,60 @@ void orc_pressure_laplacian(const double *p, double *l, const orc_grid *g, int o
             }
 }
 
+static void helmholtz3d(const double *u, double *h, const orc_grid *g, int order, double nudt, int face)
+{
+    int ng = g->ng, n0 = g->n0, n1 = g->n1, n2 = g->n2;
+    double dx = g->dx, dy = g->dy;
+    const double *dz = g->dz;
+    long X = (long) n1 * n2, Y = n2;
+    int lo = order == 2 ? 1 : 3, ktop = n2 - ng - (face ? 1 : 0);
+    for (int i = lo; i < n0 - lo; i++)
+        for (int j = lo; j < n1 - lo; j++)
+            for (int k = ng; k < ktop; k++) {
+                size_t o = ID(g, i, j, k);
+                const double *q = u + o;
+                /* z metric: cell family divides by dz_k with centre spacings
+                 * inside; the face family the reverse */
+                double zin_p = face ? dz[k + 1] : 0.5 * (dz[k + 1] + dz[k]);
+                double zin_m = face ? dz[k] : 0.5 * (dz[k - 1] + dz[k]);
+                double zout = face ? 0.5 * (dz[k + 1] + dz[k]) : dz[k];
+                if (order == 2) {
+                    h[o] = q[0] + nudt * (((q[X] - q[0]) / dx - (q[0] - q[-X]) / dx) / dx
+                                        + ((q[Y] - q[0]) / dy - (q[0] - q[-Y]) / dy) / dy
+                                        + ((q[1] - q[0]) / zin_p - (q[0] - q[-1]) / zin_m) / zout);
+                } else {
+                    double zin_p3 = face ? dz[k + 2] : 0.5 * (dz[k + 2] + dz[k + 1]);
+                    double zin_m3 = face ? dz[k - 1] : 0.5 * (dz[k - 2] + dz[k - 1]);
+                    double up3h = A1 * (q[2 * X] - q[X]) + A2 * (q[3 * X] - q[0]);
+                    double up1h = A1 * (q[X] - q[0]) + A2 * (q[2 * X] - q[-X]);
+                    double um1h = A1 * (q[0] - q[-X]) + A2 * (q[X] - q[-2 * X]);
+                    double um3h = A1 * (q[-X] - q[-2 * X]) + A2 * (q[0] - q[-3 * X]);
+                    double vp3h = A1 * (q[2 * Y] - q[Y]) + A2 * (q[3 * Y] - q[0]);
+                    double vp1h = A1 * (q[Y] - q[0]) + A2 * (q[2 * Y] - q[-Y]);
+                    double vm1h = A1 * (q[0] - q[-Y]) + A2 * (q[Y] - q[-2 * Y]);
+                    double vm3h = A1 * (q[-Y] - q[-2 * Y]) + A2 * (q[0] - q[-3 * Y]);
+                    double wp3h = A1 * (q[2] - q[1]) + A2 * (q[3] - q[0]);
+                    double wp1h = A1 * (q[1] - q[0]) + A2 * (q[2] - q[-1]);
+                    double wm1h = A1 * (q[0] - q[-1]) + A2 * (q[1] - q[-2]);
+                    double wm3h = A1 * (q[-1] - q[-2]) + A2 * (q[0] - q[-3]);
+                    h[o] = q[0] + nudt * ((A1 * (up1h / dx - um1h / dx) + A2 * (up3h / dx - um3h / dx)) / dx
+                                        + (A1 * (vp1h / dy - vm1h / dy) + A2 * (vp3h / dy - vm3h / dy)) / dy
+                                        + (A1 * (wp1h / zin_p - wm1h / zin_m)
+                                         + A2 * (wp3h / zin_p3 - wm3h / zin_m3)) / zout);
+                }
+            }
+}
+
+void orc_helmholtz3d_cell(const double *u, double *h, const orc_grid *g, int order, double nudt)
+{
+    helmholtz3d(u, h, g, order, nudt, 0);                   /* diffuseu.c:362-457 */
+}
+
+void orc_helmholtz3d_face(const double *w, double *h, const orc_grid *g, int order, double nudt)
+{
+    helmholtz3d(w, h, g, order, nudt, 1);                   /* diffusew.c:366-465 */
+}
+
 void orc_poisson(double *cout, const orc_grid *g, int order, double dt)
 {
     /* pressure_poisson2 / 4, pressure.c:293-444: per (ix, iy) the z-banded
@@ -1330,6 +1384,149 @@ void orc_poisson(double *cout, const orc_grid *g, int order, double dt)
                                 s.g + ng, s.r + ng, s.x + ng, nz);
                     cell4_fill_bot(&s, kb);
                     cell4_fill_top(&s, kt);
+                }
+                for (int k = 0; k < n2; k++) col[2 * k + rc] = s.x[k];
+            }
+        }
+    bands_free(&s);
+}
+
+/* modified wavenumbers of the 2nd- or 4th-order staggered second
+ * difference (pressure.c:318-320 and :382-389, diffuseu.c:481-484, :537-550) */
+static double mod_wavenumber(int order, int i, int n)
+{
+    if (order == 2) return 2.0 - 2.0 * cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+    double c1 = cos(2.0 * M_PI * 1.0 * (double) i / (double) n);
+    double c2 = cos(2.0 * M_PI * 2.0 * (double) i / (double) n);
+    double c3 = cos(2.0 * M_PI * 3.0 * (double) i / (double) n);
+    return (2.0 * A2 * A2 + 2.0 * A1 * A1) + (4.0 * A1 * A2 - 2.0 * A1 * A1) * c1
+         + (-4.0 * A1 * A2) * c2 + (-2.0 * A2 * A2) * c3;
+}
+
+void orc_solve_helmholtz_cell(double *cout, const orc_grid *g, int order, double nudt,
+                              double bcb, double bct, int typb, int typt)
+{
+    /* solve_helmholtz_u2 / _u4, diffuseu.c:460-614: per (ix, iy) and part,
+     * (I + nudt (d2/dz2 - lamx/dx2 - lamy/dy2)) x dvol = r dvol with
+     * dvol = dx dy dz; the wall values bcb / bct apply to the real part of
+     * the (0, 0) mode only, every other system is homogeneous */
+    int ng = g->ng, n2 = g->n2, nz = g->nz, nx = g->nx, ny = g->ny, nkx = nx / 2 + 1;
+    double dx = g->dx, dy = g->dy;
+    const double *dz = g->dz;
+    bands s;
+    if (bands_init(&s, n2)) return;
+    for (int k = 0; k < n2; k++) s.x[k] = 0.0;
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            double lamx = mod_wavenumber(order, ix, nx), lamy = mod_wavenumber(order, iy, ny);
+            double *col = cout + (size_t) (ix * ny + iy) * n2 * 2;
+            for (int rc = 0; rc < 2; rc++) {
+                cell_bands(&s, dz, ng, n2, order, nudt, dx * dy, 0.0);
+                for (int k = ng; k < n2 - ng; k++) {
+                    double dvol = dx * dy * dz[k];
+                    double *diag = order == 2 ? &s.b[k] : &s.d[k];
+                    *diag = *diag - lamx * dvol * nudt / (dx * dx) - lamy * dvol * nudt / (dy * dy) + dvol;
+                    s.r[k] = col[2 * k + rc] * dvol;
+                }
+                int kb = ng - 1, kt = n2 - ng;
+                int inh = ix == 0 && iy == 0 && rc == 0;
+                double vb = inh ? bcb : 0.0, vt = inh ? bct : 0.0;
+                if (order == 2) {
+                    cell2_rows_bot(&s, dz, kb, vb, typb);
+                    cell2_rows_top(&s, dz, kt, vt, typt);
+                    cell2_reduce_bot(&s, kb);
+                    cell2_reduce_top(&s, kt);
+                    orc_tridag(s.a + ng, s.b + ng, s.c + ng, s.r + ng, s.x + ng, nz);
+                    cell2_fill_bot(&s, kb);
+                    cell2_fill_top(&s, kt);
+                } else {
+                    cell4_rows_bot(&s, dz, kb, vb, typb);
+                    cell4_rows_top(&s, dz, kt, vt, typt);
+                    cell4_reduce_bot(&s, kb);
+                    cell4_reduce_top(&s, kt);
+                    orc_heptdag(s.a + ng, s.b + ng, s.c + ng, s.d + ng, s.e + ng, s.f + ng,
+                                s.g + ng, s.r + ng, s.x + ng, nz);
+                    cell4_fill_bot(&s, kb);
+                    cell4_fill_top(&s, kt);
+                }
+                for (int k = 0; k < n2; k++) col[2 * k + rc] = s.x[k];
+            }
+        }
+    bands_free(&s);
+}
+
+void orc_solve_helmholtz_face(double *cout, const orc_grid *g, int order, double nudt,
+                              double bcb, double bct)
+{
+    /* solve_helmholtz_w2 / _w4, diffusew.c:466-622: nz - 1 interior faces,
+     * dvol = dx dy (dz_k + dz_k+1) / 2 */
+    int ng = g->ng, n2 = g->n2, nz = g->nz, nx = g->nx, ny = g->ny, nkx = nx / 2 + 1;
+    double dx = g->dx, dy = g->dy;
+    const double *dz = g->dz;
+    bands s;
+    if (bands_init(&s, n2)) return;
+    for (int k = 0; k < n2; k++) s.x[k] = 0.0;
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            double lamx = mod_wavenumber(order, ix, nx), lamy = mod_wavenumber(order, iy, ny);
+            double *col = cout + (size_t) (ix * ny + iy) * n2 * 2;
+            for (int rc = 0; rc < 2; rc++) {
+                for (int k = ng; k < n2 - ng - 1; k++) {
+                    double dzp1 = dz[k + 1], dzm0 = dz[k];
+                    double dzp1h = 0.5 * (dz[k + 1] + dz[k]);
+                    double dvol = dx * dy * dzp1h;
+                    if (order == 2) {
+                        s.a[k] = + dvol * nudt / (dzm0 * dzp1h);
+                        s.c[k] = + dvol * nudt / (dzp1 * dzp1h);
+                        s.b[k] = - dvol * nudt / (dzm0 * dzp1h)
+                                 - dvol * nudt / (dzp1 * dzp1h)
+                                 - lamx * dvol * nudt / (dx * dx)
+                                 - lamy * dvol * nudt / (dy * dy)
+                                 + dvol;
+                    } else {
+                        double dzp2 = dz[k + 2], dzm1 = dz[k - 1];
+                        s.a[k] = + A2 * A2 * dvol * nudt / (dzm1 * dzp1h);
+                        s.g[k] = + A2 * A2 * dvol * nudt / (dzp2 * dzp1h);
+                        s.b[k] = + A1 * A2 * dvol * nudt / (dzm0 * dzp1h)
+                                 + A1 * A2 * dvol * nudt / (dzm1 * dzp1h);
+                        s.f[k] = + A1 * A2 * dvol * nudt / (dzp1 * dzp1h)
+                                 + A1 * A2 * dvol * nudt / (dzp2 * dzp1h);
+                        s.c[k] = - A1 * A2 * dvol * nudt / (dzp1 * dzp1h)
+                                 - A1 * A2 * dvol * nudt / (dzm1 * dzp1h)
+                                 + A1 * A1 * dvol * nudt / (dzm0 * dzp1h);
+                        s.e[k] = - A1 * A2 * dvol * nudt / (dzm0 * dzp1h)
+                                 - A1 * A2 * dvol * nudt / (dzp2 * dzp1h)
+                                 + A1 * A1 * dvol * nudt / (dzp1 * dzp1h);
+                        s.d[k] = - A2 * A2 * dvol * nudt / (dzm1 * dzp1h)
+                                 - A2 * A2 * dvol * nudt / (dzp2 * dzp1h)
+                                 - A1 * A1 * dvol * nudt / (dzm0 * dzp1h)
+                                 - A1 * A1 * dvol * nudt / (dzp1 * dzp1h)
+                                 - lamx * dvol * nudt / (dx * dx)
+                                 - lamy * dvol * nudt / (dy * dy)
+                                 + dvol;
+                    }
+                    s.r[k] = col[2 * k + rc] * dvol;
+                }
+                int kb = ng - 1, kt = n2 - ng - 1;
+                int inh = ix == 0 && iy == 0 && rc == 0;
+                double vb = inh ? bcb : 0.0, vt = inh ? bct : 0.0;
+                if (order == 2) {
+                    face2_rows(&s, kb, vb);
+                    face2_rows(&s, kt, vt);
+                    face2_reduce_bot(&s, kb);
+                    face2_reduce_top(&s, kt);
+                    orc_tridag(s.a + ng, s.b + ng, s.c + ng, s.r + ng, s.x + ng, nz - 1);
+                    face2_fill(&s, kb);
+                    face2_fill(&s, kt);
+                } else {
+                    face4_rows_bot(&s, dz, kb, vb);
+                    face4_rows_top(&s, dz, kt, vt);
+                    face4_reduce_bot(&s, kb);
+                    face4_reduce_top(&s, kt);
+                    orc_heptdag(s.a + ng, s.b + ng, s.c + ng, s.d + ng, s.e + ng, s.f + ng,
+                                s.g + ng, s.r + ng, s.x + ng, nz - 1);
+                    face4_fill_bot(&s, kb);
+                    face4_fill_top(&s, kt);
                 }
                 for (int k = 0; k < n2; k++) col[2 * k + rc] = s.x[k];
             }

@@ -3,7 +3,8 @@
  * one process.  TEST INFRASTRUCTURE ONLY (see orc.h).
  *
  * Follows chanfast.c:383-656 (order 2) and :1307-1580 (order 4) operation
- * by operation: f<->g swap, periodic halos + wall ghosts, explicit RHS,
+ * by operation (and, with driver ORC_CHANNEL, the fully implicit
+ * channel2 / channel4 of channel.c:383-668 and :1307-1593): f<->g swap, periodic halos + wall ghosts, explicit RHS,
  * per-field implicit z-Helmholtz solve with the boundary lift, the
  * mass-flux (and scalar-mean) constraint, w solve, divergence, FFT,
  * banded Poisson, inverse FFT, projection.  Per-step cfl/pstat/cstat
@@ -217,6 +218,32 @@ static void advance_cell(orc_run *r, double *q, double *f, double *gq, double ka
     orc_apply_bc_cell(q, g, order, bcb, bct, ORC_DIRICH_CELL, ORC_DIRICH_CELL);
 }
 
+/* one fully implicit cell-centred field update of channel2 / channel4,
+ * channel.c:453-475 (u; v and c likewise at :537-585):
+ * h = H3(q; nuadt) + gam dt f + zet dt g - H3(bc-only; -nubdt), then the
+ * spectral Helmholtz solve with -nubdt between FFT3D_FWD and FFT3D_BWD */
+static void advance_cell_implicit(orc_run *r, double *q, double *f, double *gq, double nuadt, double nubdt,
+                                  int s, double bcb, double bct)
+{
+    const orc_grid *g = &r->g;
+    int order = r->cfg.order;
+    double dt = r->dt, gamdt = RK_GAM[s] * dt, zetdt = RK_ZET[s] * dt;
+    size_t n = r->ntot;
+    double *h = r->h, *bc = r->bc, *hbc = r->hbc;
+    orc_apply_bc_cell(q, g, order, bcb, bct, ORC_DIRICH_CELL, ORC_DIRICH_CELL);
+    orc_helmholtz3d_cell(q, h, g, order, nuadt);
+    for (size_t i = 0; i < n; i++) h[i] += gamdt * f[i] + zetdt * gq[i];
+    for (size_t i = 0; i < n; i++) bc[i] = 0.0;
+    orc_apply_bc_cell(bc, g, order, bcb, bct, ORC_DIRICH_CELL, ORC_DIRICH_CELL);
+    orc_helmholtz3d_cell(bc, hbc, g, order, -nubdt);
+    for (size_t i = 0; i < n; i++) h[i] -= hbc[i];
+    orc_fft3d_fwd(h, r->spec, g);
+    orc_solve_helmholtz_cell(r->spec, g, order, -nubdt, 0.0, 0.0, ORC_DIRICH_CELL, ORC_DIRICH_CELL);
+    orc_fft3d_bwd(r->spec, q, g);
+    orc_periodic(q, g);
+    orc_apply_bc_cell(q, g, order, bcb, bct, ORC_DIRICH_CELL, ORC_DIRICH_CELL);
+}
+
 /* bulk constraint, chanfast.c:485-530 (u) and :572-617 (c): returns the
  * forcing and fills stats[0..4] = forcing, grad bot, grad top, bulk, centreline */
 static double constrain(orc_run *r, double *q, double *f, double kap, int s, int mode,
@@ -301,14 +328,20 @@ int orc_steps(orc_run *r, int nsteps, double *log)
             orc_massflux(2, r->u, r->v, r->w, r->mx, r->my, r->mz, g, order);
             orc_advect(2, r->w, r->mx, r->my, r->mz, r->fw, g, order);
 
-            orc_laplacexy_cell(r->u, r->fu, g, order, C->nu);
-            orc_laplacexy_cell(r->v, r->fv, g, order, C->nu);
-            orc_laplacexy_face(r->w, r->fw, g, order, C->nu);
+            /* chanfast: x-y diffusion explicit in f (chanfast.c:431-442);
+             * channel: all diffusion implicit, f is advection only
+             * (channel.c:418-440) */
+            const int implicit = C->driver == ORC_CHANNEL;
+            if (!implicit) {
+                orc_laplacexy_cell(r->u, r->fu, g, order, C->nu);
+                orc_laplacexy_cell(r->v, r->fv, g, order, C->nu);
+                orc_laplacexy_face(r->w, r->fw, g, order, C->nu);
+            }
 
             if (C->scalar) {
                 orc_flux_quick(r->u, r->v, r->w, r->c, r->mx, r->my, r->mz, g);
                 orc_advect_scalar(r->mx, r->my, r->mz, r->fc, g);
-                orc_laplacexy_cell(r->c, r->fc, g, order, C->nu / C->prandtl);
+                if (!implicit) orc_laplacexy_cell(r->c, r->fc, g, order, C->nu / C->prandtl);
                 orc_add_buoyancy(r->c, r->fu, r->fv, r->fw, C->betg_x, C->betg_y, C->betg_z, g, order);
             }
 
@@ -321,32 +354,47 @@ int orc_steps(orc_run *r, int nsteps, double *log)
             double abdt = (RK_ALP[s] + RK_BET[s]) * dt;
             double st[5];
 
-            advance_cell(r, r->u, r->fu, r->gu, C->nu, s, ubc, ubc);
+            const double nuadt = C->nu * RK_ALP[s] * dt, nubdt = C->nu * RK_BET[s] * dt;
+            if (implicit) advance_cell_implicit(r, r->u, r->fu, r->gu, nuadt, nubdt, s, ubc, ubc);
+            else advance_cell(r, r->u, r->fu, r->gu, C->nu, s, ubc, ubc);
             constrain(r, r->u, r->fu, C->nu, s, C->uvolavgmode, C->want_uvolavg - C->uframe, -C->dpdx, st);
             if (s == 2 && lg) for (int q = 0; q < 5; q++) lg[3 + q] = st[q];
 
-            advance_cell(r, r->v, r->fv, r->gv, C->nu, s, vbc, vbc);
+            if (implicit) advance_cell_implicit(r, r->v, r->fv, r->gv, nuadt, nubdt, s, vbc, vbc);
+            else advance_cell(r, r->v, r->fv, r->gv, C->nu, s, vbc, vbc);
 
             if (C->scalar) {
                 double kap = C->nu / C->prandtl;
-                advance_cell(r, r->c, r->fc, r->gc, kap, s, cbot, ctop);
+                if (implicit)   /* channel.c:562-581: nuadt / prandtl, -nubdt / prandtl */
+                    advance_cell_implicit(r, r->c, r->fc, r->gc, nuadt / C->prandtl, nubdt / C->prandtl, s, cbot, ctop);
+                else
+                    advance_cell(r, r->c, r->fc, r->gc, kap, s, cbot, ctop);
                 constrain(r, r->c, r->fc, kap, s, C->cvolavgmode, C->want_cvolavg, 0.0, st);
                 if (s == 2 && lg) for (int q = 0; q < 5; q++) lg[8 + q] = st[q];
             }
 
-            /* w, chanfast.c:619-637 */
+            /* w, chanfast.c:619-637; channel.c:636-662 with the 3-D
+             * operator and the spectral solve */
             {
-                double nuadt = C->nu * RK_ALP[s] * dt, nubdt = C->nu * RK_BET[s] * dt;
                 double gamdt = RK_GAM[s] * dt, zetdt = RK_ZET[s] * dt;
                 double *h = r->h, *bc = r->bc, *hbc = r->hbc;
                 orc_apply_bc_face(r->w, g, order, 0.0, 0.0);
-                orc_helmholtz1d_face(r->w, h, g, order, nuadt);
+                if (implicit) orc_helmholtz3d_face(r->w, h, g, order, nuadt);
+                else orc_helmholtz1d_face(r->w, h, g, order, nuadt);
                 for (size_t i = 0; i < n; i++) h[i] += gamdt * r->fw[i] + zetdt * r->gw[i];
                 for (size_t i = 0; i < n; i++) bc[i] = 0.0;
                 orc_apply_bc_face(bc, g, order, 0.0, 0.0);
-                orc_helmholtz1d_face(bc, hbc, g, order, -nubdt);
+                if (implicit) orc_helmholtz3d_face(bc, hbc, g, order, -nubdt);
+                else orc_helmholtz1d_face(bc, hbc, g, order, -nubdt);
                 for (size_t i = 0; i < n; i++) h[i] -= hbc[i];
-                orc_solve1d_face(r->w, h, g, order, -nubdt, 0.0, 0.0);
+                if (implicit) {
+                    orc_fft3d_fwd(h, r->spec, g);
+                    orc_solve_helmholtz_face(r->spec, g, order, -nubdt, 0.0, 0.0);
+                    orc_fft3d_bwd(r->spec, r->w, g);
+                    orc_periodic(r->w, g);
+                } else {
+                    orc_solve1d_face(r->w, h, g, order, -nubdt, 0.0, 0.0);
+                }
                 orc_apply_bc_face(r->w, g, order, 0.0, 0.0);
             }
 

@@ -36,7 +36,8 @@ class Cfg(C.Structure):
                 ("dpdx", C.c_double), ("want_uvolavg", C.c_double), ("uvolavgmode", C.c_int),
                 ("want_cvolavg", C.c_double), ("cvolavgmode", C.c_int),
                 ("uframe", C.c_double), ("vframe", C.c_double),
-                ("betg_x", C.c_double), ("betg_y", C.c_double), ("betg_z", C.c_double)]
+                ("betg_x", C.c_double), ("betg_y", C.c_double), ("betg_z", C.c_double),
+                ("driver", C.c_int)]
 
 
 _L = None
@@ -87,6 +88,10 @@ def lib():
             "orc_project": [dp, dp, dp, dp, G, C.c_int, C.c_double],
             "orc_pressure_laplacian": [dp, dp, G, C.c_int, C.c_double],
             "orc_poisson": [dp, G, C.c_int, C.c_double],
+            "orc_helmholtz3d_cell": [dp, dp, G, C.c_int, C.c_double],
+            "orc_helmholtz3d_face": [dp, dp, G, C.c_int, C.c_double],
+            "orc_solve_helmholtz_cell": [dp, G, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
+            "orc_solve_helmholtz_face": [dp, G, C.c_int, C.c_double, C.c_double, C.c_double],
             "orc_fft3d_fwd": [dp, dp, G],
             "orc_fft3d_bwd": [dp, dp, G],
             "orc_xy_avg": [dp, dp, G],
@@ -143,6 +148,8 @@ class OracleRun:
         for f, _ in Cfg._fields_:
             if f == "ng":
                 c.ng = ng
+            elif f == "driver":
+                c.driver = int(cfg.get("driver", 0))     # 0 chanfast (default), 1 channel
             else:
                 setattr(c, f, cfg[f])
         self.shape = (cfg["nx"], cfg["ny"], cfg["nz"])
@@ -184,6 +191,6 @@ class OracleRun:
 def oracle_cfg(ccfg, dz):
     """Oracle parameter dict from a chanflow.Config and its dz."""
     keys = [f for f, _ in Cfg._fields_ if f != "ng"]
-    d = {k: getattr(ccfg, k) for k in keys}
+    d = {k: getattr(ccfg, k, 0) for k in keys}
     d["dz"] = dz
     return d

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libchanflow.so")
 
 CF_OK, CF_ERR_ARG, CF_ERR_UNSUPPORTED, CF_ERR_NOMEM, CF_ERR_DEVICE, CF_ERR_NUMERIC, CF_ERR_STATE = range(7)
 CF_FP64, CF_FP32 = 0, 1
+CF_DRIVER_CHANFAST, CF_DRIVER_CHANNEL = 0, 1
 STRETCH = {"cosine": 0, "tanh": 1, "uniform": 2, "user": 3}
 FIELDS = {"u": 0, "v": 1, "w": 2, "c": 3, "p": 4}
 LOG_N = 13
@@ -48,6 +49,7 @@ class _CConfig(C.Structure):
         ("uframe", C.c_double), ("vframe", C.c_double),
         ("betg_x", C.c_double), ("betg_y", C.c_double), ("betg_z", C.c_double),
         ("device", C.c_int),
+        ("driver", C.c_int),
     ]
 
 
@@ -82,6 +84,7 @@ class Config:
     betg_y: float = 0.0
     betg_z: float = 0.0
     device: int = 0
+    driver: int = 0          # 0 chanfast (x-y diffusion explicit), 1 channel (all diffusion implicit)
 
     def to_c(self):
         c = _CConfig()

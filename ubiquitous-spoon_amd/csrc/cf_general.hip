@@ -270,7 +270,11 @@ static __device__ __forceinline__ void rhsu_body(const RunDev &R, const int s,
             f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
                   + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
         }
-        f += lapxy<ORD>(UX, UY, dx, dy, (T) R.nu);
+        /* the x-y diffusion: explicit in f (chanfast, diffuseu.c:286-359), or
+         * in the 3-D Helmholtz operator of h when every diffusion term is
+         * implicit (channel, helmholtz_operator_u, diffuseu.c:362-457) */
+        const T lxy = lapxy<ORD>(UX, UY, dx, dy, (T) 1);
+        if (!R.implicit) f += (T) R.nu * lxy;
         if (SC && R.betg[0] != 0.0) {       /* add_buoyancy2/4, scalar.c:6-76 */
             const size_t r0 = row;
             const T cx = ORD == 2 ? (T) 0.5 * (c[r0 + i] + c[r0 + ip])
@@ -279,11 +283,14 @@ static __device__ __forceinline__ void rhsu_body(const RunDev &R, const int s,
         }
         const SubCoef &C = R.sc[s];
         T h = helm_cell<ORD>(U, dz, k, C.nuadt);
+        if (R.implicit) h += (T) C.nuadt * lxy;
         if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[R.m_prev_slot[s]]);
         else h += (T) C.gamdt * f;
         h -= (T) R.liftg[((size_t) s * 4 + 0) * nz + k];
         if (s < 2) fo[q] = f;                 /* the last substep's tendency is never read */
-        ro[q] = h * dzk;                      /* the solve's right-hand side is h dvol (diffuseu.c:202) */
+        /* the z solve's right-hand side is h dvol (diffuseu.c:202); the
+         * spectral solve takes h and applies dvol itself */
+        ro[q] = R.implicit ? h : h * dzk;
         if (cflpart) {                        /* get_cfl, get.c:390-417 */
             const double dzph = 0.5 * (dz[k] + dz[k + 1]);
             cflv = fmax(fmax(fabs((double) Q0) * cfldt / R.g.dx, fabs((double) v[q]) * cfldt / R.g.dy),
@@ -385,7 +392,8 @@ static __device__ __forceinline__ void rhsv_body(const RunDev &R, const int s,
         f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
               + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
     }
-    f += lapxy<ORD>(VX, VY, dx, dy, (T) R.nu);
+    const T lxy = lapxy<ORD>(VX, VY, dx, dy, (T) 1);
+    if (!R.implicit) f += (T) R.nu * lxy;
     if (SC && R.betg[1] != 0.0) {
         const T c0 = c[q], c1 = c[rowof(k, jp, nx, P) + i];
         const T cy = ORD == 2 ? (T) 0.5 * (c0 + c1)
@@ -394,11 +402,12 @@ static __device__ __forceinline__ void rhsv_body(const RunDev &R, const int s,
     }
     const SubCoef &C = R.sc[s];
     T h = helm_cell<ORD>(V, dz, k, C.nuadt);
+    if (R.implicit) h += (T) C.nuadt * lxy;
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 1) * nz + k];
     if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
-    ro[q] = h * dzk;
+    ro[q] = R.implicit ? h : h * dzk;
 }
 
 template <int ORD, bool SC, typename T>
@@ -492,7 +501,8 @@ static __device__ __forceinline__ void rhsw_body(const RunDev &R, const int s,
         f = -((A1 * (xp - xm) + A2 * (xp3 - xm3)) + (A1 * (yp - ym) + A2 * (yp3 - ym3))
               + (A1 * (zp - zm) + A2 * (zp3 - zm3))) / den;
     }
-    f += lapxy<ORD>(WX, WY, dx, dy, (T) R.nu);
+    const T lxy = lapxy<ORD>(WX, WY, dx, dy, (T) 1);
+    if (!R.implicit) f += (T) R.nu * lxy;
     if (SC && R.betg[2] != 0.0) {
         const T c0 = ucell(i, j, k, c, R.cbot, R.ctop), c1 = ucell(i, j, k + 1, c, R.cbot, R.ctop);
         T cz;
@@ -506,11 +516,12 @@ static __device__ __forceinline__ void rhsw_body(const RunDev &R, const int s,
     }
     const SubCoef &C = R.sc[s];
     T h = helm_face<ORD>(WH, dz, k, C.nuadt);
+    if (R.implicit) h += (T) C.nuadt * lxy;
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * g[q];
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 2) * nz + k];
     if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
-    ro[q] = h * ((T) 0.5 * (DZ(k + 1) + DZ(k)));   /* times the face volume (diffusew.c:205) */
+    ro[q] = R.implicit ? h : h * ((T) 0.5 * (DZ(k + 1) + DZ(k)));   /* times the face volume (diffusew.c:205) */
 }
 
 template <int ORD, bool SC, typename T>
@@ -566,14 +577,16 @@ static __device__ __forceinline__ void rhsc_body(const RunDev &R, const int s,
     const T mv0 = quick(fv0, CY[2], CY[3], CY[4], CY[5]), mvm = quick(fvm, CY[1], CY[2], CY[3], CY[4]);
     const T mw0 = quick(fw0, Cz[2], Cz[3], Cz[4], Cz[5]), mwm = quick(fwm, Cz[1], Cz[2], Cz[3], Cz[4]);
     T f = -((mu0 - mum) + (mv0 - mvm) + (mw0 - mwm)) / (dx * dy * DZ(k));
-    f += lapxy<ORD>(CX, CY, dx, dy, (T) R.kap);
+    const T lxy = lapxy<ORD>(CX, CY, dx, dy, (T) 1);
+    if (!R.implicit) f += (T) R.kap * lxy;
     const SubCoef &C = R.sc[s];
     T h = helm_cell<ORD>(Cz, dz, k, C.kadt);
+    if (R.implicit) h += (T) C.kadt * lxy;
     if (s > 0) h += (T) C.gamdt * f + (T) C.zetdt * (g[q] + (T) scal[SC_MC0 + (s + 2) % 3]);
     else h += (T) C.gamdt * f;
     h -= (T) R.liftg[((size_t) s * 4 + 3) * nz + k];
     if (s < 2) fo[q] = f;                     /* the last substep's tendency is never read */
-    ro[q] = h * DZ(k);
+    ro[q] = R.implicit ? h : h * DZ(k);
 }
 
 template <int ORD, bool SC, typename T>

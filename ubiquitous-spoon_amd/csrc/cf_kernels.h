@@ -103,6 +103,7 @@ struct RunDev {
     double kap;                 /* scalar diffusivity nu / Pr */
     const double *liftg;        /* [s][field u, v, w, c][nz] boundary-lift profiles */
     HeptTab hp[3][3];           /* order-4 implicit z factors [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
+    int implicit;               /* 1: every diffusion term implicit (channel2/4): x-y diffusion in h, not f */
 };
 
 /* A workgroup's coordinates in one run's launch grid: blockIdx / gridDim

@@ -91,6 +91,8 @@ struct cf_run {
     int nb_plane = 1, nb_red = 1;
     RunDev rd;
     PoisTab pt[4];                   /* 3 substeps + ad-hoc dt */
+    PoisTab hs2[3][3];               /* fully implicit driver: spectral Helmholtz tables [s][type], order 2 */
+    Pois4Tab hs4[3][3];              /* ... order 4 */
     Pois4Tab pt4[4];
     size_t off_pois4_adhoc = 0;
     /* general path: order 4 and/or the scalar (cf_general.hip) */
@@ -407,6 +409,11 @@ static cf_status validate(const cf_config *c)
     if (c->scalar && (c->nx < 4 || c->ny < 4)) { cf_set_error("the scalar needs nx, ny >= 4"); return CF_ERR_ARG; }
     if (c->precision != CF_FP64 && c->precision != CF_FP32) { cf_set_error("precision must be CF_FP64 or CF_FP32"); return CF_ERR_ARG; }
     if (c->cflmode != 0 && c->cflmode != 1) { cf_set_error("cflmode must be 0 or 1"); return CF_ERR_ARG; }
+    if (c->driver != CF_DRIVER_CHANFAST && c->driver != CF_DRIVER_CHANNEL) { cf_set_error("driver must be CF_DRIVER_CHANFAST or CF_DRIVER_CHANNEL"); return CF_ERR_ARG; }
+    if (c->driver == CF_DRIVER_CHANNEL && c->precision != CF_FP64) {
+        cf_set_error("the fully implicit driver (CF_DRIVER_CHANNEL) runs in FP64 only");
+        return CF_ERR_UNSUPPORTED;
+    }
     if (c->cflmode == 1 && !(c->want_cfl > 0)) { cf_set_error("cflmode 1 needs want_cfl > 0"); return CF_ERR_ARG; }
     int rad[CF_MAX_RAD];
     if (c->nx % 1 || factor_radix(c->nx, rad) < 0 || c->nx > MAXE_LINE) {
@@ -482,6 +489,7 @@ struct TabPack {
     size_t o_dz = 0, o_th = 0, o_st = 0, o_h1 = 0, o_twx = 0, o_twy = 0, o_twx32 = 0, o_twy32 = 0, o_lx = 0, o_ly = 0;
     size_t o_rdz = 0, o_rdzh = 0, o_lift = 0;
     size_t o_p[4][6], o_sp[3][3][6], o_hp[3][3], o_p4[4];
+    size_t o_hs[3][3];               /* spectral Helmholtz z-tables (fully implicit driver) */
 };
 
 static cf_status build_tables(const cf_config *cfg, const std::vector<double> &dz, double dx, double dy, double dt,
@@ -631,6 +639,45 @@ static cf_status build_tables(const cf_config *cfg, const std::vector<double> &d
         }
     }
 
+    /* fully implicit driver (channel2/4): the spectral Helmholtz z-tables
+     * of each substep and field type (diffuseu.c:460-614, diffusew.c:466-622):
+     * the 1-D system's wall-reduced bands (homogeneous Dirichlet walls) at
+     * zero horizontal wavenumber, dvol nudt / dx^2 and dvol nudt / dy^2 for
+     * the modified wavenumbers, and dvol.  Order 2: a | c | b | ex | ey | dvol
+     * (the Poisson kernel's layout, no extra wall terms); order 4: the seven
+     * bands | ex | ey | dvol. */
+    std::vector<double> hs[3][3];
+    if (cfg->driver == CF_DRIVER_CHANNEL) {
+        for (int s = 0; s < 3; s++)
+            for (int type = 0; type < 3; type++) {
+                const int fam = type == 2 ? cft::FACE : cft::CELL;
+                const int n = fam == cft::CELL ? nz : nz - 1;
+                const double nudt = type == 1 ? -T.sc[s].kbdt : -T.sc[s].nubdt;
+                std::vector<double> &H = hs[s][type];
+                H.assign((size_t) (order == 2 ? 6 : 10) * n, 0.0);
+                const size_t bo = (size_t) (order == 2 ? 3 : 7) * n;
+                for (int k = 0; k < n; k++) {
+                    const double dvol = fam == cft::CELL ? dzl[k] : 0.5 * (dzl[k] + dzl[k + 1]);
+                    H[bo + k] = dvol * nudt / (dx * dx);
+                    H[bo + n + k] = dvol * nudt / (dy * dy);
+                    H[bo + 2 * n + k] = dvol;
+                }
+                if (order == 2) {
+                    std::vector<double> a, b, c;
+                    if (fam == cft::CELL) cell2_system(dzl, nz, nudt, a, b, c);
+                    else face2_system(dzl, nz, nudt, a, b, c);
+                    for (int k = 0; k < n; k++) { H[k] = a[k]; H[n + k] = c[k]; H[2 * n + k] = b[k]; }
+                } else {
+                    const Clos1 *cl = fam == cft::CELL ? clo.cell : clo.face;
+                    std::vector<cft::Row7> rows;
+                    cft::op_rows(fam, 4, dzl, nz, nudt, 1.0, true, rows);
+                    cft::eliminate(fam, nz, cl[0], cl[1], rows);
+                    for (int t = 0; t < 7; t++)
+                        for (int k = 0; k < n; k++) H[(size_t) t * n + k] = rows[k].c[t];
+                }
+            }
+    }
+
     /* FFT plans: radices, twiddles (host libm), modified wavenumbers */
     const int nkx = nx / 2 + 1;
     std::vector<double> twx(2 * nx), twy(2 * ny), lamx(nkx), lamy(ny);
@@ -710,6 +757,8 @@ static cf_status build_tables(const cf_config *cfg, const std::vector<double> &d
             for (int t = 0; t < 3; t++) o_hp[s][t] = put(hept[s][t].data(), hept[s][t].size());
         for (int q = 0; q < 4; q++) o_p4[q] = put(p4[q].data(), p4[q].size());
     }
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) T.o_hs[s][t] = hs[s][t].empty() ? 0 : put(hs[s][t].data(), hs[s][t].size());
 
     return CF_OK;
 }
@@ -765,6 +814,21 @@ static void apply_tables(cf_run *r, const TabPack &TP)
     r->off_pois_adhoc = TP.o_p[3][0];
     r->off_pois4_adhoc = TP.o_p4[3];
     r->pt_dt = 1.0;
+    if (r->cfg.driver == CF_DRIVER_CHANNEL)
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 3; t++) {
+                const size_t n = (size_t) (t == 2 ? nz - 1 : nz);
+                const double *b = tab + TP.o_hs[s][t];
+                if (order == 2) {
+                    PoisTab &H = r->hs2[s][t];
+                    H.a = b; H.c = b + n; H.bz = b + 2 * n; H.ex = b + 3 * n; H.ey = b + 4 * n; H.dvol = b + 5 * n;
+                    H.db_neu = H.db_dir = H.db_top = 0.0;
+                } else {
+                    Pois4Tab &H = r->hs4[s][t];
+                    H.band[0] = H.band[1] = b;
+                    H.ex = b + 7 * n; H.ey = b + 8 * n; H.dvol = b + 9 * n;
+                }
+            }
 }
 
 static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out);
@@ -811,7 +875,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     const int order = cfg->order;
     r->prec = cfg->precision;
     r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
-    r->general = force_general || order == 4 || cfg->scalar;
+    r->general = force_general || order == 4 || cfg->scalar || cfg->driver == CF_DRIVER_CHANNEL;
     r->nfield = cfg->scalar ? 4 : 3;
     r->rd.m_prev_slot[0] = SC_M0 + 2;
     r->rd.m_prev_slot[1] = SC_M0 + 0;
@@ -882,6 +946,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.scalar = cfg->scalar;
     R.betg[0] = cfg->betg_x; R.betg[1] = cfg->betg_y; R.betg[2] = cfg->betg_z;
     R.kap = cfg->scalar ? cfg->nu / cfg->prandtl : cfg->nu;
+    R.implicit = cfg->driver == CF_DRIVER_CHANNEL;
     r->kfxp = pick_fftx_plain(nx);
     r->kfy = pick_ffty<double>(ny);
     r->kfxi = pick_fftx_inv<double>(nx);
@@ -1112,8 +1177,41 @@ static cf_status enqueue_step(cf_run *r, int from)
     return r->prec == CF_FP32 ? enqueue_step_t<float>(r, from) : enqueue_step_t<double>(r, from);
 }
 
-/* one RK3 step of the general path (order 4, the scalar, or FP32 storage):
- * the state (u, v, w, c) is materialised after every substep */
+/* The fully implicit update of one field in place (channel2/4, e.g.
+ * channel.c:458-467): FFT3D_FWD of its right-hand side h, per (kx, ky) the
+ * wall-reduced banded z solve with the modified wavenumbers on the
+ * diagonal (solve_helmholtz_u/w, diffuseu.c:460-614, diffusew.c:466-622),
+ * FFT3D_BWD.  The x-y passes are the pressure path's; the z solve is the
+ * Poisson kernel of the same order on the Helmholtz tables, over nz cells
+ * or the nz - 1 interior faces. */
+static cf_status enqueue_spectral_solve(cf_run *r, int s, int type, double *fld)
+{
+    const RunDev &R = r->rd;
+    RunDev Rn = R;
+    const int n = type == 2 ? r->g.nz - 1 : r->g.nz;
+    Rn.g.nz = n;
+    double2 *X = r->X;
+    const long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
+    const unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
+    const dim3 gy(fy_blocks(r));
+    KLAUNCH("hsolve", r->kfxp, dim3(gx), dim3(256), fx_lds(r), R, (const double *) fld, X);
+    KLAUNCH("hsolve", r->kfy, gy, dim3(256), fy_lds(r), R, X, 1);
+    if (r->g.order == 4) {
+        KLAUNCH("hsolve", k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                dim3(CF_P4_THREADS), 0, Rn, r->hs4[s][type], X, r->S4);
+    } else {
+        const int L = pois2_L(n);
+        KLAUNCH("hsolve", pick_poisson2<double>(L), dim3(pois2_blocks(L, r->M)), dim3(CF_P2S_THREADS), pois2_lds(L, n),
+                Rn, r->hs2[s][type], X, r->G);
+    }
+    KLAUNCH("hsolve", r->kfy, gy, dim3(256), fy_lds(r), R, X, 0);
+    KLAUNCH("hsolve", r->kfxi, dim3(gx), dim3(256), fx_lds(r), R, (const double2 *) X, fld);
+    return CF_OK;
+}
+
+/* one RK3 step of the general path (order 4, the scalar, FP32 storage or
+ * the fully implicit driver): the state (u, v, w, c) is materialised after
+ * every substep */
 template <typename T>
 static cf_status enqueue_step_general_t(cf_run *r)
 {
@@ -1141,7 +1239,18 @@ static cf_status enqueue_step_general_t(cf_run *r)
             KLAUNCH("rhs", K.c, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[3]), (const double *) r->scal,
                     tv<T>(fnew[3]), rc, (double *) nullptr, dt);
         const unsigned nf = (unsigned) r->nfield;
-        if (r->g.order == 4)
+        if constexpr (std::is_same<T, double>::value) {
+            if (R.implicit) {
+                cf_status st = enqueue_spectral_solve(r, s, 0, ru);
+                if (st == CF_OK) st = enqueue_spectral_solve(r, s, 0, rv);
+                if (st == CF_OK && sc) st = enqueue_spectral_solve(r, s, 1, rc);
+                if (st == CF_OK) st = enqueue_spectral_solve(r, s, 2, rw);
+                if (st != CF_OK) return st;
+            }
+        }
+        if (R.implicit) {
+            /* done above */
+        } else if (r->g.order == 4)
             KLAUNCH("solve", k_gsolve4<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
         else if (r->use_spike)
             KLAUNCH("solve", pick_spike<T>(r->sp_L), dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
@@ -2379,6 +2488,11 @@ extern "C" cf_status cf_batch_create(const cf_config *cfgs, int nrun, cf_batch *
     *out = nullptr;
     if (nrun < 1 || nrun > CF_BATCH_MAX) { cf_set_error("a batch holds 1..%d runs", CF_BATCH_MAX); return CF_ERR_ARG; }
     for (int q = 0; q < nrun; q++)
+        if (cfgs[q].driver != CF_DRIVER_CHANFAST) {
+            cf_set_error("run %d: the fully implicit driver steps as a run of its own", q);
+            return CF_ERR_UNSUPPORTED;
+        }
+    for (int q = 0; q < nrun; q++)
         if (cfgs[q].cflmode != 0) {
             cf_set_error("run %d: cflmode 1 changes dt every step; step it as a run of its own", q);
             return CF_ERR_UNSUPPORTED;
@@ -2932,7 +3046,8 @@ static cf_status srv_setup(cf_server *s, cf_server::Req &r)
      * (the order-2 fast path where it applies) on their own stream; small
      * ones join their group's batched launches.  The choice depends on the
      * request alone, never on the mix. */
-    r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1;
+    r.solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1
+                            || r.cfg.driver != CF_DRIVER_CHANFAST;
     const bool fast = !r.solo && fast_eligible(r.cfg);
     if ((st = create_run(&r.cfg, !(r.solo || fast), &run)) != CF_OK) return st;
     r.gkey = group_key(r.cfg) + (fast && !run->general && run->use_spike ? 8 : 0);
@@ -2957,7 +3072,8 @@ static int srv_admit(cf_server *s)
     size_t qi = 0;
     while (qi < s->queue.size()) {
         cf_server::Req &r = s->reqs[s->queue[qi]];
-        const bool solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1;
+        const bool solo = (size_t) r.cfg.nx * r.cfg.ny * r.cfg.nz >= s->solo_points || r.cfg.cflmode == 1
+                            || r.cfg.driver != CF_DRIVER_CHANFAST;
         if (!solo && srv_group_count(s, r.cfg) >= CF_BATCH_MAX) { qi++; continue; }   /* its group is full */
         /* head-of-line for memory: a large request waits for memory rather
          * than being overtaken indefinitely */

@@ -28,6 +28,7 @@ void cf_config_default(cf_config *cfg)
     cfg->uframe = 0.0; cfg->vframe = 0.0;
     cfg->betg_x = cfg->betg_y = cfg->betg_z = 0.0;
     cfg->device = 0;
+    cfg->driver = CF_DRIVER_CHANFAST;
 }
 
 cf_status cf_grid_dz(const cf_config *cfg, double *dz)

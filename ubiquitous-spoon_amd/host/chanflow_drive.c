@@ -24,7 +24,7 @@
  *                  [--dpdx ..] [--ubulk ..] [--uvolavgmode 0|1]
  *                  [--cbulk ..] [--cvolavgmode 0|1] [--uframe ..]
  *                  [--vframe ..] [--betg bx by bz] [--stretch cosine|tanh]
- *                  [--beta ..] [--cflmode 0|1] [--want-cfl ..]
+ *                  [--beta ..] [--cflmode 0|1] [--want-cfl ..] [--driver chanfast|channel]
  *                  [--nghost ..] [--no-stats] [--project]
  */
 #include "chanflow.h"
@@ -144,6 +144,13 @@ int main(int argc, char **argv)
         if (!strcmp(k, "--betg") && a + 3 < argc) {
             cfg.betg_x = atof(argv[a + 1]); cfg.betg_y = atof(argv[a + 2]); cfg.betg_z = atof(argv[a + 3]);
             a += 3;
+            continue;
+        }
+        if (!strcmp(k, "--driver") && v) {     /* chanfast2/4 or channel2/4 (main.c:52-55) */
+            if (!strcmp(v, "channel")) cfg.driver = CF_DRIVER_CHANNEL;
+            else if (!strcmp(v, "chanfast")) cfg.driver = CF_DRIVER_CHANFAST;
+            else { fprintf(stderr, "chanflow_drive: --driver chanfast|channel\n"); return 2; }
+            a++;
             continue;
         }
         if (!strcmp(k, "--stretch") && v) {

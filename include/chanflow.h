@@ -186,6 +186,16 @@ enum { CF_STAT_AVG = 0, CF_STAT_CORR = 1, CF_STAT_MOMENT = 2, CF_STAT_SPEC = 3 }
  * work values rather than flow statistics; those are not computed. */
 cf_status cf_run_xy_stats(cf_run *run, int kind, int field_a, int field_b, double en, double *out);
 
+/* cf_run_xy_stats for kinds CF_STAT_AVG, CF_STAT_CORR and CF_STAT_MOMENT
+ * without any device allocation: the work space (partial sums, means,
+ * result) is a fixed array in device memory, one per GPU for the life of
+ * the process, sized for nz <= 2048 levels (larger runs get CF_ERR_ARG).
+ * No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Calls that overlap on one GPU share that array
+ * and race; results equal cf_run_xy_stats bit for bit. */
+cf_status cf_run_xy_stats_noalloc(cf_run *run, int kind, int field_a, int field_b, double en, double *out);
+
 /* ---- operator-level entry points (parity tests of single stages) ---- */
 
 /* Layout transpose io [nx][ny][nz] <-> device [nz][ny][nx] through the

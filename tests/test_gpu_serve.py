@@ -109,3 +109,47 @@ def test_mixed_stream_completes():
         u = srv.result(res["ids"][q], "u")
         assert np.isfinite(u).all()
     srv.close()
+
+
+def test_served_workload_grids_same_alone_and_in_the_mix():
+    """The served stream's own grids (x, wall-normal, span = 64x65x64,
+    128x97x128, 192x129x192, 256x193x256), orders 2 and 4, FP32 and FP64:
+    two large runs routed to their own graphs (solo), small and medium runs
+    in group batches, an FP32 run and an FP64 run joining while the others
+    are mid-flight.  Each result is bit-identical to the same request served
+    alone (every reduction is per run and in a fixed order; each group's
+    solve shape and each run's FFT lengths depend on the run alone), keeps
+    its bulk velocity and ends divergence free."""
+    def req(n, nzw, order, prec, seed, nu):
+        c = cf.Config(nx=n, ny=n, nz=nzw, order=order, precision=prec, stretch="tanh", lx=2 * np.pi,
+                      ly=np.pi, lz=2.0, nu=nu, dt=2.5e-4)
+        return c, cf.initial_fields(c, seed=seed)
+
+    first = [req(256, 193, 2, cf.CF_FP64, 1, 1.0 / 6900.0), req(192, 129, 4, cf.CF_FP32, 2, 1.0 / 3300.0),
+             req(64, 65, 4, cf.CF_FP64, 3, 1.0 / 2800.0), req(128, 97, 2, cf.CF_FP64, 4, 1.0 / 4000.0)]
+    late = [req(128, 97, 2, cf.CF_FP32, 5, 1.0 / 3000.0), req(64, 65, 2, cf.CF_FP64, 6, 1.0 / 2000.0)]
+    steps = [12, 8, 30, 20, 24, 36]
+    budget = 200 << 30
+    alone = []
+    for q, (c, f) in enumerate(first + late):
+        with serve.Server(budget, quantum=5) as srv:
+            rid = srv.submit(c, f, steps[q], retain=True)
+            drain(srv)
+            alone.append(({n: srv.result(rid, n) for n in "uvw"}, srv.log(rid)))
+    with serve.Server(budget, quantum=5) as srv:
+        ids = [srv.submit(c, f, steps[q], retain=True) for q, (c, f) in enumerate(first)]
+        srv.tick()
+        srv.tick()
+        for q, (c, f) in enumerate(late):          # join while the first four are mid-flight
+            ids.append(srv.submit(c, f, steps[len(first) + q], retain=True))
+        drain(srv)
+        for q, rid in enumerate(ids):
+            c = (first + late)[q][0]
+            lg = srv.log(rid)
+            assert lg.shape[0] == steps[q]
+            for n in "uvw":
+                got = srv.result(rid, n)
+                assert np.array_equal(got, alone[q][0][n]), (q, c.shape, c.order, c.precision, n)
+            assert np.array_equal(lg, alone[q][1]), q
+            tol = 1e-5 if c.precision == cf.CF_FP32 else 1e-12
+            assert np.allclose(lg[:, 6], 1.0, rtol=0, atol=tol), q

@@ -276,7 +276,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps (median; after 1 warm-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=50, help="timed steps of each other form (0: skip)")
-    ap.add_argument("--served", type=int, default=24, help="requests of the served stream (0: skip)")
+    ap.add_argument("--served", type=int, default=100, help="requests of the served stream (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()

@@ -254,8 +254,11 @@ cf_status cf_batch_tuning(cf_batch *batch, char *buf, size_t len);
  * completed (crediting steps, retiring runs that reached their count),
  * admits queued requests in arrival order while their footprint
  * (cf_run_footprint) fits the HBM budget and projects them once (dt = 1,
- * as cf_run_project), and enqueues the next quantum of every idle group and
- * solo run; if nothing moved it first waits for one quantum to complete.
+ * as cf_run_project), and enqueues the next quantum of every idle group
+ * and of the solo runs with the least remaining cells x steps while fewer
+ * than the solo limit (cf_server_set_solo_limit, default 1) are in flight;
+ * if nothing moved it first waits for one quantum to complete.  Runs with
+ * cflmode 1 or the fully implicit driver always step as solo runs.
  * Groups never wait for each other.  A request whose group is full waits
  * while later ones pass; a request that cannot be set up is marked failed
  * (cf_server_request_state) and dropped, and the server goes on.  Times
@@ -268,6 +271,9 @@ cf_status cf_server_destroy(cf_server *server);
 /* runs with nx ny nz >= points step on their own (0: every run; SIZE_MAX:
  * none); applies to requests admitted afterwards */
 cf_status cf_server_set_solo_points(cf_server *server, size_t points);
+/* at most n solo runs advance at once (default 1): each quantum goes to
+ * the solo runs with the least remaining cells x steps */
+cf_status cf_server_set_solo_limit(cf_server *server, int n);
 /* copies the io-layout initial fields (c may be NULL without the scalar);
  * retain != 0 keeps the finished run resident for cf_server_result */
 cf_status cf_server_submit(cf_server *server, const cf_config *cfg, const double *u, const double *v,

@@ -153,3 +153,26 @@ def test_served_workload_grids_same_alone_and_in_the_mix():
             assert np.array_equal(lg, alone[q][1]), q
             tol = 1e-5 if c.precision == cf.CF_FP32 else 1e-12
             assert np.allclose(lg[:, 6], 1.0, rtol=0, atol=tol), q
+
+
+def test_solo_runs_shortest_remaining_work_first():
+    """With one solo run advancing at a time, the quantum goes to the solo
+    run with the least remaining cells x steps: a short request submitted
+    after a long one finishes first, and both results are the same as when
+    each is served alone."""
+    long_c, short_c = cfg_of((32, 24, 33)), cfg_of((24, 16, 25))
+    fl, fs = cf.initial_fields(long_c, seed=11), cf.initial_fields(short_c, seed=12)
+    alone = []
+    for c, f, n in ((long_c, fl, 40), (short_c, fs, 10)):
+        with serve.Server(8 << 30, quantum=5, solo_points=0) as srv:
+            rid = srv.submit(c, f, n, retain=True)
+            drain(srv)
+            alone.append(srv.result(rid, "u"))
+    with serve.Server(8 << 30, quantum=5, solo_points=0, solo_limit=1) as srv:
+        a = srv.submit(long_c, fl, 40, retain=True)
+        b = srv.submit(short_c, fs, 10, retain=True)
+        drain(srv)
+        ta_done, tb_done = srv.info(a)[2], srv.info(b)[2]
+        assert tb_done < ta_done
+        assert np.array_equal(srv.result(a, "u"), alone[0])
+        assert np.array_equal(srv.result(b, "u"), alone[1])

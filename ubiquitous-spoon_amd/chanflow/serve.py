@@ -24,6 +24,7 @@ def _bind():
         "cf_server_create": (C.c_int, [C.c_size_t, C.c_int, C.c_int, C.POINTER(P)]),
         "cf_server_destroy": (C.c_int, [P]),
         "cf_server_set_solo_points": (C.c_int, [P, C.c_size_t]),
+        "cf_server_set_solo_limit": (C.c_int, [P, C.c_int]),
         "cf_server_submit": (C.c_int, [P, C.POINTER(_CConfig), dp, dp, dp, dp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "cf_server_tick": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "cf_server_result": (C.c_int, [P, C.c_int, C.c_int, dp]),
@@ -47,7 +48,7 @@ class Server:
     least solo_points cells (None: the library default) step through their
     own graphs beside the batches."""
 
-    def __init__(self, hbm_budget_bytes=200 << 30, device=0, quantum=20, solo_points=None):
+    def __init__(self, hbm_budget_bytes=200 << 30, device=0, quantum=20, solo_points=None, solo_limit=None):
         L = _bind()
         h = C.c_void_p()
         _check(L.cf_server_create(int(hbm_budget_bytes), device, quantum, C.byref(h)))
@@ -55,6 +56,8 @@ class Server:
         self.cfgs = {}
         if solo_points is not None:
             _check(L.cf_server_set_solo_points(h, int(solo_points)))
+        if solo_limit is not None:
+            _check(L.cf_server_set_solo_limit(h, int(solo_limit)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -153,12 +156,12 @@ def poisson_stream(n, rate=20.0, seed=7, grids=GRIDS, steps=(50, 500), p_order4=
     return out
 
 
-def serve(requests, budget=200 << 30, quantum=20, retain=False, time_scale=1.0, solo_points=None):
+def serve(requests, budget=200 << 30, quantum=20, retain=False, time_scale=1.0, solo_points=None, solo_limit=None):
     """Drive a Server in real time: each request is submitted once its
     arrival time (scaled by time_scale) has passed; tick until all are
     done.  Returns per-request latencies (completion - arrival, s), total
     wall time, cell-steps and the server for retained results."""
-    srv = Server(budget, quantum=quantum, solo_points=solo_points)
+    srv = Server(budget, quantum=quantum, solo_points=solo_points, solo_limit=solo_limit)
     pending = sorted(range(len(requests)), key=lambda q: requests[q].t_arrive)
     ids = {}
     t0 = time.perf_counter()

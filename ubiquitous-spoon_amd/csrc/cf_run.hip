@@ -2737,7 +2737,9 @@ extern "C" cf_status cf_batch_tuning(cf_batch *b, char *buf, size_t len)
  *      runs waits and lets later ones pass; one that fails to set up is
  *      marked failed and dropped);
  *   3. rebuilds the batch of every idle group whose membership changed;
- *   4. enqueues the next quantum of every idle group and solo run;
+ *   4. enqueues the next quantum of every idle group, and of the solo runs
+ *      with the least remaining work while fewer than solo_limit are in
+ *      flight;
  * and, when none of that made progress, waits for the first quantum in
  * flight to complete and does 1, 3 and 4 once more.  Groups therefore
  * never wait for each other: a group of small runs re-enqueues as soon as
@@ -2786,6 +2788,7 @@ struct cf_server {
     std::vector<std::pair<cf_run *, size_t>> retired;   /* runs to free once the device is idle */
     size_t retired_bytes = 0;
     size_t solo_points = (size_t) 4 << 20;         /* runs of at least this many cells step on their own */
+    int solo_limit = 1;                            /* solo runs advancing at once */
     double pt_steps = 0.0;                         /* completed cell-steps */
 };
 
@@ -2818,6 +2821,13 @@ static void swap_state_sets(cf_run *r)
         if (r->gexec[q]) { (void) hipGraphExecDestroy(r->gexec[q]); r->gexec[q] = nullptr; }
         if (r->graph[q]) { (void) hipGraphDestroy(r->graph[q]); r->graph[q] = nullptr; }
     }
+}
+
+extern "C" cf_status cf_server_set_solo_limit(cf_server *s, int n)
+{
+    if (!s || n < 1) { cf_set_error("cf_server_set_solo_limit: need a server and n >= 1"); return CF_ERR_ARG; }
+    s->solo_limit = n;
+    return CF_OK;
 }
 
 extern "C" cf_status cf_server_set_solo_points(cf_server *s, size_t points)
@@ -3128,11 +3138,27 @@ static cf_status srv_launch(cf_server *s)
         HIPCHK(hipEventRecord(G.ev, b->stream));
         G.inflight = k;
     }
+    /* Solo runs each fill the device, so running several at once only
+     * shares it: at most solo_limit advance, the ones with the least
+     * remaining work (cells x steps) first, ties in admission order.  Short
+     * runs then finish early instead of every large run finishing late (the
+     * mean latency of a shared server is least when it serves the shortest
+     * remaining work first); the choice is made again every quantum. */
+    int busy = 0;
+    std::vector<std::pair<double, size_t>> ready;
     for (size_t q = 0; q < s->reqs.size(); q++) {
-        cf_server::Req &r = s->reqs[q];
-        if (r.state != RQ_ACTIVE || !r.solo || r.inflight) continue;
+        const cf_server::Req &r = s->reqs[q];
+        if (r.state != RQ_ACTIVE || !r.solo) continue;
+        if (r.inflight) { busy++; continue; }
+        if (r.nsteps - r.done <= 0) continue;
+        ready.push_back({ (double) r.cfg.nx * r.cfg.ny * r.cfg.nz * (double) (r.nsteps - r.done), q });
+    }
+    std::sort(ready.begin(), ready.end());
+    for (const auto &rq : ready) {
+        if (busy >= s->solo_limit) break;
+        cf_server::Req &r = s->reqs[rq.second];
         const int k = std::min(s->quantum, r.nsteps - r.done);
-        if (k <= 0) continue;
+        busy++;
         if ((st = cf_run_step(r.run, k)) != CF_OK) return st;   /* counts its own steps */
         if (!r.ev) HIPCHK(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
         HIPCHK(hipEventRecord(r.ev, r.run->stream));

@@ -112,22 +112,32 @@ def time_form(name, cfg, steps, warmup, names="uvw", bytes_per_pt=None):
     return out
 
 
-def other_forms(steps):
+def other_forms(steps, only=None):
     forms = {}
+    if only == "order4":
+        return {"order4_scalar_192x129x192": time_form("FP64 order 4 + scalar (Pr 0.71), 192x129x192", ORDER4_CFG(),
+                                                       steps, 5, names="uvwc", bytes_per_pt=816.0)}
     c3 = cf.Config(nx=256, ny=256, nz=193, order=2, stretch="tanh", stretch_beta=2.0, lx=2 * np.pi, ly=np.pi,
                    lz=2.0, nu=1.0 / 6900.0, dt=2.5e-4, precision=cf.CF_FP32)
     forms["fp32_256x193x256"] = time_form("FP32 form (FP64 islands), 256x193x256 order 2", c3, steps, 5,
                                           bytes_per_pt=360.0)
-    c2 = cf.Config(nx=192, ny=192, nz=129, order=4, scalar=1, prandtl=0.71, stretch="tanh", stretch_beta=2.0,
-                   lx=2 * np.pi, ly=np.pi, lz=2.0, nu=1.0 / 3300.0, dt=5e-4)
-    forms["order4_scalar_192x129x192"] = time_form("FP64 order 4 + scalar (Pr 0.71), 192x129x192", c2, steps, 5,
-                                                   names="uvwc", bytes_per_pt=816.0)
+    if only == "fp32":
+        return forms
+    forms["order4_scalar_192x129x192"] = time_form("FP64 order 4 + scalar (Pr 0.71), 192x129x192", ORDER4_CFG(),
+                                                   steps, 5, names="uvwc", bytes_per_pt=816.0)
     return forms
 
 
-def run_batch(cfgs, fields, steps):
+def ORDER4_CFG():
+    """The 4th-order + scalar workload: 192 x 129 x 192, Pr 0.71, nu 1/3300, dt 5e-4."""
+    return cf.Config(nx=192, ny=192, nz=129, order=4, scalar=1, prandtl=0.71, stretch="tanh", stretch_beta=2.0,
+                     lx=2 * np.pi, ly=np.pi, lz=2.0, nu=1.0 / 3300.0, dt=5e-4)
+
+
+def run_batch(cfgs, fields, steps, total=None):
     """Create (and autotune) a batch, project, 2 warm-up steps, then time
-    `steps`; returns seconds, creation time, autotuner lines and the fields."""
+    `steps`, then step on untimed to `total` steps in all (if given);
+    returns seconds, creation time, autotuner lines and the fields."""
     t_create = time.perf_counter()
     with cf.Batch(cfgs) as b:
         t_create = time.perf_counter() - t_create
@@ -141,6 +151,9 @@ def run_batch(cfgs, fields, steps):
         b.step(steps)
         b.sync()
         secs = time.perf_counter() - t0
+        if total is not None and total > steps + 2:
+            b.step(total - steps - 2)
+            b.sync()
         tuning = b.tuning()
         out = [{n: b.download(q, n) for n in "uvw"} for q in range(len(cfgs))]
     return secs, t_create, tuning, out
@@ -158,12 +171,13 @@ def batch48(steps):
             for n in (64, 96, 128) for _ in range(16)]
     cells = sum(c.nx * c.ny * c.nz for c in cfgs)
     fields = [cf.initial_fields(c, seed=300 + q) for q, c in enumerate(cfgs)]
-    secs, t_create, tuning, ref = run_batch(cfgs, fields, steps)
+    total = 100                              # the error is stated after the workload's 100 steps
+    secs, t_create, tuning, ref = run_batch(cfgs, fields, steps, total)
     c32 = []
     for c in cfgs:
         d = cf.Config(nx=c.nx, ny=c.ny, nz=c.nz, stretch="tanh", nu=c.nu, dt=c.dt, precision=cf.CF_FP32)
         c32.append(d)
-    secs32, t_create32, tuning32, got = run_batch(c32, fields, steps)
+    secs32, t_create32, tuning32, got = run_batch(c32, fields, steps, total)
     err = {n: max(float(np.linalg.norm(got[q][n] - ref[q][n]) / np.linalg.norm(ref[q][n])) for q in range(len(cfgs)))
            for n in "uvw"}
     t1 = 0.0
@@ -187,7 +201,7 @@ def batch48(steps):
            "fp32_batch": {"value": cells / 1e6 * steps / secs32, "ms_per_step": 1e3 * secs32 / steps,
                           "create_and_autotune_s": t_create32,
                           "speedup_vs_fp64_one_at_a_time": t1 / secs32,
-                          "max_rel_l2_vs_fp64_batch": err, "after_steps": steps + 2}}
+                          "max_rel_l2_vs_fp64_batch": err, "after_steps": total}}
     print(f"[bench] batch48: {out['ms_per_step']:.3f} ms/step, FP32 {1e3 * secs32 / steps:.3f} ms/step "
           f"(err {max(err.values()):.2e}), one at a time {1e3 * t1 / steps:.3f} ms/step", file=sys.stderr, flush=True)
     return out
@@ -279,7 +293,12 @@ def main():
     ap.add_argument("--served", type=int, default=100, help="requests of the served stream (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--form", choices=["fp32", "order4"], default=None,
+                    help="time only this other form and print its record (for profiling it alone)")
     args = ap.parse_args()
+    if args.form:
+        print(json.dumps(other_forms(max(args.form_steps, 1), only=args.form)))
+        return
     if args.copy_calibrate:
         import torch
         n = args.copy_calibrate // 8

@@ -69,13 +69,23 @@ def test_fp32_against_fp64_oracle(order, scalar, shape, steps):
 
 
 def test_fp32_uploads_round_to_nearest():
+    """v and w are stored rounded to nearest FP32; u is stored as its
+    deviation from its FP64 plane-mean profile, rounded to nearest, so a u
+    with a large mean keeps far more digits than FP32 rounding of u itself."""
     cfg = cf.Config(nx=16, ny=12, nz=17, stretch="tanh", precision=cf.CF_FP32)
     rng = np.random.default_rng(2)
     a = rng.standard_normal(cfg.shape)
+    u = 1.5 + np.linspace(0.0, 0.3, cfg.nz)[None, None, :] + 0.01 * rng.standard_normal(cfg.shape)
     with cf.Run(cfg) as run:
-        run.upload("u", a)
+        run.upload("v", a)
+        assert np.array_equal(run.download("v"), a.astype(np.float32).astype(np.float64))
+        run.upload("u", u)
         back = run.download("u")
-    assert np.array_equal(back, a.astype(np.float32).astype(np.float64))
+    dev = u - u.mean(axis=(0, 1))[None, None, :]
+    # at most half an FP32 ulp of the deviation (plus the profile's own FP64 rounding)
+    assert np.all(np.abs(back - u) <= np.abs(dev) * 2.0 ** -24 + 1e-15)
+    plain = np.abs(u.astype(np.float32).astype(np.float64) - u).max()
+    assert np.abs(back - u).max() < plain / 20
 
 
 def test_fp32_form_vs_fp64_256x193x256_200_steps():

@@ -16,6 +16,7 @@ enum {
     SC_CFL = 7,     /* cfl at the start of the current step */
     SC_ZERO = 8,    /* always 0 */
     SC_CFL0 = 9,    /* cflmode 1: cfl of the state at the start of a step, with the previous dt */
+    SC_TICKET = 10, /* k_level_mass: workgroups done, as a 64-bit counter (0 between launches) */
     SC_N = 16
 };
 
@@ -104,6 +105,12 @@ struct RunDev {
     const double *liftg;        /* [s][field u, v, w, c][nz] boundary-lift profiles */
     HeptTab hp[3][3];           /* order-4 implicit z factors [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
     int implicit;               /* 1: every diffusion term implicit (channel2/4): x-y diffusion in h, not f */
+    /* FP32 storage on the fused path: u is stored as u - U(z), U a per-level
+     * FP64 profile, so storage rounding scales with the deviation from U
+     * instead of with u.  Layout: [nz + 2] U with the wall ghosts at 0 and
+     * nz + 1, then [3][nz] U's share of each substep's implicit right-hand
+     * side.  Null: no offset (FP64, or the general path). */
+    const double *uoff;
 };
 
 /* A workgroup's coordinates in one run's launch grid: blockIdx / gridDim
@@ -267,8 +274,10 @@ static __device__ __forceinline__ Blk blk_of(const GArgs<T> &a, int stage, unsig
     return b;
 }
 
-template <typename T> __global__ void k_io_to_dev(const double *io, T *dev, int nx, int ny, int nz);
-template <typename T> __global__ void k_dev_to_io(const T *dev, double *io, int nx, int ny, int nz);
+/* off (may be null): a per-level profile subtracted on the way in and
+ * added on the way out */
+template <typename T> __global__ void k_io_to_dev(const double *io, T *dev, int nx, int ny, int nz, const double *off);
+template <typename T> __global__ void k_dev_to_io(const T *dev, double *io, int nx, int ny, int nz, const double *off);
 template <typename T>
 __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrcT<T> S, const T *gu, const T *gv, const T *gw,
                         const double *scal, T *fu, T *fv, T *fw, T *ru, T *rv, T *rw, double *cflpart, double cfldt,
@@ -284,6 +293,8 @@ __global__ void k_mass_finalize(const RunDev R, const int s, const double *lsum,
 template <typename T>
 __global__ void k_cfl_plain(const RunDev R, const T *u, const T *v, const T *w, double dt, double *part);
 __global__ void k_cfl_reduce(const double *part, int nb, double *scal);
+__global__ void k_level_mass(const RunDev R, const int s, const double *plane, int nbx, double *lsum, double *scal,
+                             double *logbuf, int step_end);
 __global__ void k_cfl_finish(const RunDev R, const double *part, int nb, double dt,
                              double *scal, double *logbuf);
 __global__ void k_step_end(double *scal);
@@ -372,7 +383,7 @@ __global__ void k_poisson4(const RunDev R, const Pois4Tab T_, typename Cplx<T>::
 template <typename T, int MODE>
 __global__ void k_stat_plane(const T *a, const T *b, const double *ma, const double *mb, double en, int P, int nbl,
                              double *part);
-__global__ void k_stat_finish(const double *part, int nbl, int nz, double scale, double *out);
+__global__ void k_stat_finish(const double *part, int nbl, int nz, double scale, double *out, const double *off);
 __global__ void k_stat_spec(const double2 *A, const double2 *B, int nx, int ny, int nkx, int nz, double *specx,
                             double *specy);
 template <typename T> __global__ void k_stat_widen(const T *in, double *out, size_t n);

@@ -22,7 +22,7 @@ static constexpr double A2_ = -0.125 / 3.0;
  * ------------------------------------------------------------------- */
 template <typename T>
 __global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io, T *__restrict__ dev,
-                                                   int nx, int ny, int nz)
+                                                   int nx, int ny, int nz, const double *__restrict__ off)
 {
     __shared__ double tile[32][33];
     const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32, j = blockIdx.z;
@@ -34,20 +34,21 @@ __global__ __launch_bounds__(256) void k_io_to_dev(const double *__restrict__ io
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, i = i0 + tx;
-        if (i < nx && k < nz) dev[((size_t) k * ny + j) * nx + i] = (T) tile[tx][r];
+        if (i < nx && k < nz) dev[((size_t) k * ny + j) * nx + i] = (T) (off ? tile[tx][r] - off[k] : tile[tx][r]);
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dev_to_io(const T *__restrict__ dev, double *__restrict__ io,
-                                                   int nx, int ny, int nz)
+                                                   int nx, int ny, int nz, const double *__restrict__ off)
 {
     __shared__ double tile[32][33];
     const int i0 = blockIdx.x * 32, k0 = blockIdx.y * 32, j = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, i = i0 + tx;
-        if (i < nx && k < nz) tile[r][tx] = (double) dev[((size_t) k * ny + j) * nx + i];
+        if (i < nx && k < nz) tile[r][tx] = off ? (double) dev[((size_t) k * ny + j) * nx + i] + off[k]
+                                                : (double) dev[((size_t) k * ny + j) * nx + i];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
@@ -56,10 +57,10 @@ __global__ __launch_bounds__(256) void k_dev_to_io(const T *__restrict__ dev, do
     }
 }
 
-template __global__ void k_io_to_dev<double>(const double *, double *, int, int, int);
-template __global__ void k_io_to_dev<float>(const double *, float *, int, int, int);
-template __global__ void k_dev_to_io<double>(const double *, double *, int, int, int);
-template __global__ void k_dev_to_io<float>(const float *, double *, int, int, int);
+template __global__ void k_io_to_dev<double>(const double *, double *, int, int, int, const double *);
+template __global__ void k_io_to_dev<float>(const double *, float *, int, int, int, const double *);
+template __global__ void k_dev_to_io<double>(const double *, double *, int, int, int, const double *);
+template __global__ void k_dev_to_io<float>(const float *, double *, int, int, int, const double *);
 
 /* ---------------------------------------------------------------------
  * Explicit and implicit-solve right-hand sides of one substep, 2nd order,
@@ -124,6 +125,15 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     const cp<double> dz = (cp<double>) R.dzt + CF_NGT;
     const cp<double> rdz = (cp<double>) R.rdz + CF_NGT, rdzh = (cp<double>) R.rdzh + CF_NGT;
     const T ub = (T) R.ubc, vb = (T) R.vbc;
+    /* FP32 storage holds u - U(z) (R.uoff): the ring then holds that
+     * deviation, whose wall ghost is its negative (U's ghosts carry the
+     * wall value); full values are formed only where the advection needs
+     * them, and U's share of the implicit right-hand side is added per
+     * level.  FP64 storage holds u itself. */
+    constexpr bool OFF = sizeof(T) == 4;
+    const cp<double> uof = (cp<double>) (OFF ? R.uoff : nullptr);
+    const T ubs = (OFF && uof) ? (T) 0 : ub;
+    auto F = [&](T v, T o) -> T { if constexpr (OFF) return v + o; else return v; };
 
     /* element e of the halo tile / pressure tile -> global column offset */
     auto colo = [&](int e, int w) -> size_t {
@@ -198,7 +208,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T uc = X.u[t] + hm;
                 T up = uc - (SP[b][0][r][c + 1] - p0) * cx;
                 T vp = X.v[t] - (SP[b][0][r + 1][c] - p0) * cy;
-                if (ghost) { up = (T) 2.0 * ub - up; vp = (T) 2.0 * vb - vp; }
+                if (ghost) { up = (T) 2.0 * ubs - up; vp = (T) 2.0 * vb - vp; }
                 SU[sl][r][c] = up;
                 SV[sl][r][c] = vp;
                 SW[sl][r][c] = wok ? X.w[t] - (SP[b][1][r][c] - p0) * cz : (T) 0;
@@ -270,6 +280,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
             const T dzk = (T) dz[k], dzkp = (T) dz[k + 1];
             const T rzk = (T) rdz[k], rzkp = (T) rdz[k + 1];
             const T rzp = (T) rdzh[k], rzm = (T) rdzh[k - 1];
+            T Uk = 0, Ukp = 0, Ukm = 0, Uh = 0;    /* U at levels k, k + 1, k - 1; U's right-hand-side share */
+            if (OFF && uof) {
+                Uk = (T) uof[k + 1]; Ukp = (T) uof[k + 2]; Ukm = (T) uof[k];
+                Uh = (T) uof[(size_t) nz + 2 + (size_t) s * nz + k];
+            }
 #define LU(sl, dr, dc) SU[sl][r0 + (dr)][c0 + (dc)]
 #define LV(sl, dr, dc) SV[sl][r0 + (dr)][c0 + (dc)]
 #define LW(sl, dr, dc) SW[sl][r0 + (dr)][c0 + (dc)]
@@ -277,7 +292,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
 #define SEQ() asm volatile("" ::: "memory")
             if (cflpart) {
                 double c;
-                if (cflm < (c = fabs(LU(s0, 0, 0)) * cfldt * rdx)) cflm = c;
+                if (cflm < (c = fabs(F(LU(s0, 0, 0), Uk)) * cfldt * rdx)) cflm = c;
                 if (cflm < (c = fabs(LV(s0, 0, 0)) * cfldt * rdy)) cflm = c;
                 if (cflm < (c = fabs(LW(s0, 0, 0)) * cfldt * rzp)) cflm = c;
             }
@@ -286,17 +301,20 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T ujp = LU(s0, 1, 0), ujm = LU(s0, -1, 0), ukp = LU(sp, 0, 0), ukm = LU(sm, 0, 0);
                 const T v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vjm = LV(s0, -1, 0), vipjm = LV(s0, -1, 1);
                 const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wkm = LW(sm, 0, 0), wipkm = LW(sm, 0, 1);
-                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzk * uip);
-                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzk * u0);
+                const T u0f = F(u0, Uk), uipf = F(uip, Uk), uimf = F(uim, Uk);
+                const T ujpf = F(ujp, Uk), ujmf = F(ujm, Uk), ukpf = F(ukp, Ukp), ukmf = F(ukm, Ukm);
+                T mxp = (T) 0.5 * (dy * dzk * u0f + dy * dzk * uipf);
+                T mxm = (T) 0.5 * (dy * dzk * uimf + dy * dzk * u0f);
                 T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vip);
                 T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
                 T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wip);
                 T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * wipkm);
-                T adv = -((mxp * (T) 0.5 * (u0 + uip) - mxm * (T) 0.5 * (u0 + uim))
-                             + (myp * (T) 0.5 * (u0 + ujp) - mym * (T) 0.5 * (u0 + ujm))
-                             + (mzp * (T) 0.5 * (u0 + ukp) - mzm * (T) 0.5 * (u0 + ukm))) * (rdxdy * rzk);
+                T adv = -((mxp * (T) 0.5 * (u0f + uipf) - mxm * (T) 0.5 * (u0f + uimf))
+                             + (myp * (T) 0.5 * (u0f + ujpf) - mym * (T) 0.5 * (u0f + ujmf))
+                             + (mzp * (T) 0.5 * (u0f + ukpf) - mzm * (T) 0.5 * (u0f + ukmf))) * (rdxdy * rzk);
                 T f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
                 T h = u0 + nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
+                if constexpr (OFF) h += Uh;
                 T g = gcu + gmprev;
                 h += gamdt * f + zetdt * g;
                 if (k == 0) h -= (T) R.lift[s][0];
@@ -310,8 +328,8 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vim = LV(s0, 0, -1);
                 const T vjp = LV(s0, 1, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vkm = LV(sm, 0, 0);
                 const T w0 = LW(s0, 0, 0), wjp = LW(s0, 1, 0), wkm = LW(sm, 0, 0), wjpkm = LW(sm, 1, 0);
-                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzk * ujp);
-                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzk * uimjp);
+                T mxp = (T) 0.5 * (dy * dzk * F(u0, Uk) + dy * dzk * F(ujp, Uk));
+                T mxm = (T) 0.5 * (dy * dzk * F(uim, Uk) + dy * dzk * F(uimjp, Uk));
                 T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
                 T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * v0);
                 T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wjp);
@@ -334,8 +352,8 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wim = LW(s0, 0, -1);
                 const T wjp = LW(s0, 1, 0), wjm = LW(s0, -1, 0), wkp = LW(sp, 0, 0), wkm = LW(sm, 0, 0);
                 const T dzp1h = (T) 0.5 * (dz[k + 1] + dz[k]);
-                T mxp = (T) 0.5 * (dy * dzk * u0 + dy * dzkp * ukp);
-                T mxm = (T) 0.5 * (dy * dzk * uim + dy * dzkp * uimkp);
+                T mxp = (T) 0.5 * (dy * dzk * F(u0, Uk) + dy * dzkp * F(ukp, Ukp));
+                T mxm = (T) 0.5 * (dy * dzk * F(uim, Uk) + dy * dzkp * F(uimkp, Ukp));
                 T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
                 T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
                 T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wkp);
@@ -660,8 +678,9 @@ static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const
      * ..., then a fixed tree (deterministic, the same in every launch) */
     __shared__ double rv[256], rl[256];
     double pv = 0.0, pl = 0.0;
+    const double *uo = kind == 0 && R.uoff ? R.uoff + 1 : nullptr;   /* stored u is u - U(z) */
     for (int l = threadIdx.x; l < nz; l += 256) {
-        const double a = lsum[l] * scale;
+        const double a = uo ? lsum[l] * scale + uo[l] : lsum[l] * scale;
         avg[l + 1] = a;
         pv += a * dz[l];
         pl += dz[l];
@@ -694,6 +713,35 @@ static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const
         long idx = (long) scal[SC_STEP] % R.logcap;
         double *lg = logbuf + (size_t) idx * 13 + (kind == 0 ? 3 : 8);
         lg[0] = m; lg[1] = bot; lg[2] = top; lg[3] = vol; lg[4] = cl;
+    }
+}
+
+/* The second stage of the plane sums and the mass-flux constraint in one
+ * launch: each workgroup sums its level (level_sum_body), and the last
+ * workgroup to finish -- counted with a device-scope ticket after a
+ * release fence -- runs mass_finalize_body on all levels and resets the
+ * ticket for the next launch.  The sums are in the same fixed order as the
+ * two-launch form; only which workgroup finishes is a race, and it does
+ * not enter the arithmetic.  step_end: also count the completed step
+ * (the last substep). */
+__global__ __launch_bounds__(256) void k_level_mass(const RunDev R, const int s, const double *__restrict__ plane,
+                                                    int nbx, double *__restrict__ lsum, double *__restrict__ scal,
+                                                    double *__restrict__ logbuf, int step_end)
+{
+    level_sum_body(plane, nbx, lsum, blockIdx.x);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long *ticket = (unsigned long long *) (scal + SC_TICKET);
+        last = atomicAdd(ticket, 1ull) == (unsigned long long) (gridDim.x - 1);
+        __threadfence();
+    }
+    __syncthreads();
+    if (!last) return;
+    mass_finalize_body(R, s, lsum, scal, logbuf, 0);
+    if (threadIdx.x == 0) {
+        *(unsigned long long *) (scal + SC_TICKET) = 0ull;
+        if (step_end) scal[SC_STEP] += 1.0;
     }
 }
 
@@ -754,7 +802,8 @@ __global__ __launch_bounds__(256) void k_cfl_plain(const RunDev R, const T *__re
         const double *dz = R.dzt + CF_NGT;
         const double dzph = 0.5 * (dz[k] + dz[k + 1]);
         double c;
-        if (m < (c = fabs((double) u[q]) * dt / R.g.dx)) m = c;
+        const double uq = R.uoff ? (double) u[q] + R.uoff[k + 1] : (double) u[q];
+        if (m < (c = fabs(uq) * dt / R.g.dx)) m = c;
         if (m < (c = fabs((double) v[q]) * dt / R.g.dy)) m = c;
         if (m < (c = fabs((double) w[q]) * dt / dzph)) m = c;
     }
@@ -879,36 +928,41 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     /* every read is at a valid (clamped) address and the boundary values
      * are applied by selects, so the reads of a group of elements issue
      * together without branches */
+    /* FP32 storage: the divergence is a small sum of large nearly
+     * cancelling differences, so it is formed in FP64 from the stored
+     * values and rounded once (the pressure then corrects the velocity by
+     * a rounding of the divergence, not of its terms) */
+    typedef typename std::conditional<sizeof(T) == 4, double, T>::type D;
     auto divg = [&](int line, int i) -> T {
         const bool ok = line < NL;
         const int lc = ok ? line : NL - 1;
         const int k = lc / ny, j = lc - k * ny;
         const int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
         const size_t q = (size_t) k * P + (size_t) j * nx;
-        const T hm = h1 ? (T) h1[k + 1] * m : (T) 0;
-        const T u1 = us[q + i] + hm, u0 = us[q + im] + hm;
-        const T v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
-        const T w1r = ws[q + i], w0r = ws[(k > 0 ? q - P : q) + i];
-        const T w1 = k < nz - 1 ? w1r : (T) 0;
-        const T w0 = k > 0 ? w0r : (T) 0;
-        T d;
+        const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
+        const D u1 = (D) us[q + i] + hm, u0 = (D) us[q + im] + hm;
+        const D v1 = vs[q + i], v0 = vs[(size_t) k * P + (size_t) jm * nx + i];
+        const D w1r = ws[q + i], w0r = ws[(k > 0 ? q - P : q) + i];
+        const D w1 = k < nz - 1 ? w1r : (D) 0;
+        const D w0 = k > 0 ? w0r : (D) 0;
+        D d;
         if (ORD == 2) {
-            d = (u1 - u0) * rdx + (v1 - v0) * rdy + (w1 - w0) * (T) rdz[k];
+            d = (u1 - u0) * (D) rdx + (v1 - v0) * (D) rdy + (w1 - w0) * (D) ((T) rdz[k]);
         } else {
-            const T a1 = (T) A1_, a2 = (T) A2_;
+            const D a1 = (D) (T) A1_, a2 = (D) (T) A2_;
             const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im2 = i - 2 >= 0 ? i - 2 : i - 2 + nx;
             const int jp = j + 1 < ny ? j + 1 : j + 1 - ny, jm2 = j - 2 >= 0 ? j - 2 : j - 2 + ny;
-            const T u2 = us[q + ip] + hm, um = us[q + im2] + hm;
-            const T v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
+            const D u2 = (D) us[q + ip] + hm, um = (D) us[q + im2] + hm;
+            const D v2 = vs[(size_t) k * P + (size_t) jp * nx + i], vm = vs[(size_t) k * P + (size_t) jm2 * nx + i];
             /* w at faces k+1 and k-2: walls at -1 and nz-1 are 0, ghosts odd */
-            const T w2r = ws[(k <= nz - 3 ? q + P : (k == nz - 2 ? q : q - P)) + i];
-            const T w2 = k <= nz - 3 ? w2r : (k == nz - 2 ? (T) 0 : -w2r);
-            const T wmr = ws[(k >= 2 ? q - 2 * P : q) + i];
-            const T wm = k >= 2 ? wmr : (k == 1 ? (T) 0 : -wmr);
-            d = (a1 * (u1 - u0) + a2 * (u2 - um)) * rdx + (a1 * (v1 - v0) + a2 * (v2 - vm)) * rdy
-              + (a1 * (w1 - w0) + a2 * (w2 - wm)) * (T) rdz[k];
+            const D w2r = ws[(k <= nz - 3 ? q + P : (k == nz - 2 ? q : q - P)) + i];
+            const D w2 = k <= nz - 3 ? w2r : (k == nz - 2 ? (D) 0 : -w2r);
+            const D wmr = ws[(k >= 2 ? q - 2 * P : q) + i];
+            const D wm = k >= 2 ? wmr : (k == 1 ? (D) 0 : -wmr);
+            d = (a1 * (u1 - u0) + a2 * (u2 - um)) * (D) rdx + (a1 * (v1 - v0) + a2 * (v2 - vm)) * (D) rdy
+              + (a1 * (w1 - w0) + a2 * (w2 - wm)) * (D) ((T) rdz[k]);
         }
-        return ok ? d : (T) 0;
+        return ok ? (T) d : (T) 0;
     };
     /* groups of CF_FX_UNR elements per thread: all reads of a group are
      * in flight before the first LDS write (blockDim.x = 256) */

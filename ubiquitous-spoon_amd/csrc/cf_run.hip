@@ -61,7 +61,7 @@ static const int MAXE_LINE = 2048;      /* elements per FFT block (8 per thread 
 #define CF_P4_THREADS 64                /* order-4 Poisson: modes (threads) per workgroup of a single run */
 #endif
 #ifndef CF_FX_ELEMS
-#define CF_FX_ELEMS 2048                /* elements per x-FFT workgroup: fx_pairs = CF_FX_ELEMS / nx */
+#define CF_FX_ELEMS 1024                /* elements per x-FFT workgroup: fx_pairs = CF_FX_ELEMS / nx */
 #endif
 
 /* ------------------------------------------------------------ run */
@@ -126,6 +126,10 @@ struct cf_run {
      * projection left pending by the last step, with that step's dt */
     std::vector<double> tabhost;
     double *h1prev = nullptr;
+    /* FP32 fused path: u stored relative to the profile ubar (device copy
+     * with ghosts and its right-hand-side shares in uoffd) */
+    std::vector<double> ubar;
+    double *uoffd = nullptr;
     double dtp_prev = 0.0;
     bool prev_proj = false;
     /* profiling */
@@ -452,7 +456,7 @@ static void free_run(cf_run *r)
     }
     double *bufs[] = { r->u, r->v, r->w, r->c, r->ru, r->rv, r->rw, r->rc, r->fa[0], r->fa[1], r->fa[2], r->fa[3],
                        r->fb[0], r->fb[1], r->fb[2], r->fb[3], r->p, r->G, r->S4, r->stage, r->tab, r->scal,
-                       r->logbuf, r->part, r->lsum, r->cflpart, r->h1prev };
+                       r->logbuf, r->part, r->lsum, r->cflpart, r->h1prev, r->uoffd };
     for (double *b : bufs) if (b) (void) hipFree(b);
     if (r->X) (void) hipFree(r->X);
     for (auto &pe : r->pending) { (void) hipEventDestroy(pe.second.first); (void) hipEventDestroy(pe.second.second); }
@@ -831,6 +835,40 @@ static void apply_tables(cf_run *r, const TabPack &TP)
             }
 }
 
+/* The FP32 fused path stores u - U(z) with U = r->ubar.  The device copy
+ * holds U with its wall ghosts 2 ubc - U (apply_bc_u2 on a column: the
+ * stored deviation's ghost is then its negative) and, per substep, U's
+ * share of the implicit right-hand side, nu a dt Lz_bc U + nu b dt Lz_0 U:
+ * Lz_bc with those ghosts (how the explicit operator sees U) and Lz_0 with
+ * the homogeneous ghosts -U of the z solve, so that the solve of the
+ * stored deviation's right-hand side gives the new deviation (the solve is
+ * linear: S (u' + U) = h  <=>  S u' = h - U + nu b dt Lz_0 U). */
+static cf_status set_uoff(cf_run *r)
+{
+    if (!r->uoffd) return CF_OK;
+    const int nz = r->g.nz;
+    const double *dzl = r->dz.data() + CF_NGT;
+    const double ub = r->rd.ubc;
+    const std::vector<double> &U = r->ubar;
+    std::vector<double> H((size_t) nz + 2 + 3 * (size_t) nz, 0.0);
+    H[0] = 2.0 * ub - U[0];
+    for (int k = 0; k < nz; k++) H[(size_t) k + 1] = U[k];
+    H[(size_t) nz + 1] = 2.0 * ub - U[nz - 1];
+    for (int s = 0; s < 3; s++)
+        for (int k = 0; k < nz; k++) {
+            const double rzp = 1.0 / (0.5 * (dzl[k] + dzl[k + 1])), rzm = 1.0 / (0.5 * (dzl[k - 1] + dzl[k]));
+            const double rzk = 1.0 / dzl[k];
+            const double u0 = H[(size_t) k + 1], up = H[(size_t) k + 2], um = H[k];
+            const double lbc = ((up - u0) * rzp - (u0 - um) * rzm) * rzk;
+            const double up0 = k == nz - 1 ? -u0 : up, um0 = k == 0 ? -u0 : um;
+            const double l0 = ((up0 - u0) * rzp - (u0 - um0) * rzm) * rzk;
+            H[(size_t) nz + 2 + (size_t) s * nz + k] = r->rd.sc[s].nuadt * lbc + r->rd.sc[s].nubdt * l0;
+        }
+    HIPCHK(hipMemcpyAsync(r->uoffd, H.data(), sizeof(double) * H.size(), hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return CF_OK;
+}
+
 static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **out);
 
 extern "C" cf_status cf_run_create(const cf_config *cfg, cf_run **out)
@@ -913,6 +951,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
     ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));
     if (cfg->cflmode == 1) ok = ok && dalloc(&r->h1prev, (size_t) nz + 2);
+    if (r->prec == CF_FP32 && !r->general) ok = ok && dalloc(&r->uoffd, (size_t) nz + 2 + 3 * (size_t) nz);
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
         free_run(r); cf_set_error("stream creation failed"); return CF_ERR_DEVICE;
@@ -966,6 +1005,9 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     r->sp_L = TP.spike_ok ? spike_L(nz) : 0;
     for (int s = 0; s < 3; s++)
         for (int t = 0; t < 3; t++) r->spike_waves = std::max(r->spike_waves, R.sp[s][t].S);
+    R.uoff = r->uoffd;
+    r->ubar.assign(nz, 0.0);
+    if ((st = set_uoff(r)) != CF_OK) { free_run(r); return st; }
     *out = r;
     return CF_OK;
 }
@@ -1152,8 +1194,10 @@ static cf_status enqueue_step_t(cf_run *r, int from)
             /* the solve with the first stage of the plane sums of u* fused in */
             KLAUNCH("spike", pick_spike_sums<T>(r->sp_L), dim3((unsigned) nbx, 3), dim3(64 * r->spike_waves), 0, R, s,
                     ou, ov, ow, r->part);
-            KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, nbx,
-                    r->lsum);
+            /* the level sums and the constraint in one launch (its last
+             * workgroup finishes); the last substep also counts the step */
+            KLAUNCH("plane_sum", k_level_mass, dim3((unsigned) nz), dim3(256), sizeof(double) * (nz + 2), R, s,
+                    (const double *) r->part, nbx, r->lsum, r->scal, r->logbuf, s == 2 ? 1 : 0);
         } else {
             KLAUNCH("thomas", k_thomas<T>, dim3((unsigned) ((r->P + 255) / 256), 3), dim3(256), 0, R, s, ou, ov, ow,
                     (T *) nullptr);
@@ -1162,13 +1206,14 @@ static cf_status enqueue_step_t(cf_run *r, int from)
             KLAUNCH("plane_sum", k_level_sum, dim3((unsigned) nz), dim3(256), 0, (const double *) r->part, r->nb_plane,
                     r->lsum);
         }
-        KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
-                (const double *) r->lsum, r->scal, r->logbuf, 0);
+        if (!r->use_spike)
+            KLAUNCH("mass", k_mass_finalize, dim3(1), dim3(256), sizeof(double) * (nz + 2), R, s,
+                    (const double *) r->lsum, r->scal, r->logbuf, 0);
         const double *h1 = R.h1 + (size_t) (s * 2 + 0) * (nz + 2);
         cf_status st = enqueue_pressure<T>(r, ou, ov, ow, h1, r->scal + SC_M0 + s, s);
         if (st != CF_OK) return st;
     }
-    KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
+    if (!r->use_spike) KLAUNCH("step_end", k_step_end, dim3(1), dim3(64), 0, r->scal);
     return CF_OK;
 }
 
@@ -1334,27 +1379,41 @@ extern "C" cf_status cf_run_upload(cf_run *r, int field, const double *host)
     if (!dst) { cf_set_error("cf_run_upload: field %d is not uploadable in this configuration", field); return CF_ERR_ARG; }
     cf_status rs = reset_projection(r);
     if (rs != CF_OK) return rs;
+    const double *off = nullptr;
+    if (field == CF_U && r->uoffd) {
+        /* the stored deviation's profile: the plane means of the new u */
+        const int nx = r->g.nx, ny = r->g.ny, nz = r->g.nz;
+        std::vector<double> m((size_t) nz, 0.0);
+        for (size_t ij = 0; ij < (size_t) nx * ny; ij++)
+            for (int k = 0; k < nz; k++) m[k] += host[ij * nz + k];
+        for (int k = 0; k < nz; k++) m[k] /= (double) nx * (double) ny;
+        r->ubar.swap(m);
+        cf_status so = set_uoff(r);
+        if (so != CF_OK) return so;
+        off = r->uoffd + 1;
+    }
     HIPCHK(hipMemcpyAsync(r->stage, host, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
     if (r->prec == CF_FP32)
         hipLaunchKernelGGL(k_io_to_dev<float>, grid, dim3(256), 0, r->stream, (const double *) r->stage, tv<float>(dst),
-                           r->g.nx, r->g.ny, r->g.nz);
+                           r->g.nx, r->g.ny, r->g.nz, off);
     else
         hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, dst,
-                           r->g.nx, r->g.ny, r->g.nz);
+                           r->g.nx, r->g.ny, r->g.nz, off);
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(r->stream));
     return CF_OK;
 }
 
-static cf_status download_dev(cf_run *r, const double *src, double *host)
+static cf_status download_dev(cf_run *r, const double *src, double *host, const double *off = nullptr)
 {
     dim3 grid((r->g.nx + 31) / 32, (r->g.nz + 31) / 32, r->g.ny);
     if (r->prec == CF_FP32)
         hipLaunchKernelGGL(k_dev_to_io<float>, grid, dim3(256), 0, r->stream, (const float *) src, r->stage,
-                           r->g.nx, r->g.ny, r->g.nz);
+                           r->g.nx, r->g.ny, r->g.nz, off);
     else
-        hipLaunchKernelGGL(k_dev_to_io<double>, grid, dim3(256), 0, r->stream, src, r->stage, r->g.nx, r->g.ny, r->g.nz);
+        hipLaunchKernelGGL(k_dev_to_io<double>, grid, dim3(256), 0, r->stream, src, r->stage, r->g.nx, r->g.ny, r->g.nz,
+                           off);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(host, r->stage, sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
@@ -1374,7 +1433,7 @@ extern "C" cf_status cf_run_download(cf_run *r, int field, double *host)
     if (r->general) return download_dev(r, field == CF_U ? r->u : field == CF_V ? r->v : r->w, host);
     cf_status st = enqueue_materialise(r, r->fa[0], r->fa[1], r->fa[2]);
     if (st != CF_OK) return st;
-    return download_dev(r, r->fa[field], host);
+    return download_dev(r, r->fa[field], host, field == CF_U && r->uoffd ? r->uoffd + 1 : nullptr);
 }
 
 /* divergence -> Poisson(dt) -> projection of the general path's state, in place */
@@ -1469,7 +1528,7 @@ static cf_status set_dt(cf_run *r, double dt)
     HIPCHK(hipMemcpyAsync(r->tab, r->tabhost.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice, r->stream));
     apply_tables(r, TP);
     r->cfg.dt = dt;
-    return CF_OK;
+    return set_uoff(r);                           /* U's right-hand-side shares depend on dt */
 }
 
 /* One step with cflmode 1 (chanfast.c:383-393): the cfl of the current
@@ -1657,7 +1716,7 @@ static cf_status stat_source(cf_run *r, int field, const double **src, bool *mat
  * space; enqueued on the run's stream */
 template <typename T>
 static cf_status stat_plane_t(cf_run *r, int kind, const T *a, const T *b, double en, double *part, double *means,
-                              double *dev_out)
+                              double *dev_out, const double *aoff)
 {
     const int nz = r->g.nz, P = (int) r->P, nbl = CF_STAT_NBL;
     const double scale = 1.0 / ((double) r->g.nx * (double) r->g.ny);
@@ -1666,19 +1725,21 @@ static cf_status stat_plane_t(cf_run *r, int kind, const T *a, const T *b, doubl
     const double *nul = nullptr;
     hipLaunchKernelGGL((k_stat_plane<T, 0>), grid, dim3(256), 0, r->stream, a, (const T *) nullptr, nul, nul, 0.0, P,
                        nbl, part);
+    /* a stored relative to a profile: the average adds it back; the
+     * central statistics need only the deviations */
     hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale,
-                       kind == CF_STAT_AVG ? dev_out : ma);
+                       kind == CF_STAT_AVG ? dev_out : ma, kind == CF_STAT_AVG ? aoff : nul);
     if (kind == CF_STAT_CORR) {
         hipLaunchKernelGGL((k_stat_plane<T, 0>), grid, dim3(256), 0, r->stream, b, (const T *) nullptr, nul, nul, 0.0,
                            P, nbl, part);
-        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, mb);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, mb, nul);
         hipLaunchKernelGGL((k_stat_plane<T, 1>), grid, dim3(256), 0, r->stream, a, b, (const double *) ma,
                            (const double *) mb, 0.0, P, nbl, part);
-        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out, nul);
     } else if (kind == CF_STAT_MOMENT) {
         hipLaunchKernelGGL((k_stat_plane<T, 2>), grid, dim3(256), 0, r->stream, a, (const T *) nullptr,
                            (const double *) ma, nul, en, P, nbl, part);
-        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out);
+        hipLaunchKernelGGL(k_stat_finish, fin, dim3(256), 0, r->stream, (const double *) part, nbl, nz, scale, dev_out, nul);
     }
     LAUNCHCHK();
     return CF_OK;
@@ -1692,9 +1753,10 @@ static cf_status stat_plane(cf_run *r, int kind, int fa, int fb, double en, doub
     cf_status st = stat_source(r, fa, &a, &mat);
     if (st == CF_OK && kind == CF_STAT_CORR) st = stat_source(r, fb, &b, &mat);
     if (st != CF_OK) return st;
+    const double *aoff = fa == CF_U && r->uoffd ? r->uoffd + 1 : nullptr;
     if (r->prec == CF_FP32)
-        return stat_plane_t<float>(r, kind, (const float *) a, (const float *) b, en, part, means, dev_out);
-    return stat_plane_t<double>(r, kind, a, b, en, part, means, dev_out);
+        return stat_plane_t<float>(r, kind, (const float *) a, (const float *) b, en, part, means, dev_out, aoff);
+    return stat_plane_t<double>(r, kind, a, b, en, part, means, dev_out, aoff);
 }
 
 /* FFT3D_FWD (1/nx, 1/ny scaled) of an FP64 field in the device layout into
@@ -1827,7 +1889,7 @@ extern "C" cf_status cf_op_transpose_roundtrip(cf_run *r, const double *in_io, d
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
     hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
-                       r->g.nx, r->g.ny, r->g.nz);
+                       r->g.nx, r->g.ny, r->g.nz, (const double *) nullptr);
     LAUNCHCHK();
     if (dev_out) HIPCHK(hipMemcpyAsync(dev_out, r->fb[0], sizeof(double) * r->N, hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));
@@ -1868,7 +1930,7 @@ extern "C" cf_status cf_op_fft3d_fwd(cf_run *r, const double *in_io, double *spe
     HIPCHK(hipMemcpyAsync(r->stage, in_io, sizeof(double) * r->N, hipMemcpyHostToDevice, r->stream));
     dim3 grid((r->g.nz + 31) / 32, (r->g.nx + 31) / 32, r->g.ny);
     hipLaunchKernelGGL(k_io_to_dev<double>, grid, dim3(256), 0, r->stream, (const double *) r->stage, r->fb[0],
-                       r->g.nx, r->g.ny, r->g.nz);
+                       r->g.nx, r->g.ny, r->g.nz, (const double *) nullptr);
     LAUNCHCHK();
     const RunDev &R = r->rd;
     long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;

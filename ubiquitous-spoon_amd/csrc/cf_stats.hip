@@ -43,14 +43,15 @@ __global__ void __launch_bounds__(256) k_stat_plane(const T *a, const T *b, cons
     if (tid == 0) part[(size_t) k * nbl + bl] = red[0];
 }
 
-/* out[k] = scale * (the level's partials summed in index order) */
-__global__ void k_stat_finish(const double *part, int nbl, int nz, double scale, double *out)
+/* out[k] = scale * (the level's partials summed in index order), plus
+ * off[k] when given (a field stored relative to a per-level profile) */
+__global__ void k_stat_finish(const double *part, int nbl, int nz, double scale, double *out, const double *off)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nz) return;
     double s = 0.0;
     for (int b = 0; b < nbl; b++) s += part[(size_t) k * nbl + b];
-    out[k] = s * scale;
+    out[k] = off ? s * scale + off[k] : s * scale;
 }
 
 /* one-dimensional cross-spectra of one level (workgroup = level): A, B the

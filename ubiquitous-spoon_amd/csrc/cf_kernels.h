@@ -34,6 +34,15 @@ enum {
 #ifndef CF_RHS_KC_BATCH
 #define CF_RHS_KC_BATCH 24       /* ... of kb_rhs2t, whose launches cover many runs */
 #endif
+/* xy tile of the fused order-4 RHS (k_rhs4t); nx and ny must be multiples */
+#define CF_R4_X 32
+#define CF_R4_Y 8
+#ifndef CF_R4_KC
+#define CF_R4_KC 16              /* target levels per z-chunk of the order-4 RHS */
+#endif
+#ifndef CF_R4_WAVES
+#define CF_R4_WAVES 2            /* occupancy target (waves per SIMD) of the order-4 RHS */
+#endif
 #define CF_SPIKE_LMAX 32        /* rows per chunk held in registers */
 #define CF_SPIKE_SMAX 8         /* chunks (waves) per workgroup */
 #define CF_F64_KLOW 4           /* FP32 runs solve Poisson modes |kx|, |ky| <= this in FP64 */
@@ -329,6 +338,16 @@ template <typename T> struct GSel {
     proj_t proj;
 };
 template <typename T> GSel<T> pick_general(int order, int scalar);
+
+/* the order-4 explicit stage (cf_rhs4.hip): u and v in one launch, w and c
+ * in another; grid (nx / CF_R4_X) * (ny / CF_R4_Y) * nch blocks of 256,
+ * nch z-chunks */
+template <bool SC, typename T>
+__global__ void k_rhs4_uv(const RunDev R, const int s, const T *u, const T *v, const T *w, const T *c,
+    const T *gu, const T *gv, const double *scal, T *fu, T *fv, T *ru, T *rv, double *cflpart, double cfldt, int nch);
+template <bool SC, typename T>
+__global__ void k_rhs4_wc(const RunDev R, const int s, const T *u, const T *v, const T *w, const T *c,
+    const T *gw, const T *gc, const double *scal, T *fw, T *fc, T *rw, T *rc, int nch);
 
 /* batched launches (cf_general.hip, cf_kernels.hip): a = device array of
  * nrun GArgs, vs = that stage's virtual-block prefix (nrun + 1) */

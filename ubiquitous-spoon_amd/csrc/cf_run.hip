@@ -112,6 +112,7 @@ struct cf_run {
     bool proj_pending = false;       /* p / forcings not yet folded into the state */
     double *cflpart = nullptr;
     int rhs_blocks = 0, rhs_kc = 1;
+    int r4_nch = 0, r4_blocks = 0;   /* order 4: z-chunks and blocks of k_rhs4_uv / _wc (0: per-field kernels) */
     int steps_done = 0;
     bool use_spike = false;
     int spike_waves = 1, sp_L = 0;              /* chunks and rows per chunk of the z solves */
@@ -949,7 +950,11 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     ok = ok && dalloc(&r->part, npart) && dalloc(&r->lsum, (size_t) nz + 2);
     rhs_chunks(nx, ny, nz, CF_RHS_KC, &r->rhs_kc, &r->rhs_blocks);
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
-    ok = ok && dalloc(&r->cflpart, (size_t) std::max(r->rhs_blocks, r->gblocks));
+    if (cfg->order == 4 && nx % CF_R4_X == 0 && ny % CF_R4_Y == 0) {
+        r->r4_nch = std::max(1, (nz + CF_R4_KC / 2) / CF_R4_KC);
+        r->r4_blocks = (nx / CF_R4_X) * (ny / CF_R4_Y) * r->r4_nch;
+    }
+    ok = ok && dalloc(&r->cflpart, (size_t) std::max(std::max(r->rhs_blocks, r->gblocks), r->r4_blocks));
     if (cfg->cflmode == 1) ok = ok && dalloc(&r->h1prev, (size_t) nz + 2);
     if (r->prec == CF_FP32 && !r->general) ok = ok && dalloc(&r->uoffd, (size_t) nz + 2 + 3 * (size_t) nz);
     if (!ok) { free_run(r); cf_set_error("device allocation failed"); return CF_ERR_NOMEM; }
@@ -1271,18 +1276,41 @@ static cf_status enqueue_step_general_t(cf_run *r)
     for (int s = 0; s < 3; s++) {
         double **fnew = (s % 2 == 0) ? r->fa : r->fb;
         double **gold = (s % 2 == 0) ? r->fb : r->fa;
-        KLAUNCH("rhs", K.u, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[0]), (const double *) r->scal,
-                tv<T>(fnew[0]), ru, s == 0 ? r->cflpart : (double *) nullptr, dt);
-        if (s == 0)
-            KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart, r->gblocks, dt,
-                    r->scal, r->logbuf);
-        KLAUNCH("rhs", K.v, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[1]), (const double *) r->scal,
-                tv<T>(fnew[1]), rv, (double *) nullptr, dt);
-        KLAUNCH("rhs", K.w, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[2]), (const double *) r->scal,
-                tv<T>(fnew[2]), rw, (double *) nullptr, dt);
-        if (sc)
-            KLAUNCH("rhs", K.c, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[3]), (const double *) r->scal,
-                    tv<T>(fnew[3]), rc, (double *) nullptr, dt);
+        if (r->r4_nch > 0) {
+            /* order 4 on tiles that divide the grid: two z-marching launches */
+            const dim3 g4((unsigned) r->r4_blocks);
+            if (sc) {
+                KLAUNCH("rhs", (k_rhs4_uv<true, T>), g4, dim3(256), 0, R, s, (const T *) u, (const T *) v, (const T *) w,
+                        (const T *) c, (const T *) tv<T>(gold[0]), (const T *) tv<T>(gold[1]), (const double *) r->scal,
+                        tv<T>(fnew[0]), tv<T>(fnew[1]), ru, rv, s == 0 ? r->cflpart : (double *) nullptr, dt, r->r4_nch);
+                KLAUNCH("rhs", (k_rhs4_wc<true, T>), g4, dim3(256), 0, R, s, (const T *) u, (const T *) v, (const T *) w,
+                        (const T *) c, (const T *) tv<T>(gold[2]), (const T *) tv<T>(gold[3]), (const double *) r->scal,
+                        tv<T>(fnew[2]), tv<T>(fnew[3]), rw, rc, r->r4_nch);
+            } else {
+                KLAUNCH("rhs", (k_rhs4_uv<false, T>), g4, dim3(256), 0, R, s, (const T *) u, (const T *) v, (const T *) w,
+                        (const T *) nullptr, (const T *) tv<T>(gold[0]), (const T *) tv<T>(gold[1]), (const double *) r->scal,
+                        tv<T>(fnew[0]), tv<T>(fnew[1]), ru, rv, s == 0 ? r->cflpart : (double *) nullptr, dt, r->r4_nch);
+                KLAUNCH("rhs", (k_rhs4_wc<false, T>), g4, dim3(256), 0, R, s, (const T *) u, (const T *) v, (const T *) w,
+                        (const T *) nullptr, (const T *) tv<T>(gold[2]), (const T *) nullptr, (const double *) r->scal,
+                        tv<T>(fnew[2]), (T *) nullptr, rw, (T *) nullptr, r->r4_nch);
+            }
+            if (s == 0)
+                KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart, r->r4_blocks,
+                        dt, r->scal, r->logbuf);
+        } else {
+            KLAUNCH("rhs", K.u, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[0]), (const double *) r->scal,
+                    tv<T>(fnew[0]), ru, s == 0 ? r->cflpart : (double *) nullptr, dt);
+            if (s == 0)
+                KLAUNCH("cfl_finish", k_cfl_finish, dim3(1), dim3(256), 0, R, (const double *) r->cflpart, r->gblocks, dt,
+                        r->scal, r->logbuf);
+            KLAUNCH("rhs", K.v, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[1]), (const double *) r->scal,
+                    tv<T>(fnew[1]), rv, (double *) nullptr, dt);
+            KLAUNCH("rhs", K.w, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[2]), (const double *) r->scal,
+                    tv<T>(fnew[2]), rw, (double *) nullptr, dt);
+            if (sc)
+                KLAUNCH("rhs", K.c, cg, cb, 0, R, s, u, v, w, c, (const T *) tv<T>(gold[3]), (const double *) r->scal,
+                        tv<T>(fnew[3]), rc, (double *) nullptr, dt);
+        }
         const unsigned nf = (unsigned) r->nfield;
         if constexpr (std::is_same<T, double>::value) {
             if (R.implicit) {

@@ -56,6 +56,15 @@ struct SpikeTab {
     const double *a, *ib, *gam, *vl, *wr, *minv;
 };
 
+/* order-4 partitioned z solve of one system over n unknowns, S chunks of L
+ * rows (cf_spike4.hip): f = 7 S L chunk-local heptdag factors
+ * (a | al | be | iden | p | q | r), vw = 6 S L spikes (three left, three
+ * right), minv = the inverse interface system, 6 CF_SPIKE4_SMAX square */
+struct Spike4Tab {
+    int n, S, L;
+    const double *f, *vw, *minv;
+};
+
 /* ghost closures of cell (Dirichlet) and face fields at both walls */
 struct Clos {
     Clos1 cell[2], face[2];     /* [0] bottom, [1] top */
@@ -113,6 +122,7 @@ struct RunDev {
     double kap;                 /* scalar diffusivity nu / Pr */
     const double *liftg;        /* [s][field u, v, w, c][nz] boundary-lift profiles */
     HeptTab hp[3][3];           /* order-4 implicit z factors [s][type 0 cell-nu, 1 cell-kappa, 2 face-nu] */
+    Spike4Tab s4[3][3];         /* ... as partitioned-solve tables (n = 0: none) */
     int implicit;               /* 1: every diffusion term implicit (channel2/4): x-y diffusion in h, not f */
     /* FP32 storage on the fused path: u is stored as u - U(z), U a per-level
      * FP64 profile, so storage rounding scales with the deviation from U
@@ -173,6 +183,25 @@ static inline int spike_L(int nz)
     const int need = (nz + CF_SPIKE_SMAX - 1) / CF_SPIKE_SMAX;
     for (int l : set)
         if (l >= need) return l;
+    return 0;
+}
+
+/* order-4 spike solve: at most CF_SPIKE4_SMAX chunks of L rows, L a
+ * compiled length; one L serves the cell (nz) and face (nz - 1) systems,
+ * and every chunk but a system's last has L >= 6 rows, the last >= 3, so
+ * a chunk's three-row couplings reach only its neighbours (0: none fits,
+ * the sequential sweep instead) */
+#define CF_SPIKE4_SMAX 8
+#define CF_SPIKE4_LSET(X) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(24) X(28) X(32)
+static inline int spike4_L(int nz)
+{
+    static const int set[] = { 6, 8, 10, 12, 14, 16, 18, 20, 24, 28, 32 };
+    auto fits = [](int n, int l) {
+        const int S = (n + l - 1) / l;
+        return S <= CF_SPIKE4_SMAX && (S == 1 || n - (S - 1) * l >= 3);
+    };
+    for (int l : set)
+        if (fits(nz, l) && fits(nz - 1, l)) return l;
     return 0;
 }
 
@@ -294,6 +323,7 @@ __global__ void k_rhs2t(const RunDev R, const int s, const ProjSrcT<T> S, const 
 template <typename T> __global__ void k_thomas(const RunDev R, const int s, T *ru, T *rv, T *rw, T *rc);
 template <typename T> using k_spike_t = void (*)(const RunDev, const int, T *, T *, T *, T *);
 template <typename T> k_spike_t<T> pick_spike(int L);
+template <typename T> k_spike_t<T> pick_spike4(int L);
 template <typename T> using k_spike_sums_t = void (*)(const RunDev, const int, T *, T *, T *, double *);
 template <typename T> k_spike_sums_t<T> pick_spike_sums(int L);
 __global__ void k_level_sum(const double *plane, int nbx, double *lsum);

@@ -1330,8 +1330,8 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
  * registers and solves the interior as a Dirichlet problem in the two
  * neighbouring separators (x = y + P X_below + Q X_above, Thomas in
  * registers); the separators' Schur complement (tridiagonal, one unknown
- * per chunk) is solved in LDS by one thread per (mode, part), then every
- * thread writes its rows.  Every value is read once and written once (32 B
+ * per chunk) is solved in LDS by parallel cyclic reduction, every chunk's
+ * thread taking part, then every thread writes its rows.  Every value is read once and written once (32 B
  * per complex FP64 value), and the 16-fold parallelism over z hides the
  * latency of the level recurrences, which bounds the one-thread-per-mode
  * sweep at these mode counts.  Differs from sequential Thomas by
@@ -1427,31 +1427,49 @@ static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Re
         sh[6][chunk][g] = (double) ds;
     }
     __syncthreads();
-    if (chunk == 0) {
-        /* separator s_j = (j + 1) L - 1 above chunk j:
-         * a x_{s-1} + b x_s + c x_{s+1} = d_s with x_{s-1} chunk j's top
-         * interior row and x_{s+1} chunk j + 1's bottom one */
-        A cp = 0, dq = 0;
-        for (int j = 0; j < nch; j++) {
-            const int s = (j + 1) * L - 1;
-            const bool nxt = j < nch - 1;
-            const A as = ca(s), cs = cc(s);
-            const A sub = as * (A) sh[4][j][g];
-            const A dia = cb(s) + as * (A) sh[5][j][g] + (nxt ? cs * (A) sh[1][j + 1][g] : (A) 0);
-            const A sup = nxt ? cs * (A) sh[2][j + 1][g] : (A) 0;
-            const A rhs = (A) sh[6][j][g] - as * (A) sh[3][j][g] - (nxt ? cs * (A) sh[0][j + 1][g] : (A) 0);
-            const A iden = (A) 1 / (dia - sub * cp);
-            cp = sup * iden;
-            dq = (rhs - sub * dq) * iden;
-            sh[3][j][g] = (double) cp;            /* chunk j's top row is consumed */
-            sh[4][j][g] = (double) dq;
+    /* separator s_j = (j + 1) L - 1 above chunk j:
+     * a x_{s-1} + b x_s + c x_{s+1} = d_s with x_{s-1} chunk j's top
+     * interior row and x_{s+1} chunk j + 1's bottom one, which gives the
+     * tridiagonal system sub_j X_{j-1} + dia_j X_j + sup_j X_{j+1} = rhs_j
+     * over the separators; each chunk's lanes form their separator's row */
+    A sub = 0, dia = 1, sup = 0, rhs = 0;
+    if (live) {
+        const int j = chunk, sj = (j + 1) * L - 1;
+        const bool nxt = j < nch - 1;
+        const A as = ca(sj), cs = cc(sj);
+        sub = j > 0 ? as * (A) sh[4][j][g] : (A) 0;
+        dia = cb(sj) + as * (A) sh[5][j][g] + (nxt ? cs * (A) sh[1][j + 1][g] : (A) 0);
+        sup = nxt ? cs * (A) sh[2][j + 1][g] : (A) 0;
+        rhs = (A) sh[6][j][g] - as * (A) sh[3][j][g] - (nxt ? cs * (A) sh[0][j + 1][g] : (A) 0);
+    }
+    /* parallel cyclic reduction over the chunks (log2 nch steps, every
+     * chunk's lanes at once): row j takes rows j - d and j + d out of its
+     * equation; sh[0..3] hold the rows of the current step */
+    for (int d = 1; d < nch; d *= 2) {
+        __syncthreads();
+        if (live) {
+            sh[0][chunk][g] = (double) sub; sh[1][chunk][g] = (double) dia;
+            sh[2][chunk][g] = (double) sup; sh[3][chunk][g] = (double) rhs;
         }
-        A X = 0;
-        for (int j = nch - 1; j >= 0; j--) {
-            X = (A) sh[4][j][g] - (A) sh[3][j][g] * X;
-            xs[j][g] = (double) X;
+        __syncthreads();
+        if (live) {
+            A nsub = 0, nsup = 0, ndia = dia, nrhs = rhs;
+            if (chunk - d >= 0) {
+                const A al = -sub / (A) sh[1][chunk - d][g];
+                nsub = al * (A) sh[0][chunk - d][g];
+                ndia += al * (A) sh[2][chunk - d][g];
+                nrhs += al * (A) sh[3][chunk - d][g];
+            }
+            if (chunk + d < nch) {
+                const A ga = -sup / (A) sh[1][chunk + d][g];
+                nsup = ga * (A) sh[2][chunk + d][g];
+                ndia += ga * (A) sh[0][chunk + d][g];
+                nrhs += ga * (A) sh[3][chunk + d][g];
+            }
+            sub = nsub; sup = nsup; dia = ndia; rhs = nrhs;
         }
     }
+    if (live) xs[chunk][g] = (double) (rhs / dia);
     __syncthreads();
     if (!ld) return;
     const A XB = chunk > 0 ? (A) xs[chunk - 1][g] : (A) 0;

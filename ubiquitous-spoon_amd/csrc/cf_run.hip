@@ -116,6 +116,8 @@ struct cf_run {
     int steps_done = 0;
     bool use_spike = false;
     int spike_waves = 1, sp_L = 0;              /* chunks and rows per chunk of the z solves */
+    bool use_spike4 = false;                    /* order 4: partitioned z solves (k_spike4) */
+    int s4_L = 0, s4_waves = 1;
     KSel<double>::div_t kdiv = nullptr;
     k_fftx_plain_t kfxp = nullptr;
     KSel<double>::ffty_t kfy = nullptr;
@@ -316,6 +318,148 @@ static int spike_build(const std::vector<double> &a, const std::vector<double> &
     return 0;
 }
 
+/* Partitioned (spike) tables of one wall-reduced heptadiagonal system
+ * (rows sum_d c[d + 3] x_{i+d} = r_i, d = -3..3): chunks of L rows with
+ * their own heptdag factors (couplings that leave the chunk dropped), the
+ * left spikes V_c = A^-1 (column c of the coupling of the chunk's first
+ * three rows to the previous chunk's last three unknowns), the right
+ * spikes W_c (last three rows to the next chunk's first three), and the
+ * inverse of the interface system X + K X = Y over X = (first three, last
+ * three unknowns) of every chunk.  Device layout: chunk-major blocks of L
+ * rows, identity rows past the system's end; minv padded to
+ * 6 CF_SPIKE4_SMAX columns. */
+struct Spike4Host {
+    int n = 0, S = 0, L = 0;
+    bool ok = false;                 /* reproduces the sequential solve (checked at build) */
+    std::vector<double> f, vw, minv;
+};
+
+static int spike4_build(const std::vector<cft::Row7> &rows, int L, Spike4Host &T)
+{
+    const int n = (int) rows.size();
+    if (L <= 0) return -1;
+    const int S = (n + L - 1) / L;
+    if (S > CF_SPIKE4_SMAX || (S > 1 && n - (S - 1) * L < 3)) return -1;
+    T.n = n; T.S = S; T.L = L;
+    const size_t SL = (size_t) S * L;
+    T.f.assign(7 * SL, 0.0);
+    T.vw.assign(6 * SL, 0.0);
+    for (size_t i = 0; i < SL; i++) T.f[3 * SL + i] = 1.0;        /* identity rows: iden = 1 */
+    const int m6 = 6 * S;
+    std::vector<double> M((size_t) m6 * m6, 0.0);
+    for (int i = 0; i < m6; i++) M[(size_t) i * m6 + i] = 1.0;
+    for (int c = 0; c < S; c++) {
+        const int lo = c * L, hi = std::min(lo + L, n), m = hi - lo;
+        double *A = &T.f[(size_t) c * L], *AL = A + SL, *BE = A + 2 * SL, *ID = A + 3 * SL;
+        double *Pp = A + 4 * SL, *Qq = A + 5 * SL, *Rr = A + 6 * SL;
+        auto p_ = [&](int i) { return i >= 0 ? Pp[i] : 0.0; };
+        auto q_ = [&](int i) { return i >= 0 ? Qq[i] : 0.0; };
+        auto r_ = [&](int i) { return i >= 0 ? Rr[i] : 0.0; };
+        for (int i = 0; i < m; i++) {                 /* heptdag.c:5-61 on the chunk alone */
+            const double *cf = rows[lo + i].c;
+            const double a = i >= 3 ? cf[0] : 0.0, b = i >= 2 ? cf[1] : 0.0, cc = i >= 1 ? cf[2] : 0.0;
+            const double d = cf[3], e = i + 1 < m ? cf[4] : 0.0, ff = i + 2 < m ? cf[5] : 0.0, g = i + 3 < m ? cf[6] : 0.0;
+            const double alp = b - a * p_(i - 3);
+            const double bet = cc - a * q_(i - 3) - alp * p_(i - 2);
+            const double den = d - a * r_(i - 3) - alp * q_(i - 2) - bet * p_(i - 1);
+            if (den == 0.0) return -2;
+            Pp[i] = (e - alp * r_(i - 2) - bet * q_(i - 1)) / den;
+            Qq[i] = (ff - bet * r_(i - 1)) / den;
+            Rr[i] = g / den;
+            A[i] = a; AL[i] = alp; BE[i] = bet; ID[i] = 1.0 / den;
+        }
+        auto solve = [&](std::vector<double> &v) {   /* in place, local rows 0..m-1 */
+            double s1 = 0, s2 = 0, s3 = 0;
+            for (int i = 0; i < m; i++) {
+                const double t = (v[i] - A[i] * s3 - AL[i] * s2 - BE[i] * s1) * ID[i];
+                v[i] = t; s3 = s2; s2 = s1; s1 = t;
+            }
+            double x1 = 0, x2 = 0, x3 = 0;
+            for (int i = m - 1; i >= 0; i--) {
+                const double t = v[i] - Pp[i] * x1 - Qq[i] * x2 - Rr[i] * x3;
+                v[i] = t; x3 = x2; x2 = x1; x1 = t;
+            }
+        };
+        for (int k = 0; k < 3; k++) {
+            std::vector<double> v(m, 0.0), w(m, 0.0);
+            if (c > 0) {                              /* row lo+r couples to x_{lo-3+k} through c[k - r] */
+                for (int r = 0; r < 3 && r < m; r++) v[r] = k >= r ? rows[lo + r].c[k - r] : 0.0;
+                solve(v);
+            }
+            if (c < S - 1) {                          /* row hi-3+r couples to x_{hi+k} through c[6 + k - r] */
+                for (int r = 0; r < 3; r++) w[m - 3 + r] = k <= r ? rows[hi - 3 + r].c[6 + k - r] : 0.0;
+                solve(w);
+            }
+            for (int i = 0; i < m; i++) {
+                T.vw[(size_t) k * SL + (size_t) c * L + i] = v[i];
+                T.vw[(size_t) (3 + k) * SL + (size_t) c * L + i] = w[i];
+            }
+            for (int r = 0; r < 3; r++) {
+                const int rf = 6 * c + r, rl = 6 * c + 3 + r;
+                if (c > 0) {
+                    M[(size_t) rf * m6 + 6 * (c - 1) + 3 + k] += v[r];
+                    M[(size_t) rl * m6 + 6 * (c - 1) + 3 + k] += v[m - 3 + r];
+                }
+                if (c < S - 1) {
+                    M[(size_t) rf * m6 + 6 * (c + 1) + k] += w[r];
+                    M[(size_t) rl * m6 + 6 * (c + 1) + k] += w[m - 3 + r];
+                }
+            }
+        }
+    }
+    if (invert_dense(M, m6)) return -3;
+    const int mp = 6 * CF_SPIKE4_SMAX;
+    T.minv.assign((size_t) mp * mp, 0.0);
+    for (int r = 0; r < m6; r++)
+        for (int q = 0; q < m6; q++) T.minv[(size_t) r * mp + q] = M[(size_t) r * m6 + q];
+    return 0;
+}
+
+/* the device algorithm of k_spike4 on the host, for the self-check of a
+ * built table against the sequential solve */
+static void spike4_apply_host(const Spike4Host &T, const std::vector<double> &h, std::vector<double> &x)
+{
+    const int n = T.n, S = T.S, L = T.L, mp = 6 * CF_SPIKE4_SMAX;
+    const size_t SL = (size_t) S * L;
+    std::vector<double> y((size_t) S * L, 0.0), Y((size_t) 6 * S, 0.0);
+    for (int c = 0; c < S; c++) {
+        const int lo = c * L, m = std::min(L, n - lo);
+        const double *A = &T.f[(size_t) c * L], *AL = A + SL, *BE = A + 2 * SL, *ID = A + 3 * SL;
+        const double *Pp = A + 4 * SL, *Qq = A + 5 * SL, *Rr = A + 6 * SL;
+        double *yc = &y[(size_t) c * L];
+        double s1 = 0, s2 = 0, s3 = 0;
+        for (int i = 0; i < L; i++) {
+            const double t = ((i < m ? h[lo + i] : 0.0) - A[i] * s3 - AL[i] * s2 - BE[i] * s1) * ID[i];
+            yc[i] = t; s3 = s2; s2 = s1; s1 = t;
+        }
+        double x1 = 0, x2 = 0, x3 = 0;
+        for (int i = L - 1; i >= 0; i--) {
+            const double t = yc[i] - Pp[i] * x1 - Qq[i] * x2 - Rr[i] * x3;
+            yc[i] = t; x3 = x2; x2 = x1; x1 = t;
+        }
+        for (int r = 0; r < 3; r++) {
+            Y[6 * c + r] = yc[r];
+            Y[6 * c + 3 + r] = m - 3 + r >= 0 ? yc[m - 3 + r] : 0.0;
+        }
+    }
+    x.assign(n, 0.0);
+    for (int c = 0; c < S; c++) {
+        const int lo = c * L, m = std::min(L, n - lo);
+        double E[3] = {0, 0, 0}, F[3] = {0, 0, 0};
+        for (int k = 0; k < 3; k++)
+            for (int q = 0; q < 6 * S; q++) {
+                if (c > 0) E[k] += T.minv[(size_t) (6 * (c - 1) + 3 + k) * mp + q] * Y[q];
+                if (c < S - 1) F[k] += T.minv[(size_t) (6 * (c + 1) + k) * mp + q] * Y[q];
+            }
+        for (int i = 0; i < m; i++) {
+            double v = y[(size_t) c * L + i];
+            for (int k = 0; k < 3; k++)
+                v -= T.vw[(size_t) k * SL + (size_t) c * L + i] * E[k] + T.vw[(size_t) (3 + k) * SL + (size_t) c * L + i] * F[k];
+            x[lo + i] = v;
+        }
+    }
+}
+
 /* literal Thomas solve (division form) for the host-side profiles */
 static void tridag_host(const std::vector<double> &a, const std::vector<double> &b,
                         const std::vector<double> &c, const std::vector<double> &r, std::vector<double> &x)
@@ -488,12 +632,14 @@ struct TabPack {
     SubCoef sc[3];
     double lift[3][4];
     SpikeHost sph[3][3];
+    Spike4Host s4h[3][3];
     PoisHost ph[4];
     Clos clo;
     bool spike_ok = true;
+    bool spike4_ok = false, spike4_packed = false;
     size_t o_dz = 0, o_th = 0, o_st = 0, o_h1 = 0, o_twx = 0, o_twy = 0, o_twx32 = 0, o_twy32 = 0, o_lx = 0, o_ly = 0;
     size_t o_rdz = 0, o_rdzh = 0, o_lift = 0;
-    size_t o_p[4][6], o_sp[3][3][6], o_hp[3][3], o_p4[4];
+    size_t o_p[4][6], o_sp[3][3][6], o_hp[3][3], o_p4[4], o_s4[3][3][3];
     size_t o_hs[3][3];               /* spectral Helmholtz z-tables (fully implicit driver) */
 };
 
@@ -610,6 +756,20 @@ static cf_status build_tables(const cf_config *cfg, const std::vector<double> &d
                 if (cft::hept_factor(rows, hept[s][type])) {
                     cf_set_error("zero pivot in the order-4 implicit z system (substep %d)", s);
                         return CF_ERR_NUMERIC;
+                }
+                /* the partitioned form, kept only if it reproduces the
+                 * sequential solve on a test vector */
+                Spike4Host &S4 = T.s4h[s][type];
+                if (spike4_build(rows, spike4_L(nz), S4)) S4.n = 0;
+                else {
+                    const int n4 = (int) rows.size();
+                    std::vector<double> hv(n4), xs, xp;
+                    for (int k = 0; k < n4; k++) hv[k] = 1.0 + 0.5 * sin(0.37 * k) + (k % 3) * 0.25;
+                    cft::hept_solve(hept[s][type], n4, hv, xs);
+                    spike4_apply_host(S4, hv, xp);
+                    double num = 0.0, den = 0.0;
+                    for (int k = 0; k < n4; k++) { num += (xp[k] - xs[k]) * (xp[k] - xs[k]); den += xs[k] * xs[k]; }
+                    S4.ok = num <= 1e-26 * den;
                 }
                 if (type == 2) continue;
                 /* unit-forcing profile (chanfast.c:1422-1424) and its statistics */
@@ -760,6 +920,25 @@ static cf_status build_tables(const cf_config *cfg, const std::vector<double> &d
     if (order == 4) {
         for (int s = 0; s < 3; s++)
             for (int t = 0; t < 3; t++) o_hp[s][t] = put(hept[s][t].data(), hept[s][t].size());
+        /* the layout depends on the chunking alone (a function of nz); the
+         * checks decide whether the run uses the tables */
+        bool built = true;
+        T.spike4_ok = true;
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 3; t++) {
+                built = built && T.s4h[s][t].n > 0;
+                T.spike4_ok = T.spike4_ok && T.s4h[s][t].ok;
+            }
+        T.spike4_ok = T.spike4_ok && built;
+        T.spike4_packed = built;
+        if (built)
+            for (int s = 0; s < 3; s++)
+                for (int t = 0; t < 3; t++) {
+                    const Spike4Host &S4 = T.s4h[s][t];
+                    T.o_s4[s][t][0] = put(S4.f.data(), S4.f.size());
+                    T.o_s4[s][t][1] = put(S4.vw.data(), S4.vw.size());
+                    T.o_s4[s][t][2] = put(S4.minv.data(), S4.minv.size());
+                }
         for (int q = 0; q < 4; q++) o_p4[q] = put(p4[q].data(), p4[q].size());
     }
     for (int s = 0; s < 3; s++)
@@ -790,6 +969,13 @@ static void apply_tables(cf_run *r, const TabPack &TP)
         for (int t = 0; t < 3; t++) {
             R.hp[s][t].n = t == 2 ? nz - 1 : nz;
             R.hp[s][t].t = order == 4 ? tab + TP.o_hp[s][t] : nullptr;
+            Spike4Tab &S4 = R.s4[s][t];
+            S4.n = 0; S4.S = 0; S4.L = 0; S4.f = S4.vw = S4.minv = nullptr;
+            if (order == 4 && TP.spike4_packed) {
+                const Spike4Host &H4 = TP.s4h[s][t];
+                S4.n = H4.n; S4.S = H4.S; S4.L = H4.L;
+                S4.f = tab + TP.o_s4[s][t][0]; S4.vw = tab + TP.o_s4[s][t][1]; S4.minv = tab + TP.o_s4[s][t][2];
+            }
             SpikeTab &ST = R.sp[s][t];
             ST.n = TP.sph[s][t].n; ST.S = TP.sph[s][t].S; ST.L = TP.sph[s][t].L;
             if (!TP.spike_ok) continue;
@@ -1010,6 +1196,10 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     r->sp_L = TP.spike_ok ? spike_L(nz) : 0;
     for (int s = 0; s < 3; s++)
         for (int t = 0; t < 3; t++) r->spike_waves = std::max(r->spike_waves, R.sp[s][t].S);
+    r->use_spike4 = TP.spike4_ok;
+    r->s4_L = TP.spike4_ok ? spike4_L(nz) : 0;
+    for (int s = 0; s < 3; s++)
+        for (int t = 0; t < 3; t++) r->s4_waves = std::max(r->s4_waves, R.s4[s][t].S);
     R.uoff = r->uoffd;
     r->ubar.assign(nz, 0.0);
     if ((st = set_uoff(r)) != CF_OK) { free_run(r); return st; }
@@ -1323,7 +1513,10 @@ static cf_status enqueue_step_general_t(cf_run *r)
         }
         if (R.implicit) {
             /* done above */
-        } else if (r->g.order == 4)
+        } else if (r->g.order == 4 && r->use_spike4)
+            KLAUNCH("solve", pick_spike4<T>(r->s4_L), dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->s4_waves), 0,
+                    R, s, ru, rv, rw, rc);
+        else if (r->g.order == 4)
             KLAUNCH("solve", k_gsolve4<T>, dim3((unsigned) ((r->P + 255) / 256), nf), dim3(256), 0, R, s, ru, rv, rw, rc);
         else if (r->use_spike)
             KLAUNCH("solve", pick_spike<T>(r->sp_L), dim3((unsigned) ((r->P + 63) / 64), nf), dim3(64 * r->spike_waves), 0, R, s,
@@ -1552,6 +1745,9 @@ static cf_status set_dt(cf_run *r, double dt)
         cf_set_error("table layout changed with dt");
         return CF_ERR_STATE;
     }
+    /* the partitioned order-4 solve serves the new dt only if its tables
+     * pass their check there; the sequential sweep serves it otherwise */
+    r->use_spike4 = TP.spike4_ok && r->s4_L > 0;
     r->tabhost.swap(TP.pack);
     HIPCHK(hipMemcpyAsync(r->tab, r->tabhost.data(), sizeof(double) * r->tab_n, hipMemcpyHostToDevice, r->stream));
     apply_tables(r, TP);

@@ -16,10 +16,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("shape,scalar,steps", [
-    ((32, 8, 17), 0, 20),        # one tile, halos wrap onto the tile itself; one z-chunk
-    ((64, 16, 33), 1, 100),      # 2 x 2 tiles, two z-chunks
-    ((32, 16, 65), 1, 20),       # four z-chunks
-    ((32, 8, 13), 1, 20),        # both walls' ghost reach in one short chunk
+    ((32, 8, 17), 0, 20),        # one tile, halos wrap onto the tile itself
+    ((64, 16, 33), 1, 100),      # 2 x 2 tiles, several z-chunks
+    ((32, 16, 65), 1, 20),       # more z-chunks, boundaries away from the walls
+    ((32, 8, 13), 1, 20),        # chunks that hold a wall's whole ghost reach
 ])
 def test_order4_tiles_match_oracle(shape, scalar, steps):
     kw = dict(betg_x=0.3, betg_y=-0.2, betg_z=0.5) if scalar else {}

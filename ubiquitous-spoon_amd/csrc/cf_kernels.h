@@ -38,7 +38,7 @@ enum {
 #define CF_R4_X 32
 #define CF_R4_Y 8
 #ifndef CF_R4_KC
-#define CF_R4_KC 16              /* target levels per z-chunk of the order-4 RHS */
+#define CF_R4_KC 8               /* target levels per z-chunk of the order-4 RHS (8: 153 us, 16: 164 us, 32: 198 us per launch at 192x129x192) */
 #endif
 #ifndef CF_R4_WAVES
 #define CF_R4_WAVES 2            /* occupancy target (waves per SIMD) of the order-4 RHS */
@@ -209,6 +209,12 @@ static inline int spike4_L(int nz)
  * the n twiddles of the passes, staged once per workgroup */
 static inline __host__ __device__ int fft_ld(int n) { return n + 1; }
 
+#ifndef CF_P2S_NC
+#define CF_P2S_NC 16
+#endif
+#ifndef CF_P2S_MW
+#define CF_P2S_MW 8
+#endif
 /* order-2 Poisson: the partitioned solve (at most 16 z-chunks of L rows
  * per mode, 8 modes per 256-thread workgroup), L a compiled chunk length
  * >= nz / 16, covers nz <= 512; beyond that one thread per mode (256
@@ -223,14 +229,12 @@ static inline int pois2_L_of(int need)
     return 0;
 }
 /* chunk length for nz (0: one thread per mode) */
-static inline int pois2_L(int nz) { return pois2_L_of((nz + 15) / 16); }
+static inline int pois2_L(int nz) { return pois2_L_of((nz + CF_P2S_NC - 1) / CF_P2S_NC); }
 /* dynamic LDS of the partitioned solve: its six z tables */
 static inline size_t pois2_lds(int L, int nz) { return L > 0 ? 6 * sizeof(double) * (size_t) nz : 0; }
 /* workgroup shape: CF_P2S_NC z-chunks x CF_P2S_MW modes x 2 parts; a
  * z-level of the group is 2 CF_P2S_MW contiguous reals (128 B in FP64;
  * 16 modes per workgroup measured slower at 256x193x256) */
-#define CF_P2S_NC 16
-#define CF_P2S_MW 8
 #define CF_P2S_GW (2 * CF_P2S_MW)
 #define CF_P2S_THREADS (CF_P2S_NC * CF_P2S_GW)
 /* modes per workgroup: CF_P2S_MW partitioned, or one per thread (L = 0) */

@@ -879,24 +879,27 @@ static __device__ __forceinline__ void tw_rest(C *tws, C first, cp<cv<C>> tw, in
 
 /* unpack two packed real-line spectra and store kx-major: for each kx the
  * block's 2 nl lines are consecutive ky positions (coalesced) */
+/* the half spectra of a workgroup's line pairs to the ky-fastest layout:
+ * lane (kx, j) writes one value, so the 2 nl consecutive lines of a kx
+ * go out as one contiguous run of 16 (or 8) bytes per lane */
 template <typename C>
 static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, int nl, int nx, int nkx,
                                                           int ny, long NL, long pair0, double scd, gp<cv<C>> X)
 {
     typedef typename Real<C>::type F;
     const F sc = (F) scd;
-    for (int e = threadIdx.x; e < nl * nkx; e += blockDim.x) {
-        const int kx = e / nl, l = e - kx * nl;
-        const long la = 2 * (pair0 + l), lb = la + 1;
-        if (la >= NL) continue;
-        const bool on = fsw_on(nx);
+    const bool on = fsw_on(nx);
+    const int nj = 2 * nl;
+    for (int e = threadIdx.x; e < nj * nkx; e += blockDim.x) {
+        const int kx = e / nj, jj = e - kx * nj;
+        const int l = jj >> 1;
+        const bool second = (jj & 1) != 0;
+        const long line = 2 * (pair0 + l) + (second ? 1 : 0);
+        if (line >= NL) continue;
         const C Z = sb[l * ld + fsw(kx, on)], Wc = sb[l * ld + fsw(kx == 0 ? 0 : nx - kx, on)];
-        const long k = la / ny, j = la - k * ny;
-        X[((size_t) k * nkx + kx) * ny + j] = cv_of(mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc));
-        if (lb < NL) {
-            const long kb = lb / ny, jb = lb - kb * ny;
-            X[((size_t) kb * nkx + kx) * ny + jb] = cv_of(mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc));
-        }
+        const long k = line / ny, j = line - k * ny;
+        const C val = second ? mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc) : mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
+        X[((size_t) k * nkx + kx) * ny + j] = cv_of(val);
     }
 }
 
@@ -980,7 +983,36 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
             const int ec = e < tot ? e : tot - 1;
             const int l = ec / nx, i = ec - l * nx;
             const int pr = pair0 + l;
-            res[u] = mkc<C>(divg(2 * pr, i), divg(2 * pr + 1, i));
+            if (ORD == 2 && (nx & 63) == 0 && (ny & 1) == 0) {
+                /* the pair's lines are rows j, j + 1 of one level: the
+                 * second line's v(j - 1) is the first line's v, and
+                 * u(i - 1) comes from the neighbouring lane */
+                const int la = 2 * pr;
+                const bool ok = la < NL;
+                const int lc = ok ? la : NL - 2;
+                const int k = lc / ny, j = lc - k * ny;
+                const int jm = j > 0 ? j - 1 : ny - 1;
+                const size_t qa = (size_t) k * P + (size_t) j * nx, qb = qa + nx, qm = (size_t) k * P + (size_t) jm * nx;
+                const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
+                const D ua = (D) us[qa + i] + hm, ub = (D) us[qb + i] + hm;
+                D uam = __shfl_up(ua, 1, 64), ubm = __shfl_up(ub, 1, 64);
+                if ((threadIdx.x & 63) == 0) {
+                    const int im = i > 0 ? i - 1 : nx - 1;
+                    uam = (D) us[qa + im] + hm;
+                    ubm = (D) us[qb + im] + hm;
+                }
+                const D va = vs[qa + i], vb = vs[qb + i], vam = vs[qm + i];
+                const D war = ws[qa + i], wbr = ws[qb + i];
+                const D wamr = ws[(k > 0 ? qa - P : qa) + i], wbmr = ws[(k > 0 ? qb - P : qb) + i];
+                const D wa = k < nz - 1 ? war : (D) 0, wb = k < nz - 1 ? wbr : (D) 0;
+                const D wam = k > 0 ? wamr : (D) 0, wbm = k > 0 ? wbmr : (D) 0;
+                const D rz = (D) ((T) rdz[k]);
+                const D da = (ua - uam) * (D) rdx + (va - vam) * (D) rdy + (wa - wam) * rz;
+                const D db = (ub - ubm) * (D) rdx + (vb - va) * (D) rdy + (wb - wbm) * rz;
+                res[u] = mkc<C>(ok ? (T) da : (T) 0, ok ? (T) db : (T) 0);
+            } else {
+                res[u] = mkc<C>(divg(2 * pr, i), divg(2 * pr + 1, i));
+            }
         }
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
@@ -1337,7 +1369,9 @@ static __device__ __forceinline__ void pois2_mode(const PoisTab &T, C *__restric
  * sweep at these mode counts.  Differs from sequential Thomas by
  * round-off. */
 /* occupancy target by chunk length: the largest without register spills */
+#ifndef CF_P2S_WAVES
 #define CF_P2S_WAVES(L) ((L) == 0 ? 2 : ((L) <= 13 ? 4 : ((L) <= 20 ? 3 : 1)))
+#endif
 
 template <int L, typename A, typename T>
 static __device__ __forceinline__ void pois2s_run(const PoisTab &T_, typename Real<typename Cplx<T>::type>::type *__restrict__ yr,

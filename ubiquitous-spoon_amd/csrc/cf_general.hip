@@ -775,47 +775,71 @@ template <typename Ar, typename C>
 static __device__ __forceinline__ void pois4_mode(const Pois4Tab &T, C *__restrict__ y, double *__restrict__ S,
                                                   size_t M, int nz, double lxd, double lyd, int variant)
 {
+    /* the rows go through registers in blocks of BT: every load of a block
+     * is in flight before the first row's recurrence needs it (one wave per
+     * 64 modes leaves no other waves to hide a per-row load latency) */
+    constexpr int BT = 8;
     const Ar lx = (Ar) lxd, ly = (Ar) lyd;
     const double *bd = T.band[variant];
     double *Sp = S, *Sq = S + (size_t) nz * M, *Sr = S + (size_t) 2 * nz * M;
     Ar p1 = 0, p2 = 0, p3 = 0, q1 = 0, q2 = 0, q3 = 0, r1 = 0, r2 = 0, r3 = 0;
     Ar s1x = 0, s2x = 0, s3x = 0, s1y = 0, s2y = 0, s3y = 0;
-    for (int i = 0; i < nz; i++) {
-        const Ar a = (Ar) bd[i], b = (Ar) bd[nz + i], c = (Ar) bd[2 * nz + i];
-        const Ar d = (Ar) bd[3 * nz + i] - lx * (Ar) T.ex[i] - ly * (Ar) T.ey[i];
-        const Ar e = (Ar) bd[4 * nz + i], f = (Ar) bd[5 * nz + i], g = (Ar) bd[6 * nz + i];
-        const Ar alp = b - a * p3;
-        const Ar bet = c - a * q3 - alp * p2;
-        const Ar den = d - a * r3 - alp * q2 - bet * p1;
-        const Ar iden = (Ar) 1 / den;
-        const Ar pi = (e - alp * r2 - bet * q1) * iden;
-        const Ar qi = (f - bet * r1) * iden;
-        const Ar ri = g * iden;
-        const C rh = y[(size_t) i * M];
-        const Ar dv = (Ar) T.dvol[i];
-        const Ar six = ((Ar) rh.x * dv - a * s3x - alp * s2x - bet * s1x) * iden;
-        const Ar siy = ((Ar) rh.y * dv - a * s3y - alp * s2y - bet * s1y) * iden;
-        C o;
-        o.x = six; o.y = siy;
-        y[(size_t) i * M] = o;
-        Sp[(size_t) i * M] = pi; Sq[(size_t) i * M] = qi; Sr[(size_t) i * M] = ri;
-        p3 = p2; p2 = p1; p1 = pi;
-        q3 = q2; q2 = q1; q1 = qi;
-        r3 = r2; r2 = r1; r1 = ri;
-        s3x = s2x; s2x = s1x; s1x = six;
-        s3y = s2y; s2y = s1y; s1y = siy;
+    for (int i0 = 0; i0 < nz; i0 += BT) {
+        C rb[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) rb[t] = y[(size_t) min(i0 + t, nz - 1) * M];
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int i = i0 + t;
+            if (i < nz) {
+                const Ar a = (Ar) bd[i], b = (Ar) bd[nz + i], c = (Ar) bd[2 * nz + i];
+                const Ar d = (Ar) bd[3 * nz + i] - lx * (Ar) T.ex[i] - ly * (Ar) T.ey[i];
+                const Ar e = (Ar) bd[4 * nz + i], f = (Ar) bd[5 * nz + i], g = (Ar) bd[6 * nz + i];
+                const Ar alp = b - a * p3;
+                const Ar bet = c - a * q3 - alp * p2;
+                const Ar den = d - a * r3 - alp * q2 - bet * p1;
+                const Ar iden = (Ar) 1 / den;
+                const Ar pi = (e - alp * r2 - bet * q1) * iden;
+                const Ar qi = (f - bet * r1) * iden;
+                const Ar ri = g * iden;
+                const C rh = rb[t];
+                const Ar dv = (Ar) T.dvol[i];
+                const Ar six = ((Ar) rh.x * dv - a * s3x - alp * s2x - bet * s1x) * iden;
+                const Ar siy = ((Ar) rh.y * dv - a * s3y - alp * s2y - bet * s1y) * iden;
+                C o;
+                o.x = six; o.y = siy;
+                y[(size_t) i * M] = o;
+                Sp[(size_t) i * M] = pi; Sq[(size_t) i * M] = qi; Sr[(size_t) i * M] = ri;
+                p3 = p2; p2 = p1; p1 = pi;
+                q3 = q2; q2 = q1; q1 = qi;
+                r3 = r2; r2 = r1; r1 = ri;
+                s3x = s2x; s2x = s1x; s1x = six;
+                s3y = s2y; s2y = s1y; s1y = siy;
+            }
+        }
     }
     Ar x1x = 0, x2x = 0, x3x = 0, x1y = 0, x2y = 0, x3y = 0;
-    for (int i = nz - 1; i >= 0; i--) {
-        const Ar pi = (Ar) Sp[(size_t) i * M], qi = (Ar) Sq[(size_t) i * M], ri = (Ar) Sr[(size_t) i * M];
-        const C si = y[(size_t) i * M];
-        const Ar xix = (Ar) si.x - pi * x1x - qi * x2x - ri * x3x;
-        const Ar xiy = (Ar) si.y - pi * x1y - qi * x2y - ri * x3y;
-        C o;
-        o.x = xix; o.y = xiy;
-        y[(size_t) i * M] = o;
-        x3x = x2x; x2x = x1x; x1x = xix;
-        x3y = x2y; x2y = x1y; x1y = xiy;
+    for (int i0 = nz - 1; i0 >= 0; i0 -= BT) {
+        C sb[BT];
+        double pb[BT], qb[BT], rbv[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const size_t o = (size_t) max(i0 - t, 0) * M;
+            sb[t] = y[o]; pb[t] = Sp[o]; qb[t] = Sq[o]; rbv[t] = Sr[o];
+        }
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int i = i0 - t;
+            if (i >= 0) {
+                const Ar xix = (Ar) sb[t].x - (Ar) pb[t] * x1x - (Ar) qb[t] * x2x - (Ar) rbv[t] * x3x;
+                const Ar xiy = (Ar) sb[t].y - (Ar) pb[t] * x1y - (Ar) qb[t] * x2y - (Ar) rbv[t] * x3y;
+                C o;
+                o.x = xix; o.y = xiy;
+                y[(size_t) i * M] = o;
+                x3x = x2x; x2x = x1x; x1x = xix;
+                x3y = x2y; x2y = x1y; x1y = xiy;
+            }
+        }
     }
 }
 

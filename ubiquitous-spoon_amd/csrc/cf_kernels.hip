@@ -143,13 +143,19 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         jj %= ny; if (jj < 0) jj += ny;
         return (size_t) jj * nx + ii;
     };
-    size_t oa[2], op[2];
+    /* element offsets as 32-bit byte offsets from each array's (scalar)
+     * base: the loads take base + 32-bit vector offset directly, with no
+     * 64-bit address arithmetic (N sizeof(T) < 2^32, checked at creation) */
+    unsigned oa[2], op[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         const int e = tid + 256 * t;
-        oa[t] = e < RT_NA ? colo(e, RT_HX) : 0;
-        op[t] = e < RT_NP ? colo(e, RT_PX) : 0;
+        oa[t] = (unsigned) ((e < RT_NA ? colo(e, RT_HX) : 0) * sizeof(T));
+        op[t] = (unsigned) ((e < RT_NP ? colo(e, RT_PX) : 0) * sizeof(T));
     }
+    const unsigned Pb = (unsigned) (P * sizeof(T));
+    auto ldb = [](gp<const T> base, unsigned off) -> T { return *(gp<const T>) ((gp<const char>) base + off); };
+    auto stb = [](gp<T> base, unsigned off, T v) { *(gp<T>) ((gp<char>) base + off) = v; };
     /* a register set holds one fetched level L of the tile: u*, v*, w* on
      * the halo tile and, two levels ahead, p(L + 2) on the pressure tile
      * (clamped to the walls), so the pressure a level stages was read a
@@ -164,11 +170,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         for (int t = 0; t < 2; t++) {
             const int e = tid + 256 * t;
             if (e < RT_NA) {
-                X.u[t] = Sus[(size_t) Lc * P + oa[t]];
-                X.v[t] = Svs[(size_t) Lc * P + oa[t]];
-                X.w[t] = wok ? Sws[(size_t) L * P + oa[t]] : (T) 0;
+                X.u[t] = ldb(Sus, (unsigned) Lc * Pb + oa[t]);
+                X.v[t] = ldb(Svs, (unsigned) Lc * Pb + oa[t]);
+                X.w[t] = wok ? ldb(Sws, (unsigned) L * Pb + oa[t]) : (T) 0;
             }
-            if (e < RT_NP) X.pn[t] = Sp[(size_t) L2 * P + op[t]];
+            if (e < RT_NP) X.pn[t] = ldb(Sp, (unsigned) L2 * Pb + op[t]);
         }
     };
     /* pressure of level L into buffer (L - k0 + 1) & 1: p(clamp(L)) and,
@@ -189,7 +195,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     auto pdirect = [&](int L, T *out) {
         const int Lc = clampz(L);
 #pragma unroll
-        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? Sp[(size_t) Lc * P + op[t]] : (T) 0;
+        for (int t = 0; t < 2; t++) out[t] = tid + 256 * t < RT_NP ? ldb(Sp, (unsigned) Lc * Pb + op[t]) : (T) 0;
     };
     /* projected u, v, w of level L into ring slot (L - k0 + 1) & 3 */
     auto build = [&](int L, const RSet &X) {
@@ -252,12 +258,16 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
     double cflm = 0.0;
     /* previous tendencies of this point, one level ahead */
     const size_t qc = (size_t) j * nx + i;
+    const unsigned qcb = (unsigned) (qc * sizeof(T));
     /* gu == nullptr: the substep's zeta is 0 (no previous tendency read);
      * fu == nullptr: this substep's tendency is never read (the next
      * step's first substep has zeta = 0), so it is not stored */
     const bool useg = gu != nullptr, keepf = fu != nullptr;
     T gnu = 0, gnv = 0, gnw = 0;
-    if (act && useg) { gnu = gu[(size_t) k0 * P + qc]; gnv = gv[(size_t) k0 * P + qc]; gnw = gw[(size_t) k0 * P + qc]; }
+    if (act && useg) {
+        const unsigned o = (unsigned) k0 * Pb + qcb;
+        gnu = ldb(gu, o); gnv = ldb(gv, o); gnw = ldb(gw, o);
+    }
 
     /* level k: register set Y holds level k + 2 (its pressure staged), X
      * (level k + 1, holding p(k + 3)) receives level k + 3.  Four ring slots
@@ -269,14 +279,13 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
         T pk3[2] = { X.pn[0], X.pn[1] };          /* p(k + 3) */
         const T gcu = gnu, gcv = gnv, gcw = gnw;
         if (act && useg && k + 1 < k1) {
-            gnu = gu[(size_t) (k + 1) * P + qc];
-            gnv = gv[(size_t) (k + 1) * P + qc];
-            gnw = gw[(size_t) (k + 1) * P + qc];
+            const unsigned o = (unsigned) (k + 1) * Pb + qcb;
+            gnu = ldb(gu, o); gnv = ldb(gv, o); gnw = ldb(gw, o);
         }
         if (more3) fetch(k + 3, X);
         const int sm = (k - k0) & 3, s0 = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
         if (act) {
-            const size_t q = (size_t) k * P + (size_t) j * nx + i;
+            const unsigned qb = (unsigned) k * Pb + qcb;
             const T dzk = (T) dz[k], dzkp = (T) dz[k + 1];
             const T rzk = (T) rdz[k], rzkp = (T) rdz[k + 1];
             const T rzp = (T) rdzh[k], rzm = (T) rdzh[k - 1];
@@ -319,8 +328,8 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 h += gamdt * f + zetdt * g;
                 if (k == 0) h -= (T) R.lift[s][0];
                 if (k == nz - 1) h -= (T) R.lift[s][1];
-                if (keepf) fu[q] = f;
-                ru[q] = h * dzk;
+                if (keepf) stb(fu, qb, f);
+                stb(ru, qb, h * dzk);
             }
             SEQ();
             {   /* v */
@@ -342,8 +351,8 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 h += gamdt * f + zetdt * gcv;
                 if (k == 0) h -= (T) R.lift[s][2];
                 if (k == nz - 1) h -= (T) R.lift[s][3];
-                if (keepf) fv[q] = f;
-                rv[q] = h * dzk;
+                if (keepf) stb(fv, qb, f);
+                stb(rv, qb, h * dzk);
             }
             SEQ();
             if (k < nz - 1) {   /* w on the interior faces */
@@ -364,11 +373,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 T f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
                 T h = w0 + nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
                 h += gamdt * f + zetdt * gcw;
-                if (keepf) fw[q] = f;
-                rw[q] = h * dzp1h;
+                if (keepf) stb(fw, qb, f);
+                stb(rw, qb, h * dzp1h);
             } else {
-                if (keepf) fw[q] = (T) 0;
-                rw[q] = (T) 0;
+                if (keepf) stb(fw, qb, (T) 0);
+                stb(rw, qb, (T) 0);
             }
 #undef SEQ
 #undef LU

@@ -165,6 +165,14 @@ __device__ __forceinline__ T d2line(const T X[7])
     return A1 * (p1 - m1) + A2 * (p3 - m3);
 }
 
+/* loads and stores at a 32-bit byte offset from a scalar base: the
+ * hardware adds the base to the 32-bit vector offset, with no 64-bit
+ * address arithmetic (arrays below 2^32 bytes, checked at creation) */
+template <typename T>
+__device__ __forceinline__ T ldo(gp<const T> b, unsigned off) { return *(gp<const T>) ((gp<const char>) b + off); }
+template <typename T>
+__device__ __forceinline__ void sto(gp<T> b, unsigned off, T v) { *(gp<T>) ((gp<char>) b + off) = v; }
+
 }  // namespace
 
 
@@ -238,10 +246,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
     const int P = nx * ny;
     const Tile4 tl(nx, ny, nz, nch);
     const int lx = tl.lx, ly = tl.ly, tid = tl.tid, k0 = tl.k0, k1 = tl.k1;
-    const int oc = tl.col(lx, ly, nx, ny);
+    const unsigned oc = (unsigned) tl.col(lx, ly, nx, ny) * (unsigned) sizeof(T);
     int hx, hy;
     halo_xy(tid, hx, hy);
-    const int oh = tl.col(hx, hy, nx, ny);
+    const unsigned oh = (unsigned) tl.col(hx, hy, nx, ny) * (unsigned) sizeof(T);
+    const unsigned Pb = (unsigned) P * (unsigned) sizeof(T);
 
     const T dx = (T) R.g.dx, dy = (T) R.g.dy;
     const T rdx2 = (T) (R.rdx * R.rdx), rdy2 = (T) (R.rdy * R.rdy);
@@ -258,8 +267,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
     const Clos1 &Cb = R.cl.cell[0], &Ct = R.cl.cell[1];
     const T h5 = (T) 0.5;
 
-    auto ld_cell = [&](gp<const T> q, int L, int o) -> T { return (L >= 0 && L < nz) ? q[(size_t) L * P + o] : (T) 0; };
-    auto ld_face = [&](int L, int o) -> T { return (L >= 0 && L <= nz - 2) ? w[(size_t) L * P + o] : (T) 0; };
+    auto ld_cell = [&](gp<const T> q, int L, unsigned o) -> T { return (L >= 0 && L < nz) ? ldo(q, (unsigned) L * Pb + o) : (T) 0; };
+    auto ld_face = [&](int L, unsigned o) -> T { return (L >= 0 && L <= nz - 2) ? ldo(w, (unsigned) L * Pb + o) : (T) 0; };
 
     /* x- and y-interpolated w at faces k-3..k (before a level's update) */
     T WXw[4], WYw[4];
@@ -269,8 +278,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
         const int f = k0 - 2 + e;
         T wxv = (T) 0, wyv = (T) 0;
         if (f >= 0 && f <= nz - 2) {                   /* uniform */
-            Pw[ly + H][lx + H] = w[(size_t) f * P + oc];
-            Pw[hy + H][hx + H] = w[(size_t) f * P + oh];
+            Pw[ly + H][lx + H] = ldo(w, (unsigned) f * Pb + oc);
+            Pw[hy + H][hx + H] = ldo(w, (unsigned) f * Pb + oh);
             __syncthreads();
             wxv = B1 * (Pw[ly + H][lx + H] + Pw[ly + H][lx + H + 1]) + B2 * (Pw[ly + H][lx + H - 1] + Pw[ly + H][lx + H + 2]);
             wyv = B1 * (Pw[ly + H][lx + H] + Pw[ly + H + 1][lx + H]) + B2 * (Pw[ly + H - 1][lx + H] + Pw[ly + H + 2][lx + H]);
@@ -302,9 +311,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
         const T nhu = ld_cell(u, k + 1, oh), nhv = ld_cell(v, k + 1, oh);
         const T ncc = SC ? ld_cell(c, k + 1, oc) : (T) 0, nhc = SC ? ld_cell(c, k + 1, oh) : (T) 0;
         const T nwo = ld_face(k + 2, oc), nhw = ld_face(k + 2, oh);
-        const int q = k * P + oc;
-        const T gu_q = s > 0 ? gu[q] : (T) 0, gv_q = s > 0 ? gv[q] : (T) 0;
-        const T wq = want_cfl && k <= nz - 2 ? w[q] : (T) 0;
+        const unsigned q = (unsigned) k * Pb + oc;
+        const T gu_q = s > 0 ? ldo(gu, q) : (T) 0, gv_q = s > 0 ? ldo(gv, q) : (T) 0;
+        const T wq = want_cfl && k <= nz - 2 ? ldo(w, q) : (T) 0;
         __syncthreads();
 
         /* x/y-interpolated w at face k+1 */
@@ -369,8 +378,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
             h += gamdt * f;
             if (s > 0) h += zetdt * (gu_q + mfu);
             h -= (T) lift[0 * nz + k];
-            if (s < 2) fu[q] = f;
-            ru[q] = impl ? h : h * dzk;
+            if (s < 2) sto(fu, q, f);
+            sto(ru, q, impl ? h : h * dzk);
             if (want_cfl)
                 cflv = fmax(cflv, fmax(fmax(fabs((double) Q0) * cfldt * R.rdx, fabs((double) Vc[3]) * cfldt * R.rdy),
                                        fabs((double) wq) * cfldt * rdzh[k]));
@@ -417,8 +426,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
             h += gamdt * f;
             if (s > 0) h += zetdt * gv_q;
             h -= (T) lift[1 * nz + k];
-            if (s < 2) fv[q] = f;
-            rv[q] = impl ? h : h * dzk;
+            if (s < 2) sto(fv, q, f);
+            sto(rv, q, impl ? h : h * dzk);
         }
         __syncthreads();
 #pragma unroll
@@ -463,10 +472,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
     const int P = nx * ny;
     const Tile4 tl(nx, ny, nz, nch);
     const int lx = tl.lx, ly = tl.ly, tid = tl.tid, k0 = tl.k0, k1 = tl.k1;
-    const int oc = tl.col(lx, ly, nx, ny);
+    const unsigned oc = (unsigned) tl.col(lx, ly, nx, ny) * (unsigned) sizeof(T);
     int hx, hy;
     halo_xy(tid, hx, hy);
-    const int oh = tl.col(hx, hy, nx, ny);
+    const unsigned oh = (unsigned) tl.col(hx, hy, nx, ny) * (unsigned) sizeof(T);
+    const unsigned Pb = (unsigned) P * (unsigned) sizeof(T);
     /* a 4-level window for a halo column of the shared face fluxes: threads
      * 0..3TX-1 the vz rows y = -2, -1, TY; the next 3TY threads the uz
      * columns x = -2, -1, TX */
@@ -474,9 +484,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
     int zx = 0, zy = 0;
     if (zkind == 1) { zx = tid % TX; const int r = tid / TX; zy = r < 2 ? r - 2 : TY; }
     else if (zkind == 0) { const int e = tid - 3 * TX, q = e / TY; zx = q < 2 ? q - 2 : TX; zy = e % TY; }
-    const int oz = tl.col(zx, zy, nx, ny);
+    const unsigned oz = (unsigned) tl.col(zx, zy, nx, ny) * (unsigned) sizeof(T);
     const gp<const T> zsrc = zkind == 1 ? v : u;
-    const int oxm = tl.col(lx - 1, ly, nx, ny), oym = tl.col(lx, ly - 1, nx, ny);
+    const unsigned oxm = (unsigned) tl.col(lx - 1, ly, nx, ny) * (unsigned) sizeof(T);
+    const unsigned oym = (unsigned) tl.col(lx, ly - 1, nx, ny) * (unsigned) sizeof(T);
 
     const T dx = (T) R.g.dx, dy = (T) R.g.dy;
     const T rdx2 = (T) (R.rdx * R.rdx), rdy2 = (T) (R.rdy * R.rdy);
@@ -493,8 +504,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
     const Clos1 &Cb = R.cl.cell[0], &Ct = R.cl.cell[1], &Fb = R.cl.face[0], &Ft = R.cl.face[1];
     const T h5 = (T) 0.5;
 
-    auto ld_cell = [&](gp<const T> q, int L, int o) -> T { return (L >= 0 && L < nz) ? q[(size_t) L * P + o] : (T) 0; };
-    auto ld_face = [&](int L, int o) -> T { return (L >= 0 && L <= nz - 2) ? w[(size_t) L * P + o] : (T) 0; };
+    auto ld_cell = [&](gp<const T> q, int L, unsigned o) -> T { return (L >= 0 && L < nz) ? ldo(q, (unsigned) L * Pb + o) : (T) 0; };
+    auto ld_face = [&](int L, unsigned o) -> T { return (L >= 0 && L <= nz - 2) ? ldo(w, (unsigned) L * Pb + o) : (T) 0; };
 
     T Wc[7], Cc[7];                                     /* faces / levels k-3..k+3 of the own column */
     T U4[4], V4[4], Zh[4];                              /* levels k-1..k+2: own column, halo column */
@@ -539,10 +550,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
         const T nU = ld_cell(u, k + 3, oc), nV = ld_cell(v, k + 3, oc);
         const T nZ = zkind >= 0 ? ld_cell(zsrc, k + 3, oz) : (T) 0;
         const T nhw = ld_face(k + 1, oh), nhc = SC ? ld_cell(c, k + 1, oh) : (T) 0;
-        const int q = k * P + oc;
-        const T gw_q = (s > 0 && wface) ? gw[q] : (T) 0;
-        const T gc_q = (SC && s > 0) ? gc[q] : (T) 0;
-        const T uxm = SC ? u[(size_t) k * P + oxm] : (T) 0, vym = SC ? v[(size_t) k * P + oym] : (T) 0;
+        const unsigned q = (unsigned) k * Pb + oc;
+        const T gw_q = (s > 0 && wface) ? ldo(gw, q) : (T) 0;
+        const T gc_q = (SC && s > 0) ? ldo(gc, q) : (T) 0;
+        const T uxm = SC ? ldo(u, (unsigned) k * Pb + oxm) : (T) 0, vym = SC ? ldo(v, (unsigned) k * Pb + oym) : (T) 0;
         __syncthreads();
 
         /* ---- w at face k */
@@ -585,11 +596,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
             h += gamdt * f;
             if (s > 0) h += zetdt * gw_q;
             h -= (T) lift[2 * nz + k];
-            if (s < 2) fw[q] = f;
-            rw[q] = impl ? h : h * (h5 * ((T) dz[k + 1] + dzk));
+            if (s < 2) sto(fw, q, f);
+            sto(rw, q, impl ? h : h * (h5 * ((T) dz[k + 1] + dzk)));
         } else {                                       /* top wall face: f = 0 (advect.c:301), no unknown */
-            if (s < 2) fw[q] = (T) 0;
-            rw[q] = (T) 0;
+            if (s < 2) sto(fw, q, (T) 0);
+            sto(rw, q, (T) 0);
         }
         /* ---- c */
         if (SC) {
@@ -619,8 +630,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CF_R4_WAVES
             h += gamdt * f;
             if (s > 0) h += zetdt * (gc_q + mfc);
             h -= (T) lift[3 * nz + k];
-            if (s < 2) fc[q] = f;
-            rc[q] = impl ? h : h * dzk;
+            if (s < 2) sto(fc, q, f);
+            sto(rc, q, impl ? h : h * dzk);
         }
         __syncthreads();
 #pragma unroll

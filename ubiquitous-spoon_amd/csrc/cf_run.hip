@@ -1100,7 +1100,10 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     const int order = cfg->order;
     r->prec = cfg->precision;
     r->esz = cfg->precision == CF_FP32 ? sizeof(float) : sizeof(double);
-    r->general = force_general || order == 4 || cfg->scalar || cfg->driver == CF_DRIVER_CHANNEL;
+    /* the fused path and the order-4 tiles address their arrays with 32-bit
+     * byte offsets: a larger field takes the general path */
+    const bool small = (size_t) nx * ny * nz * r->esz < ((size_t) 1 << 32);
+    r->general = force_general || order == 4 || cfg->scalar || cfg->driver == CF_DRIVER_CHANNEL || !small;
     r->nfield = cfg->scalar ? 4 : 3;
     r->rd.m_prev_slot[0] = SC_M0 + 2;
     r->rd.m_prev_slot[1] = SC_M0 + 0;
@@ -1136,7 +1139,8 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     ok = ok && dalloc(&r->part, npart) && dalloc(&r->lsum, (size_t) nz + 2);
     rhs_chunks(nx, ny, nz, CF_RHS_KC, &r->rhs_kc, &r->rhs_blocks);
     r->gblocks = ((nx + 63) / 64) * ((ny + 3) / 4) * nz;
-    if (cfg->order == 4 && nx % CF_R4_X == 0 && ny % CF_R4_Y == 0) {
+    if (cfg->order == 4 && nx % CF_R4_X == 0 && ny % CF_R4_Y == 0
+        && (size_t) nx * ny * nz * r->esz < ((size_t) 1 << 32)) {   /* 32-bit byte offsets */
         r->r4_nch = std::max(1, (nz + CF_R4_KC / 2) / CF_R4_KC);
         r->r4_blocks = (nx / CF_R4_X) * (ny / CF_R4_Y) * r->r4_nch;
     }

@@ -316,6 +316,32 @@ static __device__ __forceinline__ void fft_lines(C *buf, int n, int ld, int nlin
     else lds_fft<MAXE>(buf, n, ld, nlines, rad, nrad, tw, sg);
 }
 
+template <int CTRL, int ROWS>
+static __device__ __forceinline__ double dpp_f64(double x)
+{
+    /* DPP moves the two 32-bit halves; lanes outside ROWS read 0 */
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+/* sum over the 64 lanes in a fixed order (the result does not depend on
+ * scheduling), returned by every lane: pairs, quads, half rows and rows by
+ * DPP permutes within each row of 16 lanes, then rows 0 + 1 and 2 + 3 and
+ * the two halves by row broadcasts; lane 63 ends with the total */
+static __device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_f64<0xB1, 0xf>(v);        /* quad_perm [1, 0, 3, 2] */
+    v += dpp_f64<0x4E, 0xf>(v);        /* quad_perm [2, 3, 0, 1] */
+    v += dpp_f64<0x141, 0xf>(v);       /* row_half_mirror */
+    v += dpp_f64<0x140, 0xf>(v);       /* row_mirror */
+    v += dpp_f64<0x142, 0xa>(v);       /* row_bcast:15 into rows 1 and 3 */
+    v += dpp_f64<0x143, 0xc>(v);       /* row_bcast:31 into rows 2 and 3 */
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
 /* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
  * L2); renumber so that consecutive logical blocks share an XCD and the
  * halo rows they exchange stay in one L2. */

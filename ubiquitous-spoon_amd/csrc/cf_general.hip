@@ -870,6 +870,213 @@ __global__ __launch_bounds__(256) void k_poisson4(const RunDev R, const Pois4Tab
 template __global__ void k_poisson4<double>(const RunDev, const Pois4Tab, double2 *, double *);
 template __global__ void k_poisson4<float>(const RunDev, const Pois4Tab, float2 *, double *);
 
+/* The same solve split at the middle row m = nz / 2 ("twisted"), for
+ * nz >= 6: wave 0 of a 128-thread workgroup eliminates rows 0..m-1 of 64
+ * modes from the top (the recurrence above), wave 1 rows nz-1..m from the
+ * bottom (the same recurrence on the mirrored bands), so each dependent
+ * chain is half as long and twice as many waves run.  Each side's last
+ * three relations,
+ *   top:    x_R + p x_{R+1} + q x_{R+2} + r x_{R+3} = s   (R = m-3..m-1)
+ *   bottom: x_R + p x_{R-1} + q x_{R-2} + r x_{R-3} = s   (R = m..m+2),
+ * couple the six unknowns x_{m-3..m+2}; both waves solve that 6x6 system
+ * (unit diagonal, no pivoting) and back-substitute their own half.  Every
+ * row is still eliminated once, from the rows on its side only, as in the
+ * sequential elimination; results differ from it by round-off. */
+template <typename Ar, typename C>
+static __device__ __forceinline__ void p4t_sweep(const Pois4Tab &T, C *__restrict__ y, double *__restrict__ S,
+                                                 size_t M, int nz, double lxd, double lyd, int variant, int side,
+                                                 double *__restrict__ xme)
+{
+    constexpr int BT = 8;
+    const int m = nz / 2, cnt = side == 0 ? m : nz - m;
+    const Ar lx = (Ar) lxd, ly = (Ar) lyd;
+    const double *bd = T.band[variant];
+    /* band offsets -3..+3 as seen from this side (mirrored for the bottom) */
+    const double *ba = bd + (side == 0 ? 0 : 6) * nz, *bb = bd + (side == 0 ? 1 : 5) * nz, *bc = bd + (side == 0 ? 2 : 4) * nz;
+    const double *be = bd + (side == 0 ? 4 : 2) * nz, *bf = bd + (side == 0 ? 5 : 1) * nz, *bg = bd + (side == 0 ? 6 : 0) * nz;
+    const double *bdd = bd + 3 * nz;
+    double *Sp = S, *Sq = S + (size_t) nz * M, *Sr = S + (size_t) 2 * nz * M;
+    Ar p1 = 0, p2 = 0, p3 = 0, q1 = 0, q2 = 0, q3 = 0, r1 = 0, r2 = 0, r3 = 0;
+    Ar s1x = 0, s2x = 0, s3x = 0, s1y = 0, s2y = 0, s3y = 0;
+    for (int j0 = 0; j0 < cnt; j0 += BT) {
+        C rb[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int j = min(j0 + t, cnt - 1);
+            rb[t] = y[(size_t) (side == 0 ? j : nz - 1 - j) * M];
+        }
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int j = j0 + t;
+            if (j < cnt) {
+                const int i = side == 0 ? j : nz - 1 - j;
+                const Ar a = (Ar) ba[i], b = (Ar) bb[i], c = (Ar) bc[i];
+                const Ar d = (Ar) bdd[i] - lx * (Ar) T.ex[i] - ly * (Ar) T.ey[i];
+                const Ar e = (Ar) be[i], f = (Ar) bf[i], g = (Ar) bg[i];
+                const Ar alp = b - a * p3;
+                const Ar bet = c - a * q3 - alp * p2;
+                const Ar den = d - a * r3 - alp * q2 - bet * p1;
+                const Ar iden = (Ar) 1 / den;
+                const Ar pi = (e - alp * r2 - bet * q1) * iden;
+                const Ar qi = (f - bet * r1) * iden;
+                const Ar ri = g * iden;
+                const Ar dv = (Ar) T.dvol[i];
+                const Ar six = ((Ar) rb[t].x * dv - a * s3x - alp * s2x - bet * s1x) * iden;
+                const Ar siy = ((Ar) rb[t].y * dv - a * s3y - alp * s2y - bet * s1y) * iden;
+                C o;
+                o.x = six; o.y = siy;
+                y[(size_t) i * M] = o;
+                Sp[(size_t) i * M] = pi; Sq[(size_t) i * M] = qi; Sr[(size_t) i * M] = ri;
+                p3 = p2; p2 = p1; p1 = pi;
+                q3 = q2; q2 = q1; q1 = qi;
+                r3 = r2; r2 = r1; r1 = ri;
+                s3x = s2x; s2x = s1x; s1x = six;
+                s3y = s2y; s2y = s1y; s1y = siy;
+            }
+        }
+    }
+    /* the relations of local rows cnt-1, cnt-2, cnt-3 (five values each) */
+    xme[0 * 64] = (double) p1; xme[1 * 64] = (double) q1; xme[2 * 64] = (double) r1;
+    xme[3 * 64] = (double) s1x; xme[4 * 64] = (double) s1y;
+    xme[5 * 64] = (double) p2; xme[6 * 64] = (double) q2; xme[7 * 64] = (double) r2;
+    xme[8 * 64] = (double) s2x; xme[9 * 64] = (double) s2y;
+    xme[10 * 64] = (double) p3; xme[11 * 64] = (double) q3; xme[12 * 64] = (double) r3;
+    xme[13 * 64] = (double) s3x; xme[14 * 64] = (double) s3y;
+}
+
+template <typename Ar, typename C>
+static __device__ __forceinline__ void p4t_finish(C *__restrict__ y, const double *__restrict__ S, size_t M, int nz,
+                                                  int side, const double *__restrict__ xt, const double *__restrict__ xb)
+{
+    constexpr int BT = 8;
+    const int m = nz / 2, cnt = side == 0 ? m : nz - m;
+    /* the 6x6 system over z_u = x_{m-3+u}: rows 0..2 the top relations of
+     * R = m-3, m-2, m-1 (exported as local rows cnt-3, cnt-2, cnt-1 of the
+     * top), rows 3..5 the bottom relations of R = m, m+1, m+2 */
+    Ar A[6][6], bx[6], by[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) A[r][c] = (Ar) 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        /* top: relation of R = m-1-k is the export slot k (local row cnt-1-k) */
+        const int r = 2 - k;                      /* matrix row and unknown index of R */
+        A[r][r] = (Ar) 1;
+        A[r][r + 1] = (Ar) xt[(5 * k + 0) * 64];
+        A[r][r + 2] = (Ar) xt[(5 * k + 1) * 64];
+        A[r][r + 3] = (Ar) xt[(5 * k + 2) * 64];
+        bx[r] = (Ar) xt[(5 * k + 3) * 64];
+        by[r] = (Ar) xt[(5 * k + 4) * 64];
+        /* bottom: relation of R = m+k is the export slot k (local row cnt_b-1-k) */
+        const int rb = 3 + k;
+        A[rb][rb] = (Ar) 1;
+        A[rb][rb - 1] = (Ar) xb[(5 * k + 0) * 64];
+        A[rb][rb - 2] = (Ar) xb[(5 * k + 1) * 64];
+        A[rb][rb - 3] = (Ar) xb[(5 * k + 2) * 64];
+        bx[rb] = (Ar) xb[(5 * k + 3) * 64];
+        by[rb] = (Ar) xb[(5 * k + 4) * 64];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const Ar ip = (Ar) 1 / A[c][c];
+#pragma unroll
+        for (int r = c + 1; r < 6; r++) {
+            const Ar f = A[r][c] * ip;
+#pragma unroll
+            for (int k = c + 1; k < 6; k++) A[r][k] -= f * A[c][k];
+            bx[r] -= f * bx[c];
+            by[r] -= f * by[c];
+        }
+    }
+    Ar zx[6], zy[6];
+#pragma unroll
+    for (int r = 5; r >= 0; r--) {
+        Ar vx = bx[r], vy = by[r];
+#pragma unroll
+        for (int k = r + 1; k < 6; k++) { vx -= A[r][k] * zx[k]; vy -= A[r][k] * zy[k]; }
+        const Ar ip = (Ar) 1 / A[r][r];
+        zx[r] = vx * ip;
+        zy[r] = vy * ip;
+    }
+    const double *Sp = S, *Sq = S + (size_t) nz * M, *Sr = S + (size_t) 2 * nz * M;
+    /* own three unknowns of the six, then back-substitution of the rest of
+     * this side, moving away from the middle */
+    Ar x1x, x2x, x3x, x1y, x2y, x3y;
+    if (side == 0) {
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            C o; o.x = zx[u]; o.y = zy[u];
+            y[(size_t) (m - 3 + u) * M] = o;
+        }
+        x1x = zx[0]; x2x = zx[1]; x3x = zx[2]; x1y = zy[0]; x2y = zy[1]; x3y = zy[2];
+    } else {
+#pragma unroll
+        for (int u = 3; u < 6; u++) {
+            C o; o.x = zx[u]; o.y = zy[u];
+            y[(size_t) (m - 3 + u) * M] = o;
+        }
+        x1x = zx[5]; x2x = zx[4]; x3x = zx[3]; x1y = zy[5]; x2y = zy[4]; x3y = zy[3];
+    }
+    for (int j0 = cnt - 4; j0 >= 0; j0 -= BT) {
+        C sb[BT];
+        double pb[BT], qb[BT], rbv[BT];
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int j = max(j0 - t, 0);
+            const size_t o = (size_t) (side == 0 ? j : nz - 1 - j) * M;
+            sb[t] = y[o]; pb[t] = Sp[o]; qb[t] = Sq[o]; rbv[t] = Sr[o];
+        }
+#pragma unroll
+        for (int t = 0; t < BT; t++) {
+            const int j = j0 - t;
+            if (j >= 0) {
+                const Ar xix = (Ar) sb[t].x - (Ar) pb[t] * x1x - (Ar) qb[t] * x2x - (Ar) rbv[t] * x3x;
+                const Ar xiy = (Ar) sb[t].y - (Ar) pb[t] * x1y - (Ar) qb[t] * x2y - (Ar) rbv[t] * x3y;
+                C o;
+                o.x = xix; o.y = xiy;
+                y[(size_t) (side == 0 ? j : nz - 1 - j) * M] = o;
+                x3x = x2x; x2x = x1x; x1x = xix;
+                x3y = x2y; x2y = x1y; x1y = xiy;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_poisson4t(const RunDev R, const Pois4Tab T_,
+                                                   typename Cplx<T>::type *__restrict__ Y, double *__restrict__ S)
+{
+    typedef typename Cplx<T>::type C;
+    __shared__ double xch[2][15 * 64];
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const size_t M = (size_t) ny * nkx;
+    const int lane = threadIdx.x & 63;
+    const int side = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   /* 0 top, 1 bottom */
+    const size_t q0 = (size_t) blockIdx.x * 64 + lane;
+    const bool act = q0 < M;
+    const size_t qm = act ? q0 : M - 1;           /* idle lanes repeat the last mode, and never write */
+    const int kx = (int) (qm / ny), ky = (int) (qm % ny);
+    const int variant = (kx == 0 && ky == 0) ? 1 : 0;
+    const bool low = kx <= CF_F64_KLOW && (ky <= CF_F64_KLOW || ny - ky <= CF_F64_KLOW);
+    const bool wide = sizeof(T) == sizeof(double) || low;
+    /* lanes past the last mode take part in the barrier only */
+    double *xme = &xch[side][lane];
+    if (act) {
+        if (wide) p4t_sweep<double, C>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant, side, xme);
+        else p4t_sweep<T, C>(T_, Y + qm, S + qm, M, nz, R.lamx[kx], R.lamy[ky], variant, side, xme);
+    }
+    __syncthreads();
+    if (!act) return;
+    const double *xt = &xch[0][lane], *xb = &xch[1][lane];
+    if (wide) p4t_finish<double, C>(Y + qm, S + qm, M, nz, side, xt, xb);
+    else p4t_finish<T, C>(Y + qm, S + qm, M, nz, side, xt, xb);
+}
+
+template __global__ void k_poisson4t<double>(const RunDev, const Pois4Tab, double2 *, double *);
+template __global__ void k_poisson4t<float>(const RunDev, const Pois4Tab, float2 *, double *);
+
 /* ------------------------------------------------------------ projection */
 
 /* project2/4 (pressure.c:120-195) of the mass-flux corrected solve

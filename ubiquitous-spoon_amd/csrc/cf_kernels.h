@@ -431,6 +431,9 @@ template <typename T> __global__ void k_gplane(const RunDev R, const T *q0, cons
 __global__ void k_gmass(const RunDev R, const int s, const double *part, int nb, double *scal, double *logbuf);
 template <typename T>
 __global__ void k_poisson4(const RunDev R, const Pois4Tab T_, typename Cplx<T>::type *Y, double *S);
+/* the twisted form (nz >= 6): grid ceil(M / 64) blocks of 128 */
+template <typename T>
+__global__ void k_poisson4t(const RunDev R, const Pois4Tab T_, typename Cplx<T>::type *Y, double *S);
 
 /* plane statistics (cf_stats.hip) */
 template <typename T, int MODE>

@@ -1305,8 +1305,11 @@ static cf_status enqueue_pressure(cf_run *r, const T *us, const T *vs, const T *
     dim3 gy(fy_blocks(r));
     KLAUNCH("ffty_fwd", Prec<T>::ffty(r), gy, dim3(256), fy_lds(r), R, X, 1);
     if (r->g.order == 4)
-        KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
-                dim3(CF_P4_THREADS), 0, R, r->pt4[tq], X, r->S4);
+        if (r->g.nz >= 6)
+            KLAUNCH("poisson", k_poisson4t<T>, dim3((unsigned) ((r->M + 63) / 64)), dim3(128), 0, R, r->pt4[tq], X, r->S4);
+        else
+            KLAUNCH("poisson", k_poisson4<T>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                    dim3(CF_P4_THREADS), 0, R, r->pt4[tq], X, r->S4);
     else
     {
         const int L = pois2_L(r->g.nz);
@@ -1441,8 +1444,12 @@ static cf_status enqueue_spectral_solve(cf_run *r, int s, int type, double *fld)
     KLAUNCH("hsolve", r->kfxp, dim3(gx), dim3(256), fx_lds(r), R, (const double *) fld, X);
     KLAUNCH("hsolve", r->kfy, gy, dim3(256), fy_lds(r), R, X, 1);
     if (r->g.order == 4) {
-        KLAUNCH("hsolve", k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
-                dim3(CF_P4_THREADS), 0, Rn, r->hs4[s][type], X, r->S4);
+        if (n >= 6)
+            KLAUNCH("hsolve", k_poisson4t<double>, dim3((unsigned) ((r->M + 63) / 64)), dim3(128), 0, Rn, r->hs4[s][type],
+                    X, r->S4);
+        else
+            KLAUNCH("hsolve", k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                    dim3(CF_P4_THREADS), 0, Rn, r->hs4[s][type], X, r->S4);
     } else {
         const int L = pois2_L(n);
         KLAUNCH("hsolve", pick_poisson2<double>(L), dim3(pois2_blocks(L, r->M)), dim3(CF_P2S_THREADS), pois2_lds(L, n),
@@ -2205,11 +2212,14 @@ extern "C" cf_status cf_op_poisson(cf_run *r, double *spec, double dt)
     std::vector<double2> d;
     spec_host_to_dev(r, spec, d);
     HIPCHK(hipMemcpyAsync(r->X, d.data(), sizeof(double2) * d.size(), hipMemcpyHostToDevice, r->stream));
-    if (r->g.order == 4)
-        hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
-                           dim3(CF_P4_THREADS), 0, r->stream,
-                           r->rd, r->pt4[3], r->X, r->S4);
-    else
+    if (r->g.order == 4) {
+        if (r->g.nz >= 6)
+            hipLaunchKernelGGL(k_poisson4t<double>, dim3((unsigned) ((r->M + 63) / 64)), dim3(128), 0, r->stream,
+                               r->rd, r->pt4[3], r->X, r->S4);
+        else
+            hipLaunchKernelGGL(k_poisson4<double>, dim3((unsigned) ((r->M + CF_P4_THREADS - 1) / CF_P4_THREADS)),
+                               dim3(CF_P4_THREADS), 0, r->stream, r->rd, r->pt4[3], r->X, r->S4);
+    } else
         hipLaunchKernelGGL(pick_poisson2<double>(pois2_L(r->g.nz)), dim3(pois2_blocks(pois2_L(r->g.nz), r->M)),
                            dim3(CF_P2S_THREADS), pois2_lds(pois2_L(r->g.nz), r->g.nz), r->stream,
                            r->rd, r->pt[3], r->X, r->G);

@@ -248,14 +248,44 @@ constexpr int ct_rad_at(int n, int idx)
     return 1;
 }
 
-template <int N, int R, int P, int MAXE, typename C>
+/* Twiddles of a compiled plan in stage order: stage s (radix R, span P)
+ * keeps w^(k m N / (P R)), k < P, 1 <= m < R, at OFF_s + (m - 1) P + k, so
+ * the lanes of a pass (consecutive k) read consecutive slots instead of a
+ * stride of m N / (P R) slots, which put up to four lanes of a 16-lane
+ * group on one LDS bank group; the stages fill N - 1 slots.  A run-time
+ * plan (N = 0) keeps the plain table w^j, j < n. */
+template <int N>
+static __device__ __forceinline__ int tw_count(int n) { return N > 0 ? N - 1 : n; }
+template <int N>
+static __device__ __forceinline__ int tw_src(int q)
+{
+    if constexpr (N > 0) {
+        int res = 0, P = 1;
+        bool found = false;
+#pragma unroll
+        for (int s = 0; s < ct_rad_count(N); s++) {
+            const int R = ct_rad_at(N, s), cnt = P * (R - 1);
+            if (!found && q < cnt) {
+                const int m = q / P + 1, k = q - (m - 1) * P;
+                res = k * m * (N / (P * R));
+                found = true;
+            }
+            if (!found) q -= cnt;
+            P *= R;
+        }
+        return res;
+    } else {
+        return q;
+    }
+}
+
+template <int N, int R, int P, int OFF, int MAXE, typename C>
 static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
                                                     const C *__restrict__ tw, double sg)
 {
     typedef typename Real<C>::type F;
     constexpr int NB = (MAXE + R - 1) / R;
     constexpr int T = N / R;
-    constexpr int TS = N / (P * R);
     constexpr bool on = (N & 63) == 0;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nb = nlines * T;
@@ -271,7 +301,7 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
             for (int m = 0; m < R; m++) {
                 C x = src[fsw(i + m * T, on)];
                 if (m > 0 && P > 1) {
-                    C w = tw[k * m * TS];
+                    C w = tw[OFF + (m - 1) * P + k];
                     w.y *= (F) sg;
                     x = cmul(x, w);
                 }
@@ -296,18 +326,19 @@ static __device__ __forceinline__ void fft_stage_ct(C *buf, int ld, int nlines,
     __syncthreads();
 }
 
-template <int N, int MAXE, int ST = 0, int P = 1, typename C>
+template <int N, int MAXE, int ST = 0, int P = 1, int OFF = 0, typename C>
 static __device__ __forceinline__ void lds_fft_ct(C *buf, int ld, int nlines,
                                                   const C *__restrict__ tw, double sg)
 {
     if constexpr (ST < ct_rad_count(N)) {
         constexpr int R = ct_rad_at(N, ST);
-        fft_stage_ct<N, R, P, MAXE>(buf, ld, nlines, tw, sg);
-        lds_fft_ct<N, MAXE, ST + 1, P * R>(buf, ld, nlines, tw, sg);
+        fft_stage_ct<N, R, P, OFF, MAXE>(buf, ld, nlines, tw, sg);
+        lds_fft_ct<N, MAXE, ST + 1, P * R, OFF + P * (R - 1)>(buf, ld, nlines, tw, sg);
     }
 }
 
-/* N > 0: compile-time plan; N == 0: runtime plan (any 2^a 3^b 5^c) */
+/* N > 0: compile-time plan (twiddles in stage order); N == 0: runtime
+ * plan (any 2^a 3^b 5^c, plain twiddle table) */
 template <int N, int MAXE, typename C>
 static __device__ __forceinline__ void fft_lines(C *buf, int n, int ld, int nlines, cp<int> rad,
                                                  int nrad, const C *__restrict__ tw, double sg)
@@ -340,6 +371,18 @@ static __device__ __forceinline__ double wave_sum(double v)
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
+}
+
+/* n / d for 0 <= n < 2^31 and 1 <= d < 2^20, given rd = 1 / d correctly
+ * rounded to double: n rd differs from n / d by at most (n / d) 2^-52 <
+ * 2^-21, less than the distance 1 / d from a non-integer n / d to the
+ * nearest integer, so its truncation is the quotient; when d divides n it
+ * may be one less, which the remainder test corrects.  A few instructions
+ * in place of the integer division sequence when d is a run-time value. */
+static __device__ __forceinline__ int qdiv(int n, int d, double rd)
+{
+    const int q = (int) ((double) n * rd);
+    return n - q * d >= d ? q + 1 : q;
 }
 
 /* Workgroups are dispatched round-robin over the 8 XCDs (each with its own

@@ -207,7 +207,15 @@ static inline int spike4_L(int nz)
 
 /* LDS of the FFT kernels: lines of n elements at stride fft_ld(n), then
  * the n twiddles of the passes, staged once per workgroup */
-static inline __host__ __device__ int fft_ld(int n) { return n + 1; }
+#ifndef CF_FFT_PAD
+#define CF_FFT_PAD 1
+#endif
+static inline __host__ __device__ int fft_ld(int n) { return n + CF_FFT_PAD; }
+/* line pairs per x-FFT workgroup: CF_FX_ELEMS complex elements (at least one pair) */
+#ifndef CF_FX_ELEMS
+#define CF_FX_ELEMS 1024
+#endif
+static constexpr __host__ __device__ int fx_pairs_of(int nx) { return CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1; }
 
 #ifndef CF_P2S_NC
 #define CF_P2S_NC 16

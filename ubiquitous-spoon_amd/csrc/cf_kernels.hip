@@ -855,23 +855,23 @@ template <typename C> static __device__ __forceinline__ const C *twsel(const dou
 template <> __device__ __forceinline__ const double2 *twsel<double2>(const double2 *d, const float2 *) { return d; }
 template <> __device__ __forceinline__ const float2 *twsel<float2>(const double2 *, const float2 *f) { return f; }
 
-/* twiddles of an n-point pass into LDS after the line data (tws): the
- * first one per thread is read into a register with the data gather
- * (tw_first) and stored after it (tw_rest stores it and copies any
- * others), so the passes read twiddles from LDS without an extra round
- * trip to memory */
-template <typename C>
+/* twiddles of an n-point pass into LDS after the line data (tws), in the
+ * order the plan reads them (tw_src): the first one per thread is read
+ * into a register with the data gather (tw_first) and stored after it
+ * (tw_rest stores it and copies any others), so the passes read twiddles
+ * from LDS without an extra round trip to memory */
+template <int N, typename C>
 static __device__ __forceinline__ C tw_first(cp<cv<C>> tw, int n)
 {
-    const int t = (int) threadIdx.x;
-    return cx_of<C>(tw[t < n ? t : n - 1]);
+    const int t = (int) threadIdx.x, cnt = tw_count<N>(n);
+    return cx_of<C>(tw[tw_src<N>(t < cnt ? t : cnt - 1)]);
 }
-template <typename C>
+template <int N, typename C>
 static __device__ __forceinline__ void tw_rest(C *tws, C first, cp<cv<C>> tw, int n)
 {
-    const int t = (int) threadIdx.x;
-    if (t < n) tws[t] = first;
-    for (int q = t + (int) blockDim.x; q < n; q += (int) blockDim.x) tws[q] = cx_of<C>(tw[q]);
+    const int t = (int) threadIdx.x, cnt = tw_count<N>(n);
+    if (t < cnt) tws[t] = first;
+    for (int q = t + (int) blockDim.x; q < cnt; q += (int) blockDim.x) tws[q] = cx_of<C>(tw[tw_src<N>(q)]);
 }
 
 #define CF_FX_UNR 3            /* elements per thread gathered together by the divergence + x-FFT */
@@ -890,6 +890,7 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
     const F sc = (F) scd;
     const bool on = fsw_on(nx);
     const int nj = 2 * nl;
+    const double rny = 1.0 / (double) ny;
     for (int e = threadIdx.x; e < nj * nkx; e += blockDim.x) {
         const int kx = e / nj, jj = e - kx * nj;
         const int l = jj >> 1;
@@ -897,7 +898,7 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
         const long line = 2 * (pair0 + l) + (second ? 1 : 0);
         if (line >= NL) continue;
         const C Z = sb[l * ld + fsw(kx, on)], Wc = sb[l * ld + fsw(kx == 0 ? 0 : nx - kx, on)];
-        const long k = line / ny, j = line - k * ny;
+        const int k = qdiv((int) line, ny, rny), j = (int) line - k * ny;
         const C val = second ? mkc<C>((Z.y + Wc.y) * sc, (Wc.x - Z.x) * sc) : mkc<C>((Z.x + Wc.x) * sc, (Z.y - Wc.y) * sc);
         X[((size_t) k * nkx + kx) * ny + j] = cv_of(val);
     }
@@ -921,9 +922,10 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = fft_ld(nx);
-    const int nl = R.fx_pairs;
+    const int nl = NX > 0 ? fx_pairs_of(NX) : R.fx_pairs;
     const int NL = ny * nz;
     const size_t P = (size_t) nx * ny;
+    const double rny = 1.0 / (double) ny;
     const cp<double> rdz = (cp<double>) R.rdz + CF_NGT;
     const T m = mptr ? (T) *mptr : (T) 0;
     const T rdx = (T) R.rdx, rdy = (T) R.rdy;
@@ -939,7 +941,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     auto divg = [&](int line, int i) -> T {
         const bool ok = line < NL;
         const int lc = ok ? line : NL - 1;
-        const int k = lc / ny, j = lc - k * ny;
+        const int k = qdiv(lc, ny, rny), j = lc - k * ny;
         const int im = i > 0 ? i - 1 : nx - 1, jm = j > 0 ? j - 1 : ny - 1;
         const size_t q = (size_t) k * P + (size_t) j * nx;
         const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
@@ -973,7 +975,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
                       : (ORD == 2 ? (sizeof(T) == 4 ? CF_FX_UNR : 2) : (sizeof(T) == 4 ? 2 : 1));   /* fit 128 VGPRs */
     const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
-    const C tw0 = tw_first<C>(twg, nx);
+    const C tw0 = tw_first<NX, C>(twg, nx);
     const int tot = nl * nx;
     for (int e0 = 0; e0 < tot; e0 += UNR * 256) {
         C res[UNR];
@@ -990,7 +992,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
                 const int la = 2 * pr;
                 const bool ok = la < NL;
                 const int lc = ok ? la : NL - 2;
-                const int k = lc / ny, j = lc - k * ny;
+                const int k = qdiv(lc, ny, rny), j = lc - k * ny;
                 const int jm = j > 0 ? j - 1 : ny - 1;
                 const size_t qa = (size_t) k * P + (size_t) j * nx, qb = qa + nx, qm = (size_t) k * P + (size_t) jm * nx;
                 const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
@@ -1023,7 +1025,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
             }
         }
     }
-    tw_rest<C>(tws, tw0, twg, nx);
+    tw_rest<NX, C>(tws, tw0, twg, nx);
     __syncthreads();
     fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, (const C *) tws, 1.0);
     store_half_spectra<C>(sb, ld, nl, nx, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) nx, X);
@@ -1050,6 +1052,9 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
     const int nl = R.fx_pairs;
     const long NL = (long) ny * nz;
     const long pair0 = (long) blockIdx.x * nl;
+    const cp<cv<double2>> twg = (cp<cv<double2>>) R.twx;
+    double2 *tws = sb + nl * ld;
+    const double2 tw0 = tw_first<NX, double2>(twg, nx);
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
         int l = e / nx, i = e - l * nx;
         long la = 2 * (pair0 + l), lb = la + 1;
@@ -1057,8 +1062,9 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
         double vb = lb < NL ? a[(size_t) lb * nx + i] : 0.0;
         sb[l * ld + fsw(i, fsw_on(nx))] = make_double2(va, vb);
     }
+    tw_rest<NX, double2>(tws, tw0, twg, nx);
     __syncthreads();
-    fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, R.twx, 1.0);
+    fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, (const double2 *) tws, 1.0);
     store_half_spectra<double2>(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, (gp<cv<double2>>) X);
 }
 
@@ -1081,7 +1087,7 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
      * (TK ny <= 8 x 256, blockDim.x = 256) */
     const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twy, R.twy32);
     C *tws = sb + TK * ld;
-    const C tw0 = tw_first<C>(twg, ny);
+    const C tw0 = tw_first<NY, C>(twg, ny);
     const int tot = TK * ny, totv = nl * ny;
     for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
         C v[8];
@@ -1100,7 +1106,7 @@ static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<
             }
         }
     }
-    tw_rest<C>(tws, tw0, twg, ny);
+    tw_rest<NY, C>(tws, tw0, twg, ny);
     __syncthreads();
     fft_lines<NY, 8>(sb, ny, ld, TK, (cp<int>) R.rady, R.nrady, (const C *) tws, fwd ? 1.0 : -1.0);
     const T sc = fwd ? (T) (1.0 / (double) ny) : (T) 1;
@@ -1132,15 +1138,16 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = fft_ld(nx);
-    const int nl = R.fx_pairs;
+    const int nl = NX > 0 ? fx_pairs_of(NX) : R.fx_pairs;
     const long NL = (long) ny * nz;
     const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
     const int half = nx / 2;
+    const double rny = 1.0 / (double) ny;
     /* groups of CF_FXI_UNR elements per thread, reads at clamped indices,
      * all in flight before the first LDS write (blockDim.x = 256) */
     const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
     C *tws = sb + nl * ld;
-    const C tw0 = tw_first<C>(twg, nx);
+    const C tw0 = tw_first<NX, C>(twg, nx);
     const int tot = nl * nkx, NLi = (int) NL;
     for (int e0 = 0; e0 < tot; e0 += CF_FXI_UNR * 256) {
         C A[CF_FXI_UNR], B[CF_FXI_UNR];
@@ -1150,8 +1157,12 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
             const int ec = e < tot ? e : tot - 1;
             const int kx = ec / nl, l = ec - kx * nl;
             const int la = 2 * ((int) pair0 + l), lb = la + 1;
-            const int lac = la < NLi ? la : NLi - 1, lbc = lb < NLi ? lb : NLi - 1;
-            const int ka = lac / ny, ja = lac - ka * ny, kb = lbc / ny, jb = lbc - kb * ny;
+            const int lac = la < NLi ? la : NLi - 1;
+            const int ka = qdiv(lac, ny, rny), ja = lac - ka * ny;
+            /* line lb = la + 1 follows la (the clamped la when lb is past
+             * the end: an in-bounds read whose value is dropped) */
+            const bool bnext = lb < NLi, wrap = ja + 1 == ny;
+            const int kb = bnext ? (wrap ? ka + 1 : ka) : ka, jb = bnext ? (wrap ? 0 : ja + 1) : ja;
             const C a = cx_of<C>(X[((size_t) ka * nkx + kx) * ny + ja]);
             const C bq = cx_of<C>(X[((size_t) kb * nkx + kx) * ny + jb]);
             A[u] = la < NLi ? a : mkc<C>((T) 0, (T) 0);
@@ -1169,7 +1180,7 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
             }
         }
     }
-    tw_rest<C>(tws, tw0, twg, nx);
+    tw_rest<NX, C>(tws, tw0, twg, nx);
     __syncthreads();
     fft_lines<NX, 8>(sb, nx, ld, nl, (cp<int>) R.radx, R.nradx, (const C *) tws, -1.0);
     for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {

@@ -60,9 +60,6 @@ static const int MAXE_LINE = 2048;      /* elements per FFT block (8 per thread 
 #ifndef CF_P4_THREADS
 #define CF_P4_THREADS 64                /* order-4 Poisson: modes (threads) per workgroup of a single run */
 #endif
-#ifndef CF_FX_ELEMS
-#define CF_FX_ELEMS 1024                /* elements per x-FFT workgroup: fx_pairs = CF_FX_ELEMS / nx */
-#endif
 
 /* ------------------------------------------------------------ run */
 
@@ -1192,7 +1189,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.cmode[0] = cfg->uvolavgmode; R.ctarget[0] = cfg->want_uvolavg - cfg->uframe; R.cfixed[0] = -cfg->dpdx;
     R.cmode[1] = cfg->cvolavgmode; R.ctarget[1] = cfg->want_cvolavg; R.cfixed[1] = 0.0;
     R.logcap = LOGCAP;
-    R.fx_pairs = CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1;
+    R.fx_pairs = fx_pairs_of(nx);
     /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
     R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
     apply_tables(r, TP);

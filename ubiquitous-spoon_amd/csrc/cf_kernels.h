@@ -211,6 +211,16 @@ static inline int spike4_L(int nz)
 #define CF_FFT_PAD 1
 #endif
 static inline __host__ __device__ int fft_ld(int n) { return n + CF_FFT_PAD; }
+/* y-FFT lengths done in two register passes, n = P1 P2 (P1 = fy2_p1(n),
+ * 0 for the Stockham passes); their lines sit in LDS as P1 rows of P2 + 1 */
+#ifndef CF_FY2
+#define CF_FY2 1
+#endif
+static constexpr __host__ __device__ int fy2_p1(int n)
+{
+    return CF_FY2 ? (n == 256 || n == 128 ? 16 : (n == 64 ? 8 : 0)) : 0;
+}
+static inline __host__ __device__ int fy_ld(int n) { return fy2_p1(n) ? n + fy2_p1(n) : fft_ld(n); }
 /* line pairs per x-FFT workgroup: CF_FX_ELEMS complex elements (at least one pair) */
 #ifndef CF_FX_ELEMS
 #define CF_FX_ELEMS 1024

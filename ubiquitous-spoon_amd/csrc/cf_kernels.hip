@@ -1068,11 +1068,137 @@ __global__ __launch_bounds__(256) void k_fftx_fwd_plain(const RunDev R, const do
     store_half_spectra<double2>(sb, ld, nl, nx, nkx, ny, NL, pair0, 0.5 / (double) nx, (gp<cv<double2>>) X);
 }
 
+/* DFT of N = 8 or 16 points in registers, natural order in and out:
+ * four-step over N = 4 (N / 4), n = (N / 4) n1 + n2, k = k1 + 4 k2 --
+ * DFT-4 over n1 for each n2, twiddle w_N^(n2 k1), DFT-(N / 4) over n2 for
+ * each k1.  w_N^j = cos(2 pi j / N) - i sg sin(2 pi j / N), the constants
+ * below being cos and sin of 2 pi j / 16. */
+template <int N, typename C>
+static __device__ __forceinline__ void dft_regs(C *a, double sgd)
+{
+    typedef typename Real<C>::type F;
+    if constexpr (N <= 5) {
+        dft_small<N>(a, sgd);
+    } else {
+        static_assert(N == 8 || N == 16, "dft_regs: 8 or 16 points");
+        constexpr double c16[16] = { 1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                     0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                     -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                                     0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613 };
+        constexpr double s16[16] = { 0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                                     1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                     0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                     -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173 };
+        constexpr int Q = N / 4;
+        const F sg = (F) sgd;
+        C y[N];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; n2++) {
+            C t[4];
+#pragma unroll
+            for (int n1 = 0; n1 < 4; n1++) t[n1] = a[Q * n1 + n2];
+            dft_small<4>(t, sgd);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) {
+                const int j = (n2 * k1) * (16 / N);       /* w_N^(n2 k1) = w_16^j */
+                C v = t[k1];
+                if (j != 0) {
+                    const F c = (F) c16[j], sn = sg * (F) s16[j];
+                    v = mkc<C>(v.x * c + v.y * sn, v.y * c - v.x * sn);
+                }
+                y[k1 * Q + n2] = v;
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            C u[Q];
+#pragma unroll
+            for (int n2 = 0; n2 < Q; n2++) u[n2] = y[k1 * Q + n2];
+            if constexpr (Q == 2) dft_small<2>(u, sgd);
+            else dft_small<4>(u, sgd);
+#pragma unroll
+            for (int k2 = 0; k2 < Q; k2++) a[k1 + 4 * k2] = u[k2];
+        }
+    }
+}
+
+/* y FFT of length NY = P1 P2 (P1, P2 <= 16) in two register passes, n =
+ * P2 n1 + n2, k = k1 + P1 k2: pass 1 reads x[P2 n1 + n2] of one line
+ * straight from memory (lanes along n2), takes the DFT-P1 over n1,
+ * multiplies by w^(n2 k1) and writes row k1 of the line's LDS image
+ * (rows of P2 + 1, so the column reads of pass 2 fall on distinct bank
+ * groups); pass 2 reads column k1, takes the DFT-P2 over n2 and writes
+ * X[k1 + P1 k2] straight to memory (lanes along k1).  One LDS round trip
+ * per element instead of one per radix-4 pass plus the staging copies. */
+template <int NY, typename T>
+static __device__ __forceinline__ void ffty2_body(const RunDev &R, typename Cplx<T>::type *__restrict__ X_, int fwd, const Blk b)
+{
+    typedef typename Cplx<T>::type C;
+    constexpr int P1 = fy2_p1(NY), P2 = NY / P1, RL = P2 + 1, LL = P1 * RL;
+    const gp<cv<C>> X = (gp<cv<C>>) X_;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
+    const long nlines = (long) R.g.nz * R.g.nkx;
+    const int TK = R.fy_cols;
+    const long L0 = (long) b.x * TK;
+    const int nl = (int) (nlines - L0 < TK ? nlines - L0 : TK);
+    const gp<cv<C>> base = X + (size_t) L0 * NY;
+    const double sgd = fwd ? 1.0 : -1.0;
+    /* twiddles w^(n2 k1) as a P1 x P2 table after the lines */
+    C *tw = sb + TK * LL;
+    const cp<cv<C>> twg = (cp<cv<C>>) twsel<C>(R.twy, R.twy32);
+    typedef typename Real<C>::type F;
+    const int t1 = (int) threadIdx.x / P2, n2 = (int) threadIdx.x - t1 * P2;
+    const bool act1 = t1 < nl;
+    C a[P1];
+    if (t1 < TK) {
+        const gp<cv<C>> src = base + (size_t) (act1 ? t1 : 0) * NY + n2;
+#pragma unroll
+        for (int n1 = 0; n1 < P1; n1++) a[n1] = cx_of<C>(src[P2 * n1]);
+    }
+    for (int q = (int) threadIdx.x; q < NY; q += (int) blockDim.x) {
+        const int k1 = q / P2, m = q - k1 * P2;
+        tw[q] = cx_of<C>(twg[k1 * m]);
+    }
+    __syncthreads();
+    if (t1 < TK) {
+        dft_regs<P1>(a, sgd);
+        C *row = sb + t1 * LL + n2;
+#pragma unroll
+        for (int k1 = 0; k1 < P1; k1++) {
+            C v = act1 ? a[k1] : mkc<C>((F) 0, (F) 0);
+            if (k1 > 0) {
+                C w = tw[k1 * P2 + n2];
+                w.y *= (F) sgd;
+                v = cmul(v, w);
+            }
+            row[k1 * RL] = v;
+        }
+    }
+    __syncthreads();
+    const int t2 = (int) threadIdx.x / P1, k1 = (int) threadIdx.x - t2 * P1;
+    if (t2 < nl) {
+        C u[P2];
+        const C *col = sb + t2 * LL + k1 * RL;
+#pragma unroll
+        for (int m = 0; m < P2; m++) u[m] = col[m];
+        dft_regs<P2>(u, sgd);
+        const F sc = fwd ? (F) (1.0 / (double) NY) : (F) 1;
+        const gp<cv<C>> dst = base + (size_t) t2 * NY + k1;
+#pragma unroll
+        for (int k2 = 0; k2 < P2; k2++) dst[P1 * k2] = cv_of(fwd ? cscale(u[k2], sc) : u[k2]);
+    }
+}
+
 /* complex y FFT in place, forward scaled 1/ny (fft.c:186-202) or unscaled
  * inverse (fft.c:205-208); a block transforms TK contiguous lines */
 template <int NY, typename T>
 static __device__ __forceinline__ void ffty_body(const RunDev &R, typename Cplx<T>::type *__restrict__ X_, int fwd, const Blk b)
 {
+    if constexpr (NY > 0 && fy2_p1(NY) > 0) {
+        ffty2_body<NY, T>(R, X_, fwd, b);
+        return;
+    }
     typedef typename Cplx<T>::type C;
     const gp<cv<C>> X = (gp<cv<C>>) X_;
     extern __shared__ __align__(16) unsigned char smem_[];

@@ -1192,6 +1192,13 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.fx_pairs = fx_pairs_of(nx);
     /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
     R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
+    /* the two-pass y FFT gives each of a workgroup's 256 threads at most
+     * one (line, column) task per pass */
+    if (fy2_p1(ny) && R.fy_cols * std::max(fy2_p1(ny), ny / fy2_p1(ny)) > 256) {
+        free_run(r);
+        cf_set_error("ny = %d: y-FFT workgroup shape", ny);
+        return CF_ERR_UNSUPPORTED;
+    }
     apply_tables(r, TP);
     r->use_spike = TP.spike_ok;
     r->sp_L = TP.spike_ok ? spike_L(nz) : 0;
@@ -1255,7 +1262,7 @@ static size_t fx_lds(const cf_run *r)
 }
 static size_t fy_lds(const cf_run *r)
 {
-    return 2 * r->esz * ((size_t) r->rd.fy_cols * fft_ld(r->g.ny) + (size_t) r->g.ny);
+    return 2 * r->esz * ((size_t) r->rd.fy_cols * fy_ld(r->g.ny) + (size_t) r->g.ny);
 }
 
 /* the kernel sets and typed views of the run's buffers for precision T */
@@ -1999,7 +2006,7 @@ static cf_status stat_fft(cf_run *r, const double *src, double2 *dst)
     const long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     const unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
     const size_t lx = 2 * sizeof(double) * ((size_t) R.fx_pairs * fft_ld(r->g.nx) + (size_t) r->g.nx);
-    const size_t ly = 2 * sizeof(double) * ((size_t) R.fy_cols * fft_ld(r->g.ny) + (size_t) r->g.ny);
+    const size_t ly = 2 * sizeof(double) * ((size_t) R.fy_cols * fy_ld(r->g.ny) + (size_t) r->g.ny);
     hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), lx, r->stream, R, src, dst);
     hipLaunchKernelGGL(r->kfy, dim3(fy_blocks(r)), dim3(256), ly, r->stream, R, dst, 1);
     LAUNCHCHK();

@@ -211,27 +211,53 @@ static inline int spike4_L(int nz)
 #define CF_FFT_PAD 1
 #endif
 static inline __host__ __device__ int fft_ld(int n) { return n + CF_FFT_PAD; }
-/* y-FFT lengths done in two register passes, n = P1 P2 (P1 = fy2_p1(n),
- * 0 for the Stockham passes); their lines sit in LDS as P1 rows of P2 + 1 */
+/* FFT lengths done in two register passes, n = P1 P2 (P1 = fft2_p1(n),
+ * 0 for the Stockham passes): the y FFT (CF_FY2), the inverse x FFT and
+ * the order-2 divergence + x FFT (CF_FX2) where they measured faster than
+ * the Stockham passes -- the inverse x FFT in FP32 and for lengths up to
+ * 128 in FP64, the divergence in FP32 at 256 (esz: bytes per stored real) */
 #ifndef CF_FY2
 #define CF_FY2 1
 #endif
-static constexpr __host__ __device__ int fy2_p1(int n)
-{
-    return CF_FY2 ? (n == 256 || n == 128 ? 16 : (n == 64 ? 8 : 0)) : 0;
-}
+#ifndef CF_FX2
+#define CF_FX2 1
+#endif
+static constexpr __host__ __device__ int fft2_p1(int n) { return n == 256 ? 16 : (n == 128 || n == 64 ? 8 : 0); }
+static constexpr __host__ __device__ int fy2_p1(int n) { return CF_FY2 ? fft2_p1(n) : 0; }
+static constexpr __host__ __device__ int fx2_p1(int n) { return CF_FX2 ? fft2_p1(n) : 0; }
+static constexpr __host__ __device__ bool fxi2_on(int n, int esz) { return fx2_p1(n) > 0 && (esz == 4 || n <= 128); }
+static constexpr __host__ __device__ bool divx2_on(int n, int esz) { return fx2_p1(n) > 0 && esz == 4 && n == 256; }
+/* y lines sit in LDS as P1 rows of P2 + 1 */
 static inline __host__ __device__ int fy_ld(int n) { return fy2_p1(n) ? n + fy2_p1(n) : fft_ld(n); }
-/* line pairs per x-FFT workgroup: CF_FX_ELEMS complex elements (at least one pair) */
+/* x line pairs: P1 rows of P2 + 1 and 2 more, so the unpacking writes of
+ * neighbouring pairs (lanes along the pair) fall on distinct bank groups */
+static constexpr __host__ __device__ int fx_ld(int n, int esz)
+{
+    return fxi2_on(n, esz) ? n + fx2_p1(n) + 2 : n + CF_FFT_PAD;
+}
+/* line pairs per x-FFT workgroup: CF_FX_ELEMS complex elements (at least
+ * one pair); 2048 where the two-pass forms run, so that a pass has 128 or
+ * 256 (pair, column) tasks.  The Stockham kernels take the pair count
+ * from the run (RunDev::fx_pairs); the two-pass ones compile it in, and
+ * for their lengths it does not depend on the element size, except at 256,
+ * whose two-pass forms are FP32 only (FP32 runs launch FP64 x FFTs only
+ * through the Stockham kernels there). */
 #ifndef CF_FX_ELEMS
 #define CF_FX_ELEMS 1024
 #endif
-static constexpr __host__ __device__ int fx_pairs_of(int nx) { return CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1; }
+static constexpr __host__ __device__ int fx_pairs_of(int nx, int esz)
+{
+    return fxi2_on(nx, esz) ? 2048 / nx : (CF_FX_ELEMS / nx > 0 ? CF_FX_ELEMS / nx : 1);
+}
 
 #ifndef CF_P2S_NC
 #define CF_P2S_NC 16
 #endif
 #ifndef CF_P2S_MW
 #define CF_P2S_MW 8
+#endif
+#ifndef CF_P2S_XCD
+#define CF_P2S_XCD 1
 #endif
 /* order-2 Poisson: the partitioned solve (at most 16 z-chunks of L rows
  * per mode, 8 modes per 256-thread workgroup), L a compiled chunk length

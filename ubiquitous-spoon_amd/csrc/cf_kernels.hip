@@ -884,11 +884,12 @@ static __device__ __forceinline__ void tw_rest(C *tws, C first, cp<cv<C>> tw, in
  * go out as one contiguous run of 16 (or 8) bytes per lane */
 template <typename C>
 static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, int nl, int nx, int nkx,
-                                                          int ny, long NL, long pair0, double scd, gp<cv<C>> X)
+                                                          int ny, long NL, long pair0, double scd, gp<cv<C>> X,
+                                                          bool swz = true)
 {
     typedef typename Real<C>::type F;
     const F sc = (F) scd;
-    const bool on = fsw_on(nx);
+    const bool on = swz && fsw_on(nx);
     const int nj = 2 * nl;
     const double rny = 1.0 / (double) ny;
     for (int e = threadIdx.x; e < nj * nkx; e += blockDim.x) {
@@ -904,6 +905,122 @@ static __device__ __forceinline__ void store_half_spectra(const C *sb, int ld, i
     }
 }
 
+template <int N, typename C>
+static __device__ __forceinline__ void dft_regs(C *a, double sgd);
+
+/* twiddles w^(m k1) of a two-pass length n = P1 P2 as a P1 x P2 table (k1
+ * rows) after a workgroup's lines */
+template <int N, typename C>
+static __device__ __forceinline__ void tw2_stage(C *tw, cp<cv<C>> twg)
+{
+    constexpr int P2 = N / fft2_p1(N);
+    for (int q = (int) threadIdx.x; q < N; q += (int) blockDim.x) {
+        const int k1 = q / P2, m = q - k1 * P2;
+        tw[q] = cx_of<C>(twg[k1 * m]);
+    }
+}
+
+/* pass 1 of a two-pass transform on registers a[0..P1) (column n2 of a
+ * line, natural order in): DFT-P1, twiddle w^(n2 k1) (conjugated for sg
+ * = -1), row k1 of the line's LDS image at row[k1 (P2 + 1)] */
+template <int P1, int P2, typename C>
+static __device__ __forceinline__ void pass1_rows(C *a, C *row, const C *tw, int n2, double sgd)
+{
+    typedef typename Real<C>::type F;
+    dft_regs<P1>(a, sgd);
+#pragma unroll
+    for (int k1 = 0; k1 < P1; k1++) {
+        C v = a[k1];
+        if (k1 > 0) {
+            C w = tw[k1 * P2 + n2];
+            w.y *= (F) sgd;
+            v = cmul(v, w);
+        }
+        row[k1 * (P2 + 1)] = v;
+    }
+}
+
+/* the order-2 divergence + forward x FFT of lengths NX = P1 P2 in two
+ * register passes (ny even: a workgroup's line pairs are rows j, j + 1 of
+ * one level).  Pass 1: the lane of (pair, column n2) forms the divergence
+ * of both rows at x = P2 n1 + n2, n1 < P1, straight from memory (the x - 1
+ * neighbours are reloads that hit the cache), takes the DFT-P1 and writes
+ * the pair's LDS rows; pass 2 (pair, k1) takes the DFT-P2 of column k1 and
+ * writes the transform in natural order over the rows, which
+ * store_half_spectra unpacks.  Same sums as divx_body's paired path. */
+template <int NX, typename T>
+static __device__ __forceinline__ void divx2_body(const RunDev &R, const T *__restrict__ us_,
+    const T *__restrict__ vs_, const T *__restrict__ ws_, const double *__restrict__ h1_,
+    const double *__restrict__ mptr_, typename Cplx<T>::type *__restrict__ X_, const Blk b)
+{
+    typedef typename Cplx<T>::type C;
+    typedef typename std::conditional<sizeof(T) == 4, double, T>::type D;
+    constexpr int P1 = fx2_p1(NX), P2 = NX / P1, LS = fx_ld(NX, sizeof(T)), nl = fx_pairs_of(NX, sizeof(T));
+    const gp<const T> us = (gp<const T>) us_, vs = (gp<const T>) vs_, ws = (gp<const T>) ws_;
+    const cp<double> h1 = (cp<double>) h1_, mptr = (cp<double>) mptr_;
+    const gp<cv<C>> X = (gp<cv<C>>) X_;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
+    C *tw = sb + nl * LS;
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const int NL = ny * nz;
+    const size_t P = (size_t) NX * ny;
+    const double rny = 1.0 / (double) ny;
+    const cp<double> rdz = (cp<double>) R.rdz + CF_NGT;
+    const T m = mptr ? (T) *mptr : (T) 0;
+    const D rdx = (D) (T) R.rdx, rdy = (D) (T) R.rdy;
+    const int pair0 = (int) xcd_block(b.x, b.nx) * nl;
+    const int t1 = (int) threadIdx.x / P2, n2 = (int) threadIdx.x - t1 * P2;
+    C a[P1];
+    if (t1 < nl) {
+        const int la = 2 * (pair0 + t1);
+        const bool ok = la < NL;
+        const int lc = ok ? la : NL - 2;
+        const int k = qdiv(lc, ny, rny), j = lc - k * ny;
+        const int jm = j > 0 ? j - 1 : ny - 1;
+        const size_t qa = (size_t) k * P + (size_t) j * NX, qb = qa + NX, qm = (size_t) k * P + (size_t) jm * NX;
+        const size_t qam = k > 0 ? qa - P : qa, qbm = k > 0 ? qb - P : qb;
+        const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
+        const D rz = (D) ((T) rdz[k]);
+        const bool top = k < nz - 1, bot = k > 0;
+#pragma unroll
+        for (int n1 = 0; n1 < P1; n1++) {
+            const int x = P2 * n1 + n2, xm = x > 0 ? x - 1 : NX - 1;
+            const D ua = (D) us[qa + x] + hm, ub = (D) us[qb + x] + hm;
+            const D uam = (D) us[qa + xm] + hm, ubm = (D) us[qb + xm] + hm;
+            const D va = vs[qa + x], vb = vs[qb + x], vam = vs[qm + x];
+            const D war = ws[qa + x], wbr = ws[qb + x], wamr = ws[qam + x], wbmr = ws[qbm + x];
+            const D wa = top ? war : (D) 0, wb = top ? wbr : (D) 0;
+            const D wam = bot ? wamr : (D) 0, wbm = bot ? wbmr : (D) 0;
+            const D da = (ua - uam) * rdx + (va - vam) * rdy + (wa - wam) * rz;
+            const D db = (ub - ubm) * rdx + (vb - va) * rdy + (wb - wbm) * rz;
+            a[n1] = mkc<C>(ok ? (T) da : (T) 0, ok ? (T) db : (T) 0);
+            /* at most four columns' loads in flight: a compiler memory
+             * barrier keeps the next group's loads below this one */
+            if ((n1 & 3) == 3) __asm__ volatile("" ::: "memory");
+        }
+    }
+    tw2_stage<NX, C>(tw, (cp<cv<C>>) twsel<C>(R.twx, R.twx32));
+    __syncthreads();
+    if (t1 < nl) pass1_rows<P1, P2>(a, sb + t1 * LS + n2, tw, n2, 1.0);
+    __syncthreads();
+    const int t2 = (int) threadIdx.x / P1, k1 = (int) threadIdx.x - t2 * P1;
+    C u[P2];
+    if (t2 < nl) {
+        const C *col = sb + t2 * LS + k1 * (P2 + 1);
+#pragma unroll
+        for (int q = 0; q < P2; q++) u[q] = col[q];
+        dft_regs<P2>(u, 1.0);
+    }
+    __syncthreads();
+    if (t2 < nl) {
+#pragma unroll
+        for (int k2 = 0; k2 < P2; k2++) sb[t2 * LS + k1 + P1 * k2] = u[k2];
+    }
+    __syncthreads();
+    store_half_spectra<C>(sb, LS, nl, NX, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) NX, X, false);
+}
+
 /* divergence2 / divergence4 (pressure.c:40-118) of the mass-flux-corrected
  * velocity (u + h1 mdpdx, chanfast.c:519-522) fused with the forward r2c x
  * FFT scaled 1/nx (fft.c:161-177).  Order 4 sees the w ghost faces
@@ -913,6 +1030,12 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     const T *__restrict__ vs_, const T *__restrict__ ws_, const double *__restrict__ h1_,
     const double *__restrict__ mptr_, typename Cplx<T>::type *__restrict__ X_, const Blk b)
 {
+    if constexpr (NX > 0 && ORD == 2 && divx2_on(NX, sizeof(T))) {
+        if ((R.g.ny & 1) == 0) {                  /* workgroup-uniform */
+            divx2_body<NX, T>(R, us_, vs_, ws_, h1_, mptr_, X_, b);
+            return;
+        }
+    }
     typedef typename Cplx<T>::type C;
     const gp<const T> us = (gp<const T>) us_, vs = (gp<const T>) vs_, ws = (gp<const T>) ws_;
     const cp<double> h1 = (cp<double>) h1_, mptr = (cp<double>) mptr_;
@@ -922,7 +1045,9 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = fft_ld(nx);
-    const int nl = NX > 0 ? fx_pairs_of(NX) : R.fx_pairs;
+    /* compiled lengths run only on runs of element size sizeof(T), whose
+     * pair count is fx_pairs_of(NX, sizeof(T)) */
+    const int nl = NX > 0 ? fx_pairs_of(NX, sizeof(T)) : R.fx_pairs;
     const int NL = ny * nz;
     const size_t P = (size_t) nx * ny;
     const double rny = 1.0 / (double) ny;
@@ -1032,7 +1157,7 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
 }
 
 template <int NX, int ORD, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_div_fftx(const RunDev R, const T *__restrict__ us,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((ORD == 2 && divx2_on(NX, sizeof(T))) ? 2 : 4, 8))) void k_div_fftx(const RunDev R, const T *__restrict__ us,
     const T *__restrict__ vs, const T *__restrict__ ws, const double *__restrict__ h1,
     const double *__restrict__ mptr, typename Cplx<T>::type *__restrict__ X)
 {
@@ -1249,6 +1374,85 @@ __global__ __launch_bounds__(256) void k_ffty(const RunDev R, typename Cplx<T>::
     ffty_body<NY, T>(R, X, fwd, blk_native());
 }
 
+/* the inverse x FFT of lengths NX = P1 P2 in two register passes: the
+ * pairs' packed spectra are unpacked into LDS in natural order as in
+ * fftxi_body; pass 1 (pair, n2) reads column n2, takes the inverse DFT-P1
+ * and writes the pair's rows; pass 2 (pair, k1) takes the inverse DFT-P2
+ * of column k1 and writes x = k1 + P1 k2 of both real lines straight to p
+ * (lanes along k1). */
+template <int NX, typename T>
+static __device__ __forceinline__ void fftxi2_body(const RunDev &R, const typename Cplx<T>::type *__restrict__ X_,
+                                                   T *__restrict__ p_, const Blk b)
+{
+    typedef typename Cplx<T>::type C;
+    constexpr int P1 = fx2_p1(NX), P2 = NX / P1, LS = fx_ld(NX, sizeof(T)), nl = fx_pairs_of(NX, sizeof(T)), half = NX / 2;
+    const gp<const cv<C>> X = (gp<const cv<C>>) X_;
+    const gp<T> p = (gp<T>) p_;
+    extern __shared__ __align__(16) unsigned char smem_[];
+    C *sb = (C *) smem_;
+    C *tw = sb + nl * LS;
+    const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
+    const int NLi = ny * nz;
+    const int pair0 = (int) xcd_block(b.x, b.nx) * nl;
+    const double rny = 1.0 / (double) ny;
+    const int tot = nl * nkx;
+    for (int e0 = 0; e0 < tot; e0 += CF_FXI_UNR * 256) {
+        C A[CF_FXI_UNR], B[CF_FXI_UNR];
+#pragma unroll
+        for (int u = 0; u < CF_FXI_UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            const int ec = e < tot ? e : tot - 1;
+            const int kx = ec / nl, l = ec - kx * nl;
+            const int la = 2 * (pair0 + l), lb = la + 1;
+            const int lac = la < NLi ? la : NLi - 1;
+            const int ka = qdiv(lac, ny, rny), ja = lac - ka * ny;
+            const bool bnext = lb < NLi, wrap = ja + 1 == ny;
+            const int kb = bnext ? (wrap ? ka + 1 : ka) : ka, jb = bnext ? (wrap ? 0 : ja + 1) : ja;
+            const C av = cx_of<C>(X[((size_t) ka * nkx + kx) * ny + ja]);
+            const C bv = cx_of<C>(X[((size_t) kb * nkx + kx) * ny + jb]);
+            A[u] = la < NLi ? av : mkc<C>((T) 0, (T) 0);
+            B[u] = lb < NLi ? bv : mkc<C>((T) 0, (T) 0);
+        }
+#pragma unroll
+        for (int u = 0; u < CF_FXI_UNR; u++) {
+            const int e = e0 + (int) threadIdx.x + u * 256;
+            if (e < tot) {
+                const int kx = e / nl, l = e - kx * nl;
+                C a = A[u], bq = B[u];
+                if (kx == 0 || kx == half) { a.y = (T) 0; bq.y = (T) 0; }
+                sb[l * LS + kx] = mkc<C>(a.x - bq.y, a.y + bq.x);
+                if (kx > 0 && kx < NX - kx) sb[l * LS + NX - kx] = mkc<C>(a.x + bq.y, bq.x - a.y);
+            }
+        }
+    }
+    tw2_stage<NX, C>(tw, (cp<cv<C>>) twsel<C>(R.twx, R.twx32));
+    __syncthreads();
+    const int t1 = (int) threadIdx.x / P2, n2 = (int) threadIdx.x - t1 * P2;
+    C a[P1];
+    if (t1 < nl) {
+#pragma unroll
+        for (int n1 = 0; n1 < P1; n1++) a[n1] = sb[t1 * LS + P2 * n1 + n2];
+    }
+    __syncthreads();
+    if (t1 < nl) pass1_rows<P1, P2>(a, sb + t1 * LS + n2, tw, n2, -1.0);
+    __syncthreads();
+    const int t2 = (int) threadIdx.x / P1, k1 = (int) threadIdx.x - t2 * P1;
+    if (t2 < nl) {
+        C u[P2];
+        const C *col = sb + t2 * LS + k1 * (P2 + 1);
+#pragma unroll
+        for (int q = 0; q < P2; q++) u[q] = col[q];
+        dft_regs<P2>(u, -1.0);
+        const int la = 2 * (pair0 + t2), lb = la + 1;
+        const gp<T> pa = p + (size_t) la * NX + k1, pb = pa + NX;
+#pragma unroll
+        for (int k2 = 0; k2 < P2; k2++) {
+            if (la < NLi) pa[P1 * k2] = u[k2].x;
+            if (lb < NLi) pb[P1 * k2] = u[k2].y;
+        }
+    }
+}
+
 /* inverse complex-to-real x FFT (fft.c:180-183; FFTW c2r semantics: the
  * imaginary parts of the kx = 0 and kx = nx/2 terms are ignored), two real
  * lines per complex transform, writing p */
@@ -1256,6 +1460,12 @@ template <int NX, typename T>
 static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typename Cplx<T>::type *__restrict__ X_,
                                                   T *__restrict__ p_, const Blk b)
 {
+    /* a run's pair count is fx_pairs_of(nx, its element size), which for
+     * the lengths compiled here in two passes is the same in FP32 and FP64 */
+    if constexpr (NX > 0 && fxi2_on(NX, sizeof(T))) {
+        fftxi2_body<NX, T>(R, X_, p_, b);
+        return;
+    }
     typedef typename Cplx<T>::type C;
     const gp<const cv<C>> X = (gp<const cv<C>>) X_;
     const gp<T> p = (gp<T>) p_;
@@ -1264,7 +1474,8 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
     const int nx = NX > 0 ? NX : R.g.nx;
     const int ny = R.g.ny, nz = R.g.nz, nkx = R.g.nkx;
     const int ld = fft_ld(nx);
-    const int nl = NX > 0 ? fx_pairs_of(NX) : R.fx_pairs;
+    const int nl = NX > 0 ? fx_pairs_of(NX, sizeof(T)) : R.fx_pairs;
+    const double rnl = 1.0 / (double) nl;
     const long NL = (long) ny * nz;
     const long pair0 = (long) xcd_block(b.x, b.nx) * nl;
     const int half = nx / 2;
@@ -1281,7 +1492,7 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
         for (int u = 0; u < CF_FXI_UNR; u++) {
             const int e = e0 + (int) threadIdx.x + u * 256;
             const int ec = e < tot ? e : tot - 1;
-            const int kx = ec / nl, l = ec - kx * nl;
+            const int kx = NX > 0 ? ec / nl : qdiv(ec, nl, rnl), l = ec - kx * nl;
             const int la = 2 * ((int) pair0 + l), lb = la + 1;
             const int lac = la < NLi ? la : NLi - 1;
             const int ka = qdiv(lac, ny, rny), ja = lac - ka * ny;
@@ -1298,7 +1509,7 @@ static __device__ __forceinline__ void fftxi_body(const RunDev &R, const typenam
         for (int u = 0; u < CF_FXI_UNR; u++) {
             const int e = e0 + (int) threadIdx.x + u * 256;
             if (e < tot) {
-                const int kx = e / nl, l = e - kx * nl;
+                const int kx = NX > 0 ? e / nl : qdiv(e, nl, rnl), l = e - kx * nl;
                 C a = A[u], bq = B[u];
                 if (kx == 0 || (nx % 2 == 0 && kx == half)) { a.y = (T) 0; bq.y = (T) 0; }
                 sb[l * ld + fsw(kx, fsw_on(nx))] = mkc<C>(a.x - bq.y, a.y + bq.x);
@@ -1674,7 +1885,12 @@ static __device__ __forceinline__ void pois2_body(const RunDev &R, const PoisTab
         __shared__ double sh[7][CF_P2S_NC][CF_P2S_GW];
         __shared__ double xs[CF_P2S_NC][CF_P2S_GW];
         const int nch = (nz + L - 1) / (L > 0 ? L : 1);
-        const size_t m0 = (size_t) b.x * CF_P2S_MW;
+        /* FP32 rows are half a cache line per workgroup: with CF_P2S_XCD the
+         * workgroups holding neighbouring modes share an XCD (and its L2;
+         * FP32 step 1.267 -> 1.243 ms at 256x193x256); the island launch
+         * covers only the leading workgroups and keeps the plain order */
+        const unsigned bx = (CF_P2S_XCD && ISL != 1) ? xcd_block(b.x, b.nx) : b.x;
+        const size_t m0 = (size_t) bx * CF_P2S_MW;
         const size_t mode = m0 + ((threadIdx.x >> 1) & (CF_P2S_MW - 1));
         const bool act = mode < M;
         const size_t mc = act ? mode : M - 1;
@@ -1853,7 +2069,7 @@ template __global__ void k_maxdiv_part<float>(const RunDev, const float *, const
 /* compiled lengths gather 2 elements per thread in FP32 (not 3) so that
  * the batch loop's extra registers still fit 4 waves per SIMD */
 template <int NX, int ORD, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && ORD == 2) ? 4 : 1, 8))) void kb_divx(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && ORD == 2) ? (divx2_on(NX, sizeof(T)) ? 2 : 4) : 1, 8))) void kb_divx(
     const GArgs<T> *__restrict__ A, const unsigned *__restrict__ vs, int nrun, int s)
 {
     constexpr int U = (NX > 0 && ORD == 2 && sizeof(T) == 4) ? 2 : 0;
@@ -1884,7 +2100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NY > 0 ? 4 
 }
 
 template <int NX, typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && NX <= 128) ? 4 : 1, 8))) void kb_fftxi(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NX > 0 && NX <= 128) ? (fxi2_on(NX, sizeof(T)) && sizeof(T) == 8 ? 2 : 4) : 1, 8))) void kb_fftxi(const GArgs<T> *__restrict__ a_, const unsigned *__restrict__ vs, int nrun, int s)
 {
     (void) s;
     batch_loop(vs, nrun, [&](int r, unsigned lb) {

@@ -1189,7 +1189,7 @@ static cf_status create_run(const cf_config *cfg, bool force_general, cf_run **o
     R.cmode[0] = cfg->uvolavgmode; R.ctarget[0] = cfg->want_uvolavg - cfg->uframe; R.cfixed[0] = -cfg->dpdx;
     R.cmode[1] = cfg->cvolavgmode; R.ctarget[1] = cfg->want_cvolavg; R.cfixed[1] = 0.0;
     R.logcap = LOGCAP;
-    R.fx_pairs = fx_pairs_of(nx);
+    R.fx_pairs = fx_pairs_of(nx, (int) r->esz);
     /* y lines per workgroup: 8 elements per thread (up to 32 short lines) */
     R.fy_cols = MAXE_LINE / ny >= 32 ? 32 : (MAXE_LINE / ny > 0 ? MAXE_LINE / ny : 1);
     /* the two-pass y FFT gives each of a workgroup's 256 threads at most
@@ -1258,7 +1258,7 @@ static dim3 cell_grid(const cf_run *r) { return dim3((r->g.nx + 63) / 64, (r->g.
 
 static size_t fx_lds(const cf_run *r)
 {
-    return 2 * r->esz * ((size_t) r->rd.fx_pairs * fft_ld(r->g.nx) + (size_t) r->g.nx);
+    return 2 * r->esz * ((size_t) r->rd.fx_pairs * fx_ld(r->g.nx, (int) r->esz) + (size_t) r->g.nx);
 }
 static size_t fy_lds(const cf_run *r)
 {
@@ -2005,7 +2005,7 @@ static cf_status stat_fft(cf_run *r, const double *src, double2 *dst)
     const RunDev &R = r->rd;
     const long NL = (long) r->g.ny * r->g.nz, npairs = (NL + 1) / 2;
     const unsigned gx = (unsigned) ((npairs + R.fx_pairs - 1) / R.fx_pairs);
-    const size_t lx = 2 * sizeof(double) * ((size_t) R.fx_pairs * fft_ld(r->g.nx) + (size_t) r->g.nx);
+    const size_t lx = 2 * sizeof(double) * ((size_t) R.fx_pairs * fx_ld(r->g.nx, (int) r->esz) + (size_t) r->g.nx);
     const size_t ly = 2 * sizeof(double) * ((size_t) R.fy_cols * fy_ld(r->g.ny) + (size_t) r->g.ny);
     hipLaunchKernelGGL(r->kfxp, dim3(gx), dim3(256), lx, r->stream, R, src, dst);
     hipLaunchKernelGGL(r->kfy, dim3(fy_blocks(r)), dim3(256), ly, r->stream, R, dst, 1);

@@ -373,6 +373,35 @@ static __device__ __forceinline__ double wave_sum(double v)
     return __hiloint2double(hi, lo);
 }
 
+/* the sums over a wave's 64 lanes of each of a lane's LM <= 32 values
+ * v[t], by recursive halving: at the exchange over lane bit m = 32, 16,
+ * ..., 2 a lane keeps the half of its values that bit selects, adds its
+ * partner's copy of that half and passes on the other half; a last
+ * exchange over bit 0 completes the sums.  Lanes 2t and 2t + 1 return the
+ * sum of v[t] (t < 32), each the same fixed tree over the 64 lanes.  32
+ * exchanges in place of 6 per value. */
+template <int LM>
+static __device__ __forceinline__ double wave_sums(const double *v)
+{
+    static_assert(LM <= 32, "wave_sums: at most 32 values per lane");
+    double a[32];
+#pragma unroll
+    for (int t = 0; t < 32; t++) a[t] = t < LM ? v[t] : 0.0;
+    const int lane = (int) threadIdx.x & 63;
+#pragma unroll
+    for (int st = 0; st < 5; st++) {
+        const int m = 32 >> st, half = 16 >> st;
+        const bool up = (lane & m) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const double send = up ? a[i] : a[i + half];
+            const double keep = up ? a[i + half] : a[i];
+            a[i] = keep + __shfl_xor(send, m, 64);
+        }
+    }
+    return a[0] + __shfl_xor(a[0], 1, 64);
+}
+
 /* n / d for 0 <= n < 2^31 and 1 <= d < 2^20, given rd = 1 / d correctly
  * rounded to double: n rd differs from n / d by at most (n / d) 2^-52 <
  * 2^-21, less than the distance 1 / d from a non-integer n / d to the

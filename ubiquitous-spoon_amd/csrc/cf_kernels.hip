@@ -574,16 +574,20 @@ static __device__ __forceinline__ void spike_body(const RunDev &R, const int s,
 #pragma unroll
         for (int q = 0; q < MP; q++) F += (T) row[q] * (q < m ? itf[q][lane] : (T) 0);
     }
+    double sv[LM];
 #pragma unroll
     for (int t = 0; t < LM; t++) {
+        sv[t] = 0.0;
         if (t < Lc) {                             /* wave-uniform */
             const T val = y[t] - E * (T) tvl[t] - F * (T) twr[t];
             if (act) xc[(size_t) (lo + t) * P] = val;
-            if (sums) {
-                const double tot = wave_sum(act ? (double) val : 0.0);
-                if (lane == 0) pl[(size_t) (lo + t) * b.nx + b.x] = tot;
-            }
+            sv[t] = act ? (double) val : 0.0;
         }
+    }
+    if (sums) {                                   /* wave-uniform; every lane takes part */
+        const double tot = wave_sums<LM>(sv);
+        const int t = lane >> 1;
+        if ((lane & 1) == 0 && t < Lc) pl[(size_t) (lo + t) * b.nx + b.x] = tot;
     }
     if (act && f == 2 && chunk == ST.S - 1) xc[(size_t) (nz - 1) * P] = (T) 0;   /* top wall face */
 }
@@ -597,8 +601,11 @@ __global__ __launch_bounds__(512) void k_spike(const RunDev R, const int s,
 
 /* the fast path's solve: k_spike with the plane sums of u* fused in
  * (accumulated in FP64 whatever the storage) */
+#ifndef CF_SPIKE_WPE
+#define CF_SPIKE_WPE 1
+#endif
 template <typename T, int LM>
-__global__ __launch_bounds__(512) void k_spike_sums(const RunDev R, const int s, T *__restrict__ ru,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(CF_SPIKE_WPE, 8))) void k_spike_sums(const RunDev R, const int s, T *__restrict__ ru,
     T *__restrict__ rv, T *__restrict__ rw, double *__restrict__ plane)
 {
     spike_body<T, LM>(R, s, ru, rv, rw, nullptr, blk_native(), plane);

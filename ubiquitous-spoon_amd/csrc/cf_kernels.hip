@@ -249,8 +249,7 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
 
     const SubCoef C = R.sc[s];
     const T nu = (T) R.nu;
-    const T dx = (T) R.g.dx, dy = (T) R.g.dy;
-    const T rdx = (T) R.rdx, rdy = (T) R.rdy, rdxdy = (T) R.rdxdy;
+    const T rdx = (T) R.rdx, rdy = (T) R.rdy;
     const T rdx2 = rdx * rdx, rdy2 = rdy * rdy;
     const T gmprev = (T) scal[R.m_prev_slot[s]];
     const T nuadt = (T) C.nuadt, gamdt = (T) C.gamdt, zetdt = (T) C.zetdt;
@@ -312,15 +311,13 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wkm = LW(sm, 0, 0), wipkm = LW(sm, 0, 1);
                 const T u0f = F(u0, Uk), uipf = F(uip, Uk), uimf = F(uim, Uk);
                 const T ujpf = F(ujp, Uk), ujmf = F(ujm, Uk), ukpf = F(ukp, Ukp), ukmf = F(ukm, Ukm);
-                T mxp = (T) 0.5 * (dy * dzk * u0f + dy * dzk * uipf);
-                T mxm = (T) 0.5 * (dy * dzk * uimf + dy * dzk * u0f);
-                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vip);
-                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * vipjm);
-                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wip);
-                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * wipkm);
-                T adv = -((mxp * (T) 0.5 * (u0f + uipf) - mxm * (T) 0.5 * (u0f + uimf))
-                             + (myp * (T) 0.5 * (u0f + ujpf) - mym * (T) 0.5 * (u0f + ujmf))
-                             + (mzp * (T) 0.5 * (u0f + ukpf) - mzm * (T) 0.5 * (u0f + ukmf))) * (rdxdy * rzk);
+                /* the face mass fluxes (dy dz, dx dz, dx dy times the face
+                 * velocity) times the face value of u, over the cell volume:
+                 * the cell-size factors cancel to 1/dx, 1/dy, 1/dz */
+                const T sxp = u0f + uipf, sxm = u0f + uimf;
+                T adv = -(T) 0.25 * ((sxp * sxp - sxm * sxm) * rdx
+                                     + ((v0 + vip) * (u0f + ujpf) - (vjm + vipjm) * (u0f + ujmf)) * rdy
+                                     + ((w0 + wip) * (u0f + ukpf) - (wkm + wipkm) * (u0f + ukmf)) * rzk);
                 T f = adv + nu * (((uip - u0) - (u0 - uim)) * rdx2 + ((ujp - u0) - (u0 - ujm)) * rdy2);
                 T h = u0 + nuadt * (((ukp - u0) * rzp - (u0 - ukm) * rzm) * rzk);
                 if constexpr (OFF) h += Uh;
@@ -337,15 +334,10 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T v0 = LV(s0, 0, 0), vip = LV(s0, 0, 1), vim = LV(s0, 0, -1);
                 const T vjp = LV(s0, 1, 0), vjm = LV(s0, -1, 0), vkp = LV(sp, 0, 0), vkm = LV(sm, 0, 0);
                 const T w0 = LW(s0, 0, 0), wjp = LW(s0, 1, 0), wkm = LW(sm, 0, 0), wjpkm = LW(sm, 1, 0);
-                T mxp = (T) 0.5 * (dy * dzk * F(u0, Uk) + dy * dzk * F(ujp, Uk));
-                T mxm = (T) 0.5 * (dy * dzk * F(uim, Uk) + dy * dzk * F(uimjp, Uk));
-                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzk * vjp);
-                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzk * v0);
-                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wjp);
-                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * wjpkm);
-                T adv = -((mxp * (T) 0.5 * (v0 + vip) - mxm * (T) 0.5 * (v0 + vim))
-                             + (myp * (T) 0.5 * (v0 + vjp) - mym * (T) 0.5 * (v0 + vjm))
-                             + (mzp * (T) 0.5 * (v0 + vkp) - mzm * (T) 0.5 * (v0 + vkm))) * (rdxdy * rzk);
+                const T syp = v0 + vjp, sym = vjm + v0;
+                T adv = -(T) 0.25 * (((F(u0, Uk) + F(ujp, Uk)) * (v0 + vip) - (F(uim, Uk) + F(uimjp, Uk)) * (v0 + vim)) * rdx
+                                     + (syp * syp - sym * sym) * rdy
+                                     + ((w0 + wjp) * (v0 + vkp) - (wkm + wjpkm) * (v0 + vkm)) * rzk);
                 T f = adv + nu * (((vip - v0) - (v0 - vim)) * rdx2 + ((vjp - v0) - (v0 - vjm)) * rdy2);
                 T h = v0 + nuadt * (((vkp - v0) * rzp - (v0 - vkm) * rzm) * rzk);
                 h += gamdt * f + zetdt * gcv;
@@ -361,15 +353,11 @@ static __device__ __forceinline__ void rhs2t_body(const RunDev &R, const int s, 
                 const T w0 = LW(s0, 0, 0), wip = LW(s0, 0, 1), wim = LW(s0, 0, -1);
                 const T wjp = LW(s0, 1, 0), wjm = LW(s0, -1, 0), wkp = LW(sp, 0, 0), wkm = LW(sm, 0, 0);
                 const T dzp1h = (T) 0.5 * (dz[k + 1] + dz[k]);
-                T mxp = (T) 0.5 * (dy * dzk * F(u0, Uk) + dy * dzkp * F(ukp, Ukp));
-                T mxm = (T) 0.5 * (dy * dzk * F(uim, Uk) + dy * dzkp * F(uimkp, Ukp));
-                T myp = (T) 0.5 * (dx * dzk * v0 + dx * dzkp * vkp);
-                T mym = (T) 0.5 * (dx * dzk * vjm + dx * dzkp * vjmkp);
-                T mzp = (T) 0.5 * (dx * dy * w0 + dx * dy * wkp);
-                T mzm = (T) 0.5 * (dx * dy * wkm + dx * dy * w0);
-                T adv = -((mxp * (T) 0.5 * (w0 + wip) - mxm * (T) 0.5 * (w0 + wim))
-                             + (myp * (T) 0.5 * (w0 + wjp) - mym * (T) 0.5 * (w0 + wjm))
-                             + (mzp * (T) 0.5 * (w0 + wkp) - mzm * (T) 0.5 * (w0 + wkm))) * (rdxdy * rzp);
+                const T szp = w0 + wkp, szm = wkm + w0;
+                T adv = -(T) 0.25 * rzp * (((dzk * F(u0, Uk) + dzkp * F(ukp, Ukp)) * (w0 + wip)
+                                            - (dzk * F(uim, Uk) + dzkp * F(uimkp, Ukp)) * (w0 + wim)) * rdx
+                                           + ((dzk * v0 + dzkp * vkp) * (w0 + wjp) - (dzk * vjm + dzkp * vjmkp) * (w0 + wjm)) * rdy
+                                           + (szp * szp - szm * szm));
                 T f = adv + nu * (((wip - w0) - (w0 - wim)) * rdx2 + ((wjp - w0) - (w0 - wjm)) * rdy2);
                 T h = w0 + nuadt * (((wkp - w0) * rzkp - (w0 - wkm) * rzk) * rzp);
                 h += gamdt * f + zetdt * gcw;

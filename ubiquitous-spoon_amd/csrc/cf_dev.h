@@ -373,6 +373,17 @@ static __device__ __forceinline__ double wave_sum(double v)
     return __hiloint2double(hi, lo);
 }
 
+/* the sum of one value per thread over a 256-thread workgroup: a DPP sum
+ * per wave, then the four wave sums in order (one barrier, fixed order);
+ * every thread returns the total.  sh: 4 doubles of LDS. */
+static __device__ __forceinline__ double block_sum256(double v, double *sh)
+{
+    const double w = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
+    __syncthreads();
+    return ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 /* the sums over a wave's 64 lanes of each of a lane's LM <= 32 values
  * v[t], by recursive halving: at the exchange over lane bit m = 32, 16,
  * ..., 2 a lane keeps the half of its values that bit selects, adds its

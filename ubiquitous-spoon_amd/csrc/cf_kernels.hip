@@ -630,16 +630,11 @@ template k_spike_sums_t<float> pick_spike_sums<float>(int);
 static __device__ __forceinline__ void level_sum_body(const double *__restrict__ plane, int nbx,
                                                       double *__restrict__ lsum, const int l)
 {
-    __shared__ double red[256];
+    __shared__ double red[4];
     double acc = 0.0;
     for (int q = threadIdx.x; q < nbx; q += 256) acc += plane[(size_t) l * nbx + q];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) lsum[l] = red[0];
+    const double tot = block_sum256(acc, red);
+    if (threadIdx.x == 0) lsum[l] = tot;
 }
 
 __global__ __launch_bounds__(256) void k_level_sum(const double *__restrict__ plane, int nbx,
@@ -671,7 +666,7 @@ static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const
     const double scale = 1.0 / ((double) nx * (double) ny);
     /* volume integral and height: per-thread partials over k = t, t + 256,
      * ..., then a fixed tree (deterministic, the same in every launch) */
-    __shared__ double rv[256], rl[256];
+    __shared__ double rv[4], rl[4];
     double pv = 0.0, pl = 0.0;
     const double *uo = kind == 0 && R.uoff ? R.uoff + 1 : nullptr;   /* stored u is u - U(z) */
     for (int l = threadIdx.x; l < nz; l += 256) {
@@ -680,19 +675,13 @@ static __device__ __forceinline__ void mass_finalize_body(const RunDev &R, const
         pv += a * dz[l];
         pl += dz[l];
     }
-    rv[threadIdx.x] = pv;
-    rl[threadIdx.x] = pl;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) { rv[threadIdx.x] += rv[threadIdx.x + w]; rl[threadIdx.x] += rl[threadIdx.x + w]; }
-        __syncthreads();
-    }
+    const double volsum = block_sum256(pv, rv), hsum = block_sum256(pl, rl);
     if (threadIdx.x != 0) return;
     double *A = avg + 1;                          /* A[-1 .. nz] */
     const double bc = kind == 0 ? R.ubc : R.cbot;
     A[-1] = 2.0 * bc - A[0];                      /* apply_bc_u2 ghosts of the solved field */
     A[nz] = 2.0 * (kind == 0 ? R.ubc : R.ctop) - A[nz - 1];
-    double vol = rv[0] / rl[0];
+    double vol = volsum / hsum;
     const double kap = kind == 0 ? R.nu : R.nu / R.prandtl;
     double bot = kap * (A[0] - A[-1]) / (0.5 * (dz[0] + dz[-1]));
     double top = kap * (A[nz] - A[nz - 1]) / (0.5 * (dz[nz] + dz[nz - 1]));

@@ -1078,6 +1078,61 @@ static __device__ __forceinline__ void divx_body(const RunDev &R, const T *__res
         }
         return ok ? (T) d : (T) 0;
     };
+    /* compiled order-2 lengths whose first pass is radix 4 with one
+     * butterfly per thread (n = 256, 4 line pairs): the lane of (pair l,
+     * i < n / 4) forms the divergence at x = i + m n / 4, m < 4 -- the
+     * first pass's inputs -- and takes that butterfly in registers, so the
+     * values go to LDS once, as the first pass's outputs (one LDS round
+     * trip and two barriers fewer); the other passes follow as usual */
+    constexpr int NQ = NX > 0 ? NX / 4 : 1;
+    if constexpr (NX > 0 && ORD == 2 && ct_rad_at(NX, 0) == 4 && fx_pairs_of(NX, sizeof(T)) * NQ == 256 && NQ == 64) {
+        if ((ny & 1) == 0) {                      /* workgroup-uniform */
+            const cp<cv<C>> twg0 = (cp<cv<C>>) twsel<C>(R.twx, R.twx32);
+            C *tws0 = sb + nl * ld;
+            const C tw0f = tw_first<NX, C>(twg0, nx);
+            const int l = (int) threadIdx.x >> 6, i = (int) threadIdx.x & 63;
+            const int la = 2 * (pair0 + l);
+            const bool ok = la < NL;
+            const int lc = ok ? la : NL - 2;
+            const int k = qdiv(lc, ny, rny), j = lc - k * ny;
+            const int jm = j > 0 ? j - 1 : ny - 1;
+            const size_t qa = (size_t) k * P + (size_t) j * nx, qb = qa + nx, qm = (size_t) k * P + (size_t) jm * nx;
+            const D hm = h1 ? (D) ((T) h1[k + 1] * m) : (D) 0;
+            const D rz = (D) ((T) rdz[k]);
+            C z[4];
+#pragma unroll
+            for (int g = 0; g < 4; g += 2) {
+#pragma unroll
+                for (int mm = g; mm < g + 2; mm++) {
+                    const int x = i + NQ * mm;
+                    const D ua = (D) us[qa + x] + hm, ub = (D) us[qb + x] + hm;
+                    D uam = __shfl_up(ua, 1, 64), ubm = __shfl_up(ub, 1, 64);
+                    if (i == 0) {
+                        const int xm = x > 0 ? x - 1 : nx - 1;
+                        uam = (D) us[qa + xm] + hm;
+                        ubm = (D) us[qb + xm] + hm;
+                    }
+                    const D va = vs[qa + x], vb = vs[qb + x], vam = vs[qm + x];
+                    const D war = ws[qa + x], wbr = ws[qb + x];
+                    const D wamr = ws[(k > 0 ? qa - P : qa) + x], wbmr = ws[(k > 0 ? qb - P : qb) + x];
+                    const D wa = k < nz - 1 ? war : (D) 0, wb = k < nz - 1 ? wbr : (D) 0;
+                    const D wam = k > 0 ? wamr : (D) 0, wbm = k > 0 ? wbmr : (D) 0;
+                    const D da = (ua - uam) * (D) rdx + (va - vam) * (D) rdy + (wa - wam) * rz;
+                    const D db = (ub - ubm) * (D) rdx + (vb - va) * (D) rdy + (wb - wbm) * rz;
+                    z[mm] = mkc<C>(ok ? (T) da : (T) 0, ok ? (T) db : (T) 0);
+                }
+            }
+            dft_small<4>(z, 1.0);
+            constexpr bool on = (NX & 63) == 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) sb[l * ld + fsw(4 * i + q, on)] = z[q];
+            tw_rest<NX, C>(tws0, tw0f, twg0, nx);
+            __syncthreads();
+            lds_fft_ct<NX, 8, 1, 4, 3>(sb, ld, nl, (const C *) tws0, 1.0);
+            store_half_spectra<C>(sb, ld, nl, nx, nkx, ny, (long) NL, (long) pair0, 0.5 / (double) nx, X);
+            return;
+        }
+    }
     /* groups of CF_FX_UNR elements per thread: all reads of a group are
      * in flight before the first LDS write (blockDim.x = 256) */
     constexpr int UNR = UNRX > 0 ? UNRX

@@ -97,10 +97,13 @@ def test_projection_matches_oracle():
         assert run.max_divergence() <= 1e-12
 
 
-@pytest.mark.parametrize("shape,steps", [((16, 12, 17), 3), ((32, 24, 33), 10), ((32, 24, 129), 5)])
+@pytest.mark.parametrize("shape,steps", [((16, 12, 17), 3), ((32, 24, 33), 10), ((32, 24, 129), 5),
+                                         ((256, 16, 17), 3), ((64, 256, 9), 3)])
 def test_step_parity_small(shape, steps):
     """Includes nz = 129, whose z solve runs as six spike chunks of 24 rows
-    with a short last chunk."""
+    with a short last chunk; nx = 256 (the divergence + x FFT taking its
+    first radix-4 pass in registers), and ny = 256 with nx = 64 (the 16 x 16
+    two-pass y FFT, the two-pass inverse x FFT at 64)."""
     cfg, dz, orc, init, f = projected_case(*shape)
     olog = orc.steps(steps)
     with cf.Run(cfg) as run:

@@ -719,8 +719,13 @@ static __device__ __forceinline__ void mass_body(const RunDev &R, const int s, c
         avg[l + 3] = acc * scale;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
     double *A = avg + 3;
+    /* the volume integral and the height as fixed-order block sums */
+    __shared__ double shv[4], shl[4];
+    double pv = 0.0, pl = 0.0;
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) { pv += A[k] * dz[k]; pl += dz[k]; }
+    const double volsum = block_sum256(pv, shv), lzsum = block_sum256(pl, shl);
+    if (threadIdx.x != 0) return;
     const double bcb = kind == 0 ? R.ubc : R.cbot, bct = kind == 0 ? R.ubc : R.ctop;
     for (int m = 0; m < 3; m++) {
         const Clos1 &B = R.cl.cell[0], &T = R.cl.cell[1];
@@ -729,9 +734,7 @@ static __device__ __forceinline__ void mass_body(const RunDev &R, const int s, c
         A[-1 - m] = B.G[m][0] * A[0] + B.G[m][1] * b1 + B.G[m][2] * b2 + B.B[m] * bcb;
         A[nz + m] = T.G[m][0] * A[nz - 1] + T.G[m][1] * t1 + T.G[m][2] * t2 + T.B[m] * bct;
     }
-    double vol = 0.0, lz = 0.0;
-    for (int k = 0; k < nz; k++) { vol += A[k] * dz[k]; lz += dz[k]; }
-    vol = vol / lz;
+    double vol = volsum / lzsum;
     const double kap = kind == 0 ? R.nu : R.kap;
     double nb_, nt_;
     if (ord == 2) {

@@ -672,7 +672,7 @@ template <typename T>
 static __device__ __forceinline__ void plane_body(const RunDev &R, const T *__restrict__ q0,
                                                 const T *__restrict__ q1, double *__restrict__ part, int nb, const Blk b)
 {
-    __shared__ double red[256];
+    __shared__ double red[4];
     const int nx = R.g.nx, ny = R.g.ny, nz = R.g.nz;
     const size_t P = (size_t) nx * ny;
     const int lev = b.y, kind = b.z, bb = b.x;
@@ -681,14 +681,11 @@ static __device__ __forceinline__ void plane_body(const RunDev &R, const T *__re
     const size_t lo = (size_t) bb * chunk, hi = lo + chunk < P ? lo + chunk : P;
     const T *pl = q + (size_t) lev * P;
     double acc = 0.0;                             /* FP64 accumulation for either storage */
+    /* unrolled so that a thread's loads are in flight together */
+#pragma unroll 8
     for (size_t e = lo + threadIdx.x; e < hi; e += 256) acc += (double) pl[e];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[((size_t) kind * nz + lev) * nb + bb] = red[0];
+    const double tot = block_sum256(acc, red);    /* DPP wave sums, then the four in order */
+    if (threadIdx.x == 0) part[((size_t) kind * nz + lev) * nb + bb] = tot;
 }
 
 template <typename T>

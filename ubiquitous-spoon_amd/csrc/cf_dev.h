@@ -373,6 +373,29 @@ static __device__ __forceinline__ double wave_sum(double v)
     return __hiloint2double(hi, lo);
 }
 
+/* the maximum over a 64-lane wave of non-negative values (lanes a DPP
+ * step does not reach read 0, which leaves such a maximum unchanged),
+ * and over a 256-thread workgroup (sh: 4 doubles of LDS, one barrier) */
+static __device__ __forceinline__ double wave_max_nonneg(double v)
+{
+    v = fmax(v, dpp_f64<0xB1, 0xf>(v));
+    v = fmax(v, dpp_f64<0x4E, 0xf>(v));
+    v = fmax(v, dpp_f64<0x141, 0xf>(v));
+    v = fmax(v, dpp_f64<0x140, 0xf>(v));
+    v = fmax(v, dpp_f64<0x142, 0xa>(v));
+    v = fmax(v, dpp_f64<0x143, 0xc>(v));
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+static __device__ __forceinline__ double block_max256_nonneg(double v, double *sh)
+{
+    const double w = wave_max_nonneg(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = w;
+    __syncthreads();
+    return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
 /* the sum of one value per thread over a 256-thread workgroup: a DPP sum
  * per wave, then the four wave sums in order (one barrier, fixed order);
  * every thread returns the total.  sh: 4 doubles of LDS. */

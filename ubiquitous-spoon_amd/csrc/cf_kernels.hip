@@ -743,22 +743,18 @@ __global__ __launch_bounds__(256) void k_mass_finalize(const RunDev R, const int
 static __device__ __forceinline__ void cfl_body(const RunDev &R, const double *__restrict__ part, int nb,
                                                     double dt, double *__restrict__ scal, double *__restrict__ logbuf, const Blk b)
 {
-    __shared__ double red[256];
-    double m = 0.0;
+    __shared__ double red[4];
+    double m = 0.0;                               /* the block maxima are >= 0 */
+#pragma unroll 4
     for (int b = threadIdx.x; b < nb; b += 256) m = fmax(m, part[b]);
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
+    const double mx = block_max256_nonneg(m, red);
     if (threadIdx.x != 0) return;
     long idx = (long) scal[SC_STEP] % R.logcap;
     double *lg = logbuf + (size_t) idx * 13;
     for (int t = 0; t < 13; t++) lg[t] = 0.0;
     lg[0] = scal[SC_STEP] + 1.0;
     lg[1] = dt;
-    lg[2] = R.cfl_log > 0.0 ? R.cfl_log : red[0];
+    lg[2] = R.cfl_log > 0.0 ? R.cfl_log : mx;
     scal[SC_CFL] = lg[2];
 }
 
